@@ -1,0 +1,111 @@
+// cc_common.hpp -- shared constants, device helpers and launch arguments of
+// the MI355X exact-transform engine (gfx950, wave64).
+//
+// The engine holds one problem cell: a mixed-radix product space
+// X = Z_{a_0} x ... x Z_{a_{n-1}} (little-endian index, axis 0 contiguous),
+// L = R+1 int32 distance-layer planes (1 plane for the L-inf families), a
+// uint16 multiplicity plane `cnt` padded to an even length so packed 32-bit
+// atomics never leave it, and the v1 support table.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#define CC_MAXN 16      // axes
+#define CC_MAXAX 16     // axis size (nibble-packed support deltas)
+#define CC_MAXR 6       // Hamming / Lee radius (layer planes R+1 <= 7)
+#define CC_MAXL (CC_MAXR + 1)
+#define CC_MAXPAT 8     // bytes per packed v1 pattern
+#define CC_CSTRIDE 15   // per-position slots of the walk displacement table
+#define CC_BLOCK 256    // threads per workgroup (4 waves of 64)
+
+// problem ids (ABI values, reference src/include/coldcore/plugin.h:45-53)
+#define CC_HAMMING 0
+#define CC_TORUS 1
+#define CC_LEE 2
+#define CC_GRID 3
+
+// cnt predicates (reference src/plugins/covering/dctcore_mu.cuh:29-40)
+#define CC_EQ 0     // cnt == target
+#define CC_DEF 1    // cnt < mu          (deficient: gain role)
+#define CC_CRIT 2   // 1 <= cnt <= mu    (critical: loss role)
+
+__device__ __forceinline__ bool cc_pred(int mode, int c, int target, int mu) {
+  if (mode == CC_DEF) return c < mu;
+  if (mode == CC_CRIT) return c >= 1 && c <= mu;
+  return c == target;
+}
+
+// ---------------------------------------------------------------- wave64
+__device__ __forceinline__ long long cc_wave_sum(long long v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ __forceinline__ int cc_wave_max(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// inclusive prefix sum over the 64 lanes of a wave
+__device__ __forceinline__ int cc_wave_incl_scan(int v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    int u = __shfl_up(v, o, 64);
+    if (lane >= o) v += u;
+  }
+  return v;
+}
+
+// block-wide sum for CC_BLOCK threads; the result is valid in every thread
+__device__ __forceinline__ long long cc_block_sum(long long v) {
+  __shared__ long long red[CC_BLOCK / 64];
+  v = cc_wave_sum(v);
+  const int wid = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) red[wid] = v;
+  __syncthreads();
+  long long t = 0;
+#pragma unroll
+  for (int i = 0; i < CC_BLOCK / 64; i++) t += red[i];
+  __syncthreads();
+  return t;
+}
+
+// ---------------------------------------------------------------- sweeps
+// One sweep applies the per-axis passes of a group of consecutive axes
+// [j0, j0 + nax) to every tile of the space.  A tile is the set of words
+// base + p * s + tl, p < P (the group's sub-space, in its own mixed radix)
+// and tl < TL consecutive columns (TL = 1 when the group starts at axis 0,
+// where the tile is one contiguous run).  The tile's live layers are staged
+// in LDS as S[d][p * TL + tl]; every axis of the group runs in LDS, so one
+// HBM round trip covers the whole group.
+#define CC_IN_LAYERS 0   // layers 0..Lin-1 come from the planes
+#define CC_IN_PRED 1     // layer 0 = predicate(cnt), the rest zero (first sweep)
+#define CC_IN_CODE 2     // layer 0 = code indicator already in A0 (first sweep)
+#define CC_OUT_LAYERS 0  // write back the layers this sweep produced
+#define CC_OUT_MAP 1     // write sum_d A_d into A0 (gain / loss map)
+#define CC_OUT_CNT 2     // write min(sum_d A_d, 65535) into cnt (recount)
+
+struct SweepArgs {
+  int32_t *A[CC_MAXL];
+  uint16_t *cnt;
+  long long s;       // stride (words) of the group's first axis
+  long long ncolb;   // column blocks per outer index = ceil(s / TL)
+  long long ntiles;  // tiles in the space = (|X| / (s * P)) * ncolb
+  int P;             // words per column inside the group
+  int TL;            // columns per tile
+  int nax;           // axes in the group
+  int c0;            // axes transformed before this sweep
+  int R;
+  int in_kind, out_kind, pmode, mu;
+  int write_l0;      // CC_OUT_LAYERS: also store layer 0 (first sweep only)
+  int Q[CC_MAXN];    // axis sizes of the group
+  int sk[CC_MAXN];   // in-tile stride (units of p) of each group axis
+};
+
+// ---------------------------------------------------------------- walks
+struct WordsArg {
+  long long w[128];  // small batches travel by value (no staging, no sync)
+};

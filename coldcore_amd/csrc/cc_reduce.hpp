@@ -1,0 +1,228 @@
+// cc_reduce.hpp -- whole-field reductions and queries.
+//
+// Semantics follow reference src/coldcore/protocol.py:181-219 and
+// src/plugins/covering/dctcore_fast.cuh:199-327:
+//   count_eq(t)   sum [cnt == t]                   (int64)
+//   deficit       sum max(0, mu - cnt)             (int64)
+//   map_max       max(0, max A0)
+//   map_hist      positive v -> bin (v-1)*nbins/vmax, clipped to nbins-1
+//   map_extract   (idx, val) of A0 >= thr; found counts every hit
+// The extraction here is a deterministic compaction: blocks own contiguous
+// chunks, a scan of the per-chunk counts gives each chunk its output offset,
+// and hits are written in ascending index order.  That is a strict
+// refinement of the reference's unordered contract and equals the host
+// backends' order (protocol.py:201-208), which keeps search trajectories
+// identical across engines.
+#pragma once
+#include "cc_common.hpp"
+
+__global__ void k_fill32(int32_t *p, int32_t v, long long n) {
+  const long long gs = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += gs)
+    p[i] = v;
+}
+
+__global__ void k_fill16(uint16_t *p, uint16_t v, long long n) {
+  const long long gs = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += gs)
+    p[i] = v;
+}
+
+__global__ void k_fill_u64(unsigned long long *p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+
+// A0[idx[i]] += 1 : the code multiset indicator
+__global__ void k_scatter_code(int32_t *A0, const long long *idx, int M) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < M) atomicAdd(&A0[idx[i]], 1);
+}
+
+// MODE CC_EQ: sum [cnt == target];  CC_DEF: sum max(0, mu - cnt)
+template <int MODE>
+__global__ void __launch_bounds__(CC_BLOCK)
+    k_cnt_reduce(const uint16_t *__restrict__ cnt, long long N, int target,
+                 int mu, unsigned long long *out) {
+  const long long nvec = N >> 3;
+  const long long gs = (long long)gridDim.x * CC_BLOCK;
+  const uint4 *v = (const uint4 *)cnt;
+  long long acc = 0;
+  for (long long i = (long long)blockIdx.x * CC_BLOCK + threadIdx.x; i < nvec;
+       i += gs) {
+    const uint4 q = v[i];
+    const uint32_t wv[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+    for (int h = 0; h < 8; h++) {
+      const int c = (int)((wv[h >> 1] >> (16 * (h & 1))) & 0xFFFFu);
+      if (MODE == CC_EQ) acc += (c == target);
+      else { const int d = mu - c; acc += d > 0 ? d : 0; }
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x < (int)(N & 7)) {
+    const int c = (int)cnt[nvec * 8 + threadIdx.x];
+    if (MODE == CC_EQ) acc += (c == target);
+    else { const int d = mu - c; acc += d > 0 ? d : 0; }
+  }
+  acc = cc_block_sum(acc);
+  if (threadIdx.x == 0 && acc) atomicAdd(out, (unsigned long long)acc);
+}
+
+__global__ void __launch_bounds__(CC_BLOCK)
+    k_max_i32(const int32_t *__restrict__ A, long long N, int *out) {
+  const long long nvec = N >> 2;
+  const long long gs = (long long)gridDim.x * CC_BLOCK;
+  const int4 *v = (const int4 *)A;
+  int m = 0;
+  for (long long i = (long long)blockIdx.x * CC_BLOCK + threadIdx.x; i < nvec;
+       i += gs) {
+    const int4 q = v[i];
+    m = max(m, max(max(q.x, q.y), max(q.z, q.w)));
+  }
+  if (blockIdx.x == 0 && threadIdx.x < (int)(N & 3))
+    m = max(m, A[nvec * 4 + threadIdx.x]);
+  m = cc_wave_max(m);
+  if ((threadIdx.x & 63) == 0 && m > 0) atomicMax(out, m);
+}
+
+__device__ __forceinline__ void cc_hist_one(int32_t v, int nbins, int32_t vmax,
+                                            int32_t *sh) {
+  if (v > 0) {
+    long long b = (long long)(v - 1) * nbins / vmax;
+    if (b >= nbins) b = nbins - 1;
+    atomicAdd(&sh[b], 1);
+  }
+}
+
+__global__ void __launch_bounds__(CC_BLOCK)
+    k_hist(const int32_t *__restrict__ A, long long N, int nbins, int32_t vmax,
+           int32_t *hist) {
+  extern __shared__ int32_t sh[];
+  for (int i = threadIdx.x; i < nbins; i += CC_BLOCK) sh[i] = 0;
+  __syncthreads();
+  const long long nvec = N >> 2;
+  const long long gs = (long long)gridDim.x * CC_BLOCK;
+  const int4 *v = (const int4 *)A;
+  for (long long i = (long long)blockIdx.x * CC_BLOCK + threadIdx.x; i < nvec;
+       i += gs) {
+    const int4 q = v[i];
+    cc_hist_one(q.x, nbins, vmax, sh);
+    cc_hist_one(q.y, nbins, vmax, sh);
+    cc_hist_one(q.z, nbins, vmax, sh);
+    cc_hist_one(q.w, nbins, vmax, sh);
+  }
+  if (blockIdx.x == 0 && threadIdx.x < (int)(N & 3))
+    cc_hist_one(A[nvec * 4 + threadIdx.x], nbins, vmax, sh);
+  __syncthreads();
+  for (int i = threadIdx.x; i < nbins; i += CC_BLOCK)
+    if (sh[i]) atomicAdd(&hist[i], sh[i]);
+}
+
+// ---- ordered extraction: chunk counts -> exclusive scan -> ordered write
+// Chunk b covers [b*CH, min(N, (b+1)*CH)); CH is a multiple of 4*CC_BLOCK.
+__device__ __forceinline__ int cc_hits4(const int32_t *A, long long N,
+                                        long long i, int32_t thr,
+                                        int32_t e[4]) {
+  if (i + 3 < N) {
+    const int4 q = *(const int4 *)(A + i);
+    e[0] = q.x; e[1] = q.y; e[2] = q.z; e[3] = q.w;
+  } else {
+#pragma unroll
+    for (int j = 0; j < 4; j++) e[j] = (i + j < N) ? A[i + j] : INT32_MIN;
+  }
+  int m = 0;
+#pragma unroll
+  for (int j = 0; j < 4; j++) m |= (e[j] >= thr && i + j < N) ? (1 << j) : 0;
+  return m;
+}
+
+__global__ void __launch_bounds__(CC_BLOCK)
+    k_extract_count(const int32_t *__restrict__ A, long long N, int32_t thr,
+                    long long CH, long long *bcnt) {
+  const long long c0 = (long long)blockIdx.x * CH;
+  const long long c1 = min(N, c0 + CH);
+  long long acc = 0;
+  for (long long i = c0 + 4LL * threadIdx.x; i < c1; i += 4LL * CC_BLOCK) {
+    int32_t e[4];
+    acc += __popc(cc_hits4(A, c1, i, thr, e));
+  }
+  acc = cc_block_sum(acc);
+  if (threadIdx.x == 0) bcnt[blockIdx.x] = acc;
+}
+
+// exclusive scan of nb chunk counts by one workgroup of 1024 threads;
+// boff[nb] = total
+__global__ void __launch_bounds__(1024)
+    k_scan_offsets(const long long *bcnt, long long nb, long long *boff) {
+  __shared__ long long part[1024];
+  const int t = threadIdx.x;
+  const long long per = (nb + 1023) / 1024;
+  const long long lo = t * per, hi = min(nb, lo + per);
+  long long s = 0;
+  for (long long i = lo; i < hi; i++) s += bcnt[i];
+  part[t] = s;
+  __syncthreads();
+  if (t == 0) {
+    long long run = 0;
+    for (int i = 0; i < 1024; i++) {
+      const long long v = part[i];
+      part[i] = run;
+      run += v;
+    }
+    boff[nb] = run;
+  }
+  __syncthreads();
+  long long run = part[t];
+  for (long long i = lo; i < hi; i++) {
+    boff[i] = run;
+    run += bcnt[i];
+  }
+}
+
+template <typename IDX>
+__global__ void __launch_bounds__(CC_BLOCK)
+    k_extract_write(const int32_t *__restrict__ A, long long N, int32_t thr,
+                    long long CH, const long long *boff, long long cap,
+                    IDX *oidx, int32_t *oval) {
+  __shared__ int wtot[CC_BLOCK / 64];
+  const long long c0 = (long long)blockIdx.x * CH;
+  const long long c1 = min(N, c0 + CH);
+  long long run = boff[blockIdx.x];
+  if (run >= cap) return;  // block-uniform: nothing of this chunk is stored
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (long long b = c0; b < c1; b += 4LL * CC_BLOCK) {
+    const long long i = b + 4LL * threadIdx.x;
+    int32_t e[4] = {0, 0, 0, 0};
+    const int m = (i < c1) ? cc_hits4(A, c1, i, thr, e) : 0;
+    const int c = __popc(m);
+    const int incl = cc_wave_incl_scan(c);
+    if (lane == 63) wtot[wid] = incl;
+    __syncthreads();
+    int before = 0, tot = 0;
+#pragma unroll
+    for (int k = 0; k < CC_BLOCK / 64; k++) {
+      before += (k < wid) ? wtot[k] : 0;
+      tot += wtot[k];
+    }
+    long long pos = run + before + incl - c;
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+      if ((m >> j) & 1) {
+        if (pos < cap) {
+          oidx[pos] = (IDX)(i + j);
+          oval[pos] = e[j];
+        }
+        pos++;
+      }
+    run += tot;
+    __syncthreads();
+  }
+}
+
+__global__ void k_read_at(const int32_t *A, const long long *idx, int k,
+                          int32_t *out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < k) out[i] = A[idx[i]];
+}

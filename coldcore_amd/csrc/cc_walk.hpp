@@ -1,0 +1,95 @@
+// cc_walk.hpp -- support-set walks: exact incremental updates and exact
+// per-candidate gathers over one word's ball.
+//
+// The ball of centre w is w + D(p) for every v1 pattern p, where D sums the
+// per-(position, delta) board displacement ((dig_pos + delta) mod ax_pos -
+// dig_pos) * stride_pos (reference src/plugins/covering/dctcore_core.cuh:
+// 158-212).  The open grid resolves the same circular delta against the
+// centre digit and drops a pattern whose displacement leaves the L-inf
+// radius, which is a wrap past the board edge (dctcore_grid.cuh:239-286).
+//
+// One workgroup walks one contiguous segment of one word's table
+// (blockIdx = word * split + segment).  The displacement table lives in LDS
+// (16 x 15 int64); each digit is computed by the thread that needs it, so
+// no thread serialises the mixed-radix decomposition.  Updates are packed
+// uint16 +-1 on the aligned 32-bit word (the plane is padded to an even
+// length); gathers reduce over the wave with DPP shuffles.
+#pragma once
+#include "cc_common.hpp"
+
+struct Geom {
+  long long pw[CC_MAXN];
+  int ax[CC_MAXN];
+  int n, R, mu;
+};
+
+#define CC_WALK_UPDATE 0  // cnt += delta over the ball
+#define CC_WALK_TRANS 1   // ... and count deficit transitions (update_delta)
+#define CC_WALK_GATHER 2  // count predicate(cnt) over the ball
+
+template <int KIND, bool GRID>
+__global__ void __launch_bounds__(CC_BLOCK)
+    k_walk(Geom g, WordsArg wa, const long long *__restrict__ wptr, int split,
+           int delta, uint16_t *cnt, const uint64_t *__restrict__ pat,
+           long long blen, int pmode, int target, int32_t *out,
+           unsigned long long *trans) {
+  __shared__ long long disp[CC_MAXN * CC_CSTRIDE];
+  __shared__ unsigned char okf[CC_MAXN * CC_CSTRIDE];
+  const int wi = blockIdx.x / split, seg = blockIdx.x - wi * split;
+  const long long w = wptr ? wptr[wi] : wa.w[wi];
+  for (int t = threadIdx.x; t < CC_MAXN * CC_CSTRIDE; t += CC_BLOCK) {
+    const int p = t / CC_CSTRIDE, dl = t - p * CC_CSTRIDE + 1;
+    long long off = 0;
+    unsigned char ok = 0;
+    if (p < g.n && dl < g.ax[p]) {
+      const int a = g.ax[p];
+      const int dig = (int)((w / g.pw[p]) % a);
+      const int sh = (dig + dl < a) ? dl : dl - a;
+      off = (long long)sh * g.pw[p];
+      ok = (sh <= g.R && sh >= -g.R) ? 1 : 0;
+    }
+    disp[t] = off;
+    okf[t] = ok;
+  }
+  __syncthreads();
+  long long acc = 0;
+  const long long step = (long long)split * CC_BLOCK;
+  for (long long t = (long long)seg * CC_BLOCK + threadIdx.x; t < blen;
+       t += step) {
+    const uint64_t pt = pat[t];
+    long long x = w;
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < CC_MAXPAT; k++) {
+      const unsigned b = (unsigned)(pt >> (8 * k)) & 255u;
+      if (!b) break;
+      const int e = (int)(b >> 4) * CC_CSTRIDE + (int)(b & 15u) - 1;
+      x += disp[e];
+      if (GRID) ok = ok && okf[e];
+    }
+    if (GRID && !ok) continue;
+    if (KIND == CC_WALK_GATHER) {
+      acc += cc_pred(pmode, (int)cnt[x], target, g.mu) ? 1 : 0;
+    } else {
+      uint32_t *cell = (uint32_t *)(cnt + (x & ~1LL));
+      const int sh = (int)(x & 1) * 16;
+      const uint32_t add = 1u << sh;
+      const uint32_t old = delta > 0 ? atomicAdd(cell, add) : atomicSub(cell, add);
+      if (KIND == CC_WALK_TRANS) {
+        const int c = (int)((old >> sh) & 0xFFFFu);
+        acc += delta > 0 ? (c < g.mu) : (c <= g.mu);
+      }
+    }
+  }
+  if (KIND != CC_WALK_UPDATE) {
+    acc = cc_block_sum(acc);
+    if (threadIdx.x == 0) {
+      if (KIND == CC_WALK_GATHER) {
+        if (split == 1) out[wi] = (int32_t)acc;
+        else if (acc) atomicAdd(&out[wi], (int32_t)acc);
+      } else if (acc) {
+        atomicAdd(trans, (unsigned long long)acc);
+      }
+    }
+  }
+}

@@ -1,0 +1,926 @@
+// dct_engine.hip -- the plugin ABI v1 (`dct_*`) over HIP on MI355X (gfx950).
+//
+// Drop-in for the reference's libcoldcore_covering.so
+// (reference src/plugins/covering/dct_hc.cu, contract src/include/coldcore/
+// plugin.h:62-138 and docs/PLUGIN.md:71-160): same flat symbols, argument
+// meaning, return codes (0 ok, -1 invalid / rejected, -2 space > 2^40,
+// -3/-4/-5 allocation, -(100 + hipError_t) device error with a message on
+// stderr) and validation-before-mutation rules.  One cell per process
+// (singleton); every device operation runs in call order on one stream
+// owned by the engine; calls that return data synchronise once; small
+// ball_update batches return without waiting.
+//
+// Everything is HBM-resident (288 GB per MI355X): the layers_mode/cnt_mode
+// placement arguments are accepted for ABI compatibility and all planes go
+// to device memory, so dct_host_bytes() is 0.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <climits>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <unordered_map>
+#include <vector>
+
+#include "cc_common.hpp"
+#include "cc_reduce.hpp"
+#include "cc_sweep.hpp"
+#include "cc_walk.hpp"
+
+#define CK(x)                                                               \
+  do {                                                                      \
+    hipError_t e_ = (x);                                                    \
+    if (e_ != hipSuccess) {                                                 \
+      fprintf(stderr, "HIP error %s at %s:%d\n", hipGetErrorString(e_),     \
+              __FILE__, __LINE__);                                          \
+      return -(100 + (int)e_);                                              \
+    }                                                                       \
+  } while (0)
+
+namespace {
+
+constexpr int MAX_CODE = 65535;        // live occurrences (uint16 exactness)
+constexpr int CODECAP = 1 << 16;       // staging words per chunk
+constexpr int ASYNC_UPDATE_MAX = 1024; // larger update batches synchronise
+constexpr int NV_WORDS = 128;          // words per by-value launch
+constexpr long long EXTRACT_CH = 1 << 14;  // words per extraction chunk
+
+struct Sweep {
+  int j0, nax, c0, P, TL;
+  long long s, ncolb, ntiles;
+  int lmax;
+  size_t lds;
+};
+
+struct Engine {
+  bool live = false;
+  int prob = 0, n = 0, R = 0, L = 1, mu = 1;
+  int ax[CC_MAXN] = {};
+  long long pw[CC_MAXN + 1] = {};
+  long long X = 0;
+  int32_t *A[CC_MAXL] = {};
+  uint16_t *cnt = nullptr;
+  long long cnt_len = 0;
+  uint64_t *pat = nullptr, *pat2 = nullptr;
+  long long blen = 0;
+  int fmt = 0, use_owner = 0;
+  long long *d_code = nullptr;
+  int codeM = 0;
+  long long *d_words = nullptr;
+  int32_t *d_out = nullptr;
+  unsigned long long *d_scal = nullptr;  // [0] reductions [1] transitions
+  int32_t *d_hist = nullptr;
+  int64_t *d_eidx64 = nullptr;
+  uint32_t *d_eidx32 = nullptr;
+  int32_t *d_eval = nullptr;
+  int ecap = 0;
+  long long *d_bcnt = nullptr, *d_boff = nullptr;
+  long long nchunks = 0;
+  long long *h_words = nullptr;
+  int32_t *h_out = nullptr;
+  unsigned long long *h_scal = nullptr;
+  hipStream_t st = nullptr;
+  long long bytes_dev = 0;
+  long long stats[8] = {};
+  unsigned long long trans_last = 0;
+  std::unordered_map<long long, int> uploaded, active;
+  int active_size = 0;
+  std::vector<Sweep> plan;
+  int lds_budget = 64 * 1024;
+  // optional per-sweep timing of the last transform (additive ABI)
+  int prof = 0;
+  hipEvent_t ev[2 * CC_MAXN] = {};
+  float sweep_ms[CC_MAXN] = {};
+  int nswept = 0;
+};
+
+Engine E;
+
+Geom make_geom() {
+  Geom g;
+  memset(&g, 0, sizeof(g));
+  for (int i = 0; i < E.n; i++) {
+    g.pw[i] = E.pw[i];
+    g.ax[i] = E.ax[i];
+  }
+  g.n = E.n;
+  g.R = E.R;
+  g.mu = E.mu;
+  return g;
+}
+
+bool valid_words(const long long *w, int n) {
+  if (n < 0 || n > MAX_CODE || (n && !w)) return false;
+  for (int i = 0; i < n; i++)
+    if (w[i] < 0 || w[i] >= E.X) return false;
+  return true;
+}
+
+int pass_blocks(long long items) {
+  long long b = (items + CC_BLOCK * 8LL - 1) / (CC_BLOCK * 8LL);
+  return (int)std::max(1LL, std::min(b, 8192LL));
+}
+
+// Split blocks per word so a single-word walk still spreads over the chip
+// (reference dct_hc.cu:523-531): min(8192/nwords, ceil(|B|/256), 65535).
+int pick_split(int nwords) {
+  long long cover = (E.blen + CC_BLOCK - 1) / CC_BLOCK;
+  long long split = 8192 / std::max(nwords, 1);
+  split = std::max(1LL, std::min(split, cover));
+  return (int)std::min(split, 65535LL);
+}
+
+// A v1 table is walked without per-entry bounds checks, so it is proven
+// well-formed first: positions < n and distinct, 1 <= delta < ax_pos, zero
+// byte terminates.  Any such pattern keeps every walked index inside X.
+bool table_ok(const uint64_t *pat, long long len) {
+  for (long long t = 0; t < len; t++) {
+    const uint64_t p = pat[t];
+    unsigned seen = 0;
+    bool ended = false;
+    for (int k = 0; k < CC_MAXPAT; k++) {
+      const unsigned b = (unsigned)(p >> (8 * k)) & 255u;
+      if (!b) { ended = true; continue; }
+      if (ended) return false;
+      const int pos = (int)(b >> 4), d = (int)(b & 15u);
+      if (pos >= E.n || ((seen >> pos) & 1u) || d < 1 || d >= E.ax[pos])
+        return false;
+      seen |= 1u << pos;
+    }
+  }
+  return true;
+}
+
+// ---- sweep schedule ------------------------------------------------------
+// Greedy grouping of consecutive axes.  The first group starts at axis 0,
+// so its tile is one contiguous run of P words; later groups stage TL = 32
+// columns (a 128-byte line per row).  A group grows while its LDS tile
+// (live layers x P x TL int32) stays within the budget.
+int layers_after(int c) { return E.L == 1 ? 1 : std::min(c, E.R) + 1; }
+
+void build_plan() {
+  E.plan.clear();
+  int j = 0;
+  while (j < E.n) {
+    Sweep sw;
+    sw.j0 = j;
+    sw.c0 = j;
+    sw.s = E.pw[j];
+    sw.TL = (j == 0) ? 1 : (int)std::min<long long>(32, sw.s);
+    long long P = 1;
+    int k = j;
+    while (k < E.n) {
+      long long P2 = P * E.ax[k];
+      size_t need = (size_t)layers_after(k + 1) * P2 * sw.TL * 4;
+      if (k > j && (need > (size_t)E.lds_budget || P2 > (1LL << 20))) break;
+      P = P2;
+      k++;
+    }
+    sw.nax = k - j;
+    sw.P = (int)P;
+    sw.lmax = layers_after(k);
+    sw.ncolb = (sw.s + sw.TL - 1) / sw.TL;
+    sw.ntiles = (E.X / (sw.s * P)) * sw.ncolb;
+    sw.lds = (size_t)sw.lmax * P * sw.TL * 4;
+    E.plan.push_back(sw);
+    j = k;
+  }
+}
+
+template <int PROB, int R1>
+hipError_t launch_sweep_t(const SweepArgs &a, int blocks, size_t lds) {
+  static bool attr = false;
+  if (!attr && lds > 64 * 1024) {
+    hipError_t e = hipFuncSetAttribute((const void *)k_sweep<PROB, R1>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds);
+    if (e != hipSuccess) return e;
+    attr = true;
+  }
+  hipLaunchKernelGGL((k_sweep<PROB, R1>), dim3(blocks), dim3(CC_BLOCK), lds,
+                     E.st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_sweep(const SweepArgs &a, int blocks, size_t lds) {
+  switch (E.prob) {
+    case CC_TORUS: return launch_sweep_t<CC_TORUS, 1>(a, blocks, lds);
+    case CC_GRID: return launch_sweep_t<CC_GRID, 1>(a, blocks, lds);
+    case CC_HAMMING:
+      switch (E.L) {
+        case 2: return launch_sweep_t<CC_HAMMING, 2>(a, blocks, lds);
+        case 3: return launch_sweep_t<CC_HAMMING, 3>(a, blocks, lds);
+        case 4: return launch_sweep_t<CC_HAMMING, 4>(a, blocks, lds);
+        case 5: return launch_sweep_t<CC_HAMMING, 5>(a, blocks, lds);
+        case 6: return launch_sweep_t<CC_HAMMING, 6>(a, blocks, lds);
+        case 7: return launch_sweep_t<CC_HAMMING, 7>(a, blocks, lds);
+      }
+      break;
+    case CC_LEE:
+      switch (E.L) {
+        case 2: return launch_sweep_t<CC_LEE, 2>(a, blocks, lds);
+        case 3: return launch_sweep_t<CC_LEE, 3>(a, blocks, lds);
+        case 4: return launch_sweep_t<CC_LEE, 4>(a, blocks, lds);
+        case 5: return launch_sweep_t<CC_LEE, 5>(a, blocks, lds);
+        case 6: return launch_sweep_t<CC_LEE, 6>(a, blocks, lds);
+        case 7: return launch_sweep_t<CC_LEE, 7>(a, blocks, lds);
+      }
+      break;
+  }
+  return hipErrorInvalidValue;
+}
+
+// Run the whole transform: in_kind seeds layer 0, out_kind says where the
+// result goes.  pmode is the predicate for CC_IN_PRED.
+int run_transform(int in_kind, int out_kind, int pmode) {
+  const int ns = (int)E.plan.size();
+  E.nswept = 0;
+  for (int i = 0; i < ns; i++) {
+    const Sweep &sw = E.plan[i];
+    SweepArgs a;
+    memset(&a, 0, sizeof(a));
+    for (int d = 0; d < CC_MAXL; d++) a.A[d] = E.A[d];
+    a.cnt = E.cnt;
+    a.s = sw.s;
+    a.ncolb = sw.ncolb;
+    a.ntiles = sw.ntiles;
+    a.P = sw.P;
+    a.TL = sw.TL;
+    a.nax = sw.nax;
+    a.c0 = sw.c0;
+    a.R = E.R;
+    a.mu = E.mu;
+    a.pmode = pmode;
+    a.in_kind = (i == 0) ? in_kind : CC_IN_LAYERS;
+    a.out_kind = (i == ns - 1) ? out_kind : CC_OUT_LAYERS;
+    // the layered families never change layer 0 after the first sweep
+    // seeds it; the single-plane families rewrite it every sweep
+    a.write_l0 = (E.L == 1) || (i == 0 && in_kind == CC_IN_PRED);
+    long long sk = 1;
+    for (int k = 0; k < sw.nax; k++) {
+      a.Q[k] = E.ax[sw.j0 + k];
+      a.sk[k] = (int)sk;
+      sk *= a.Q[k];
+    }
+    const int blocks = (int)std::min<long long>(sw.ntiles, 1LL << 20);
+    if (E.prof) CK(hipEventRecord(E.ev[2 * i], E.st));
+    CK(launch_sweep(a, blocks, sw.lds));
+    if (E.prof) CK(hipEventRecord(E.ev[2 * i + 1], E.st));
+  }
+  E.nswept = ns;
+  return 0;
+}
+
+int sync_stream() {
+  CK(hipStreamSynchronize(E.st));
+  return 0;
+}
+
+// zero the reduction scalar; n = 8 also clears the running transition
+// counter (init only: update_delta reads it as a difference)
+int zero_scalars(int n = 1) {
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, E.st, E.d_scal, n);
+  CK(hipGetLastError());
+  return 0;
+}
+
+int fetch_scalar(int i, unsigned long long *r) {
+  CK(hipMemcpyAsync(E.h_scal, E.d_scal + i, 8, hipMemcpyDeviceToHost, E.st));
+  CK(hipStreamSynchronize(E.st));
+  *r = E.h_scal[0];
+  return 0;
+}
+
+// copy nb words to d_words through pinned staging (caller synchronises
+// before the staging buffer is reused)
+int stage_words(const long long *w, int nb) {
+  memcpy(E.h_words, w, (size_t)nb * 8);
+  CK(hipMemcpyAsync(E.d_words, E.h_words, (size_t)nb * 8,
+                    hipMemcpyHostToDevice, E.st));
+  return 0;
+}
+
+const uint64_t *walk_table() { return E.fmt == 1 ? E.pat2 : E.pat; }
+
+template <int KIND>
+hipError_t launch_walk(const WordsArg &wa, const long long *wptr, int nb,
+                       int split, int delta, int pmode, int target,
+                       int32_t *out) {
+  const Geom g = make_geom();
+  dim3 grid((unsigned)nb * split);
+  if (E.prob == CC_GRID)
+    hipLaunchKernelGGL((k_walk<KIND, true>), grid, dim3(CC_BLOCK), 0, E.st, g,
+                       wa, wptr, split, delta, E.cnt, walk_table(), E.blen,
+                       pmode, target, out, E.d_scal + 1);
+  else
+    hipLaunchKernelGGL((k_walk<KIND, false>), grid, dim3(CC_BLOCK), 0, E.st,
+                       g, wa, wptr, split, delta, E.cnt, walk_table(), E.blen,
+                       pmode, target, out, E.d_scal + 1);
+  return hipGetLastError();
+}
+
+// cnt += delta over the balls of nwords validated words; no wait for
+// batches of at most ASYNC_UPDATE_MAX words
+template <int KIND>
+int update_inner(const long long *words, int nwords, int delta) {
+  if (nwords <= 0) return 0;
+  for (int off = 0; off < nwords; off += CODECAP) {
+    const int nb = std::min(nwords - off, CODECAP);
+    if (nb <= ASYNC_UPDATE_MAX) {
+      for (int o2 = 0; o2 < nb; o2 += NV_WORDS) {
+        const int k = std::min(nb - o2, NV_WORDS);
+        WordsArg wa;
+        memset(&wa, 0, sizeof(wa));
+        memcpy(wa.w, words + off + o2, (size_t)k * 8);
+        CK((launch_walk<KIND>(wa, nullptr, k, pick_split(k), delta, 0, 0,
+                              nullptr)));
+      }
+    } else {
+      WordsArg wa;
+      memset(&wa, 0, sizeof(wa));
+      if (int rc = stage_words(words + off, nb)) return rc;
+      CK((launch_walk<KIND>(wa, E.d_words, nb, pick_split(nb), delta, 0, 0,
+                            nullptr)));
+      CK(hipStreamSynchronize(E.st));
+    }
+  }
+  return 0;
+}
+
+int gather_impl(const long long *words, int nwords, int pmode, int target,
+                int32_t *out) {
+  if (nwords <= 0) return 0;
+  if (!out) return -1;
+  for (int off = 0; off < nwords; off += CODECAP) {
+    const int nb = std::min(nwords - off, CODECAP);
+    if (int rc = stage_words(words + off, nb)) return rc;
+    const int split = pick_split(nb);
+    if (split > 1) {
+      hipLaunchKernelGGL(k_fill32, dim3((nb + 255) / 256), dim3(256), 0, E.st,
+                         E.d_out, 0, (long long)nb);
+      CK(hipGetLastError());
+    }
+    WordsArg wa;
+    memset(&wa, 0, sizeof(wa));
+    CK((launch_walk<CC_WALK_GATHER>(wa, E.d_words, nb, split, 0, pmode, target,
+                                    E.d_out)));
+    CK(hipMemcpyAsync(E.h_out, E.d_out, (size_t)nb * 4, hipMemcpyDeviceToHost,
+                      E.st));
+    CK(hipStreamSynchronize(E.st));
+    memcpy(out + off, E.h_out, (size_t)nb * 4);
+    E.stats[1]++;
+  }
+  return 0;
+}
+
+// host-side validation of an update (reference dct_hc.cu:567-577)
+bool update_ok(const long long *words, int nwords, int delta,
+               std::unordered_map<long long, int> &batch) {
+  if (!valid_words(words, nwords) || (delta != 1 && delta != -1)) return false;
+  if (delta == 1 && nwords > MAX_CODE - E.active_size) return false;
+  for (int i = 0; i < nwords; i++) ++batch[words[i]];
+  if (delta == -1)
+    for (const auto &kv : batch) {
+      auto it = E.active.find(kv.first);
+      if (it == E.active.end() || it->second < kv.second) return false;
+    }
+  return true;
+}
+
+void apply_batch(const std::unordered_map<long long, int> &batch, int delta,
+                 int nwords) {
+  for (const auto &kv : batch) {
+    int &c = E.active[kv.first];
+    c += delta * kv.second;
+    if (c == 0) E.active.erase(kv.first);
+  }
+  E.active_size += delta * nwords;
+}
+
+template <typename T>
+int dalloc(T **p, size_t bytes) {
+  if (hipMalloc((void **)p, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    *p = nullptr;
+    return -1;
+  }
+  E.bytes_dev += (long long)bytes;
+  return 0;
+}
+
+void release_all() {
+  if (E.st) (void)hipStreamSynchronize(E.st);
+  for (int d = 0; d < CC_MAXL; d++)
+    if (E.A[d]) { (void)hipFree(E.A[d]); E.A[d] = nullptr; }
+  void *bufs[] = {E.cnt, E.pat, E.pat2, E.d_code, E.d_words, E.d_out,
+                  E.d_scal, E.d_hist, E.d_eidx64, E.d_eidx32, E.d_eval,
+                  E.d_bcnt, E.d_boff};
+  for (void *b : bufs)
+    if (b) (void)hipFree(b);
+  E.cnt = nullptr; E.pat = E.pat2 = nullptr; E.d_code = E.d_words = nullptr;
+  E.d_out = nullptr; E.d_scal = nullptr; E.d_hist = nullptr;
+  E.d_eidx64 = nullptr; E.d_eidx32 = nullptr; E.d_eval = nullptr;
+  E.d_bcnt = E.d_boff = nullptr;
+  if (E.h_words) { (void)hipHostFree(E.h_words); E.h_words = nullptr; }
+  if (E.h_out) { (void)hipHostFree(E.h_out); E.h_out = nullptr; }
+  if (E.h_scal) { (void)hipHostFree(E.h_scal); E.h_scal = nullptr; }
+  for (auto &e : E.ev)
+    if (e) { (void)hipEventDestroy(e); e = nullptr; }
+  if (E.st) { (void)hipStreamDestroy(E.st); E.st = nullptr; }
+  E.uploaded.clear();
+  E.active.clear();
+  E.active_size = 0;
+  E.codeM = 0;
+  E.live = false;
+  E.mu = 1;
+  E.fmt = 0;
+  E.nswept = 0;
+}
+
+}  // namespace
+
+namespace {
+template <typename IDX>
+long long extract_impl(int32_t thr, IDX *dev_idx, IDX *out_idx, int32_t *out_val,
+                       int cap) {
+  if (!E.live || cap < 0) return -1;
+  if (cap > E.ecap) cap = E.ecap;
+  const unsigned nb = (unsigned)E.nchunks;
+  hipLaunchKernelGGL(k_extract_count, dim3(nb), dim3(CC_BLOCK), 0, E.st,
+                     E.A[0], E.X, thr, EXTRACT_CH, E.d_bcnt);
+  CK(hipGetLastError());
+  hipLaunchKernelGGL(k_scan_offsets, dim3(1), dim3(1024), 0, E.st, E.d_bcnt,
+                     E.nchunks, E.d_boff);
+  CK(hipGetLastError());
+  hipLaunchKernelGGL((k_extract_write<IDX>), dim3(nb), dim3(CC_BLOCK), 0, E.st,
+                     E.A[0], E.X, thr, EXTRACT_CH, E.d_boff, (long long)cap,
+                     dev_idx, E.d_eval);
+  CK(hipGetLastError());
+  long long found = 0;
+  CK(hipMemcpyAsync(E.h_scal, E.d_boff + E.nchunks, 8, hipMemcpyDeviceToHost,
+                    E.st));
+  CK(hipStreamSynchronize(E.st));
+  found = (long long)E.h_scal[0];
+  const long long m = std::min<long long>(found, cap);
+  if (m > 0) {
+    if (!out_idx || !out_val) return -1;
+    CK(hipMemcpy(out_idx, dev_idx, (size_t)m * sizeof(IDX), hipMemcpyDeviceToHost));
+    CK(hipMemcpy(out_val, E.d_eval, (size_t)m * 4, hipMemcpyDeviceToHost));
+  }
+  return found;
+}
+}  // namespace
+
+#pragma GCC visibility push(default)
+extern "C" {
+
+// ---- lifecycle (reference dct_hc.cu:224-428) -------------------------------
+int dct_init3(int problem, int n, const int *ax, int R,
+              const uint64_t *ballpat, long long balllen, int extract_cap,
+              long long *bytes, int layers_mode, int cnt_mode, int use_owner) {
+  (void)layers_mode;
+  (void)cnt_mode;
+  if (n < 2 || n > CC_MAXN || R < 1 || !ax || !ballpat || balllen < 1 ||
+      balllen > INT_MAX || extract_cap < 1)
+    return -1;
+  int L;
+  if (problem == CC_HAMMING) {
+    if (R > CC_MAXR) return -1;
+    for (int i = 0; i < n; i++)
+      if (ax[i] < 2 || ax[i] > 10) return -1;
+    L = R + 1;
+  } else if (problem == CC_TORUS) {
+    for (int i = 0; i < n; i++)
+      if (ax[i] < 3 || ax[i] > CC_MAXAX || R > (ax[i] - 1) / 2) return -1;
+    if (n > CC_MAXPAT) return -1;
+    L = 1;
+  } else if (problem == CC_LEE) {
+    if (R > CC_MAXR) return -1;
+    for (int i = 0; i < n; i++)
+      if (ax[i] < 2 || ax[i] > CC_MAXAX) return -1;
+    L = R + 1;
+  } else if (problem == CC_GRID) {
+    for (int i = 0; i < n; i++)
+      if (ax[i] < 2 || ax[i] > CC_MAXAX) return -1;
+    if (n > CC_MAXPAT) return -1;
+    L = 1;
+  } else {
+    return -1;
+  }
+  long long X = 1;
+  for (int i = 0; i < n; i++) {
+    if (X > (1LL << 40) / ax[i]) return -2;
+    X *= ax[i];
+  }
+  if (E.live) release_all();
+  E.prob = problem; E.n = n; E.R = R; E.L = L; E.X = X; E.mu = 1;
+  E.pw[0] = 1;
+  for (int i = 0; i < n; i++) {
+    E.ax[i] = ax[i];
+    E.pw[i + 1] = E.pw[i] * ax[i];
+  }
+  if (!table_ok(ballpat, balllen)) return -1;
+  E.use_owner = use_owner;
+  E.bytes_dev = 0;
+  memset(E.stats, 0, sizeof(E.stats));
+  const char *lb = getenv("CC_SWEEP_LDS");
+  E.lds_budget = lb ? std::max(4096, atoi(lb)) : 64 * 1024;
+  CK(hipStreamCreateWithFlags(&E.st, hipStreamNonBlocking));
+  for (int d = 0; d < L; d++)
+    if (dalloc(&E.A[d], (size_t)X * 4)) { release_all(); return -3; }
+  E.cnt_len = (X + 1) & ~1LL;
+  if (dalloc(&E.cnt, (size_t)E.cnt_len * 2)) { release_all(); return -4; }
+  hipLaunchKernelGGL(k_fill16, dim3(pass_blocks(E.cnt_len)), dim3(CC_BLOCK), 0,
+                     E.st, E.cnt, (uint16_t)0, E.cnt_len);
+  CK(hipGetLastError());
+  E.blen = balllen;
+  if (dalloc(&E.pat, (size_t)balllen * 8)) { release_all(); return -3; }
+  CK(hipMemcpy(E.pat, ballpat, (size_t)balllen * 8, hipMemcpyHostToDevice));
+  if (dalloc(&E.d_code, (size_t)CODECAP * 8) ||
+      dalloc(&E.d_words, (size_t)CODECAP * 8) ||
+      dalloc(&E.d_out, (size_t)CODECAP * 4) || dalloc(&E.d_scal, 64) ||
+      dalloc(&E.d_hist, 4096 * 4)) {
+    release_all();
+    return -3;
+  }
+  E.ecap = extract_cap;
+  E.nchunks = (X + EXTRACT_CH - 1) / EXTRACT_CH;
+  if (dalloc(&E.d_eidx64, (size_t)extract_cap * 8) ||
+      dalloc(&E.d_eidx32, (size_t)extract_cap * 4) ||
+      dalloc(&E.d_eval, (size_t)extract_cap * 4) ||
+      dalloc(&E.d_bcnt, (size_t)E.nchunks * 8) ||
+      dalloc(&E.d_boff, (size_t)(E.nchunks + 1) * 8)) {
+    release_all();
+    return -3;
+  }
+  CK(hipHostMalloc((void **)&E.h_words, (size_t)CODECAP * 8, hipHostMallocDefault));
+  CK(hipHostMalloc((void **)&E.h_out, (size_t)std::max(CODECAP, 4096) * 4,
+                   hipHostMallocDefault));
+  CK(hipHostMalloc((void **)&E.h_scal, 64, hipHostMallocDefault));
+  for (auto &e : E.ev) CK(hipEventCreate(&e));
+  const char *pe = getenv("DCT_TIMING");
+  E.prof = pe ? atoi(pe) : 0;
+  if (int rc = zero_scalars(8)) return rc;
+  E.trans_last = 0;
+  build_plan();
+  E.uploaded.clear();
+  E.active.clear();
+  E.active_size = 0;
+  E.codeM = 0;
+  E.fmt = 0;
+  E.live = true;
+  CK(hipStreamSynchronize(E.st));
+  if (bytes) *bytes = E.bytes_dev;
+  return 0;
+}
+
+int dct_init2(int q, int n, int R, const uint64_t *ballpat, long long balllen,
+              int extract_cap, long long *bytes, int layers_mode, int cnt_mode,
+              int use_owner) {
+  int ax[CC_MAXN];
+  if (n < 2 || n > CC_MAXN) return -1;
+  for (int i = 0; i < n; i++) ax[i] = q;
+  return dct_init3(CC_HAMMING, n, ax, R, ballpat, balllen, extract_cap, bytes,
+                   layers_mode, cnt_mode, use_owner);
+}
+
+int dct_init(int q, int n, int R, const uint64_t *ballpat, long long balllen,
+             int extract_cap, long long *bytes) {
+  return dct_init2(q, n, R, ballpat, balllen, extract_cap, bytes, 0, 0, 0);
+}
+
+int dct_free(void) {
+  if (E.live) release_all();
+  E.mu = 1;
+  return 0;
+}
+
+long long dct_host_bytes(void) { return 0; }
+
+// ---- mu-fold covering (reference dct_hc.cu:210-222) -----------------------
+int dct_set_mu(int mu) {
+  if (mu < 1 || mu > 65535) return -1;
+  E.mu = mu;
+  return 0;
+}
+
+int dct_get_mu(void) { return E.mu; }
+
+// ---- solution and transforms (reference dct_hc.cu:452-521) ----------------
+int dct_set_code(const long long *idx, int M) {
+  if (!E.live || !valid_words(idx, M)) return -1;
+  for (int off = 0; off < M; off += CODECAP) {
+    const int nb = std::min(M - off, CODECAP);
+    memcpy(E.h_words, idx + off, (size_t)nb * 8);
+    CK(hipMemcpyAsync(E.d_code + off, E.h_words, (size_t)nb * 8,
+                      hipMemcpyHostToDevice, E.st));
+    CK(hipStreamSynchronize(E.st));
+  }
+  E.codeM = M;
+  E.uploaded.clear();
+  for (int i = 0; i < M; i++) ++E.uploaded[idx[i]];
+  return 0;
+}
+
+// init_mode: 0 = code multiset, 1 = [cnt < mu] (deficient), 2 = [1 <= cnt <=
+// mu] (critical); out_mode: 1 = layer sum -> A0, 2 = clamped sum -> cnt
+// (init_mode 0 only).  At mu = 1 the indicators are [cnt==0] / [cnt==1].
+int dct_transform(int init_mode, int out_mode) {
+  if (!E.live) return -1;
+  if (init_mode < 0 || init_mode > 2 || out_mode < 1 || out_mode > 2 ||
+      (out_mode == 2 && init_mode != 0))
+    return -1;
+  int in_kind = CC_IN_PRED, pmode = CC_DEF;
+  if (init_mode == 0) {
+    in_kind = CC_IN_CODE;
+    hipLaunchKernelGGL(k_fill32, dim3(pass_blocks(E.X)), dim3(CC_BLOCK), 0,
+                       E.st, E.A[0], 0, E.X);
+    CK(hipGetLastError());
+    if (E.codeM > 0) {
+      hipLaunchKernelGGL(k_scatter_code, dim3((E.codeM + 255) / 256), dim3(256),
+                         0, E.st, E.A[0], E.d_code, E.codeM);
+      CK(hipGetLastError());
+    }
+  } else if (init_mode == 2) {
+    pmode = CC_CRIT;
+  }
+  if (int rc = run_transform(in_kind, out_mode == 1 ? CC_OUT_MAP : CC_OUT_CNT,
+                             pmode))
+    return rc;
+  CK(hipStreamSynchronize(E.st));
+  if (E.prof)
+    for (int i = 0; i < E.nswept; i++)
+      CK(hipEventElapsedTime(&E.sweep_ms[i], E.ev[2 * i], E.ev[2 * i + 1]));
+  if (out_mode == 2) {
+    E.active = E.uploaded;
+    E.active_size = E.codeM;
+  }
+  return 0;
+}
+
+// ---- incremental updates and exact gathers (reference dct_hc.cu:533-799) --
+int dct_ball_update(const long long *words, int nwords, int delta) {
+  if (!E.live) return -1;
+  std::unordered_map<long long, int> batch;
+  if (!update_ok(words, nwords, delta, batch)) return -1;
+  if (int rc = update_inner<CC_WALK_UPDATE>(words, nwords, delta)) return rc;
+  apply_batch(batch, delta, nwords);
+  return 0;
+}
+
+int dct_ball_gather(const long long *words, int nwords, int target,
+                    int32_t *out) {
+  if (!E.live) return -1;
+  if (nwords <= 0) return 0;
+  if (!words) return -1;
+  for (int i = 0; i < nwords; i++)
+    if (words[i] < 0 || words[i] >= E.X) return -1;
+  return gather_impl(words, nwords, CC_EQ, target, out);
+}
+
+int dct_ball_gather_mu(const long long *words, int nwords, int role,
+                       int32_t *out) {
+  if (!E.live) return -1;
+  if (nwords <= 0) return 0;
+  if (role < 0 || role > 1 || !words) return -1;
+  for (int i = 0; i < nwords; i++)
+    if (words[i] < 0 || words[i] >= E.X) return -1;
+  return gather_impl(words, nwords, role == 0 ? CC_DEF : CC_CRIT, 0, out);
+}
+
+int dct_ball_update_delta(const long long *words, int nwords, int delta,
+                          long long *dd) {
+  if (!dd) return -1;
+  *dd = 0;
+  if (!E.live) return -1;
+  std::unordered_map<long long, int> batch;
+  if (!update_ok(words, nwords, delta, batch)) return -1;
+  if (nwords == 0) return 0;
+  if (int rc = update_inner<CC_WALK_TRANS>(words, nwords, delta)) return rc;
+  unsigned long long tr;
+  if (int rc = fetch_scalar(1, &tr)) return rc;
+  const long long diff = (long long)(tr - E.trans_last);
+  E.trans_last = tr;
+  *dd = delta > 0 ? -diff : diff;
+  apply_batch(batch, delta, nwords);
+  return 0;
+}
+
+// read-only batched swap evaluation (reference dct_hc.cu:868-977): the exact
+// drop / gather / undo sequence, cnt restored before returning
+int dct_swap_eval(const long long *drops, int k, const long long *cand,
+                  const long long *off, int32_t *gain, int32_t *loss) {
+  if (!E.live) return -1;
+  if (k <= 0) return 0;
+  if (k > CODECAP || !off || off[0] != 0 || !drops || !loss) return -1;
+  for (int i = 0; i < k; i++)
+    if (off[i + 1] < off[i]) return -1;
+  const long long nc = off[k];
+  if (nc > CODECAP || (nc && (!cand || !gain))) return -1;
+  if (!valid_words(drops, k) || !valid_words(cand, (int)nc)) return -1;
+  for (int i = 0; i < k; i++)
+    if (E.active.find(drops[i]) == E.active.end()) return -1;
+  const int lmode = E.mu == 1 ? CC_EQ : CC_CRIT, lt = E.mu == 1 ? 1 : 0;
+  const int gmode = E.mu == 1 ? CC_EQ : CC_DEF;
+  for (int i = 0; i < k; i++) {
+    if (int rc = gather_impl(drops + i, 1, lmode, lt, loss + i)) return rc;
+    const long long ni = off[i + 1] - off[i];
+    if (!ni) continue;
+    if (int rc = update_inner<CC_WALK_UPDATE>(drops + i, 1, -1)) return rc;
+    int rc = gather_impl(cand + off[i], (int)ni, gmode, 0, gain + off[i]);
+    int rc2 = update_inner<CC_WALK_UPDATE>(drops + i, 1, +1);
+    if (rc) return rc;
+    if (rc2) return rc2;
+  }
+  CK(hipStreamSynchronize(E.st));
+  E.stats[5]++;
+  return 0;
+}
+
+int dct_stats(long long *out, int n) {
+  if (!out) return -1;
+  if (n > 8) n = 8;
+  for (int i = 0; i < n; i++) out[i] = E.stats[i];
+  return n;
+}
+
+// owner-trick loss (reference dct_hc.cu:994-1022): rebuild cnt from the full
+// solution, then per-codeword critical counts.  A word is critical to
+// exactly the codewords covering it, so the critical gather reproduces the
+// owner scan's integers (and is what the reference itself runs for mu > 1).
+int dct_loss_owner(const long long *words, int M, int32_t *out) {
+  if (!E.live || !E.use_owner) return -1;
+  if (!valid_words(words, M) || (M && !out)) return -2;
+  hipLaunchKernelGGL(k_fill16, dim3(pass_blocks(E.cnt_len)), dim3(CC_BLOCK), 0,
+                     E.st, E.cnt, (uint16_t)0, E.cnt_len);
+  CK(hipGetLastError());
+  if (int rc = update_inner<CC_WALK_UPDATE>(words, M, +1)) return rc;
+  E.active.clear();
+  for (int i = 0; i < M; i++) ++E.active[words[i]];
+  E.active_size = M;
+  if (M == 0) return sync_stream();
+  return gather_impl(words, M, E.mu == 1 ? CC_EQ : CC_CRIT, E.mu == 1 ? 1 : 0,
+                     out);
+}
+
+// ---- reductions and queries (reference dct_hc.cu:979-1157) ---------------
+long long dct_count_eq(int target) {
+  if (!E.live) return -1;
+  if (int rc = zero_scalars()) return rc;
+  hipLaunchKernelGGL(k_cnt_reduce<CC_EQ>, dim3(pass_blocks(E.X / 8 + 1)),
+                     dim3(CC_BLOCK), 0, E.st, E.cnt, E.X, target, E.mu,
+                     E.d_scal);
+  CK(hipGetLastError());
+  unsigned long long r;
+  if (int rc = fetch_scalar(0, &r)) return rc;
+  return (long long)r;
+}
+
+long long dct_count_deficit(void) {
+  if (!E.live) return -1;
+  if (int rc = zero_scalars()) return rc;
+  hipLaunchKernelGGL(k_cnt_reduce<CC_DEF>, dim3(pass_blocks(E.X / 8 + 1)),
+                     dim3(CC_BLOCK), 0, E.st, E.cnt, E.X, 0, E.mu, E.d_scal);
+  CK(hipGetLastError());
+  unsigned long long r;
+  if (int rc = fetch_scalar(0, &r)) return rc;
+  return (long long)r;
+}
+
+int dct_map_max(int32_t *out_max) {
+  if (!E.live || !out_max) return -1;
+  if (int rc = zero_scalars()) return rc;
+  hipLaunchKernelGGL(k_max_i32, dim3(pass_blocks(E.X / 4 + 1)), dim3(CC_BLOCK),
+                     0, E.st, E.A[0], E.X, (int *)E.d_scal);
+  CK(hipGetLastError());
+  unsigned long long r;
+  if (int rc = fetch_scalar(0, &r)) return rc;
+  *out_max = (int32_t)(r & 0xFFFFFFFFull);
+  return 0;
+}
+
+int dct_map_hist(int nbins, int32_t vmax, int32_t *out_hist) {
+  if (!E.live || nbins > 4096 || nbins < 1 || !out_hist) return -1;
+  hipLaunchKernelGGL(k_fill32, dim3((nbins + 255) / 256), dim3(256), 0, E.st,
+                     E.d_hist, 0, (long long)nbins);
+  CK(hipGetLastError());
+  if (vmax > 0) {
+    const int hb = std::min(pass_blocks(E.X / 4 + 1), 1024);
+    hipLaunchKernelGGL(k_hist, dim3(hb), dim3(CC_BLOCK), (size_t)nbins * 4,
+                       E.st, E.A[0], E.X, nbins, vmax, E.d_hist);
+    CK(hipGetLastError());
+  }
+  CK(hipMemcpyAsync(E.h_out, E.d_hist, (size_t)nbins * 4,
+                    hipMemcpyDeviceToHost, E.st));
+  CK(hipStreamSynchronize(E.st));
+  memcpy(out_hist, E.h_out, (size_t)nbins * 4);
+  return 0;
+}
+
+
+long long dct_map_extract(int32_t thr, uint32_t *out_idx, int32_t *out_val,
+                          int cap) {
+  return extract_impl<uint32_t>(thr, E.d_eidx32, out_idx, out_val, cap);
+}
+
+long long dct_map_extract64(int32_t thr, int64_t *out_idx, int32_t *out_val,
+                            int cap) {
+  return extract_impl<int64_t>(thr, E.d_eidx64, out_idx, out_val, cap);
+}
+
+int dct_map_read_at(const long long *idx, int k, int32_t *out) {
+  if (!E.live) return -1;
+  if (k <= 0) return 0;
+  if (!idx || !out) return -1;
+  for (int i = 0; i < k; i++)
+    if (idx[i] < 0 || idx[i] >= E.X) return -1;
+  for (int off = 0; off < k; off += CODECAP) {
+    const int nb = std::min(k - off, CODECAP);
+    if (int rc = stage_words(idx + off, nb)) return rc;
+    hipLaunchKernelGGL(k_read_at, dim3((nb + 255) / 256), dim3(256), 0, E.st,
+                       E.A[0], E.d_words, nb, E.d_out);
+    CK(hipGetLastError());
+    CK(hipMemcpyAsync(E.h_out, E.d_out, (size_t)nb * 4, hipMemcpyDeviceToHost,
+                      E.st));
+    CK(hipStreamSynchronize(E.st));
+    memcpy(out + off, E.h_out, (size_t)nb * 4);
+  }
+  return 0;
+}
+
+int dct_read_cnt(long long off, long long len, uint16_t *out) {
+  if (!E.live || off < 0 || len < 0 || off + len > E.X || (len && !out))
+    return -1;
+  CK(hipStreamSynchronize(E.st));
+  if (len) CK(hipMemcpy(out, E.cnt + off, (size_t)len * 2, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+// ---- optional ox table (reference dct_hc.cu:430-450) ----------------------
+// The ox table must enumerate the same support set; walks are order-free, so
+// it is converted to the v1 byte form once and walked by the same kernels.
+int dct_set_tbl2(const uint16_t *mask, const uint64_t *offs, long long len) {
+  if (!E.live || len != E.blen || !mask || !offs) return -1;
+  std::vector<uint64_t> v1((size_t)len);
+  for (long long t = 0; t < len; t++) {
+    uint64_t p = 0;
+    int k = 0;
+    for (int pos = 0; pos < E.n; pos++)
+      if ((mask[t] >> pos) & 1u) {
+        if (k >= CC_MAXPAT) return -1;
+        const uint64_t d = (offs[t] >> (4 * pos)) & 0xFull;
+        p |= (((uint64_t)pos << 4) | d) << (8 * k);
+        k++;
+      }
+    if (mask[t] >> E.n) return -1;
+    v1[(size_t)t] = p;
+  }
+  if (!table_ok(v1.data(), len)) return -1;
+  if (!E.pat2 && dalloc(&E.pat2, (size_t)len * 8)) return -3;
+  CK(hipMemcpy(E.pat2, v1.data(), (size_t)len * 8, hipMemcpyHostToDevice));
+  return 0;
+}
+
+int dct_use_fmt(int fmt) {
+  if (!E.live || fmt < 0 || fmt > 1) return -1;
+  if (fmt == 1 && !E.pat2) return -1;
+  if (fmt == 1 && E.prob == CC_GRID) return -1;  // the grid walk needs v1
+  E.fmt = fmt;
+  return 0;
+}
+
+// ---- additive (not in the reference ABI): schedule and sweep timing ------
+// dct_sweep_plan: per sweep 6 ints (first axis, axes, P, TL, layers in,
+// layers out); returns the sweep count.
+int dct_sweep_plan(int *out, int cap) {
+  if (!E.live || !out) return -1;
+  const int ns = (int)E.plan.size();
+  for (int i = 0; i < ns && i < cap; i++) {
+    const Sweep &s = E.plan[i];
+    out[6 * i + 0] = s.j0;
+    out[6 * i + 1] = s.nax;
+    out[6 * i + 2] = s.P;
+    out[6 * i + 3] = s.TL;
+    out[6 * i + 4] = layers_after(s.c0);
+    out[6 * i + 5] = s.lmax;
+  }
+  return ns;
+}
+
+// dct_sweep_profile(1): record HIP events around every sweep of later
+// transforms (on the engine's stream); dct_sweep_times copies the last
+// transform's per-sweep milliseconds and returns the sweep count.
+int dct_sweep_profile(int on) {
+  if (!E.live) return -1;
+  E.prof = on ? 1 : 0;
+  return 0;
+}
+
+int dct_sweep_times(float *ms, int cap) {
+  if (!E.live || !ms) return -1;
+  for (int i = 0; i < E.nswept && i < cap; i++) ms[i] = E.sweep_ms[i];
+  return E.nswept;
+}
+
+}  // extern "C"
+#pragma GCC visibility pop

@@ -1,0 +1,361 @@
+"""GPU backend: ctypes binding of the MI355X engine (plugin ABI v1).
+
+Mirrors reference src/coldcore/gpu.py (``_bind`` 52-103, ``locality_order``
+106-129, ``GpuBackend`` 207-537): same class, method names, argument meaning
+and error behaviour (a nonzero rc raises ``RuntimeError``; ``-100-e`` is
+device error e).  The library is the in-tree ``libcoldcore_covering.so``
+built from ``coldcore_amd/csrc`` (or ``COLDCORE_PLUGIN_PATH``).  There is no
+CPU fallback: a missing or unloadable library raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from .patterns import PROBLEM_IDS, support_patterns
+from .space import ProductSpace
+
+_LL = ctypes.c_longlong
+_LLP = ctypes.POINTER(ctypes.c_longlong)
+_I32P = ctypes.POINTER(ctypes.c_int32)
+_U64P = ctypes.POINTER(ctypes.c_uint64)
+_libs = {}
+
+#: every symbol the library must export (the v1 surface plus the optional
+#: and additive ones this engine implements)
+REQUIRED = ("dct_init3", "dct_free", "dct_set_code", "dct_transform",
+            "dct_ball_update", "dct_ball_gather", "dct_count_eq",
+            "dct_map_max", "dct_map_extract", "dct_map_extract64",
+            "dct_map_read_at", "dct_read_cnt", "dct_init", "dct_init2")
+OPTIONAL = ("dct_map_hist", "dct_set_mu", "dct_get_mu", "dct_ball_gather_mu",
+            "dct_count_deficit", "dct_host_bytes", "dct_loss_owner",
+            "dct_set_tbl2", "dct_use_fmt", "dct_ball_update_delta",
+            "dct_swap_eval", "dct_stats")
+ADDITIVE = ("dct_sweep_plan", "dct_sweep_profile", "dct_sweep_times")
+
+
+def _rc_msg(rc):
+    if rc <= -100:
+        return f"rc={rc} (HIP error {-(rc + 100)})"
+    return f"rc={rc}"
+
+
+def _check(name, rc):
+    if rc != 0:
+        raise RuntimeError(f"{name} {_rc_msg(rc)}")
+
+
+def default_plugin_path():
+    env = os.environ.get("COLDCORE_PLUGIN_PATH")
+    if env:
+        return env
+    return os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                        "libcoldcore_covering.so")
+
+
+def _bind(path):
+    if not os.path.exists(path):
+        raise RuntimeError(
+            f"engine library {path} is missing: run python -m coldcore_amd.build")
+    lib = ctypes.CDLL(path)
+    for f in REQUIRED + OPTIONAL + ADDITIVE:
+        getattr(lib, f)                      # every symbol must resolve
+    for f in ("dct_free", "dct_set_code", "dct_transform", "dct_ball_update",
+              "dct_ball_gather", "dct_map_max", "dct_map_read_at",
+              "dct_read_cnt", "dct_map_hist", "dct_set_mu", "dct_get_mu",
+              "dct_ball_gather_mu", "dct_loss_owner", "dct_set_tbl2",
+              "dct_use_fmt", "dct_stats", "dct_sweep_plan",
+              "dct_sweep_profile", "dct_sweep_times"):
+        getattr(lib, f).restype = ctypes.c_int
+    for f in ("dct_count_eq", "dct_count_deficit", "dct_map_extract",
+              "dct_map_extract64", "dct_host_bytes"):
+        getattr(lib, f).restype = _LL
+    lib.dct_read_cnt.argtypes = [_LL, _LL, ctypes.POINTER(ctypes.c_uint16)]
+    lib.dct_init.restype = ctypes.c_int
+    lib.dct_init.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _U64P,
+                             _LL, ctypes.c_int, _LLP]
+    lib.dct_init2.restype = ctypes.c_int
+    lib.dct_init2.argtypes = lib.dct_init.argtypes + [ctypes.c_int] * 3
+    lib.dct_init3.restype = ctypes.c_int
+    lib.dct_init3.argtypes = [ctypes.c_int, ctypes.c_int,
+                              ctypes.POINTER(ctypes.c_int), ctypes.c_int,
+                              _U64P, _LL, ctypes.c_int, _LLP,
+                              ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    lib.dct_ball_update_delta.restype = ctypes.c_int
+    lib.dct_ball_update_delta.argtypes = [_LLP, ctypes.c_int, ctypes.c_int,
+                                          _LLP]
+    lib.dct_swap_eval.restype = ctypes.c_int
+    lib.dct_swap_eval.argtypes = [_LLP, ctypes.c_int, _LLP, _LLP, _I32P,
+                                  _I32P]
+    return lib
+
+
+def load_plugin(path=None):
+    path = os.path.abspath(path or default_plugin_path())
+    if path not in _libs:
+        _libs[path] = _bind(path)
+    return _libs[path]
+
+
+def locality_order(pats, axes):
+    """Sort a v1 table by unwrapped offset so neighbouring table entries
+    touch neighbouring words (reference gpu.py:106-129).  Walks are
+    order-free, so results are unchanged; entry 0 stays the centre.
+    ``COLDCORE_BALL_ORDER=native`` keeps the builder's order."""
+    if os.environ.get("COLDCORE_BALL_ORDER", "locality") == "native":
+        return pats
+    pats = np.ascontiguousarray(pats, dtype=np.uint64)
+    stride = np.cumprod([1] + list(axes[:-1])).astype(np.int64)
+    key = np.zeros(len(pats), dtype=np.int64)
+    for k in range(8):
+        byte = ((pats >> np.uint64(8 * k)) & np.uint64(255)).astype(np.int64)
+        pos, delta = byte >> 4, byte & 15
+        key += np.where(byte != 0,
+                        delta * stride[np.minimum(pos, len(axes) - 1)], 0)
+    return np.ascontiguousarray(pats[np.argsort(key, kind="stable")])
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(t)
+
+
+class GpuBackend:
+    """One problem cell resident in MI355X HBM (reference gpu.py:207-537).
+
+    problem: "hamming" | "torus_linf" | "lee" | "grid_linf"; mu: covering
+    multiplicity.  layers_mode / cnt_mode are accepted for compatibility;
+    every plane lives in HBM.
+    """
+
+    def __init__(self, axes, R, problem="hamming", plugin_path=None,
+                 extract_cap=1 << 22, layers_mode=None, cnt_mode=None,
+                 use_owner=False, mu=1):
+        self.problem = problem
+        self.R = R
+        self.mu = mu = int(mu)
+        if not 1 <= mu <= 65535:
+            raise ValueError("mu must be in 1..65535")
+        self.space = ProductSpace(axes)
+        self.extract_cap = extract_cap
+        self.indep_dist = 2 * R if problem in ("hamming", "lee") else None
+        self.supports_hist = True
+        self._lib = lib = load_plugin(plugin_path)
+        self._ext_bufs = None
+        pats = locality_order(support_patterns(self.space.axes, R, problem),
+                              self.space.axes)
+        self._pats = pats
+        self.layers_mode = layers_mode or 0
+        self.cnt_mode = cnt_mode or 0
+        by = _LL(0)
+        ax = (ctypes.c_int * self.space.n)(*self.space.axes)
+        rc = lib.dct_init3(PROBLEM_IDS[problem], self.space.n, ax, R,
+                           _ptr(pats, _U64P), len(pats), extract_cap,
+                           ctypes.byref(by), self.layers_mode, self.cnt_mode,
+                           1 if use_owner else 0)
+        _check("init", rc)
+        _check("set_mu", lib.dct_set_mu(mu))
+        self.gpu_bytes = by.value
+        self.host_bytes = int(lib.dct_host_bytes())
+        self._closed = False
+
+    # ---------------- lifecycle ----------------
+    def close(self):
+        if not self._closed:
+            self._lib.dct_free()
+            self._closed = True
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def word_index(self, words):
+        return self.space.word_index(words)
+
+    def index_word(self, idx):
+        return self.space.index_word(idx)
+
+    # ---------------- solution / fields ----------------
+    def _set_code(self, idx):
+        idx = np.ascontiguousarray(np.atleast_1d(idx), dtype=np.int64)
+        _check("set_code", self._lib.dct_set_code(_ptr(idx, _LLP), len(idx)))
+
+    def recount(self, idx):
+        self._set_code(idx)
+        _check("transform(code->cnt)", self._lib.dct_transform(0, 2))
+
+    def gain_map(self):
+        _check("transform(uncov->gain)", self._lib.dct_transform(1, 1))
+
+    def loss_map(self):
+        _check("transform(cnt1->loss)", self._lib.dct_transform(2, 1))
+
+    # ---------------- incremental / exact ----------------
+    def ball_update(self, idx, delta):
+        idx = np.ascontiguousarray(np.atleast_1d(idx), dtype=np.int64)
+        _check("ball_update", self._lib.dct_ball_update(
+            _ptr(idx, _LLP), len(idx), int(delta)))
+
+    def ball_gather(self, idx, target):
+        """Exact per-candidate support count; target is a coverage ROLE
+        (0 deficient = placement gain, 1 critical = removal loss), else a
+        literal cnt value (reference gpu.py:323-348)."""
+        idx = np.ascontiguousarray(np.atleast_1d(idx), dtype=np.int64)
+        out = np.empty(len(idx), dtype=np.int32)
+        if self.mu == 1 or target not in (0, 1):
+            _check("ball_gather", self._lib.dct_ball_gather(
+                _ptr(idx, _LLP), len(idx), int(target), _ptr(out, _I32P)))
+        else:
+            _check("ball_gather_mu", self._lib.dct_ball_gather_mu(
+                _ptr(idx, _LLP), len(idx), int(target), _ptr(out, _I32P)))
+        return out
+
+    def ball_update_delta(self, idx, delta):
+        idx = np.ascontiguousarray(np.atleast_1d(idx), dtype=np.int64)
+        dd = _LL(0)
+        _check("ball_update_delta", self._lib.dct_ball_update_delta(
+            _ptr(idx, _LLP), len(idx), int(delta), ctypes.byref(dd)))
+        return int(dd.value)
+
+    def swap_eval(self, drops, cands):
+        """Exact costs of many swaps without changing state
+        (reference gpu.py:368-440)."""
+        drops = np.ascontiguousarray(np.atleast_1d(drops), dtype=np.int64)
+        k = len(drops)
+        shared = not isinstance(cands, (list, tuple))
+        if shared:
+            c1 = np.ascontiguousarray(np.atleast_1d(cands), dtype=np.int64)
+            parts = [c1] * k
+        else:
+            if len(cands) != k:
+                raise ValueError("need one candidate array per drop")
+            parts = [np.ascontiguousarray(np.atleast_1d(c), dtype=np.int64)
+                     for c in cands]
+        cap = (1 << 16) - 1
+        loss = np.empty(k, dtype=np.int32)
+        gains = [None] * k
+        i = 0
+        while i < k:
+            j, tot = i, 0
+            while j < k and j - i < cap and tot + len(parts[j]) <= cap:
+                tot += len(parts[j])
+                j += 1
+            if j == i:
+                raise ValueError(f"over {cap} candidates for one drop")
+            cat = (np.concatenate(parts[i:j]) if tot
+                   else np.zeros(1, dtype=np.int64))
+            off = np.zeros(j - i + 1, dtype=np.int64)
+            off[1:] = np.cumsum([len(p) for p in parts[i:j]])
+            g = np.empty(max(tot, 1), dtype=np.int32)
+            lo = np.empty(j - i, dtype=np.int32)
+            dsub = np.ascontiguousarray(drops[i:j])
+            _check("swap_eval", self._lib.dct_swap_eval(
+                _ptr(dsub, _LLP), j - i, _ptr(cat, _LLP), _ptr(off, _LLP),
+                _ptr(g, _I32P), _ptr(lo, _I32P)))
+            loss[i:j] = lo
+            for t in range(j - i):
+                gains[i + t] = g[off[t]:off[t + 1]].copy()
+            i = j
+        if shared:
+            return loss, (np.stack(gains) if k
+                          else np.zeros((0, len(c1)), dtype=np.int32))
+        return loss, gains
+
+    def gpu_stats(self):
+        buf = (_LL * 8)()
+        self._lib.dct_stats(buf, 8)
+        keys = ("gather_set_side", "gather_walk", "set_scans",
+                "set_overflows", "swap_eval_fast", "swap_eval_fallback",
+                "count_eq0_listed", "gather_set_tiled")
+        return {k: int(v) for k, v in zip(keys, buf)}
+
+    # ---------------- queries ----------------
+    def count_eq(self, target):
+        r = int(self._lib.dct_count_eq(int(target)))
+        if r < 0:
+            _check("count_eq", r)
+        return r
+
+    def uncovered(self):
+        """Total deficit sum_x max(0, mu - cnt(x)); 0 iff feasible."""
+        r = int(self._lib.dct_count_eq(0) if self.mu == 1
+                else self._lib.dct_count_deficit())
+        if r < 0:
+            _check("uncovered", r)
+        return r
+
+    def map_max(self):
+        v = ctypes.c_int32(0)
+        _check("map_max", self._lib.dct_map_max(ctypes.byref(v)))
+        return int(v.value)
+
+    def map_hist(self, nbins, vmax):
+        out = np.zeros(nbins, dtype=np.int32)
+        _check("map_hist", self._lib.dct_map_hist(
+            int(nbins), int(vmax), _ptr(out, _I32P)))
+        return out
+
+    def map_extract(self, thr, cap=None):
+        """(idx, val, found) of map >= thr, ascending index order; found may
+        exceed cap (only cap entries are returned)."""
+        cap = min(cap or self.extract_cap, self.extract_cap)
+        wide = self.space.size > 2 ** 32 - 2
+        oi = np.empty(max(cap, 1), dtype=np.int64 if wide else np.uint32)
+        ov = np.empty(max(cap, 1), dtype=np.int32)
+        if wide:
+            found = self._lib.dct_map_extract64(
+                int(thr), _ptr(oi, ctypes.POINTER(ctypes.c_int64)),
+                _ptr(ov, _I32P), cap)
+        else:
+            found = self._lib.dct_map_extract(
+                int(thr), _ptr(oi, ctypes.POINTER(ctypes.c_uint32)),
+                _ptr(ov, _I32P), cap)
+        if found < 0:
+            _check("map_extract", int(found))
+        m = min(int(found), cap)
+        return oi[:m].astype(np.int64), ov[:m], int(found)
+
+    def map_read_at(self, idx):
+        idx = np.ascontiguousarray(np.atleast_1d(idx), dtype=np.int64)
+        out = np.empty(len(idx), dtype=np.int32)
+        _check("map_read_at", self._lib.dct_map_read_at(
+            _ptr(idx, _LLP), len(idx), _ptr(out, _I32P)))
+        return out
+
+    def map_all(self):
+        return self.map_read_at(np.arange(self.space.size, dtype=np.int64))
+
+    def read_cnt(self, off=0, length=None):
+        length = self.space.size - off if length is None else length
+        out = np.empty(length, dtype=np.uint16)
+        _check("read_cnt", self._lib.dct_read_cnt(
+            int(off), int(length), _ptr(out, ctypes.POINTER(ctypes.c_uint16))))
+        return out
+
+    # ---------------- additive: schedule and per-sweep timing -------------
+    def sweep_plan(self):
+        buf = (ctypes.c_int * (6 * 16))()
+        ns = self._lib.dct_sweep_plan(buf, 16)
+        if ns < 0:
+            _check("sweep_plan", ns)
+        keys = ("first_axis", "axes", "P", "TL", "layers_in", "layers_out")
+        return [dict(zip(keys, buf[6 * i:6 * i + 6])) for i in range(ns)]
+
+    def sweep_profile(self, on=True):
+        _check("sweep_profile", self._lib.dct_sweep_profile(1 if on else 0))
+
+    def sweep_times(self):
+        buf = (ctypes.c_float * 16)()
+        ns = self._lib.dct_sweep_times(buf, 16)
+        if ns < 0:
+            _check("sweep_times", ns)
+        return [float(buf[i]) for i in range(ns)]
+
+
+class GpuCoveringBackend(GpuBackend):
+    """Homogeneous covering-code cell K_q(n,R) (reference gpu.py:540-545)."""
+
+    def __init__(self, q, n, R, **kw):
+        super().__init__([q] * n, R, problem="hamming", **kw)
+        self.q, self.n = q, n

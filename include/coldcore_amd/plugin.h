@@ -1,0 +1,139 @@
+/* coldcore_amd/plugin.h -- C ABI of the MI355X exact-transform engine.
+ *
+ * The engine library libcoldcore_covering.so (built from
+ * coldcore_amd/csrc/dct_engine.hip for gfx950) is a drop-in for the
+ * reference's CUDA plugin of the same name.  Every entry point keeps the
+ * reference's name, signature, argument meaning and return codes; the
+ * comment on each cites the reference declaration it replaces
+ * (reference src/include/coldcore/plugin.h, implementation
+ * src/plugins/covering/dct_hc.cu).
+ *
+ * Conventions (reference plugin.h:7-19, docs/PLUGIN.md:71-140):
+ *  - singleton: one cell per process; not thread-safe;
+ *  - return codes: 0 ok, -1 invalid argument or rejected input (state left
+ *    untouched), -2 space above 2^40, -3/-4/-5 allocation failure,
+ *    -(100 + hipError_t) device error (message on stderr);
+ *  - caller-allocated outputs; inputs are read only during the call;
+ *  - all device work runs in call order on one stream owned by the engine;
+ *    calls that return data synchronise once, ball_update of <= 1024 words
+ *    returns without waiting (a device fault surfaces at the next
+ *    synchronising call);
+ *  - every plane lives in HBM: layers_mode / cnt_mode are accepted and
+ *    ignored, dct_host_bytes() returns 0.
+ * Strict refinements of the reference contract: map_extract(64) returns the
+ * hits in ascending index order; read_cnt and the gathers validate their
+ * ranges and return -1 instead of reading out of bounds; init rejects a
+ * support table whose patterns are malformed (position >= n, repeated
+ * position, delta outside 1..ax-1).
+ */
+#ifndef COLDCORE_AMD_PLUGIN_H
+#define COLDCORE_AMD_PLUGIN_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define COLDCORE_PLUGIN_ABI_VERSION 1
+
+/* problem ids (reference plugin.h:45-53) */
+#define CCP_PROB_HAMMING    0
+#define CCP_PROB_TORUS_LINF 1
+#define CCP_PROB_LEE        2
+#define CCP_PROB_GRID_LINF  3
+
+/* ---- lifecycle ---------------------------------------------------------- */
+/* reference plugin.h:62-65, dct_hc.cu:230-374.  ballpat: v1 support table,
+ * one uint64 per pattern, byte k = pos<<4 | delta; entry 0 = the centre. */
+int  dct_init3(int problem, int n, const int *ax, int R,
+               const uint64_t *ballpat, long long balllen, int extract_cap,
+               long long *bytes, int layers_mode, int cnt_mode,
+               int use_owner);
+/* reference plugin.h:66, dct_hc.cu:393-428 (also resets mu to 1) */
+int  dct_free(void);
+/* reference plugin.h:67, dct_hc.cu:208 */
+long long dct_host_bytes(void);
+/* reference plugin.h:70-73, dct_hc.cu:376-391 (homogeneous Hamming) */
+int  dct_init2(int q, int n, int R, const uint64_t *ballpat,
+               long long balllen, int extract_cap, long long *bytes,
+               int layers_mode, int cnt_mode, int use_owner);
+int  dct_init (int q, int n, int R, const uint64_t *ballpat,
+               long long balllen, int extract_cap, long long *bytes);
+
+/* ---- solution / transforms ---------------------------------------------- */
+/* reference plugin.h:79, dct_hc.cu:452-459: pending code multiset (<= 65535
+ * occurrences); it becomes the represented solution at transform(0, 2). */
+int  dct_set_code(const long long *idx, int M);
+/* reference plugin.h:80, dct_hc.cu:465-521.  init_mode 0 = code multiset,
+ * 1 = [cnt < mu], 2 = [1 <= cnt <= mu]; out_mode 1 = sum -> map (A0),
+ * 2 = clamped sum -> cnt (init_mode 0 only). */
+int  dct_transform(int init_mode, int out_mode);
+
+/* ---- mu-fold covering ---------------------------------------------------- */
+/* reference plugin.h:89-90, dct_hc.cu:214-222 */
+int  dct_set_mu(int mu);
+int  dct_get_mu(void);
+
+/* ---- incremental / exact per-candidate ---------------------------------- */
+/* reference plugin.h:93, dct_hc.cu:567-587 */
+int  dct_ball_update(const long long *words, int nwords, int delta);
+/* reference plugin.h:94-95, dct_hc.cu:785-788: literal cnt == target */
+int  dct_ball_gather(const long long *words, int nwords, int target,
+                     int32_t *out);
+/* reference plugin.h:98-99, dct_hc.cu:794-799: role 0 deficient, 1 critical */
+int  dct_ball_gather_mu(const long long *words, int nwords, int role,
+                        int32_t *out);
+/* reference plugin.h:100, dct_hc.cu:997-1022 (needs use_owner = 1) */
+int  dct_loss_owner(const long long *words, int M, int32_t *out);
+
+/* ---- v0.3 optional ------------------------------------------------------- */
+/* reference plugin.h:110-111, dct_hc.cu:826-866 */
+int  dct_ball_update_delta(const long long *words, int nwords, int delta,
+                           long long *dd);
+/* reference plugin.h:112-114, dct_hc.cu:882-977 */
+int  dct_swap_eval(const long long *drops, int k, const long long *cand,
+                   const long long *off, int32_t *gain, int32_t *loss);
+/* reference plugin.h:115, dct_hc.cu:779-783 */
+int  dct_stats(long long *out, int n);
+
+/* ---- reductions / map queries ------------------------------------------- */
+/* reference plugin.h:121, dct_hc.cu:1024-1054 */
+long long dct_count_eq(int target);
+/* reference plugin.h:124, dct_hc.cu:981-992 */
+long long dct_count_deficit(void);
+/* reference plugin.h:125, dct_hc.cu:1056-1069 */
+int  dct_map_max(int32_t *out);
+/* reference plugin.h:126, dct_hc.cu:1071-1089 */
+int  dct_map_hist(int nbins, int32_t vmax, int32_t *out);
+/* reference plugin.h:127-130, dct_hc.cu:1091-1134 (ascending order here) */
+long long dct_map_extract  (int32_t thr, uint32_t *idx, int32_t *val,
+                            int cap);
+long long dct_map_extract64(int32_t thr, int64_t *idx, int32_t *val,
+                            int cap);
+/* reference plugin.h:131, dct_hc.cu:1136-1150 */
+int  dct_map_read_at(const long long *idx, int k, int32_t *out);
+/* reference plugin.h:132, dct_hc.cu:1152-1157 */
+int  dct_read_cnt(long long off, long long len, uint16_t *out);
+
+/* ---- optional ox support-table format ----------------------------------- */
+/* reference plugin.h:135-137, dct_hc.cu:430-450 */
+int  dct_set_tbl2(const uint16_t *mask, const uint64_t *offs,
+                  long long len);
+int  dct_use_fmt(int fmt);
+
+/* ---- additive, no counterpart in the reference -------------------------- */
+/* Schedule of the LDS-tiled sweeps: 6 ints per sweep (first axis, axes in
+ * the group, P, TL, layers in, layers out); returns the sweep count.  For
+ * benchmarks and the roofline accounting of each sweep's traffic. */
+int  dct_sweep_plan(int *out, int cap);
+/* Record HIP events around every sweep of later transforms (on the
+ * engine's stream; also enabled by DCT_TIMING=1 at init). */
+int  dct_sweep_profile(int on);
+/* Per-sweep milliseconds of the last transform; returns the sweep count. */
+int  dct_sweep_times(float *ms, int cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* COLDCORE_AMD_PLUGIN_H */

@@ -1,0 +1,125 @@
+"""Host-side logic (no GPU): support tables, the binding's table order, and
+the C-ABI library's exported surface."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from coldcore_amd.patterns import (ball_patterns, ball_size, decode_v1,
+                                   lee_ball_size, support_patterns,
+                                   support_patterns_ox)
+from coldcore_amd.space import ProductSpace
+from oracle.dp import NumpyDP
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "coldcore_amd", "plugin.h")
+LIB = os.path.join(ROOT, "coldcore_amd", "libcoldcore_covering.so")
+
+
+def test_index_roundtrip():
+    sp = ProductSpace(5, 6)
+    idx = np.array([0, 1, 5, 5 ** 6 - 1, 12345], dtype=np.int64)
+    assert (sp.word_index(sp.index_word(idx)) == idx).all()
+    mixed = ProductSpace([2, 3, 4])
+    assert mixed.size == 24 and list(mixed.pow_q) == [1, 2, 6]
+
+
+@pytest.mark.parametrize("q,n,R", [(3, 4, 2), (2, 8, 3), (4, 5, 2), (5, 3, 1)])
+def test_hamming_table_is_the_ball(q, n, R):
+    pats = ball_patterns(q, n, R)
+    assert len(pats) == ball_size(q, n, R) and pats[0] == 0
+    sp = ProductSpace(q, n)
+    centre = np.arange(n) % q
+    seen = set()
+    for p in pats:
+        d = decode_v1(p)
+        assert len(d) <= R
+        w = centre.copy()
+        for pos, delta in d.items():
+            w[pos] = (w[pos] + delta) % q
+        seen.add(int(sp.word_index(w[None, :])[0]))
+    assert len(seen) == ball_size(q, n, R)
+
+
+def test_radius_above_length_is_whole_space():
+    for q, n, R in [(3, 2, 3), (2, 3, 5)]:
+        assert len(ball_patterns(q, n, R)) == q ** n == ball_size(q, n, R)
+
+
+def test_ox_matches_v1():
+    pats = ball_patterns(4, 5, 2)
+    masks, offs = support_patterns_ox([4] * 5, 2)
+    assert len(masks) == len(pats)
+    for k in range(len(pats)):
+        d = decode_v1(pats[k])
+        assert sum(1 << p for p in d) == int(masks[k])
+        for p, dl in d.items():
+            assert (int(offs[k]) >> (4 * p)) & 0xF == dl
+
+
+@pytest.mark.parametrize("axes,R", [([5, 5], 1), ([7, 7], 2), ([4, 5, 6], 3),
+                                    ([8, 8], 4), ([9, 9, 9], 2)])
+def test_lee_table_size(axes, R):
+    assert len(support_patterns(axes, R, "lee")) == lee_ball_size(axes, R)
+
+
+@pytest.mark.parametrize("problem,axes,R", [
+    ("hamming", [2, 3, 4, 5], 2), ("lee", [4, 5, 6], 3),
+    ("torus_linf", [7, 7, 7], 1), ("grid_linf", [5, 6, 4], 2),
+    ("grid_linf", [7, 7], 5), ("hamming", [3] * 5, 2)])
+def test_table_walks_the_oracle_ball(problem, axes, R):
+    """Decoding the packed bytes and walking them from a centre gives the
+    oracle's ball (the same check the reference runs,
+    tests/test_dp_backend.py:163-194)."""
+    pats = support_patterns(axes, R, problem)
+    dp = NumpyDP(axes, R, problem)
+    sp = ProductSpace(axes)
+    for w in (0, sp.size // 3, sp.size - 1):
+        dig = sp.index_word([w])[0].astype(np.int64)
+        got = set()
+        for v in pats.tolist():
+            x, ok = w, True
+            for p, d in decode_v1(v).items():
+                nd = (int(dig[p]) + d) % axes[p]
+                sh = nd - int(dig[p])
+                if problem == "grid_linf" and abs(sh) > R:
+                    ok = False
+                    break
+                x += sh * int(sp.pow_q[p])
+            if ok:
+                got.add(x)
+        assert got == set(dp.ball_indices(w).tolist())
+
+
+def test_locality_order_is_permutation():
+    from coldcore_amd.gpu import locality_order
+    for problem, axes, R in [("hamming", [5] * 6, 3), ("lee", [7, 7, 7], 3),
+                             ("grid_linf", [4, 5, 6], 3)]:
+        pats = support_patterns(axes, R, problem)
+        out = locality_order(pats, axes)
+        assert out[0] == 0 and (np.sort(out) == np.sort(pats)).all()
+
+
+def _declared_symbols():
+    with open(HEADER) as f:
+        text = f.read()
+    return sorted(set(re.findall(r"\b(dct_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_the_v1_surface():
+    from coldcore_amd.gpu import ADDITIVE, OPTIONAL, REQUIRED
+    declared = _declared_symbols()
+    assert set(REQUIRED) | set(OPTIONAL) | set(ADDITIVE) <= set(declared)
+
+
+def test_library_exports_every_declared_symbol():
+    """Load the built library (no device call) and resolve every symbol the
+    header declares."""
+    if not os.path.exists(LIB):
+        pytest.fail("libcoldcore_covering.so is not built "
+                    "(python -m coldcore_amd.build)")
+    lib = ctypes.CDLL(LIB)
+    for name in _declared_symbols():
+        assert hasattr(lib, name), name
