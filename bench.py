@@ -1,0 +1,248 @@
+"""Benchmark: field passes/s and achieved HBM GB/s on K_2(24,3).
+
+Workload (the hot path at its published single-GPU scale): the binary
+covering-code cell K_2(24,3), |X| = 2^24 words, R = 3 (4 int32 distance
+layers), code = flatnonzero(default_rng(0).random(2^24) < 1e-4) (1,678
+words, synthetic).  One step = one full exact gain-field pass
+(dct_transform(1, 1): predicate cnt < mu seeded into layer 0, all 24 axis
+passes, layer sum into the map).  Inputs are resident in HBM before the
+timed region; each pass is one ABI call that synchronises on return, so
+the time is the real per-call wall time through the C ABI.
+
+Also reported: loss and recount passes, 10^5 single-word updates
+(alternating +1 / -1, seed 1), 10^6 exact gain queries (seed 2, chunks of
+65,536), the per-sweep schedule with its algorithmic bytes, the roofline of
+the dominant sweep kernel (HIP events on the engine's stream), and the CPU
+baseline: the numpy axis-DP oracle on one core over a bounded sample.
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0        # MI355X HBM3E, published
+
+
+def sweep_bytes(plan, kind):
+    """Algorithmic bytes per word of each sweep of one pass.
+
+    kind 'field' (gain / loss: reads the uint16 cnt predicate, writes the
+    int32 map) or 'recount' (reads the int32 code indicator, writes uint16
+    cnt).  First sweep reads its seed and writes every layer it produced
+    (gain: layer 0 too, recount: layers 1..); middle sweeps read their live
+    layers and write the layers above 0; the last reads its live layers and
+    writes the result."""
+    out = []
+    ns = len(plan)
+    for i, sw in enumerate(plan):
+        lin, lout = sw["layers_in"], sw["layers_out"]
+        rd = (2 if kind == "field" else 4) if i == 0 else 4 * lin
+        if i == ns - 1:
+            wr = 4 if kind == "field" else 2
+        elif i == 0:
+            wr = 4 * lout if kind == "field" else 4 * (lout - 1)
+        else:
+            wr = 4 * (lout - 1)
+        out.append(rd + wr)
+    return out
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def run_cpu_baseline(code, passes):
+    """numpy axis-DP oracle, one thread, a bounded sample of gain passes."""
+    from oracle.dp import NumpyDP
+    try:
+        os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[0]})
+    except (AttributeError, OSError):
+        pass
+    c = NumpyDP([2] * 24, 3)
+    c.recount(code)
+    c.gain_map()                      # warm-up
+    t = time.perf_counter()
+    for _ in range(passes):
+        c.gain_map()
+    dt = (time.perf_counter() - t) / passes
+    return dict(value=round(1.0 / dt, 4), unit="field passes/s", cores=1,
+                kind="port",
+                sample=f"{passes} gain passes of K_2(24,3) on the numpy "
+                       f"axis-DP oracle after one warm-up, 1 thread "
+                       f"({dt * 1e3:.0f} ms/pass); host {cpu_model()}, "
+                       f"os.cpu_count()={os.cpu_count()}")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--cpu-passes", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs the hot path on a "
+              "single MI355X (multi-GPU sharding is a design note, not "
+              "code); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from coldcore_amd.gpu import GpuBackend
+    n, R = 24, 3
+    size = 1 << n
+    code = np.flatnonzero(np.random.default_rng(0).random(size) < 1e-4)
+    code = code.astype(np.int64)
+    g = GpuBackend([2] * n, R)
+    try:
+        g.recount(code)
+        plan = g.sweep_plan()
+        for _ in range(args.warmup):
+            g.gain_map()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            g.gain_map()
+        t1 = time.perf_counter()
+        ms = (t1 - t0) * 1e3 / args.steps
+        passes_s = 1e3 / ms
+
+        # per-sweep event timing on the engine's stream (separate passes, so
+        # the events do not perturb the timed region above)
+        g.sweep_profile(True)
+        per = []
+        for _ in range(args.steps):
+            g.gain_map()
+            per.append(g.sweep_times())
+        g.sweep_profile(False)
+        per = np.array(per)                       # (steps, sweeps) ms
+        sweep_ms = per.mean(axis=0)
+        bpw = sweep_bytes(plan, "field")
+        dom = int(np.argmax(sweep_ms))
+        dom_bytes = bpw[dom] * size
+        dom_gbs = dom_bytes / (sweep_ms[dom] * 1e-3) / 1e9
+        pass_bytes = sum(bpw) * size
+        kernel_ms = float(per.sum(axis=1).mean())
+
+        extra = {}
+        if not args.no_extra:
+            # loss and recount passes
+            for name, fn in (("loss", g.loss_map),
+                             ("recount", lambda: g.recount(code))):
+                fn()
+                t = time.perf_counter()
+                for _ in range(10):
+                    fn()
+                extra[f"{name}_pass_ms"] = round(
+                    (time.perf_counter() - t) * 100, 4)
+            rb = sweep_bytes(plan, "recount")
+            extra["recount_bytes_per_word"] = sum(rb)
+            # 10^5 single-word updates, alternating +1 (random non-code word)
+            # and -1 (random current codeword)
+            rng = np.random.default_rng(1)
+            live = list(code.tolist())
+            liveset = set(live)
+            adds = rng.integers(0, size, 100000)
+            picks = rng.random(100000)
+            g.count_eq(0)
+            t = time.perf_counter()
+            for i in range(100000):
+                if i % 2 == 0:
+                    w = int(adds[i])
+                    while w in liveset:
+                        w = (w + 1) % size
+                    g.ball_update(np.array([w], dtype=np.int64), +1)
+                    live.append(w)
+                    liveset.add(w)
+                else:
+                    j = int(picks[i] * len(live))
+                    w = live[j]
+                    live[j] = live[-1]
+                    live.pop()
+                    liveset.discard(w)
+                    g.ball_update(np.array([w], dtype=np.int64), -1)
+            g.count_eq(0)                          # drains the stream
+            du = time.perf_counter() - t
+            extra["updates_per_s"] = round(100000 / du, 1)
+            extra["update_us"] = round(du / 100000 * 1e6, 3)
+            # batches of 1,024 words (+1 then -1)
+            wb = rng.choice(size, 1024, replace=False).astype(np.int64)
+            t = time.perf_counter()
+            for _ in range(20):
+                g.ball_update(wb, +1)
+                g.ball_update(wb, -1)
+            g.count_eq(0)
+            extra["update_words_per_s_batch1024"] = round(
+                40 * 1024 / (time.perf_counter() - t), 1)
+            # 10^6 exact gain queries in chunks of 65,536
+            q = np.random.default_rng(2).integers(0, size, 10 ** 6)
+            q = q.astype(np.int64)
+            g.ball_gather(q[:65536], 0)
+            t = time.perf_counter()
+            for lo in range(0, len(q), 65536):
+                g.ball_gather(q[lo:lo + 65536], 0)
+            dq = time.perf_counter() - t
+            extra["queries_per_s"] = round(1e6 / dq, 1)
+            extra["query_cnt_reads_per_s"] = round(1e6 * 2325 / dq, 1)
+    finally:
+        g.close()
+
+    rec = {
+        "metric": "field passes/s + HBM GB/s on K_2(24,3)",
+        "value": round(passes_s, 3),
+        "unit": "field passes/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic: seeded random code, p=1e-4 (1,678 words)",
+        "config": {
+            "workload": "K_2(24,3) exact gain-field pass, 2^24 words, R=3",
+            "axes": "[2]*24", "R": 3, "code_words": int(len(code)),
+            "schedule": [f"axes {s['first_axis']}..{s['first_axis'] + s['axes'] - 1}"
+                         f" P={s['P']} TL={s['TL']}" for s in plan],
+        },
+        "pass_bytes_algorithmic": int(pass_bytes),
+        "pass_bytes_per_word": int(sum(bpw)),
+        "pass_GBps_algorithmic": round(pass_bytes / (ms * 1e-3) / 1e9, 1),
+        "pass_kernel_ms": round(kernel_ms, 5),
+        "sweep_ms": [round(float(x), 5) for x in sweep_ms],
+        "sweep_bytes_per_word": bpw,
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(dom_gbs, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(dom_gbs / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "kernel": f"k_sweep sweep {dom} (axes {plan[dom]['first_axis']}.."
+                      f"{plan[dom]['first_axis'] + plan[dom]['axes'] - 1}, "
+                      f"{bpw[dom]} B/word x 2^24 words per launch)",
+        },
+    }
+    rec.update(extra)
+    if not args.no_cpu:
+        rec["cpu_baseline"] = run_cpu_baseline(code, args.cpu_passes)
+        rec["speedup_vs_cpu"] = round(passes_s / rec["cpu_baseline"]["value"], 1)
+    print(json.dumps(rec), flush=True)
+
+
+if __name__ == "__main__":
+    main()

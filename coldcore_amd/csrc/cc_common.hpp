@@ -10,11 +10,18 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
-#define CC_MAXN 16      // axes
+#define CC_MAXN 40      // axes (|X| <= 2^40); the v1 table format caps n at 16
+#define CC_MAXN_V1 16   // axes addressable by the v1 byte format
+#define CC_MAXSW 24     // axes inside one sweep (its tile has <= 2^20 words)
 #define CC_MAXAX 16     // axis size (nibble-packed support deltas)
 #define CC_MAXR 6       // Hamming / Lee radius (layer planes R+1 <= 7)
 #define CC_MAXL (CC_MAXR + 1)
 #define CC_MAXPAT 8     // bytes per packed v1 pattern
+#define CC_MAXPAT_W 6   // 10-bit fields per packed wide pattern
+
+// support-table formats the walks decode
+#define CC_FMT_V1 0     // uint64, byte k = pos<<4 | delta (pos < 16), <= 8
+#define CC_FMT_WIDE 1   // uint64, bits [10k, 10k+10) = pos<<4 | delta, <= 6
 #define CC_CSTRIDE 15   // per-position slots of the walk displacement table
 #define CC_BLOCK 256    // threads per workgroup (4 waves of 64)
 
@@ -101,8 +108,12 @@ struct SweepArgs {
   int R;
   int in_kind, out_kind, pmode, mu;
   int write_l0;      // CC_OUT_LAYERS: also store layer 0 (first sweep only)
-  int Q[CC_MAXN];    // axis sizes of the group
-  int sk[CC_MAXN];   // in-tile stride (units of p) of each group axis
+  int Q[CC_MAXSW];   // axis sizes of the group
+  int sk[CC_MAXSW];  // in-tile stride (units of p) of each group axis
+  // live[k] = layers that can be nonzero before group axis k (live[nax]
+  // after the group): min(sum of the per-axis maximal distances so far, R)+1,
+  // one per axis for Hamming and floor(ax/2) for Lee
+  int live[CC_MAXSW + 1];
 };
 
 // ---------------------------------------------------------------- walks

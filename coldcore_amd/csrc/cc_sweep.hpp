@@ -10,9 +10,10 @@
 //   torus    newA[a] = sum_{|e|<=R} A[(a+e) mod Q]     (dctcore_torus.cuh:25-48)
 //   grid     newA[a] = sum_{b=max(0,a-R)}^{min(Q-1,a+R)} A[b]
 //            (dctcore_grid.cuh:44-66)
-// After axis c only layers 0..min(c,R) can be nonzero, so a pass reads
-// lv = min(c,R)+1 layers and writes lw = min(c+1,R)+1 (reference
-// dctcore_hamming.cuh:98-110).  Layer 0 never changes for the two layered
+// Only the layers up to the largest distance the axes done so far can
+// produce are nonzero (Hamming: one per axis, reference dctcore_hamming.cuh:
+// 98-110; Lee: floor(Q/2) per axis), so a pass reads only the live layers
+// and writes only the ones it can make nonzero.  Layer 0 never changes for the two layered
 // families.  Arithmetic is modulo 2^32, so int32 wrap-around (if a cell ever
 // reached it) is reproduced exactly.
 //
@@ -112,8 +113,8 @@ __global__ void __launch_bounds__(CC_BLOCK) k_sweep(SweepArgs a) {
   extern __shared__ int32_t S[];
   constexpr int QMIN = CcQRange<PROB>::lo, QMAX = CcQRange<PROB>::hi;
   const int PT = a.P * a.TL;
-  const int Lmax = (R1 == 1) ? 1 : min(a.c0 + a.nax, a.R) + 1;
-  const int Lin = (R1 == 1) ? 1 : min(a.c0, a.R) + 1;
+  const int Lmax = a.live[a.nax];
+  const int Lin = a.live[0];
   for (long long tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
     const long long h = tile / a.ncolb;
     const long long l0 = (tile - h * a.ncolb) * a.TL;
@@ -135,9 +136,7 @@ __global__ void __launch_bounds__(CC_BLOCK) k_sweep(SweepArgs a) {
     __syncthreads();
     // ---- every axis of the group, in LDS
     for (int k = 0; k < a.nax; k++) {
-      const int c = a.c0 + k;
-      const int lv = (R1 == 1) ? 1 : min(c, a.R) + 1;
-      const int lw = (R1 == 1) ? 1 : min(c + 1, a.R) + 1;
+      const int lv = a.live[k], lw = a.live[k + 1];
       const int Q = a.Q[k], sk = a.sk[k];
       const int nf = (a.P / Q) * a.TL;
       switch (Q) {
@@ -171,3 +170,160 @@ __global__ void __launch_bounds__(CC_BLOCK) k_sweep(SweepArgs a) {
   }
 }
 #undef CC_STEP_CASE
+
+// ---------------------------------------------------------------------------
+// Round-based sweep for a homogeneous group (every axis of the group has size
+// Q): each thread holds a sub-cube of RR consecutive group axes (C = Q^RR
+// words x R1 layers) in registers and applies those RR axis passes there, so
+// one LDS round trip covers RR axes instead of one.  The first round loads
+// straight from HBM, the last stores straight to HBM; only the rounds in
+// between exchange through LDS (a padded layout, idx + idx/16, keeps the
+// stride-C accesses of the first round off a single bank).  A group whose
+// axis count m is not a multiple of RR ends with a cube placed at the top
+// RR axes that applies only the axes not yet done.  Layers above the live
+// count are zero in registers and stay zero under every recurrence, so the
+// arithmetic is the per-fiber arithmetic of cc_fiber, unconditioned.
+__device__ __forceinline__ int cc_pad(int i) { return i + (i >> 4); }
+
+template <int PROB, int R1, int Q, int C, int ST>
+__device__ __forceinline__ void cc_cube_axis(uint32_t (&v)[R1][C], int R) {
+#pragma unroll
+  for (int b = 0; b < C; b++) {
+    if ((b / ST) % Q != 0) continue;
+    uint32_t f[R1][Q];
+#pragma unroll
+    for (int d = 0; d < R1; d++)
+#pragma unroll
+      for (int a = 0; a < Q; a++) f[d][a] = v[d][b + a * ST];
+    if (PROB == CC_HAMMING) {
+#pragma unroll
+      for (int d = 1; d < R1; d++) {
+        uint32_t t = 0;
+#pragma unroll
+        for (int a = 0; a < Q; a++) t += f[d - 1][a];
+#pragma unroll
+        for (int a = 0; a < Q; a++) v[d][b + a * ST] = f[d][a] + t - f[d - 1][a];
+      }
+    } else if (PROB == CC_LEE) {
+      constexpr int W = Q / 2;
+#pragma unroll
+      for (int d = 1; d < R1; d++)
+#pragma unroll
+        for (int a = 0; a < Q; a++) {
+          uint32_t s = f[d][a];
+#pragma unroll
+          for (int w = 1; w <= W; w++)
+            if (w <= d) {
+              const int am = (a - w + Q) % Q, ap = (a + w) % Q;
+              s += f[d - w][am];
+              if (ap != am) s += f[d - w][ap];
+            }
+          v[d][b + a * ST] = s;
+        }
+    } else {
+#pragma unroll
+      for (int a = 0; a < Q; a++) {
+        uint32_t s = 0;
+#pragma unroll
+        for (int bb = 0; bb < Q; bb++) {
+          int dd = a > bb ? a - bb : bb - a;
+          if (PROB == CC_TORUS && Q - dd < dd) dd = Q - dd;
+          s += (dd <= R) ? f[0][bb] : 0u;
+        }
+        v[0][b + a * ST] = s;
+      }
+    }
+  }
+}
+
+template <int PROB, int R1, int Q, int C, int I, int RR>
+__device__ __forceinline__ void cc_cube_round(uint32_t (&v)[R1][C], int ilo,
+                                              int R) {
+  if constexpr (I < RR) {
+    constexpr int ST = I == 0 ? 1 : (I == 1 ? Q : (I == 2 ? Q * Q : Q * Q * Q));
+    if (I >= ilo) cc_cube_axis<PROB, R1, Q, C, ST>(v, R);
+    cc_cube_round<PROB, R1, Q, C, I + 1, RR>(v, ilo, R);
+  }
+}
+
+template <int PROB, int R1, int Q, int RR>
+__global__ void __launch_bounds__(CC_BLOCK) k_sweep_r(SweepArgs a) {
+  extern __shared__ int32_t S[];
+  constexpr int C = RR == 1 ? Q : (RR == 2 ? Q * Q : (RR == 3 ? Q * Q * Q : Q * Q * Q * Q));
+  const int m = a.nax;
+  const int PT = a.P * a.TL;
+  const int PTP = cc_pad(PT) + 1;               // padded layer pitch
+  const int Lmax = a.live[m];
+  const int Lin = a.live[0];
+  const int nrounds = (m + RR - 1) / RR;
+  const int ncube = PT / C;
+  for (long long tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
+    const long long h = tile / a.ncolb;
+    const long long l0 = (tile - h * a.ncolb) * a.TL;
+    const int ncol = (int)min((long long)a.TL, a.s - l0);
+    const long long base = h * a.s * a.P + l0;
+    for (int rd = 0; rd < nrounds; rd++) {
+      const bool first = rd == 0, last = rd == nrounds - 1;
+      const int lo = rd * RR;                     // first axis of this round
+      const int ca = min(lo, m - RR);             // cube's lowest group axis
+      int qa = 1;                                 // Q^ca: in-tile stride
+      for (int i = 0; i < ca; i++) qa *= Q;
+      for (int cu = threadIdx.x; cu < ncube; cu += CC_BLOCK) {
+        const int tl = cu % a.TL, u = cu / a.TL;
+        const int pb = (u / qa) * qa * C + (u % qa);
+        const bool live = tl < ncol;
+        uint32_t v[R1][C];
+#pragma unroll
+        for (int c = 0; c < C; c++) {
+          const int p = pb + c * qa;
+          if (first) {
+            const long long w = base + (long long)p * a.s + tl;
+            if (a.in_kind == CC_IN_PRED)
+              v[0][c] = (live && cc_pred(a.pmode, (int)a.cnt[w], 0, a.mu)) ? 1u : 0u;
+            else
+              v[0][c] = live ? (uint32_t)a.A[0][w] : 0u;
+#pragma unroll
+            for (int d = 1; d < R1; d++)
+              v[d][c] = (live && d < Lin && a.in_kind == CC_IN_LAYERS)
+                            ? (uint32_t)a.A[d][w] : 0u;
+          } else {
+            const int e = cc_pad(p * a.TL + tl);
+#pragma unroll
+            for (int d = 0; d < R1; d++)
+              v[d][c] = d < Lmax ? (uint32_t)S[d * PTP + e] : 0u;
+          }
+        }
+        cc_cube_round<PROB, R1, Q, C, 0, RR>(v, lo - ca, a.R);
+#pragma unroll
+        for (int c = 0; c < C; c++) {
+          const int p = pb + c * qa;
+          if (last) {
+            if (!live) continue;
+            const long long w = base + (long long)p * a.s + tl;
+            if (a.out_kind == CC_OUT_LAYERS) {
+#pragma unroll
+              for (int d = 0; d < R1; d++)
+                if (d < Lmax && (d > 0 || a.write_l0)) a.A[d][w] = (int32_t)v[d][c];
+            } else {
+              uint32_t sum = 0;
+#pragma unroll
+              for (int d = 0; d < R1; d++) sum += v[d][c];
+              if (a.out_kind == CC_OUT_MAP) {
+                a.A[0][w] = (int32_t)sum;
+              } else {
+                const int32_t si = (int32_t)sum;
+                a.cnt[w] = si > 65535 ? (uint16_t)65535 : (uint16_t)si;
+              }
+            }
+          } else {
+            const int e = cc_pad(p * a.TL + tl);
+#pragma unroll
+            for (int d = 0; d < R1; d++)
+              if (d < Lmax && (first || d > 0 || R1 == 1)) S[d * PTP + e] = (int32_t)v[d][c];
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}

@@ -1,12 +1,15 @@
 // cc_walk.hpp -- support-set walks: exact incremental updates and exact
 // per-candidate gathers over one word's ball.
 //
-// The ball of centre w is w + D(p) for every v1 pattern p, where D sums the
+// The ball of centre w is w + D(p) for every pattern p, where D sums the
 // per-(position, delta) board displacement ((dig_pos + delta) mod ax_pos -
 // dig_pos) * stride_pos (reference src/plugins/covering/dctcore_core.cuh:
 // 158-212).  The open grid resolves the same circular delta against the
 // centre digit and drops a pattern whose displacement leaves the L-inf
 // radius, which is a wrap past the board edge (dctcore_grid.cuh:239-286).
+// Two packed formats are decoded: v1 (8-bit pos<<4|delta fields, pos < 16,
+// the reference's wire format) and wide (10-bit fields, pos < 64, at most 6
+// per pattern) for the Hamming / Lee cells with more than 16 axes.
 //
 // One workgroup walks one contiguous segment of one word's table
 // (blockIdx = word * split + segment).  The displacement table lives in LDS
@@ -27,7 +30,7 @@ struct Geom {
 #define CC_WALK_TRANS 1   // ... and count deficit transitions (update_delta)
 #define CC_WALK_GATHER 2  // count predicate(cnt) over the ball
 
-template <int KIND, bool GRID>
+template <int KIND, bool GRID, int FMT>
 __global__ void __launch_bounds__(CC_BLOCK)
     k_walk(Geom g, WordsArg wa, const long long *__restrict__ wptr, int split,
            int delta, uint16_t *cnt, const uint64_t *__restrict__ pat,
@@ -59,9 +62,12 @@ __global__ void __launch_bounds__(CC_BLOCK)
     const uint64_t pt = pat[t];
     long long x = w;
     bool ok = true;
+    constexpr int FB = FMT == CC_FMT_V1 ? 8 : 10;
+    constexpr int NF = FMT == CC_FMT_V1 ? CC_MAXPAT : CC_MAXPAT_W;
+    constexpr unsigned FM = (1u << FB) - 1u;
 #pragma unroll
-    for (int k = 0; k < CC_MAXPAT; k++) {
-      const unsigned b = (unsigned)(pt >> (8 * k)) & 255u;
+    for (int k = 0; k < NF; k++) {
+      const unsigned b = (unsigned)(pt >> (FB * k)) & FM;
       if (!b) break;
       const int e = (int)(b >> 4) * CC_CSTRIDE + (int)(b & 15u) - 1;
       x += disp[e];

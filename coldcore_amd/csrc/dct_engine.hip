@@ -49,6 +49,7 @@ struct Sweep {
   int j0, nax, c0, P, TL;
   long long s, ncolb, ntiles;
   int lmax;
+  int rq;       // round kernel: axis size Q of the homogeneous group, else 0
   size_t lds;
 };
 
@@ -63,6 +64,7 @@ struct Engine {
   long long cnt_len = 0;
   uint64_t *pat = nullptr, *pat2 = nullptr;
   long long blen = 0;
+  int tfmt = CC_FMT_V1;  // packed format of the init table
   int fmt = 0, use_owner = 0;
   long long *d_code = nullptr;
   int codeM = 0;
@@ -87,6 +89,8 @@ struct Engine {
   int active_size = 0;
   std::vector<Sweep> plan;
   int lds_budget = 64 * 1024;
+  int lds_max = 64 * 1024;   // device limit per workgroup
+  int generic_only = 0;      // CC_SWEEP_KERNEL=generic: no round kernel
   // optional per-sweep timing of the last transform (additive ABI)
   int prof = 0;
   hipEvent_t ev[2 * CC_MAXN] = {};
@@ -133,19 +137,23 @@ int pick_split(int nwords) {
 // A v1 table is walked without per-entry bounds checks, so it is proven
 // well-formed first: positions < n and distinct, 1 <= delta < ax_pos, zero
 // byte terminates.  Any such pattern keeps every walked index inside X.
-bool table_ok(const uint64_t *pat, long long len) {
+bool table_ok(const uint64_t *pat, long long len, int fmt) {
+  const int fb = fmt == CC_FMT_V1 ? 8 : 10;
+  const int nf = fmt == CC_FMT_V1 ? CC_MAXPAT : CC_MAXPAT_W;
+  const uint64_t fm = (1ull << fb) - 1ull;
   for (long long t = 0; t < len; t++) {
     const uint64_t p = pat[t];
-    unsigned seen = 0;
+    if (fmt == CC_FMT_WIDE && (p >> (fb * nf))) return false;
+    uint64_t seen = 0;
     bool ended = false;
-    for (int k = 0; k < CC_MAXPAT; k++) {
-      const unsigned b = (unsigned)(p >> (8 * k)) & 255u;
+    for (int k = 0; k < nf; k++) {
+      const unsigned b = (unsigned)((p >> (fb * k)) & fm);
       if (!b) { ended = true; continue; }
       if (ended) return false;
       const int pos = (int)(b >> 4), d = (int)(b & 15u);
-      if (pos >= E.n || ((seen >> pos) & 1u) || d < 1 || d >= E.ax[pos])
+      if (pos >= E.n || ((seen >> pos) & 1ull) || d < 1 || d >= E.ax[pos])
         return false;
-      seen |= 1u << pos;
+      seen |= 1ull << pos;
     }
   }
   return true;
@@ -156,7 +164,19 @@ bool table_ok(const uint64_t *pat, long long len) {
 // so its tile is one contiguous run of P words; later groups stage TL = 32
 // columns (a 128-byte line per row).  A group grows while its LDS tile
 // (live layers x P x TL int32) stays within the budget.
-int layers_after(int c) { return E.L == 1 ? 1 : std::min(c, E.R) + 1; }
+// layers that can be nonzero after axes 0..c-1: the largest distance those
+// axes can produce is one per axis (Hamming) or floor(ax/2) per axis (Lee)
+int layers_after(int c) {
+  if (E.L == 1) return 1;
+  long long w = 0;
+  for (int j = 0; j < c; j++) w += E.prob == CC_LEE ? E.ax[j] / 2 : 1;
+  return (int)std::min<long long>(w, E.R) + 1;
+}
+
+// axes per register round of the round kernel for axis size q (q^RR <= 16)
+int round_axes(int q) { return q == 2 ? 4 : (q <= 4 ? 2 : 1); }
+
+int padded(int i) { return i + (i >> 4); }
 
 void build_plan() {
   E.plan.clear();
@@ -166,40 +186,85 @@ void build_plan() {
     sw.j0 = j;
     sw.c0 = j;
     sw.s = E.pw[j];
-    sw.TL = (j == 0) ? 1 : (int)std::min<long long>(32, sw.s);
+    const int TL0 = (j == 0) ? 1 : (int)std::min<long long>(32, sw.s);
     long long P = 1;
     int k = j;
     while (k < E.n) {
       long long P2 = P * E.ax[k];
-      size_t need = (size_t)layers_after(k + 1) * P2 * sw.TL * 4;
-      if (k > j && (need > (size_t)E.lds_budget || P2 > (1LL << 20))) break;
+      size_t need = (size_t)layers_after(k + 1) * P2 * TL0 * 4;
+      if (k > j && (need > (size_t)E.lds_budget || P2 > (1LL << 20) ||
+                    k - j >= CC_MAXSW))
+        break;
       P = P2;
       k++;
     }
     sw.nax = k - j;
     sw.P = (int)P;
     sw.lmax = layers_after(k);
+    // widen short strided tiles to ~4096 words so every thread has work
+    int TL = TL0;
+    while (j > 0 && TL * 2LL <= sw.s && P * TL < 4096 &&
+           (size_t)sw.lmax * P * TL * 2 * 4 <= (size_t)E.lds_budget)
+      TL *= 2;
+    sw.TL = TL;
     sw.ncolb = (sw.s + sw.TL - 1) / sw.TL;
     sw.ntiles = (E.X / (sw.s * P)) * sw.ncolb;
-    sw.lds = (size_t)sw.lmax * P * sw.TL * 4;
+    // round kernel: a homogeneous Hamming group of at least one full round
+    sw.rq = 0;
+    bool homog = true;
+    for (int i = j; i < k; i++) homog = homog && E.ax[i] == E.ax[j];
+    const int q = E.ax[j];
+    const size_t lds_r = (size_t)sw.lmax * (padded(P * sw.TL) + 1) * 4;
+    if (!E.generic_only && E.prob == CC_HAMMING && homog && q <= 5 &&
+        sw.nax >= round_axes(q) && lds_r <= (size_t)E.lds_max)
+      sw.rq = q;
+    if (sw.rq)
+      sw.lds = (sw.nax <= round_axes(q)) ? 0 : lds_r;
+    else
+      sw.lds = (size_t)sw.lmax * P * sw.TL * 4;
     E.plan.push_back(sw);
     j = k;
   }
 }
 
-template <int PROB, int R1>
-hipError_t launch_sweep_t(const SweepArgs &a, int blocks, size_t lds) {
-  static bool attr = false;
-  if (!attr && lds > 64 * 1024) {
-    hipError_t e = hipFuncSetAttribute((const void *)k_sweep<PROB, R1>,
+template <typename K>
+hipError_t launch_k(K kf, const SweepArgs &a, int blocks, size_t lds) {
+  if (lds > 64 * 1024) {
+    hipError_t e = hipFuncSetAttribute((const void *)kf,
                                        hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)lds);
     if (e != hipSuccess) return e;
-    attr = true;
   }
-  hipLaunchKernelGGL((k_sweep<PROB, R1>), dim3(blocks), dim3(CC_BLOCK), lds,
-                     E.st, a);
+  hipLaunchKernelGGL(kf, dim3(blocks), dim3(CC_BLOCK), lds, E.st, a);
   return hipGetLastError();
+}
+
+template <int PROB, int R1>
+hipError_t launch_sweep_t(const SweepArgs &a, int blocks, size_t lds) {
+  return launch_k(k_sweep<PROB, R1>, a, blocks, lds);
+}
+
+template <int R1>
+hipError_t launch_round_t(int q, const SweepArgs &a, int blocks, size_t lds) {
+  switch (q) {
+    case 2: return launch_k(k_sweep_r<CC_HAMMING, R1, 2, 4>, a, blocks, lds);
+    case 3: return launch_k(k_sweep_r<CC_HAMMING, R1, 3, 2>, a, blocks, lds);
+    case 4: return launch_k(k_sweep_r<CC_HAMMING, R1, 4, 2>, a, blocks, lds);
+    case 5: return launch_k(k_sweep_r<CC_HAMMING, R1, 5, 1>, a, blocks, lds);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_round(int q, const SweepArgs &a, int blocks, size_t lds) {
+  switch (E.L) {
+    case 2: return launch_round_t<2>(q, a, blocks, lds);
+    case 3: return launch_round_t<3>(q, a, blocks, lds);
+    case 4: return launch_round_t<4>(q, a, blocks, lds);
+    case 5: return launch_round_t<5>(q, a, blocks, lds);
+    case 6: return launch_round_t<6>(q, a, blocks, lds);
+    case 7: return launch_round_t<7>(q, a, blocks, lds);
+  }
+  return hipErrorInvalidValue;
 }
 
 hipError_t launch_sweep(const SweepArgs &a, int blocks, size_t lds) {
@@ -262,9 +327,11 @@ int run_transform(int in_kind, int out_kind, int pmode) {
       a.sk[k] = (int)sk;
       sk *= a.Q[k];
     }
+    for (int k = 0; k <= sw.nax; k++) a.live[k] = layers_after(sw.j0 + k);
     const int blocks = (int)std::min<long long>(sw.ntiles, 1LL << 20);
     if (E.prof) CK(hipEventRecord(E.ev[2 * i], E.st));
-    CK(launch_sweep(a, blocks, sw.lds));
+    if (sw.rq) CK(launch_round(sw.rq, a, blocks, sw.lds));
+    else CK(launch_sweep(a, blocks, sw.lds));
     if (E.prof) CK(hipEventRecord(E.ev[2 * i + 1], E.st));
   }
   E.nswept = ns;
@@ -308,14 +375,19 @@ hipError_t launch_walk(const WordsArg &wa, const long long *wptr, int nb,
                        int32_t *out) {
   const Geom g = make_geom();
   dim3 grid((unsigned)nb * split);
-  if (E.prob == CC_GRID)
-    hipLaunchKernelGGL((k_walk<KIND, true>), grid, dim3(CC_BLOCK), 0, E.st, g,
-                       wa, wptr, split, delta, E.cnt, walk_table(), E.blen,
-                       pmode, target, out, E.d_scal + 1);
-  else
-    hipLaunchKernelGGL((k_walk<KIND, false>), grid, dim3(CC_BLOCK), 0, E.st,
-                       g, wa, wptr, split, delta, E.cnt, walk_table(), E.blen,
-                       pmode, target, out, E.d_scal + 1);
+  const bool wide = E.fmt == 0 && E.tfmt == CC_FMT_WIDE;
+#define CC_WALK_LAUNCH(GR, FM)                                              \
+  hipLaunchKernelGGL((k_walk<KIND, GR, FM>), grid, dim3(CC_BLOCK), 0, E.st, \
+                     g, wa, wptr, split, delta, E.cnt, walk_table(), E.blen,\
+                     pmode, target, out, E.d_scal + 1)
+  if (E.prob == CC_GRID) {
+    CC_WALK_LAUNCH(true, CC_FMT_V1);   // grid cells have n <= 8
+  } else if (wide) {
+    CC_WALK_LAUNCH(false, CC_FMT_WIDE);
+  } else {
+    CC_WALK_LAUNCH(false, CC_FMT_V1);
+  }
+#undef CC_WALK_LAUNCH
   return hipGetLastError();
 }
 
@@ -475,12 +547,11 @@ long long extract_impl(int32_t thr, IDX *dev_idx, IDX *out_idx, int32_t *out_val
 extern "C" {
 
 // ---- lifecycle (reference dct_hc.cu:224-428) -------------------------------
-int dct_init3(int problem, int n, const int *ax, int R,
-              const uint64_t *ballpat, long long balllen, int extract_cap,
-              long long *bytes, int layers_mode, int cnt_mode, int use_owner) {
-  (void)layers_mode;
-  (void)cnt_mode;
-  if (n < 2 || n > CC_MAXN || R < 1 || !ax || !ballpat || balllen < 1 ||
+static int init_cell(int problem, int n, const int *ax, int R,
+                     const uint64_t *ballpat, long long balllen, int tfmt,
+                     int extract_cap, long long *bytes, int use_owner) {
+  const int nmax = tfmt == CC_FMT_V1 ? CC_MAXN_V1 : CC_MAXN;
+  if (n < 2 || n > nmax || R < 1 || !ax || !ballpat || balllen < 1 ||
       balllen > INT_MAX || extract_cap < 1)
     return -1;
   int L;
@@ -519,12 +590,22 @@ int dct_init3(int problem, int n, const int *ax, int R,
     E.ax[i] = ax[i];
     E.pw[i + 1] = E.pw[i] * ax[i];
   }
-  if (!table_ok(ballpat, balllen)) return -1;
+  if (!table_ok(ballpat, balllen, tfmt)) return -1;
+  E.tfmt = tfmt;
   E.use_owner = use_owner;
   E.bytes_dev = 0;
   memset(E.stats, 0, sizeof(E.stats));
+  {
+    int dev = 0, mx = 0;
+    CK(hipGetDevice(&dev));
+    CK(hipDeviceGetAttribute(&mx, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    E.lds_max = mx > 0 ? mx : 64 * 1024;
+  }
   const char *lb = getenv("CC_SWEEP_LDS");
   E.lds_budget = lb ? std::max(4096, atoi(lb)) : 64 * 1024;
+  E.lds_budget = std::min(E.lds_budget, E.lds_max);
+  const char *sk = getenv("CC_SWEEP_KERNEL");
+  E.generic_only = sk && !strcmp(sk, "generic");
   CK(hipStreamCreateWithFlags(&E.st, hipStreamNonBlocking));
   for (int d = 0; d < L; d++)
     if (dalloc(&E.A[d], (size_t)X * 4)) { release_all(); return -3; }
@@ -574,11 +655,32 @@ int dct_init3(int problem, int n, const int *ax, int R,
   return 0;
 }
 
+int dct_init3(int problem, int n, const int *ax, int R,
+              const uint64_t *ballpat, long long balllen, int extract_cap,
+              long long *bytes, int layers_mode, int cnt_mode, int use_owner) {
+  (void)layers_mode;
+  (void)cnt_mode;
+  return init_cell(problem, n, ax, R, ballpat, balllen, CC_FMT_V1,
+                   extract_cap, bytes, use_owner);
+}
+
+// additive: the same cell with a wide-format table (10-bit pos<<4|delta
+// fields, at most 6 per pattern), for Hamming / Lee spaces of up to 40 axes
+int dct_init3w(int problem, int n, const int *ax, int R,
+               const uint64_t *widepat, long long balllen, int extract_cap,
+               long long *bytes, int layers_mode, int cnt_mode,
+               int use_owner) {
+  (void)layers_mode;
+  (void)cnt_mode;
+  return init_cell(problem, n, ax, R, widepat, balllen, CC_FMT_WIDE,
+                   extract_cap, bytes, use_owner);
+}
+
 int dct_init2(int q, int n, int R, const uint64_t *ballpat, long long balllen,
               int extract_cap, long long *bytes, int layers_mode, int cnt_mode,
               int use_owner) {
-  int ax[CC_MAXN];
-  if (n < 2 || n > CC_MAXN) return -1;
+  int ax[CC_MAXN_V1];
+  if (n < 2 || n > CC_MAXN_V1) return -1;
   for (int i = 0; i < n; i++) ax[i] = q;
   return dct_init3(CC_HAMMING, n, ax, R, ballpat, balllen, extract_cap, bytes,
                    layers_mode, cnt_mode, use_owner);
@@ -860,7 +962,7 @@ int dct_read_cnt(long long off, long long len, uint16_t *out) {
 // The ox table must enumerate the same support set; walks are order-free, so
 // it is converted to the v1 byte form once and walked by the same kernels.
 int dct_set_tbl2(const uint16_t *mask, const uint64_t *offs, long long len) {
-  if (!E.live || len != E.blen || !mask || !offs) return -1;
+  if (!E.live || len != E.blen || !mask || !offs || E.n > CC_MAXN_V1) return -1;
   std::vector<uint64_t> v1((size_t)len);
   for (long long t = 0; t < len; t++) {
     uint64_t p = 0;
@@ -875,7 +977,7 @@ int dct_set_tbl2(const uint16_t *mask, const uint64_t *offs, long long len) {
     if (mask[t] >> E.n) return -1;
     v1[(size_t)t] = p;
   }
-  if (!table_ok(v1.data(), len)) return -1;
+  if (!table_ok(v1.data(), len, CC_FMT_V1)) return -1;
   if (!E.pat2 && dalloc(&E.pat2, (size_t)len * 8)) return -3;
   CK(hipMemcpy(E.pat2, v1.data(), (size_t)len * 8, hipMemcpyHostToDevice));
   return 0;
@@ -890,19 +992,21 @@ int dct_use_fmt(int fmt) {
 }
 
 // ---- additive (not in the reference ABI): schedule and sweep timing ------
-// dct_sweep_plan: per sweep 6 ints (first axis, axes, P, TL, layers in,
-// layers out); returns the sweep count.
+// dct_sweep_plan: per sweep 8 ints (first axis, axes, P, TL, layers in,
+// layers out, round-kernel axis size or 0, LDS bytes); returns the count.
 int dct_sweep_plan(int *out, int cap) {
   if (!E.live || !out) return -1;
   const int ns = (int)E.plan.size();
   for (int i = 0; i < ns && i < cap; i++) {
     const Sweep &s = E.plan[i];
-    out[6 * i + 0] = s.j0;
-    out[6 * i + 1] = s.nax;
-    out[6 * i + 2] = s.P;
-    out[6 * i + 3] = s.TL;
-    out[6 * i + 4] = layers_after(s.c0);
-    out[6 * i + 5] = s.lmax;
+    out[8 * i + 0] = s.j0;
+    out[8 * i + 1] = s.nax;
+    out[8 * i + 2] = s.P;
+    out[8 * i + 3] = s.TL;
+    out[8 * i + 4] = layers_after(s.c0);
+    out[8 * i + 5] = s.lmax;
+    out[8 * i + 6] = s.rq;
+    out[8 * i + 7] = (int)s.lds;
   }
   return ns;
 }

@@ -12,7 +12,7 @@ import os
 
 import numpy as np
 
-from .patterns import PROBLEM_IDS, support_patterns
+from .patterns import PROBLEM_IDS, support_patterns, support_patterns_wide
 from .space import ProductSpace
 
 _LL = ctypes.c_longlong
@@ -31,7 +31,8 @@ OPTIONAL = ("dct_map_hist", "dct_set_mu", "dct_get_mu", "dct_ball_gather_mu",
             "dct_count_deficit", "dct_host_bytes", "dct_loss_owner",
             "dct_set_tbl2", "dct_use_fmt", "dct_ball_update_delta",
             "dct_swap_eval", "dct_stats")
-ADDITIVE = ("dct_sweep_plan", "dct_sweep_profile", "dct_sweep_times")
+ADDITIVE = ("dct_init3w", "dct_sweep_plan", "dct_sweep_profile",
+            "dct_sweep_times")
 
 
 def _rc_msg(rc):
@@ -76,11 +77,12 @@ def _bind(path):
                              _LL, ctypes.c_int, _LLP]
     lib.dct_init2.restype = ctypes.c_int
     lib.dct_init2.argtypes = lib.dct_init.argtypes + [ctypes.c_int] * 3
-    lib.dct_init3.restype = ctypes.c_int
-    lib.dct_init3.argtypes = [ctypes.c_int, ctypes.c_int,
-                              ctypes.POINTER(ctypes.c_int), ctypes.c_int,
-                              _U64P, _LL, ctypes.c_int, _LLP,
-                              ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    for f in ("dct_init3", "dct_init3w"):
+        getattr(lib, f).restype = ctypes.c_int
+        getattr(lib, f).argtypes = [ctypes.c_int, ctypes.c_int,
+                                    ctypes.POINTER(ctypes.c_int), ctypes.c_int,
+                                    _U64P, _LL, ctypes.c_int, _LLP,
+                                    ctypes.c_int, ctypes.c_int, ctypes.c_int]
     lib.dct_ball_update_delta.restype = ctypes.c_int
     lib.dct_ball_update_delta.argtypes = [_LLP, ctypes.c_int, ctypes.c_int,
                                           _LLP]
@@ -97,7 +99,7 @@ def load_plugin(path=None):
     return _libs[path]
 
 
-def locality_order(pats, axes):
+def locality_order(pats, axes, bits=8):
     """Sort a v1 table by unwrapped offset so neighbouring table entries
     touch neighbouring words (reference gpu.py:106-129).  Walks are
     order-free, so results are unchanged; entry 0 stays the centre.
@@ -107,8 +109,9 @@ def locality_order(pats, axes):
     pats = np.ascontiguousarray(pats, dtype=np.uint64)
     stride = np.cumprod([1] + list(axes[:-1])).astype(np.int64)
     key = np.zeros(len(pats), dtype=np.int64)
-    for k in range(8):
-        byte = ((pats >> np.uint64(8 * k)) & np.uint64(255)).astype(np.int64)
+    for k in range(64 // bits):
+        byte = ((pats >> np.uint64(bits * k))
+                & np.uint64((1 << bits) - 1)).astype(np.int64)
         pos, delta = byte >> 4, byte & 15
         key += np.where(byte != 0,
                         delta * stride[np.minimum(pos, len(axes) - 1)], 0)
@@ -141,14 +144,24 @@ class GpuBackend:
         self.supports_hist = True
         self._lib = lib = load_plugin(plugin_path)
         self._ext_bufs = None
-        pats = locality_order(support_patterns(self.space.axes, R, problem),
-                              self.space.axes)
+        # v1 tables address 16 axes; larger Hamming / Lee spaces use the
+        # additive wide format (10-bit fields) through dct_init3w
+        self.wide = self.space.n > 16
+        if self.wide:
+            pats = locality_order(
+                support_patterns_wide(self.space.axes, R, problem),
+                self.space.axes, bits=10)
+            init = lib.dct_init3w
+        else:
+            pats = locality_order(
+                support_patterns(self.space.axes, R, problem), self.space.axes)
+            init = lib.dct_init3
         self._pats = pats
         self.layers_mode = layers_mode or 0
         self.cnt_mode = cnt_mode or 0
         by = _LL(0)
         ax = (ctypes.c_int * self.space.n)(*self.space.axes)
-        rc = lib.dct_init3(PROBLEM_IDS[problem], self.space.n, ax, R,
+        rc = init(PROBLEM_IDS[problem], self.space.n, ax, R,
                            _ptr(pats, _U64P), len(pats), extract_cap,
                            ctypes.byref(by), self.layers_mode, self.cnt_mode,
                            1 if use_owner else 0)
@@ -335,12 +348,13 @@ class GpuBackend:
 
     # ---------------- additive: schedule and per-sweep timing -------------
     def sweep_plan(self):
-        buf = (ctypes.c_int * (6 * 16))()
+        buf = (ctypes.c_int * (8 * 16))()
         ns = self._lib.dct_sweep_plan(buf, 16)
         if ns < 0:
             _check("sweep_plan", ns)
-        keys = ("first_axis", "axes", "P", "TL", "layers_in", "layers_out")
-        return [dict(zip(keys, buf[6 * i:6 * i + 6])) for i in range(ns)]
+        keys = ("first_axis", "axes", "P", "TL", "layers_in", "layers_out",
+                "round_q", "lds_bytes")
+        return [dict(zip(keys, buf[8 * i:8 * i + 8])) for i in range(ns)]
 
     def sweep_profile(self, on=True):
         _check("sweep_profile", self._lib.dct_sweep_profile(1 if on else 0))

@@ -87,22 +87,40 @@ def _enumerate(axes, R, problem):
                 yield row[None, :], d
 
 
-def support_patterns(axes, R, problem="hamming"):
-    """Packed v1 support table (any problem, mixed radix)."""
+def _pack(axes, R, problem, bits, nfields, maxpos):
     axes = [int(a) for a in axes]
     if problem not in PROBLEM_IDS:
         raise ValueError(problem)
     if max(axes) > 16:
-        raise ValueError("v1 tables need ax_j <= 16")
+        raise ValueError("packed tables need ax_j <= 16")
+    if len(axes) > maxpos:
+        raise ValueError(f"this table format addresses at most {maxpos} axes")
     parts = [np.zeros(1, dtype=np.uint64)]
     for subs, d in _enumerate(axes, R, problem):
         w = subs.shape[1]
+        if w > nfields:
+            raise ValueError(f"this table format holds at most {nfields} "
+                             "changed positions per pattern")
         out = np.zeros((len(subs), len(d)), dtype=np.uint64)
         for k in range(w):
-            byte = ((subs[:, k][:, None] << 4) | d[None, :, k]).astype(np.uint64)
-            out |= byte << np.uint64(8 * k)
+            fld = ((subs[:, k][:, None] << 4) | d[None, :, k]).astype(np.uint64)
+            out |= fld << np.uint64(bits * k)
         parts.append(out.ravel())
     return np.concatenate(parts)
+
+
+def support_patterns(axes, R, problem="hamming"):
+    """Packed v1 support table (any problem, mixed radix, n <= 16)."""
+    return _pack(axes, R, problem, 8, 8, 16)
+
+
+def support_patterns_wide(axes, R, problem="hamming"):
+    """Wide support table: the same enumeration, each pattern a uint64 of up
+    to six 10-bit fields (pos << 4 | delta) at bits [10k, 10k+10), zero
+    field terminates.  Positions up to 63, so Hamming and Lee cells with
+    more than 16 axes (K_2(24,3), K_2(30,4)) have a table; the engine takes
+    it through the additive ``dct_init3w``."""
+    return _pack(axes, R, problem, 10, 6, 64)
 
 
 def support_patterns_ox(axes, R, problem="hamming"):
@@ -147,10 +165,16 @@ def lee_ball_size(axes, R):
 
 def decode_v1(pat):
     """One packed v1 pattern -> {pos: delta}."""
+    return decode_fields(pat, 8)
+
+
+def decode_fields(pat, bits):
+    """One packed pattern of ``bits``-wide fields -> {pos: delta}."""
     out = {}
     p = int(pat)
+    m = (1 << bits) - 1
     while p:
-        b = p & 0xFF
+        b = p & m
         out[b >> 4] = b & 0xF
-        p >>= 8
+        p >>= bits
     return out

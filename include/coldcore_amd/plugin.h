@@ -123,9 +123,20 @@ int  dct_set_tbl2(const uint16_t *mask, const uint64_t *offs,
 int  dct_use_fmt(int fmt);
 
 /* ---- additive, no counterpart in the reference -------------------------- */
-/* Schedule of the LDS-tiled sweeps: 6 ints per sweep (first axis, axes in
- * the group, P, TL, layers in, layers out); returns the sweep count.  For
- * benchmarks and the roofline accounting of each sweep's traffic. */
+/* dct_init3 with a wide-format table: each pattern a uint64 of up to six
+ * 10-bit fields (pos<<4 | delta) at bits [10k, 10k+10), zero field ends.
+ * v1 bytes hold a position in a nibble, so the reference ABI stops at 16
+ * axes (reference plugin.h:22, dct_hc.cu:233); this entry accepts up to 40
+ * (|X| <= 2^40), which the binary benchmark cells K_2(24,3) and K_2(30,4)
+ * need.  Same cell, same semantics, same return codes otherwise. */
+int  dct_init3w(int problem, int n, const int *ax, int R,
+                const uint64_t *widepat, long long balllen, int extract_cap,
+                long long *bytes, int layers_mode, int cnt_mode,
+                int use_owner);
+/* Schedule of the LDS-tiled sweeps: 8 ints per sweep (first axis, axes in
+ * the group, P, TL, layers in, layers out, axis size of the register-round
+ * kernel or 0 for the per-axis kernel, LDS bytes); returns the sweep count.
+ * For benchmarks and the roofline accounting of each sweep's traffic. */
 int  dct_sweep_plan(int *out, int cap);
 /* Record HIP events around every sweep of later transforms (on the
  * engine's stream; also enabled by DCT_TIMING=1 at init). */

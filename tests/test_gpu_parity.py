@@ -14,10 +14,22 @@ from oracle.dp import NumpyDP
 pytestmark = pytest.mark.gpu
 
 
+# schedule variants: the default plan (register-round kernel on homogeneous
+# Hamming groups), the per-axis kernel only, and a tiny LDS budget that
+# forces one- and two-axis sweeps (the multi-sweep paths on small cells)
+VARIANTS = {"default": {}, "generic": {"CC_SWEEP_KERNEL": "generic"},
+            "small-tiles": {"CC_SWEEP_LDS": "4096"}}
+
+
+@pytest.mark.parametrize("variant", list(VARIANTS))
 @pytest.mark.parametrize("axes,R,mu", TRANSFORM_CELLS, ids=TRANSFORM_IDS)
-def test_hamming_transform_matches_oracle(gpu_cell, axes, R, mu):
+def test_hamming_transform_matches_oracle(gpu_cell, monkeypatch, axes, R, mu,
+                                          variant):
     """cnt from a multiset code, the full gain map and the full loss map,
-    including R >= n cells, odd and even n and mixed radix."""
+    including R >= n cells, odd and even n and mixed radix, under every
+    schedule variant (reference tests/test_gpu_fast_paths.py:190-222)."""
+    for k, v in VARIANTS[variant].items():
+        monkeypatch.setenv(k, v)
     c = NumpyDP(axes, R, mu=mu)
     rng = np.random.default_rng(sum(axes) + R)
     size = c.space.size
