@@ -35,21 +35,24 @@ def sweep_bytes(plan, kind):
 
     kind 'field' (gain / loss: reads the uint16 cnt predicate, writes the
     int32 map) or 'recount' (reads the int32 code indicator, writes uint16
-    cnt).  First sweep reads its seed and writes every layer it produced
-    (gain: layer 0 too, recount: layers 1..); middle sweeps read their live
-    layers and write the layers above 0; the last reads its live layers and
+    cnt).  Inter-sweep layers are lb = 2 (uint16 planes, when every layer
+    value is provably < 2^16) or 4 bytes.  The first sweep reads its seed
+    and writes every live layer (layer 0 too, except an int32 recount whose
+    layer 0 stays in the code-indicator plane); middle sweeps read their
+    live layers and write those above 0; the last reads its live layers and
     writes the result."""
     out = []
     ns = len(plan)
+    lb = plan[0]["layer_bytes_field" if kind == "field" else "layer_bytes_recount"]
     for i, sw in enumerate(plan):
         lin, lout = sw["layers_in"], sw["layers_out"]
-        rd = (2 if kind == "field" else 4) if i == 0 else 4 * lin
+        rd = (2 if kind == "field" else 4) if i == 0 else lb * lin
         if i == ns - 1:
             wr = 4 if kind == "field" else 2
         elif i == 0:
-            wr = 4 * lout if kind == "field" else 4 * (lout - 1)
+            wr = lb * lout if (kind == "field" or lb == 2) else lb * (lout - 1)
         else:
-            wr = 4 * (lout - 1)
+            wr = lb * (lout - 1)
         out.append(rd + wr)
     return out
 
@@ -217,7 +220,10 @@ def main():
             "workload": "K_2(24,3) exact gain-field pass, 2^24 words, R=3",
             "axes": "[2]*24", "R": 3, "code_words": int(len(code)),
             "schedule": [f"axes {s['first_axis']}..{s['first_axis'] + s['axes'] - 1}"
-                         f" P={s['P']} TL={s['TL']}" for s in plan],
+                         f" P={s['P']} TL={s['TL']}"
+                         f" {'round-kernel Q=' + str(s['round_q']) if s['round_q'] else 'per-axis kernel'}"
+                         f" LDS={s['lds_bytes']}" for s in plan],
+            "layer_bytes": plan[0]["layer_bytes_field"],
         },
         "pass_bytes_algorithmic": int(pass_bytes),
         "pass_bytes_per_word": int(sum(bpw)),

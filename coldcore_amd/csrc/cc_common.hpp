@@ -96,7 +96,8 @@ __device__ __forceinline__ long long cc_block_sum(long long v) {
 #define CC_OUT_CNT 2     // write min(sum_d A_d, 65535) into cnt (recount)
 
 struct SweepArgs {
-  int32_t *A[CC_MAXL];
+  int32_t *A[CC_MAXL];   // int32 planes: A[0] is the map / code indicator
+  uint16_t *H[CC_MAXL];  // uint16 inter-sweep layer planes (16-bit mode)
   uint16_t *cnt;
   long long s;       // stride (words) of the group's first axis
   long long ncolb;   // column blocks per outer index = ceil(s / TL)

@@ -108,7 +108,26 @@ struct CcQRange {
   static constexpr int hi = PROB == CC_HAMMING ? 10 : 16;
 };
 
-template <int PROB, int R1>
+// Inter-sweep layer planes are int32 (LT = int32_t, planes A[d]) or, when
+// every layer value is provably below 2^16, uint16 (LT = uint16_t, planes
+// H[d]).  A layer value is a count of solution elements at one partial
+// distance, so it is bounded by the ball size for the gain / loss
+// indicators and by the code size M <= 65,535 for a recount.
+template <typename LT>
+__device__ __forceinline__ uint32_t cc_lay_ld(const SweepArgs &a, int d,
+                                              long long w) {
+  if constexpr (sizeof(LT) == 2) return (uint32_t)a.H[d][w];
+  else return (uint32_t)a.A[d][w];
+}
+
+template <typename LT>
+__device__ __forceinline__ void cc_lay_st(const SweepArgs &a, int d,
+                                          long long w, uint32_t v) {
+  if constexpr (sizeof(LT) == 2) a.H[d][w] = (uint16_t)v;
+  else a.A[d][w] = (int32_t)v;
+}
+
+template <int PROB, int R1, typename LT>
 __global__ void __launch_bounds__(CC_BLOCK) k_sweep(SweepArgs a) {
   extern __shared__ int32_t S[];
   constexpr int QMIN = CcQRange<PROB>::lo, QMAX = CcQRange<PROB>::hi;
@@ -127,11 +146,13 @@ __global__ void __launch_bounds__(CC_BLOCK) k_sweep(SweepArgs a) {
       const long long w = base + (long long)p * a.s + tl;
       if (a.in_kind == CC_IN_PRED)
         S[e] = (live && cc_pred(a.pmode, (int)a.cnt[w], 0, a.mu)) ? 1 : 0;
-      else
+      else if (a.in_kind == CC_IN_CODE)
         S[e] = live ? a.A[0][w] : 0;
+      else
+        S[e] = live ? (int32_t)cc_lay_ld<LT>(a, 0, w) : 0;
       for (int d = 1; d < Lmax; d++)
-        S[d * PT + e] =
-            (live && d < Lin && a.in_kind == CC_IN_LAYERS) ? a.A[d][w] : 0;
+        S[d * PT + e] = (live && d < Lin && a.in_kind == CC_IN_LAYERS)
+                            ? (int32_t)cc_lay_ld<LT>(a, d, w) : 0;
     }
     __syncthreads();
     // ---- every axis of the group, in LDS
@@ -154,7 +175,8 @@ __global__ void __launch_bounds__(CC_BLOCK) k_sweep(SweepArgs a) {
       if (tl >= ncol) continue;
       const long long w = base + (long long)p * a.s + tl;
       if (a.out_kind == CC_OUT_LAYERS) {
-        for (int d = a.write_l0 ? 0 : 1; d < Lmax; d++) a.A[d][w] = S[d * PT + e];
+        for (int d = a.write_l0 ? 0 : 1; d < Lmax; d++)
+          cc_lay_st<LT>(a, d, w, (uint32_t)S[d * PT + e]);
       } else {
         uint32_t sum = 0;
         for (int d = 0; d < Lmax; d++) sum += (uint32_t)S[d * PT + e];
@@ -246,7 +268,84 @@ __device__ __forceinline__ void cc_cube_round(uint32_t (&v)[R1][C], int ilo,
   }
 }
 
-template <int PROB, int R1, int Q, int RR>
+// First-round inputs of one cube: predicate(cnt), the code indicator, or the
+// live layers.  With TL = 1 the first round's cube is C consecutive words,
+// read with 16-byte vector loads.
+template <int R1, int C, typename LT>
+__device__ __forceinline__ void cc_cube_load(const SweepArgs &a, long long base,
+                                             int pb, int tl, int ncol,
+                                             int Lin, uint32_t (&v)[R1][C]) {
+  const bool live = tl < ncol;
+  if (a.TL == 1 && C % 8 == 0) {
+    const long long w0 = base + pb;              // C-aligned, 16-byte aligned
+    if (a.in_kind == CC_IN_PRED) {
+#pragma unroll
+      for (int c8 = 0; c8 < C; c8 += 8) {
+        const uint4 q = *(const uint4 *)(a.cnt + w0 + c8);
+        const uint32_t wv[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+        for (int h = 0; h < 8; h++)
+          v[0][c8 + h] = cc_pred(a.pmode, (int)((wv[h >> 1] >> (16 * (h & 1))) & 0xFFFFu),
+                                 0, a.mu) ? 1u : 0u;
+      }
+    } else if (a.in_kind == CC_IN_CODE) {
+#pragma unroll
+      for (int c4 = 0; c4 < C; c4 += 4) {
+        const int4 q = *(const int4 *)(a.A[0] + w0 + c4);
+        v[0][c4] = (uint32_t)q.x; v[0][c4 + 1] = (uint32_t)q.y;
+        v[0][c4 + 2] = (uint32_t)q.z; v[0][c4 + 3] = (uint32_t)q.w;
+      }
+    } else {
+#pragma unroll
+      for (int d = 0; d < R1; d++) {
+        if (d >= Lin) {
+#pragma unroll
+          for (int c = 0; c < C; c++) v[d][c] = 0u;
+          continue;
+        }
+        if constexpr (sizeof(LT) == 2) {
+#pragma unroll
+          for (int c8 = 0; c8 < C; c8 += 8) {
+            const uint4 q = *(const uint4 *)(a.H[d] + w0 + c8);
+            const uint32_t wv[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+            for (int h = 0; h < 8; h++)
+              v[d][c8 + h] = (wv[h >> 1] >> (16 * (h & 1))) & 0xFFFFu;
+          }
+        } else {
+#pragma unroll
+          for (int c4 = 0; c4 < C; c4 += 4) {
+            const int4 q = *(const int4 *)(a.A[d] + w0 + c4);
+            v[d][c4] = (uint32_t)q.x; v[d][c4 + 1] = (uint32_t)q.y;
+            v[d][c4 + 2] = (uint32_t)q.z; v[d][c4 + 3] = (uint32_t)q.w;
+          }
+        }
+      }
+      return;
+    }
+#pragma unroll
+    for (int d = 1; d < R1; d++)
+#pragma unroll
+      for (int c = 0; c < C; c++) v[d][c] = 0u;
+    return;
+  }
+#pragma unroll
+  for (int c = 0; c < C; c++) {
+    const long long w = base + (long long)(pb + c) * a.s + tl;
+    if (a.in_kind == CC_IN_PRED)
+      v[0][c] = (live && cc_pred(a.pmode, (int)a.cnt[w], 0, a.mu)) ? 1u : 0u;
+    else if (a.in_kind == CC_IN_CODE)
+      v[0][c] = live ? (uint32_t)a.A[0][w] : 0u;
+    else
+      v[0][c] = live ? cc_lay_ld<LT>(a, 0, w) : 0u;
+#pragma unroll
+    for (int d = 1; d < R1; d++)
+      v[d][c] = (live && d < Lin && a.in_kind == CC_IN_LAYERS)
+                    ? cc_lay_ld<LT>(a, d, w) : 0u;
+  }
+}
+
+template <int PROB, int R1, int Q, int RR, typename LT>
 __global__ void __launch_bounds__(CC_BLOCK) k_sweep_r(SweepArgs a) {
   extern __shared__ int32_t S[];
   constexpr int C = RR == 1 ? Q : (RR == 2 ? Q * Q : (RR == 3 ? Q * Q * Q : Q * Q * Q * Q));
@@ -257,6 +356,17 @@ __global__ void __launch_bounds__(CC_BLOCK) k_sweep_r(SweepArgs a) {
   const int Lin = a.live[0];
   const int nrounds = (m + RR - 1) / RR;
   const int ncube = PT / C;
+  // one cube per thread per round: the next tile's first-round inputs are
+  // loaded while the current tile runs its LDS rounds
+  const bool pf = ncube <= CC_BLOCK;
+  uint32_t nxt[R1][C];
+  if (pf && (long long)blockIdx.x < a.ntiles && (int)threadIdx.x < ncube) {
+    const long long t0 = blockIdx.x;
+    const long long h = t0 / a.ncolb, l0 = (t0 - h * a.ncolb) * a.TL;
+    const int cu = threadIdx.x;
+    cc_cube_load<R1, C, LT>(a, h * a.s * a.P + l0, (cu / a.TL) * C, cu % a.TL,
+                            (int)min((long long)a.TL, a.s - l0), Lin, nxt);
+  }
   for (long long tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
     const long long h = tile / a.ncolb;
     const long long l0 = (tile - h * a.ncolb) * a.TL;
@@ -273,21 +383,27 @@ __global__ void __launch_bounds__(CC_BLOCK) k_sweep_r(SweepArgs a) {
         const int pb = (u / qa) * qa * C + (u % qa);
         const bool live = tl < ncol;
         uint32_t v[R1][C];
+        if (first) {
+          if (pf) {
 #pragma unroll
-        for (int c = 0; c < C; c++) {
-          const int p = pb + c * qa;
-          if (first) {
-            const long long w = base + (long long)p * a.s + tl;
-            if (a.in_kind == CC_IN_PRED)
-              v[0][c] = (live && cc_pred(a.pmode, (int)a.cnt[w], 0, a.mu)) ? 1u : 0u;
-            else
-              v[0][c] = live ? (uint32_t)a.A[0][w] : 0u;
+            for (int d = 0; d < R1; d++)
 #pragma unroll
-            for (int d = 1; d < R1; d++)
-              v[d][c] = (live && d < Lin && a.in_kind == CC_IN_LAYERS)
-                            ? (uint32_t)a.A[d][w] : 0u;
+              for (int c = 0; c < C; c++) v[d][c] = nxt[d][c];
+            const long long t2 = tile + gridDim.x;
+            if (t2 < a.ntiles) {
+              const long long h2 = t2 / a.ncolb;
+              const long long l2 = (t2 - h2 * a.ncolb) * a.TL;
+              cc_cube_load<R1, C, LT>(a, h2 * a.s * a.P + l2, pb, tl,
+                                      (int)min((long long)a.TL, a.s - l2), Lin,
+                                      nxt);
+            }
           } else {
-            const int e = cc_pad(p * a.TL + tl);
+            cc_cube_load<R1, C, LT>(a, base, pb, tl, ncol, Lin, v);
+          }
+        } else {
+#pragma unroll
+          for (int c = 0; c < C; c++) {
+            const int e = cc_pad((pb + c * qa) * a.TL + tl);
 #pragma unroll
             for (int d = 0; d < R1; d++)
               v[d][c] = d < Lmax ? (uint32_t)S[d * PTP + e] : 0u;
@@ -303,7 +419,7 @@ __global__ void __launch_bounds__(CC_BLOCK) k_sweep_r(SweepArgs a) {
             if (a.out_kind == CC_OUT_LAYERS) {
 #pragma unroll
               for (int d = 0; d < R1; d++)
-                if (d < Lmax && (d > 0 || a.write_l0)) a.A[d][w] = (int32_t)v[d][c];
+                if (d < Lmax && (d > 0 || a.write_l0)) cc_lay_st<LT>(a, d, w, v[d][c]);
             } else {
               uint32_t sum = 0;
 #pragma unroll

@@ -59,7 +59,12 @@ struct Engine {
   int ax[CC_MAXN] = {};
   long long pw[CC_MAXN + 1] = {};
   long long X = 0;
-  int32_t *A[CC_MAXL] = {};
+  int32_t *A[CC_MAXL] = {};   // A[0] map plane; A[1..R] views of `lay`
+  uint16_t *H[CC_MAXL] = {};  // uint16 views of `lay` (16-bit mode)
+  int32_t *lay = nullptr;     // one block for layers 1..R (pitch-aligned)
+  long long pitch = 0;        // plane pitch in words (multiple of 64)
+  int l16_field = 0;          // gain / loss run 16-bit inter-sweep planes
+  int l16_count = 0;          // recount runs 16-bit inter-sweep planes
   uint16_t *cnt = nullptr;
   long long cnt_len = 0;
   uint64_t *pat = nullptr, *pat2 = nullptr;
@@ -88,6 +93,7 @@ struct Engine {
   std::unordered_map<long long, int> uploaded, active;
   int active_size = 0;
   std::vector<Sweep> plan;
+  double ball = 0;           // exact metric ball size (closed form)
   int lds_budget = 64 * 1024;
   int lds_max = 64 * 1024;   // device limit per workgroup
   int generic_only = 0;      // CC_SWEEP_KERNEL=generic: no round kernel
@@ -186,7 +192,9 @@ void build_plan() {
     sw.j0 = j;
     sw.c0 = j;
     sw.s = E.pw[j];
-    const int TL0 = (j == 0) ? 1 : (int)std::min<long long>(32, sw.s);
+    // strided groups read whole 128-byte rows: 32 int32 or 64 uint16 columns
+    const int TL0 = (j == 0) ? 1
+                             : (int)std::min<long long>(E.l16_field ? 64 : 32, sw.s);
     long long P = 1;
     int k = j;
     while (k < E.n) {
@@ -240,55 +248,67 @@ hipError_t launch_k(K kf, const SweepArgs &a, int blocks, size_t lds) {
 }
 
 template <int PROB, int R1>
-hipError_t launch_sweep_t(const SweepArgs &a, int blocks, size_t lds) {
-  return launch_k(k_sweep<PROB, R1>, a, blocks, lds);
+hipError_t launch_sweep_t(const SweepArgs &a, int blocks, size_t lds,
+                          bool l16) {
+  if (l16 && R1 > 1)
+    return launch_k(k_sweep<PROB, R1, uint16_t>, a, blocks, lds);
+  return launch_k(k_sweep<PROB, R1, int32_t>, a, blocks, lds);
+}
+
+template <int R1, int Q, int RR>
+hipError_t launch_round_q(const SweepArgs &a, int blocks, size_t lds,
+                          bool l16) {
+  if (l16) return launch_k(k_sweep_r<CC_HAMMING, R1, Q, RR, uint16_t>, a, blocks, lds);
+  return launch_k(k_sweep_r<CC_HAMMING, R1, Q, RR, int32_t>, a, blocks, lds);
 }
 
 template <int R1>
-hipError_t launch_round_t(int q, const SweepArgs &a, int blocks, size_t lds) {
+hipError_t launch_round_t(int q, const SweepArgs &a, int blocks, size_t lds,
+                          bool l16) {
   switch (q) {
-    case 2: return launch_k(k_sweep_r<CC_HAMMING, R1, 2, 4>, a, blocks, lds);
-    case 3: return launch_k(k_sweep_r<CC_HAMMING, R1, 3, 2>, a, blocks, lds);
-    case 4: return launch_k(k_sweep_r<CC_HAMMING, R1, 4, 2>, a, blocks, lds);
-    case 5: return launch_k(k_sweep_r<CC_HAMMING, R1, 5, 1>, a, blocks, lds);
+    case 2: return launch_round_q<R1, 2, 4>(a, blocks, lds, l16);
+    case 3: return launch_round_q<R1, 3, 2>(a, blocks, lds, l16);
+    case 4: return launch_round_q<R1, 4, 2>(a, blocks, lds, l16);
+    case 5: return launch_round_q<R1, 5, 1>(a, blocks, lds, l16);
   }
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_round(int q, const SweepArgs &a, int blocks, size_t lds) {
+hipError_t launch_round(int q, const SweepArgs &a, int blocks, size_t lds,
+                        bool l16) {
   switch (E.L) {
-    case 2: return launch_round_t<2>(q, a, blocks, lds);
-    case 3: return launch_round_t<3>(q, a, blocks, lds);
-    case 4: return launch_round_t<4>(q, a, blocks, lds);
-    case 5: return launch_round_t<5>(q, a, blocks, lds);
-    case 6: return launch_round_t<6>(q, a, blocks, lds);
-    case 7: return launch_round_t<7>(q, a, blocks, lds);
+    case 2: return launch_round_t<2>(q, a, blocks, lds, l16);
+    case 3: return launch_round_t<3>(q, a, blocks, lds, l16);
+    case 4: return launch_round_t<4>(q, a, blocks, lds, l16);
+    case 5: return launch_round_t<5>(q, a, blocks, lds, l16);
+    case 6: return launch_round_t<6>(q, a, blocks, lds, l16);
+    case 7: return launch_round_t<7>(q, a, blocks, lds, l16);
   }
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_sweep(const SweepArgs &a, int blocks, size_t lds) {
+hipError_t launch_sweep(const SweepArgs &a, int blocks, size_t lds, bool l16) {
   switch (E.prob) {
-    case CC_TORUS: return launch_sweep_t<CC_TORUS, 1>(a, blocks, lds);
-    case CC_GRID: return launch_sweep_t<CC_GRID, 1>(a, blocks, lds);
+    case CC_TORUS: return launch_sweep_t<CC_TORUS, 1>(a, blocks, lds, false);
+    case CC_GRID: return launch_sweep_t<CC_GRID, 1>(a, blocks, lds, false);
     case CC_HAMMING:
       switch (E.L) {
-        case 2: return launch_sweep_t<CC_HAMMING, 2>(a, blocks, lds);
-        case 3: return launch_sweep_t<CC_HAMMING, 3>(a, blocks, lds);
-        case 4: return launch_sweep_t<CC_HAMMING, 4>(a, blocks, lds);
-        case 5: return launch_sweep_t<CC_HAMMING, 5>(a, blocks, lds);
-        case 6: return launch_sweep_t<CC_HAMMING, 6>(a, blocks, lds);
-        case 7: return launch_sweep_t<CC_HAMMING, 7>(a, blocks, lds);
+        case 2: return launch_sweep_t<CC_HAMMING, 2>(a, blocks, lds, l16);
+        case 3: return launch_sweep_t<CC_HAMMING, 3>(a, blocks, lds, l16);
+        case 4: return launch_sweep_t<CC_HAMMING, 4>(a, blocks, lds, l16);
+        case 5: return launch_sweep_t<CC_HAMMING, 5>(a, blocks, lds, l16);
+        case 6: return launch_sweep_t<CC_HAMMING, 6>(a, blocks, lds, l16);
+        case 7: return launch_sweep_t<CC_HAMMING, 7>(a, blocks, lds, l16);
       }
       break;
     case CC_LEE:
       switch (E.L) {
-        case 2: return launch_sweep_t<CC_LEE, 2>(a, blocks, lds);
-        case 3: return launch_sweep_t<CC_LEE, 3>(a, blocks, lds);
-        case 4: return launch_sweep_t<CC_LEE, 4>(a, blocks, lds);
-        case 5: return launch_sweep_t<CC_LEE, 5>(a, blocks, lds);
-        case 6: return launch_sweep_t<CC_LEE, 6>(a, blocks, lds);
-        case 7: return launch_sweep_t<CC_LEE, 7>(a, blocks, lds);
+        case 2: return launch_sweep_t<CC_LEE, 2>(a, blocks, lds, l16);
+        case 3: return launch_sweep_t<CC_LEE, 3>(a, blocks, lds, l16);
+        case 4: return launch_sweep_t<CC_LEE, 4>(a, blocks, lds, l16);
+        case 5: return launch_sweep_t<CC_LEE, 5>(a, blocks, lds, l16);
+        case 6: return launch_sweep_t<CC_LEE, 6>(a, blocks, lds, l16);
+        case 7: return launch_sweep_t<CC_LEE, 7>(a, blocks, lds, l16);
       }
       break;
   }
@@ -299,12 +319,16 @@ hipError_t launch_sweep(const SweepArgs &a, int blocks, size_t lds) {
 // result goes.  pmode is the predicate for CC_IN_PRED.
 int run_transform(int in_kind, int out_kind, int pmode) {
   const int ns = (int)E.plan.size();
+  const bool l16 = out_kind == CC_OUT_CNT ? E.l16_count : E.l16_field;
   E.nswept = 0;
   for (int i = 0; i < ns; i++) {
     const Sweep &sw = E.plan[i];
     SweepArgs a;
     memset(&a, 0, sizeof(a));
-    for (int d = 0; d < CC_MAXL; d++) a.A[d] = E.A[d];
+    for (int d = 0; d < CC_MAXL; d++) {
+      a.A[d] = E.A[d];
+      a.H[d] = E.H[d];
+    }
     a.cnt = E.cnt;
     a.s = sw.s;
     a.ncolb = sw.ncolb;
@@ -320,7 +344,9 @@ int run_transform(int in_kind, int out_kind, int pmode) {
     a.out_kind = (i == ns - 1) ? out_kind : CC_OUT_LAYERS;
     // the layered families never change layer 0 after the first sweep
     // seeds it; the single-plane families rewrite it every sweep
-    a.write_l0 = (E.L == 1) || (i == 0 && in_kind == CC_IN_PRED);
+    // 16-bit mode keeps layer 0 in H[0] too, so the first sweep writes it
+    // whatever seeded it (A[0] holds the code indicator only as int32)
+    a.write_l0 = (E.L == 1) || (i == 0 && (in_kind == CC_IN_PRED || l16));
     long long sk = 1;
     for (int k = 0; k < sw.nax; k++) {
       a.Q[k] = E.ax[sw.j0 + k];
@@ -330,8 +356,8 @@ int run_transform(int in_kind, int out_kind, int pmode) {
     for (int k = 0; k <= sw.nax; k++) a.live[k] = layers_after(sw.j0 + k);
     const int blocks = (int)std::min<long long>(sw.ntiles, 1LL << 20);
     if (E.prof) CK(hipEventRecord(E.ev[2 * i], E.st));
-    if (sw.rq) CK(launch_round(sw.rq, a, blocks, sw.lds));
-    else CK(launch_sweep(a, blocks, sw.lds));
+    if (sw.rq) CK(launch_round(sw.rq, a, blocks, sw.lds, l16));
+    else CK(launch_sweep(a, blocks, sw.lds, l16));
     if (E.prof) CK(hipEventRecord(E.ev[2 * i + 1], E.st));
   }
   E.nswept = ns;
@@ -482,8 +508,10 @@ int dalloc(T **p, size_t bytes) {
 
 void release_all() {
   if (E.st) (void)hipStreamSynchronize(E.st);
-  for (int d = 0; d < CC_MAXL; d++)
-    if (E.A[d]) { (void)hipFree(E.A[d]); E.A[d] = nullptr; }
+  if (E.A[0]) (void)hipFree(E.A[0]);
+  if (E.lay) (void)hipFree(E.lay);
+  E.lay = nullptr;
+  for (int d = 0; d < CC_MAXL; d++) { E.A[d] = nullptr; E.H[d] = nullptr; }
   void *bufs[] = {E.cnt, E.pat, E.pat2, E.d_code, E.d_words, E.d_out,
                   E.d_scal, E.d_hist, E.d_eidx64, E.d_eidx32, E.d_eval,
                   E.d_bcnt, E.d_boff};
@@ -607,8 +635,17 @@ static int init_cell(int problem, int n, const int *ax, int R,
   const char *sk = getenv("CC_SWEEP_KERNEL");
   E.generic_only = sk && !strcmp(sk, "generic");
   CK(hipStreamCreateWithFlags(&E.st, hipStreamNonBlocking));
-  for (int d = 0; d < L; d++)
-    if (dalloc(&E.A[d], (size_t)X * 4)) { release_all(); return -3; }
+  // A[0] (map / code indicator) on its own; layers 1..R as one block whose
+  // planes are 64-word aligned, viewed as int32 planes A[1..R] or, in 16-bit
+  // mode, as L uint16 planes H[0..R] (2 L pitch <= 4 R pitch for R >= 1)
+  E.pitch = (X + 63) & ~63LL;
+  if (dalloc(&E.A[0], (size_t)X * 4)) { release_all(); return -3; }
+  if (L > 1) {
+    if (dalloc(&E.lay, (size_t)(L - 1) * E.pitch * 4)) { release_all(); return -3; }
+    for (int d = 1; d < L; d++) E.A[d] = E.lay + (size_t)(d - 1) * E.pitch;
+    for (int d = 0; d < L; d++)
+      E.H[d] = (uint16_t *)E.lay + (size_t)d * E.pitch;
+  }
   E.cnt_len = (X + 1) & ~1LL;
   if (dalloc(&E.cnt, (size_t)E.cnt_len * 2)) { release_all(); return -4; }
   hipLaunchKernelGGL(k_fill16, dim3(pass_blocks(E.cnt_len)), dim3(CC_BLOCK), 0,
@@ -643,6 +680,30 @@ static int init_cell(int problem, int n, const int *ax, int R,
   E.prof = pe ? atoi(pe) : 0;
   if (int rc = zero_scalars(8)) return rc;
   E.trans_last = 0;
+  {
+    // exact ball size: coefficients 0..R of prod_j (per-axis enumerator)
+    std::vector<double> poly(R + 1, 0.0);
+    poly[0] = 1;
+    for (int i = 0; i < n; i++) {
+      std::vector<double> nx(R + 1, 0.0);
+      for (int w = 0; w <= R; w++) {
+        if (poly[w] == 0) continue;
+        nx[w] += poly[w];
+        for (int dl = 1; dl < ax[i]; dl++) {
+          const int c = std::min(dl, ax[i] - dl);
+          const int e = problem == CC_LEE ? c : 1;
+          if (w + e <= R) nx[w + e] += poly[w];
+        }
+      }
+      poly = nx;
+    }
+    E.ball = 0;
+    for (double v : poly) E.ball += v;
+    const char *l16 = getenv("CC_LAYER16");
+    const bool on = L > 1 && !(l16 && !strcmp(l16, "0"));
+    E.l16_field = on && E.ball <= 65535.0;
+    E.l16_count = on;          // layer values never exceed M <= 65,535
+  }
   build_plan();
   E.uploaded.clear();
   E.active.clear();
@@ -993,7 +1054,9 @@ int dct_use_fmt(int fmt) {
 
 // ---- additive (not in the reference ABI): schedule and sweep timing ------
 // dct_sweep_plan: per sweep 8 ints (first axis, axes, P, TL, layers in,
-// layers out, round-kernel axis size or 0, LDS bytes); returns the count.
+// layers out, round-kernel axis size or 0, LDS bytes) and, after the last
+// sweep, 2 ints: inter-sweep layer bytes of gain/loss and of recount (2 or
+// 4).  Returns the sweep count.
 int dct_sweep_plan(int *out, int cap) {
   if (!E.live || !out) return -1;
   const int ns = (int)E.plan.size();
@@ -1007,6 +1070,10 @@ int dct_sweep_plan(int *out, int cap) {
     out[8 * i + 5] = s.lmax;
     out[8 * i + 6] = s.rq;
     out[8 * i + 7] = (int)s.lds;
+  }
+  if (ns < cap) {
+    out[8 * ns + 0] = E.l16_field ? 2 : 4;
+    out[8 * ns + 1] = E.l16_count ? 2 : 4;
   }
   return ns;
 }

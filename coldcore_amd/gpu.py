@@ -348,13 +348,17 @@ class GpuBackend:
 
     # ---------------- additive: schedule and per-sweep timing -------------
     def sweep_plan(self):
-        buf = (ctypes.c_int * (8 * 16))()
+        buf = (ctypes.c_int * (8 * 17))()
         ns = self._lib.dct_sweep_plan(buf, 16)
         if ns < 0:
             _check("sweep_plan", ns)
         keys = ("first_axis", "axes", "P", "TL", "layers_in", "layers_out",
                 "round_q", "lds_bytes")
-        return [dict(zip(keys, buf[8 * i:8 * i + 8])) for i in range(ns)]
+        out = [dict(zip(keys, buf[8 * i:8 * i + 8])) for i in range(ns)]
+        for sw in out:
+            sw["layer_bytes_field"] = buf[8 * ns]
+            sw["layer_bytes_recount"] = buf[8 * ns + 1]
+        return out
 
     def sweep_profile(self, on=True):
         _check("sweep_profile", self._lib.dct_sweep_profile(1 if on else 0))

@@ -17,6 +17,7 @@ from itertools import combinations
 
 import numpy as np
 
+from .protocol import DenseQueries
 from .space import Space
 
 PROBLEMS = ("hamming", "torus_linf", "lee", "grid_linf")
@@ -48,7 +49,7 @@ def max_weight(n, R, problem):
     return n
 
 
-class NumpyDP:
+class NumpyDP(DenseQueries):
     def __init__(self, axes, R, problem="hamming", mu=1, dtype=np.int32):
         if problem not in PROBLEMS:
             raise ValueError(f"unknown problem {problem!r}")
@@ -216,13 +217,6 @@ class NumpyDP:
                 row = off[k] if good is None else off[k][good[k]]
                 self.cnt[row] += delta
 
-    def role_mask(self, target):
-        if target == 0:
-            return self.cnt < self.mu
-        if target == 1:
-            return (self.cnt >= 1) & (self.cnt <= self.mu)
-        return self.cnt == target
-
     def ball_gather(self, idx, target):
         idx = np.atleast_1d(np.asarray(idx, dtype=np.int64))
         out = np.empty(len(idx), dtype=np.int32)
@@ -237,34 +231,3 @@ class NumpyDP:
                 hit &= good
             out[lo:lo + step] = hit.sum(axis=1)
         return out
-
-    # ------------------------------------------------------------ queries
-    def count_eq(self, target):
-        return int((self.cnt == target).sum())
-
-    def uncovered(self):
-        return int(np.maximum(self.mu - self.cnt.astype(np.int64), 0).sum())
-
-    def map_max(self):
-        return int(self.map.max(initial=0))
-
-    def map_hist(self, nbins, vmax):
-        out = np.zeros(nbins, dtype=np.int32)
-        v = self.map[self.map > 0].astype(np.int64)
-        if len(v) and vmax > 0:
-            b = np.minimum((v - 1) * nbins // vmax, nbins - 1)
-            np.add.at(out, b, 1)
-        return out
-
-    def map_extract(self, thr, cap=1 << 22):
-        idx = np.flatnonzero(self.map >= thr).astype(np.int64)
-        found = len(idx)
-        idx = idx[:cap]
-        return idx, self.map[idx].astype(np.int32), found
-
-    def map_read_at(self, idx):
-        return self.map[np.atleast_1d(np.asarray(idx, dtype=np.int64))].astype(np.int32)
-
-    def read_cnt(self, off=0, length=None):
-        length = self.space.size - off if length is None else length
-        return self.cnt[off:off + length].astype(np.uint16)

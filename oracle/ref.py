@@ -15,6 +15,7 @@ Fields (reference base.py:65-106, protocol.py:163-219):
 """
 import numpy as np
 
+from .protocol import DenseQueries
 from .space import Space
 
 MAX_SPACE = 1 << 16
@@ -39,7 +40,7 @@ def covers(problem, axes, R, a, b):
     raise ValueError(problem)
 
 
-class BruteForce:
+class BruteForce(DenseQueries):
     def __init__(self, axes, R, problem="hamming", mu=1):
         if problem not in PROBLEMS:
             raise ValueError(problem)
@@ -76,13 +77,6 @@ class BruteForce:
             hi = min(size, lo + chunk)
             m = self._cov(self._digs[lo:hi, None, :], self._digs[None, sel, :])
             self.map[lo:hi] = m.sum(axis=1)
-
-    def role_mask(self, target):
-        if target == 0:
-            return self.cnt < self.mu
-        if target == 1:
-            return (self.cnt >= 1) & (self.cnt <= self.mu)
-        return self.cnt == target
 
     def gain_map(self):
         self._field(self.role_mask(0))

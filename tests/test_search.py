@@ -1,0 +1,111 @@
+"""The search counterpart (caller side of the hot path).
+
+Pinned by the reference's frozen move-trace digests (reference
+tests/test_search_determinism.py:24-50, 110-118): greedy build, peel and a
+fixed ruin/recreate sequence over every ruin operator, on the brute-force
+backend.  Then the same traces on the numpy axis-DP oracle (cpu == ref,
+reference tests/test_dp_backend.py:124-160) and, on the GPU, through the
+engine's C ABI: identical trajectories across all three.
+"""
+import hashlib
+
+import numpy as np
+import pytest
+
+from coldcore_amd.search import OperatorBandit, Searcher
+from oracle.dp import NumpyDP
+from oracle.ref import BruteForce
+
+
+def digest(trace):
+    h = hashlib.sha256()
+    for u, code in trace:
+        h.update(f"{u}|{','.join(map(str, code))};".encode())
+    return h.hexdigest()[:16]
+
+
+def greedy_ruin_trace(backend, seed, rounds=16, notch=2):
+    s = Searcher(backend, seed=seed)
+    s.load(np.zeros(0, dtype=np.int64))
+    s.greedy_fill(10 ** 9)
+    trace = [(int(s.uncov), s.code.tolist())]
+    s.peel()
+    trace.append((int(s.uncov), s.code.tolist()))
+    target = max(1, len(s.code) - notch)
+    modes = ["low", "hole", "cluster", "random", "deficit"]
+    for i in range(rounds):
+        s.ruin(1 + i % 4, modes[i % len(modes)])
+        s.greedy_fill(target)
+        trace.append((int(s.uncov), s.code.tolist()))
+    return trace
+
+
+#: (id, (axes, R, problem, mu), seed, frozen digest of the reference)
+PINS = [
+    ("greedy-K3(4,1)", ([3] * 4, 1, "hamming", 1), 4, "7fe8ede9952ee50e"),
+    ("greedy-K2(6,1)", ([2] * 6, 1, "hamming", 1), 11, "57c1e4d063f33278"),
+    ("greedy-lee5x5", ([5, 5], 1, "lee", 1), 3, "3f8e8af3d3acc2ac"),
+    ("greedy-mu2", ([2] * 6, 1, "hamming", 2), 5, "3f4b42018740787c"),
+]
+
+
+@pytest.mark.parametrize("label,cell,seed,pin", PINS, ids=[p[0] for p in PINS])
+def test_frozen_digest_on_brute_force(label, cell, seed, pin):
+    axes, R, prob, mu = cell
+    assert digest(greedy_ruin_trace(BruteForce(axes, R, prob, mu), seed)) == pin
+
+
+@pytest.mark.parametrize("label,cell,seed,pin", PINS, ids=[p[0] for p in PINS])
+def test_frozen_digest_on_numpy_dp(label, cell, seed, pin):
+    axes, R, prob, mu = cell
+    assert digest(greedy_ruin_trace(NumpyDP(axes, R, prob, mu), seed)) == pin
+
+
+def lns_trace(backend, seed, rounds, select="fixed"):
+    log = []
+    s = Searcher(backend, seed=seed,
+                 on_round=lambda s, r: log.append((r["mode"], r["post"],
+                                                   len(s.code))))
+    s.load(np.zeros(0, dtype=np.int64))
+    s.greedy_fill(10 ** 9)
+    s.peel()
+    M = len(s.code) - 1
+    s.ruin(1, "low")                 # one notch below the feasible size
+    s.lns(M, kmin=1, kmax=3, select=select, max_rounds=rounds)
+    return log, s.code.tolist(), int(s.uncov)
+
+
+@pytest.mark.parametrize("select", ["fixed", "adaptive"])
+def test_lns_rounds_identical_cpu_and_ref(select):
+    a = lns_trace(BruteForce([3] * 5, 1), 7, 60, select)
+    b = lns_trace(NumpyDP([3] * 5, 1), 7, 60, select)
+    assert a == b and len(a[0]) > 0
+
+
+def test_bandit_probabilities():
+    rng = np.random.default_rng(0)
+    b = OperatorBandit(rng)
+    p = b.probabilities()
+    assert abs(p.sum() - 1) < 1e-12 and p[1] > p[0]       # hole prior x2
+    for _ in range(30):
+        b.update("random", b.REWARD_BEST)
+    assert b.probabilities()[3] > p[3]
+    assert min(b.probabilities()) >= b.floor / len(b.ops) - 1e-12
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("label,cell,seed,pin", PINS, ids=[p[0] for p in PINS])
+def test_frozen_digest_on_gpu_engine(gpu_cell, label, cell, seed, pin):
+    """The engine through its C ABI reproduces the reference's frozen
+    search trace: every field, gather, update and extraction order agree."""
+    axes, R, prob, mu = cell
+    g = gpu_cell(axes, R, problem=prob, mu=mu)
+    assert digest(greedy_ruin_trace(g, seed)) == pin
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("select", ["fixed", "adaptive"])
+def test_lns_rounds_identical_gpu_and_cpu(gpu_cell, select):
+    g = gpu_cell([3] * 6, 1)
+    assert lns_trace(g, 7, 80, select) == lns_trace(NumpyDP([3] * 6, 1), 7,
+                                                    80, select)
