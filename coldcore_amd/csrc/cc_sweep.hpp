@@ -200,7 +200,11 @@ __global__ void __launch_bounds__(CC_BLOCK) k_sweep(SweepArgs a) {
 // one LDS round trip covers RR axes instead of one.  The first round loads
 // straight from HBM, the last stores straight to HBM; only the rounds in
 // between exchange through LDS (a padded layout, idx + idx/16, keeps the
-// stride-C accesses of the first round off a single bank).  A group whose
+// stride-C accesses of the first round off a single bank; in 16-bit mode
+// two layers share one LDS dword, halving the tile's LDS and doubling the
+// workgroups a CU holds).  The grid is persistent (occupancy x CUs) and each
+// thread loads the next tile's first-round inputs while the current tile
+// runs its LDS rounds.  A group whose
 // axis count m is not a multiple of RR ends with a cube placed at the top
 // RR axes that applies only the axes not yet done.  Layers above the live
 // count are zero in registers and stay zero under every recurrence, so the
@@ -217,7 +221,14 @@ __device__ __forceinline__ void cc_cube_axis(uint32_t (&v)[R1][C], int R) {
     for (int d = 0; d < R1; d++)
 #pragma unroll
       for (int a = 0; a < Q; a++) f[d][a] = v[d][b + a * ST];
-    if (PROB == CC_HAMMING) {
+    if (PROB == CC_HAMMING && Q == 2) {
+      // binary fiber: T_{d-1} - A_{d-1}[a] is the other word's A_{d-1}
+#pragma unroll
+      for (int d = 1; d < R1; d++) {
+        v[d][b] = f[d][0] + f[d - 1][1];
+        v[d][b + ST] = f[d][1] + f[d - 1][0];
+      }
+    } else if (PROB == CC_HAMMING) {
 #pragma unroll
       for (int d = 1; d < R1; d++) {
         uint32_t t = 0;
@@ -404,9 +415,19 @@ __global__ void __launch_bounds__(CC_BLOCK) k_sweep_r(SweepArgs a) {
 #pragma unroll
           for (int c = 0; c < C; c++) {
             const int e = cc_pad((pb + c * qa) * a.TL + tl);
+            if constexpr (sizeof(LT) == 2) {
+              // two 16-bit layers per LDS dword: plane k = layers 2k, 2k+1
 #pragma unroll
-            for (int d = 0; d < R1; d++)
-              v[d][c] = d < Lmax ? (uint32_t)S[d * PTP + e] : 0u;
+              for (int d = 0; d < R1; d += 2) {
+                const uint32_t wv = d < Lmax ? (uint32_t)S[(d >> 1) * PTP + e] : 0u;
+                v[d][c] = wv & 0xFFFFu;
+                if (d + 1 < R1) v[d + 1][c] = wv >> 16;
+              }
+            } else {
+#pragma unroll
+              for (int d = 0; d < R1; d++)
+                v[d][c] = d < Lmax ? (uint32_t)S[d * PTP + e] : 0u;
+            }
           }
         }
         cc_cube_round<PROB, R1, Q, C, 0, RR>(v, lo - ca, a.R);
@@ -433,9 +454,17 @@ __global__ void __launch_bounds__(CC_BLOCK) k_sweep_r(SweepArgs a) {
             }
           } else {
             const int e = cc_pad(p * a.TL + tl);
+            if constexpr (sizeof(LT) == 2) {
 #pragma unroll
-            for (int d = 0; d < R1; d++)
-              if (d < Lmax && (first || d > 0 || R1 == 1)) S[d * PTP + e] = (int32_t)v[d][c];
+              for (int d = 0; d < R1; d += 2)
+                if (d < Lmax)
+                  S[(d >> 1) * PTP + e] =
+                      (int32_t)(v[d][c] | (d + 1 < R1 ? v[d + 1][c] << 16 : 0u));
+            } else {
+#pragma unroll
+              for (int d = 0; d < R1; d++)
+                if (d < Lmax && (first || d > 0 || R1 == 1)) S[d * PTP + e] = (int32_t)v[d][c];
+            }
           }
         }
       }

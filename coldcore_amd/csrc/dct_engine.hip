@@ -50,7 +50,8 @@ struct Sweep {
   long long s, ncolb, ntiles;
   int lmax;
   int rq;       // round kernel: axis size Q of the homogeneous group, else 0
-  size_t lds;
+  size_t lds;   // LDS bytes with int32 layers
+  size_t lds16; // LDS bytes in 16-bit mode (round kernel packs layer pairs)
 };
 
 struct Engine {
@@ -96,6 +97,7 @@ struct Engine {
   double ball = 0;           // exact metric ball size (closed form)
   int lds_budget = 64 * 1024;
   int lds_max = 64 * 1024;   // device limit per workgroup
+  int ncu = 256;             // compute units
   int generic_only = 0;      // CC_SWEEP_KERNEL=generic: no round kernel
   // optional per-sweep timing of the last transform (additive ABI)
   int prof = 0;
@@ -226,15 +228,22 @@ void build_plan() {
     if (!E.generic_only && E.prob == CC_HAMMING && homog && q <= 5 &&
         sw.nax >= round_axes(q) && lds_r <= (size_t)E.lds_max)
       sw.rq = q;
-    if (sw.rq)
+    if (sw.rq) {
       sw.lds = (sw.nax <= round_axes(q)) ? 0 : lds_r;
-    else
-      sw.lds = (size_t)sw.lmax * P * sw.TL * 4;
+      sw.lds16 = (sw.nax <= round_axes(q))
+                     ? 0
+                     : (size_t)((sw.lmax + 1) / 2) * (padded(P * sw.TL) + 1) * 4;
+    } else {
+      sw.lds = sw.lds16 = (size_t)sw.lmax * P * sw.TL * 4;
+    }
     E.plan.push_back(sw);
     j = k;
   }
 }
 
+// Launch a sweep kernel on a persistent grid: as many workgroups as the
+// CUs hold at once (occupancy from the kernel's registers and LDS), each
+// looping over tiles, capped by the tile count.
 template <typename K>
 hipError_t launch_k(K kf, const SweepArgs &a, int blocks, size_t lds) {
   if (lds > 64 * 1024) {
@@ -243,7 +252,13 @@ hipError_t launch_k(K kf, const SweepArgs &a, int blocks, size_t lds) {
                                        (int)lds);
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(kf, dim3(blocks), dim3(CC_BLOCK), lds, E.st, a);
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kf, CC_BLOCK, lds) !=
+          hipSuccess || per_cu < 1)
+    per_cu = 1;
+  const long long resident = (long long)per_cu * E.ncu;
+  const int g = (int)std::min<long long>(blocks, resident);
+  hipLaunchKernelGGL(kf, dim3(g), dim3(CC_BLOCK), lds, E.st, a);
   return hipGetLastError();
 }
 
@@ -356,7 +371,7 @@ int run_transform(int in_kind, int out_kind, int pmode) {
     for (int k = 0; k <= sw.nax; k++) a.live[k] = layers_after(sw.j0 + k);
     const int blocks = (int)std::min<long long>(sw.ntiles, 1LL << 20);
     if (E.prof) CK(hipEventRecord(E.ev[2 * i], E.st));
-    if (sw.rq) CK(launch_round(sw.rq, a, blocks, sw.lds, l16));
+    if (sw.rq) CK(launch_round(sw.rq, a, blocks, l16 ? sw.lds16 : sw.lds, l16));
     else CK(launch_sweep(a, blocks, sw.lds, l16));
     if (E.prof) CK(hipEventRecord(E.ev[2 * i + 1], E.st));
   }
@@ -628,6 +643,9 @@ static int init_cell(int problem, int n, const int *ax, int R,
     CK(hipGetDevice(&dev));
     CK(hipDeviceGetAttribute(&mx, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
     E.lds_max = mx > 0 ? mx : 64 * 1024;
+    int ncu = 0;
+    CK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    E.ncu = ncu > 0 ? ncu : 256;
   }
   const char *lb = getenv("CC_SWEEP_LDS");
   E.lds_budget = lb ? std::max(4096, atoi(lb)) : 64 * 1024;
@@ -1069,7 +1087,7 @@ int dct_sweep_plan(int *out, int cap) {
     out[8 * i + 4] = layers_after(s.c0);
     out[8 * i + 5] = s.lmax;
     out[8 * i + 6] = s.rq;
-    out[8 * i + 7] = (int)s.lds;
+    out[8 * i + 7] = (int)(E.l16_field ? s.lds16 : s.lds);
   }
   if (ns < cap) {
     out[8 * ns + 0] = E.l16_field ? 2 : 4;
