@@ -1,0 +1,46 @@
+// cc_launch.hpp -- sweep launchers.  Each family's kernel instantiations
+// live in their own translation unit (sweep_*.hip) so the library builds in
+// parallel; the engine calls them through these declarations.
+#pragma once
+#include "cc_common.hpp"
+
+struct LaunchCtx {
+  hipStream_t st;
+  int ncu;
+  int grid;   // workgroups per CU for the persistent grid; 0 = one per tile,
+              // < 0 = the occupancy the runtime reports
+};
+
+// Launch a sweep kernel on a persistent grid: as many workgroups as the
+// CUs hold at once (occupancy from the kernel's registers and LDS), each
+// looping over tiles, capped by the tile count.
+template <typename K>
+hipError_t cc_launch_k(K kf, const SweepArgs &a, int blocks, size_t lds,
+                       const LaunchCtx &lc) {
+  if (lds > 64 * 1024) {
+    hipError_t e = hipFuncSetAttribute((const void *)kf,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  int per_cu = lc.grid;
+  if (per_cu < 0 &&
+      (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kf, CC_BLOCK, lds) !=
+           hipSuccess || per_cu < 1))
+    per_cu = 1;
+  const long long resident = per_cu == 0 ? blocks : (long long)per_cu * lc.ncu;
+  const int g = (int)(blocks < resident ? blocks : resident);
+  hipLaunchKernelGGL(kf, dim3(g), dim3(CC_BLOCK), lds, lc.st, a);
+  return hipGetLastError();
+}
+
+// per-axis sweep kernels: L = layer planes (1 for the L-inf families)
+hipError_t cc_sweep_hamming(const SweepArgs &a, int L, int blocks, size_t lds,
+                            bool l16, const LaunchCtx &lc);
+hipError_t cc_sweep_lee(const SweepArgs &a, int L, int blocks, size_t lds,
+                        bool l16, const LaunchCtx &lc);
+hipError_t cc_sweep_linf(const SweepArgs &a, int prob, int blocks, size_t lds,
+                         const LaunchCtx &lc);
+// register-round kernel for homogeneous Hamming groups, axis size q <= 5
+hipError_t cc_sweep_round(const SweepArgs &a, int L, int q, int blocks,
+                          size_t lds, bool l16, const LaunchCtx &lc);

@@ -280,13 +280,17 @@ __device__ __forceinline__ void cc_cube_round(uint32_t (&v)[R1][C], int ilo,
 }
 
 // First-round inputs of one cube: predicate(cnt), the code indicator, or the
-// live layers.  With TL = 1 the first round's cube is C consecutive words,
-// read with 16-byte vector loads.
+// live layers (the seed kind is block-uniform and branched on once per
+// cube).  With TL = 1 the first round's cube is C consecutive words, read
+// with 16-byte vector loads.
 template <int R1, int C, typename LT>
 __device__ __forceinline__ void cc_cube_load(const SweepArgs &a, long long base,
-                                             int pb, int tl, int ncol,
+                                             int pb, int tl, bool live,
                                              int Lin, uint32_t (&v)[R1][C]) {
-  const bool live = tl < ncol;
+#pragma unroll
+  for (int d = 0; d < R1; d++)
+#pragma unroll
+    for (int c = 0; c < C; c++) v[d][c] = 0u;
   if (a.TL == 1 && C % 8 == 0) {
     const long long w0 = base + pb;              // C-aligned, 16-byte aligned
     if (a.in_kind == CC_IN_PRED) {
@@ -309,11 +313,7 @@ __device__ __forceinline__ void cc_cube_load(const SweepArgs &a, long long base,
     } else {
 #pragma unroll
       for (int d = 0; d < R1; d++) {
-        if (d >= Lin) {
-#pragma unroll
-          for (int c = 0; c < C; c++) v[d][c] = 0u;
-          continue;
-        }
+        if (d >= Lin) break;
         if constexpr (sizeof(LT) == 2) {
 #pragma unroll
           for (int c8 = 0; c8 < C; c8 += 8) {
@@ -332,27 +332,115 @@ __device__ __forceinline__ void cc_cube_load(const SweepArgs &a, long long base,
           }
         }
       }
-      return;
     }
-#pragma unroll
-    for (int d = 1; d < R1; d++)
-#pragma unroll
-      for (int c = 0; c < C; c++) v[d][c] = 0u;
     return;
   }
+  if (!live) return;
+  const long long w0 = base + (long long)pb * a.s + tl;
+  if (a.in_kind == CC_IN_PRED) {
+#pragma unroll
+    for (int c = 0; c < C; c++)
+      v[0][c] = cc_pred(a.pmode, (int)a.cnt[w0 + c * a.s], 0, a.mu) ? 1u : 0u;
+  } else if (a.in_kind == CC_IN_CODE) {
+#pragma unroll
+    for (int c = 0; c < C; c++) v[0][c] = (uint32_t)a.A[0][w0 + c * a.s];
+  } else {
+#pragma unroll
+    for (int d = 0; d < R1; d++) {
+      if (d >= Lin) break;
+#pragma unroll
+      for (int c = 0; c < C; c++) v[d][c] = cc_lay_ld<LT>(a, d, w0 + c * a.s);
+    }
+  }
+}
+
+// Last-round output of one cube (pb, stride qa in p): the produced layers,
+// or the layer sum into the map / clamped into cnt.
+template <int R1, int C, typename LT>
+__device__ __forceinline__ void cc_cube_store(const SweepArgs &a, long long base,
+                                              int pb, int qa, int tl,
+                                              int Lmax, const uint32_t (&v)[R1][C]) {
+  const long long sq = (long long)qa * a.s;
+  const long long w0 = base + (long long)pb * a.s + tl;
+  if (a.out_kind == CC_OUT_LAYERS) {
+#pragma unroll
+    for (int d = 0; d < R1; d++) {
+      if (d >= Lmax) break;
+      if (d == 0 && !a.write_l0) continue;
+#pragma unroll
+      for (int c = 0; c < C; c++) cc_lay_st<LT>(a, d, w0 + c * sq, v[d][c]);
+    }
+  } else if (a.out_kind == CC_OUT_MAP) {
+#pragma unroll
+    for (int c = 0; c < C; c++) {
+      uint32_t sum = 0;
+#pragma unroll
+      for (int d = 0; d < R1; d++) sum += v[d][c];
+      a.A[0][w0 + c * sq] = (int32_t)sum;
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < C; c++) {
+      uint32_t sum = 0;
+#pragma unroll
+      for (int d = 0; d < R1; d++) sum += v[d][c];
+      const int32_t si = (int32_t)sum;
+      a.cnt[w0 + c * sq] = si > 65535 ? (uint16_t)65535 : (uint16_t)si;
+    }
+  }
+}
+
+// LDS exchange of one cube: all R1 layers (two 16-bit layers per dword in
+// 16-bit mode), so the unrolled code has no per-layer conditions.
+template <int R1, int C, typename LT>
+__device__ __forceinline__ void cc_cube_lds_st(int32_t *__restrict__ S, int PTP,
+                                               int pb, int qa, int TL, int tl,
+                                               const uint32_t (&v)[R1][C]) {
 #pragma unroll
   for (int c = 0; c < C; c++) {
-    const long long w = base + (long long)(pb + c) * a.s + tl;
-    if (a.in_kind == CC_IN_PRED)
-      v[0][c] = (live && cc_pred(a.pmode, (int)a.cnt[w], 0, a.mu)) ? 1u : 0u;
-    else if (a.in_kind == CC_IN_CODE)
-      v[0][c] = live ? (uint32_t)a.A[0][w] : 0u;
-    else
-      v[0][c] = live ? cc_lay_ld<LT>(a, 0, w) : 0u;
+    const int e = cc_pad((pb + c * qa) * TL + tl);
+    if constexpr (sizeof(LT) == 2) {
 #pragma unroll
-    for (int d = 1; d < R1; d++)
-      v[d][c] = (live && d < Lin && a.in_kind == CC_IN_LAYERS)
-                    ? cc_lay_ld<LT>(a, d, w) : 0u;
+      for (int d = 0; d < R1; d += 2)
+        S[(d >> 1) * PTP + e] = (int32_t)(v[d][c] | (d + 1 < R1 ? v[d + 1][c] << 16 : 0u));
+    } else {
+#pragma unroll
+      for (int d = 0; d < R1; d++) S[d * PTP + e] = (int32_t)v[d][c];
+    }
+  }
+}
+
+template <int R1, int C, typename LT>
+__device__ __forceinline__ void cc_cube_lds_ld(const int32_t *__restrict__ S,
+                                               int PTP, int pb, int qa, int TL,
+                                               int tl, uint32_t (&v)[R1][C]) {
+#pragma unroll
+  for (int c = 0; c < C; c++) {
+    const int e = cc_pad((pb + c * qa) * TL + tl);
+    if constexpr (sizeof(LT) == 2) {
+#pragma unroll
+      for (int d = 0; d < R1; d += 2) {
+        const uint32_t wv = (uint32_t)S[(d >> 1) * PTP + e];
+        v[d][c] = wv & 0xFFFFu;
+        if (d + 1 < R1) v[d + 1][c] = wv >> 16;
+      }
+    } else {
+#pragma unroll
+      for (int d = 0; d < R1; d++) v[d][c] = (uint32_t)S[d * PTP + e];
+    }
+  }
+}
+
+// the axis passes of one round; ilo (block-uniform) skips cube axes an
+// earlier round already applied
+template <int PROB, int R1, int Q, int C, int RR>
+__device__ __forceinline__ void cc_cube_apply(uint32_t (&v)[R1][C], int ilo,
+                                              int R) {
+  switch (ilo) {
+    case 0: cc_cube_round<PROB, R1, Q, C, 0, RR>(v, 0, R); break;
+    case 1: cc_cube_round<PROB, R1, Q, C, 1, RR>(v, 0, R); break;
+    case 2: cc_cube_round<PROB, R1, Q, C, 2, RR>(v, 0, R); break;
+    default: cc_cube_round<PROB, R1, Q, C, 3, RR>(v, 0, R); break;
   }
 }
 
@@ -367,24 +455,30 @@ __global__ void __launch_bounds__(CC_BLOCK) k_sweep_r(SweepArgs a) {
   const int Lin = a.live[0];
   const int nrounds = (m + RR - 1) / RR;
   const int ncube = PT / C;
-  // one cube per thread per round: the next tile's first-round inputs are
-  // loaded while the current tile runs its LDS rounds
-  const bool pf = ncube <= CC_BLOCK;
-  uint32_t nxt[R1][C];
-  if (pf && (long long)blockIdx.x < a.ntiles && (int)threadIdx.x < ncube) {
-    const long long t0 = blockIdx.x;
-    const long long h = t0 / a.ncolb, l0 = (t0 - h * a.ncolb) * a.TL;
-    const int cu = threadIdx.x;
-    cc_cube_load<R1, C, LT>(a, h * a.s * a.P + l0, (cu / a.TL) * C, cu % a.TL,
-                            (int)min((long long)a.TL, a.s - l0), Lin, nxt);
-  }
   for (long long tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
     const long long h = tile / a.ncolb;
     const long long l0 = (tile - h * a.ncolb) * a.TL;
     const int ncol = (int)min((long long)a.TL, a.s - l0);
     const long long base = h * a.s * a.P + l0;
-    for (int rd = 0; rd < nrounds; rd++) {
-      const bool first = rd == 0, last = rd == nrounds - 1;
+    // ---- round 0: HBM -> registers (cube = group axes 0..RR-1)
+    {
+      const int ca = min(0, m - RR);
+      for (int cu = threadIdx.x; cu < ncube; cu += CC_BLOCK) {
+        const int tl = cu % a.TL, pb = (cu / a.TL) * C;
+        uint32_t v[R1][C];
+        cc_cube_load<R1, C, LT>(a, base, pb, tl, tl < ncol, Lin, v);
+        cc_cube_apply<PROB, R1, Q, C, RR>(v, -ca, a.R);
+        if (nrounds == 1) {
+          if (tl < ncol) cc_cube_store<R1, C, LT>(a, base, pb, 1, tl, Lmax, v);
+        } else {
+          cc_cube_lds_st<R1, C, LT>(S, PTP, pb, 1, a.TL, tl, v);
+        }
+      }
+      __syncthreads();
+    }
+    // ---- rounds 1..: LDS -> registers -> LDS, the last one -> HBM
+    for (int rd = 1; rd < nrounds; rd++) {
+      const bool last = rd == nrounds - 1;
       const int lo = rd * RR;                     // first axis of this round
       const int ca = min(lo, m - RR);             // cube's lowest group axis
       int qa = 1;                                 // Q^ca: in-tile stride
@@ -392,80 +486,13 @@ __global__ void __launch_bounds__(CC_BLOCK) k_sweep_r(SweepArgs a) {
       for (int cu = threadIdx.x; cu < ncube; cu += CC_BLOCK) {
         const int tl = cu % a.TL, u = cu / a.TL;
         const int pb = (u / qa) * qa * C + (u % qa);
-        const bool live = tl < ncol;
         uint32_t v[R1][C];
-        if (first) {
-          if (pf) {
-#pragma unroll
-            for (int d = 0; d < R1; d++)
-#pragma unroll
-              for (int c = 0; c < C; c++) v[d][c] = nxt[d][c];
-            const long long t2 = tile + gridDim.x;
-            if (t2 < a.ntiles) {
-              const long long h2 = t2 / a.ncolb;
-              const long long l2 = (t2 - h2 * a.ncolb) * a.TL;
-              cc_cube_load<R1, C, LT>(a, h2 * a.s * a.P + l2, pb, tl,
-                                      (int)min((long long)a.TL, a.s - l2), Lin,
-                                      nxt);
-            }
-          } else {
-            cc_cube_load<R1, C, LT>(a, base, pb, tl, ncol, Lin, v);
-          }
+        cc_cube_lds_ld<R1, C, LT>(S, PTP, pb, qa, a.TL, tl, v);
+        cc_cube_apply<PROB, R1, Q, C, RR>(v, lo - ca, a.R);
+        if (last) {
+          if (tl < ncol) cc_cube_store<R1, C, LT>(a, base, pb, qa, tl, Lmax, v);
         } else {
-#pragma unroll
-          for (int c = 0; c < C; c++) {
-            const int e = cc_pad((pb + c * qa) * a.TL + tl);
-            if constexpr (sizeof(LT) == 2) {
-              // two 16-bit layers per LDS dword: plane k = layers 2k, 2k+1
-#pragma unroll
-              for (int d = 0; d < R1; d += 2) {
-                const uint32_t wv = d < Lmax ? (uint32_t)S[(d >> 1) * PTP + e] : 0u;
-                v[d][c] = wv & 0xFFFFu;
-                if (d + 1 < R1) v[d + 1][c] = wv >> 16;
-              }
-            } else {
-#pragma unroll
-              for (int d = 0; d < R1; d++)
-                v[d][c] = d < Lmax ? (uint32_t)S[d * PTP + e] : 0u;
-            }
-          }
-        }
-        cc_cube_round<PROB, R1, Q, C, 0, RR>(v, lo - ca, a.R);
-#pragma unroll
-        for (int c = 0; c < C; c++) {
-          const int p = pb + c * qa;
-          if (last) {
-            if (!live) continue;
-            const long long w = base + (long long)p * a.s + tl;
-            if (a.out_kind == CC_OUT_LAYERS) {
-#pragma unroll
-              for (int d = 0; d < R1; d++)
-                if (d < Lmax && (d > 0 || a.write_l0)) cc_lay_st<LT>(a, d, w, v[d][c]);
-            } else {
-              uint32_t sum = 0;
-#pragma unroll
-              for (int d = 0; d < R1; d++) sum += v[d][c];
-              if (a.out_kind == CC_OUT_MAP) {
-                a.A[0][w] = (int32_t)sum;
-              } else {
-                const int32_t si = (int32_t)sum;
-                a.cnt[w] = si > 65535 ? (uint16_t)65535 : (uint16_t)si;
-              }
-            }
-          } else {
-            const int e = cc_pad(p * a.TL + tl);
-            if constexpr (sizeof(LT) == 2) {
-#pragma unroll
-              for (int d = 0; d < R1; d += 2)
-                if (d < Lmax)
-                  S[(d >> 1) * PTP + e] =
-                      (int32_t)(v[d][c] | (d + 1 < R1 ? v[d + 1][c] << 16 : 0u));
-            } else {
-#pragma unroll
-              for (int d = 0; d < R1; d++)
-                if (d < Lmax && (first || d > 0 || R1 == 1)) S[d * PTP + e] = (int32_t)v[d][c];
-            }
-          }
+          cc_cube_lds_st<R1, C, LT>(S, PTP, pb, qa, a.TL, tl, v);
         }
       }
       __syncthreads();

@@ -24,7 +24,7 @@
 
 #include "cc_common.hpp"
 #include "cc_reduce.hpp"
-#include "cc_sweep.hpp"
+#include "cc_launch.hpp"
 #include "cc_walk.hpp"
 
 #define CK(x)                                                               \
@@ -98,6 +98,7 @@ struct Engine {
   int lds_budget = 64 * 1024;
   int lds_max = 64 * 1024;   // device limit per workgroup
   int ncu = 256;             // compute units
+  int grid = -1;             // CC_SWEEP_GRID: workgroups per CU (0 = per tile)
   int generic_only = 0;      // CC_SWEEP_KERNEL=generic: no round kernel
   // optional per-sweep timing of the last transform (additive ABI)
   int prof = 0;
@@ -224,7 +225,7 @@ void build_plan() {
     bool homog = true;
     for (int i = j; i < k; i++) homog = homog && E.ax[i] == E.ax[j];
     const int q = E.ax[j];
-    const size_t lds_r = (size_t)sw.lmax * (padded(P * sw.TL) + 1) * 4;
+    const size_t lds_r = (size_t)E.L * (padded(P * sw.TL) + 1) * 4;
     if (!E.generic_only && E.prob == CC_HAMMING && homog && q <= 5 &&
         sw.nax >= round_axes(q) && lds_r <= (size_t)E.lds_max)
       sw.rq = q;
@@ -232,7 +233,7 @@ void build_plan() {
       sw.lds = (sw.nax <= round_axes(q)) ? 0 : lds_r;
       sw.lds16 = (sw.nax <= round_axes(q))
                      ? 0
-                     : (size_t)((sw.lmax + 1) / 2) * (padded(P * sw.TL) + 1) * 4;
+                     : (size_t)((E.L + 1) / 2) * (padded(P * sw.TL) + 1) * 4;
     } else {
       sw.lds = sw.lds16 = (size_t)sw.lmax * P * sw.TL * 4;
     }
@@ -241,91 +242,18 @@ void build_plan() {
   }
 }
 
-// Launch a sweep kernel on a persistent grid: as many workgroups as the
-// CUs hold at once (occupancy from the kernel's registers and LDS), each
-// looping over tiles, capped by the tile count.
-template <typename K>
-hipError_t launch_k(K kf, const SweepArgs &a, int blocks, size_t lds) {
-  if (lds > 64 * 1024) {
-    hipError_t e = hipFuncSetAttribute((const void *)kf,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)lds);
-    if (e != hipSuccess) return e;
-  }
-  int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kf, CC_BLOCK, lds) !=
-          hipSuccess || per_cu < 1)
-    per_cu = 1;
-  const long long resident = (long long)per_cu * E.ncu;
-  const int g = (int)std::min<long long>(blocks, resident);
-  hipLaunchKernelGGL(kf, dim3(g), dim3(CC_BLOCK), lds, E.st, a);
-  return hipGetLastError();
-}
-
-template <int PROB, int R1>
-hipError_t launch_sweep_t(const SweepArgs &a, int blocks, size_t lds,
-                          bool l16) {
-  if (l16 && R1 > 1)
-    return launch_k(k_sweep<PROB, R1, uint16_t>, a, blocks, lds);
-  return launch_k(k_sweep<PROB, R1, int32_t>, a, blocks, lds);
-}
-
-template <int R1, int Q, int RR>
-hipError_t launch_round_q(const SweepArgs &a, int blocks, size_t lds,
-                          bool l16) {
-  if (l16) return launch_k(k_sweep_r<CC_HAMMING, R1, Q, RR, uint16_t>, a, blocks, lds);
-  return launch_k(k_sweep_r<CC_HAMMING, R1, Q, RR, int32_t>, a, blocks, lds);
-}
-
-template <int R1>
-hipError_t launch_round_t(int q, const SweepArgs &a, int blocks, size_t lds,
-                          bool l16) {
-  switch (q) {
-    case 2: return launch_round_q<R1, 2, 4>(a, blocks, lds, l16);
-    case 3: return launch_round_q<R1, 3, 2>(a, blocks, lds, l16);
-    case 4: return launch_round_q<R1, 4, 2>(a, blocks, lds, l16);
-    case 5: return launch_round_q<R1, 5, 1>(a, blocks, lds, l16);
-  }
-  return hipErrorInvalidValue;
-}
-
 hipError_t launch_round(int q, const SweepArgs &a, int blocks, size_t lds,
                         bool l16) {
-  switch (E.L) {
-    case 2: return launch_round_t<2>(q, a, blocks, lds, l16);
-    case 3: return launch_round_t<3>(q, a, blocks, lds, l16);
-    case 4: return launch_round_t<4>(q, a, blocks, lds, l16);
-    case 5: return launch_round_t<5>(q, a, blocks, lds, l16);
-    case 6: return launch_round_t<6>(q, a, blocks, lds, l16);
-    case 7: return launch_round_t<7>(q, a, blocks, lds, l16);
-  }
-  return hipErrorInvalidValue;
+  return cc_sweep_round(a, E.L, q, blocks, lds, l16, LaunchCtx{E.st, E.ncu, E.grid});
 }
 
 hipError_t launch_sweep(const SweepArgs &a, int blocks, size_t lds, bool l16) {
+  const LaunchCtx lc{E.st, E.ncu, E.grid};
   switch (E.prob) {
-    case CC_TORUS: return launch_sweep_t<CC_TORUS, 1>(a, blocks, lds, false);
-    case CC_GRID: return launch_sweep_t<CC_GRID, 1>(a, blocks, lds, false);
-    case CC_HAMMING:
-      switch (E.L) {
-        case 2: return launch_sweep_t<CC_HAMMING, 2>(a, blocks, lds, l16);
-        case 3: return launch_sweep_t<CC_HAMMING, 3>(a, blocks, lds, l16);
-        case 4: return launch_sweep_t<CC_HAMMING, 4>(a, blocks, lds, l16);
-        case 5: return launch_sweep_t<CC_HAMMING, 5>(a, blocks, lds, l16);
-        case 6: return launch_sweep_t<CC_HAMMING, 6>(a, blocks, lds, l16);
-        case 7: return launch_sweep_t<CC_HAMMING, 7>(a, blocks, lds, l16);
-      }
-      break;
-    case CC_LEE:
-      switch (E.L) {
-        case 2: return launch_sweep_t<CC_LEE, 2>(a, blocks, lds, l16);
-        case 3: return launch_sweep_t<CC_LEE, 3>(a, blocks, lds, l16);
-        case 4: return launch_sweep_t<CC_LEE, 4>(a, blocks, lds, l16);
-        case 5: return launch_sweep_t<CC_LEE, 5>(a, blocks, lds, l16);
-        case 6: return launch_sweep_t<CC_LEE, 6>(a, blocks, lds, l16);
-        case 7: return launch_sweep_t<CC_LEE, 7>(a, blocks, lds, l16);
-      }
-      break;
+    case CC_TORUS:
+    case CC_GRID: return cc_sweep_linf(a, E.prob, blocks, lds, lc);
+    case CC_HAMMING: return cc_sweep_hamming(a, E.L, blocks, lds, l16, lc);
+    case CC_LEE: return cc_sweep_lee(a, E.L, blocks, lds, l16, lc);
   }
   return hipErrorInvalidValue;
 }
@@ -650,6 +578,8 @@ static int init_cell(int problem, int n, const int *ax, int R,
   const char *lb = getenv("CC_SWEEP_LDS");
   E.lds_budget = lb ? std::max(4096, atoi(lb)) : 64 * 1024;
   E.lds_budget = std::min(E.lds_budget, E.lds_max);
+  const char *sg = getenv("CC_SWEEP_GRID");
+  E.grid = sg ? atoi(sg) : 0;   // measured: one workgroup per tile is best
   const char *sk = getenv("CC_SWEEP_KERNEL");
   E.generic_only = sk && !strcmp(sk, "generic");
   CK(hipStreamCreateWithFlags(&E.st, hipStreamNonBlocking));
