@@ -90,6 +90,65 @@ def run_cpu_baseline(code, passes):
                        f"os.cpu_count()={os.cpu_count()}")
 
 
+def copy_calibration(mib):
+    """hipMemcpy device-to-device of a known size: its dispatch
+    (__amd_rocclr_copyBuffer) reads and writes exactly mib MiB."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(n)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(n)) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3) == 0   # D2D
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"calib_copy_bytes": n, "copies": 3}), flush=True)
+
+
+#: further configured cells, timed as gain passes (name, axes, R, code recipe)
+OTHER_CELLS = [
+    ("K_5(10,3)", [5] * 10, 3, ("p", 1e-4)),
+    ("mixed [4,5,6,7,8,4,5,6,7] R=3", [4, 5, 6, 7, 8, 4, 5, 6, 7], 3, ("p", 1e-4)),
+    ("K_2(30,4)", [2] * 30, 4, ("choice", 40000)),
+    ("K_3(8,2)", [3] * 8, 2, ("choice", 40)),
+    ("K_3(10,2)", [3] * 10, 2, ("choice", 400)),
+    ("K_4(8,2)", [4] * 8, 2, ("choice", 400)),
+]
+
+
+def time_cells(GpuBackend, steps):
+    out = []
+    for name, axes, R, (kind, arg) in OTHER_CELLS:
+        size = int(np.prod(np.array(axes, dtype=np.int64)))
+        rng = np.random.default_rng(0)
+        if kind == "p":
+            code = np.flatnonzero(rng.random(size) < arg)
+        else:
+            code = rng.choice(size, arg, replace=False)
+        g = GpuBackend(axes, R)
+        try:
+            g.recount(code.astype(np.int64))
+            plan = g.sweep_plan()
+            g.gain_map()
+            t = time.perf_counter()
+            for _ in range(steps):
+                g.gain_map()
+            ms = (time.perf_counter() - t) * 1e3 / steps
+            bpw = sum(sweep_bytes(plan, "field"))
+            out.append({"cell": name, "words": size, "code_words": int(len(code)),
+                        "gain_pass_ms": round(ms, 4),
+                        "passes_per_s": round(1e3 / ms, 1),
+                        "bytes_per_word": bpw,
+                        "GBps_algorithmic": round(bpw * size / (ms * 1e-3) / 1e9, 1),
+                        "sweeps": len(plan)})
+        finally:
+            g.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,7 +157,14 @@ def main():
     ap.add_argument("--cpu-passes", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB "
+                         "(3 times): the reference kernel for scaling the "
+                         "FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
+    if args.calib_copy:
+        copy_calibration(args.calib_copy)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the hot path on a "
               "single MI355X (multi-GPU sharding is a design note, not "
@@ -244,6 +310,8 @@ def main():
         },
     }
     rec.update(extra)
+    if not args.no_extra:
+        rec["other_cells"] = time_cells(GpuBackend, 10)
     if not args.no_cpu:
         rec["cpu_baseline"] = run_cpu_baseline(code, args.cpu_passes)
         rec["speedup_vs_cpu"] = round(passes_s / rec["cpu_baseline"]["value"], 1)
