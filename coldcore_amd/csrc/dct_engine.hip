@@ -198,13 +198,23 @@ void build_plan() {
     // strided groups read whole 128-byte rows: 32 int32 or 64 uint16 columns
     const int TL0 = (j == 0) ? 1
                              : (int)std::min<long long>(E.l16_field ? 64 : 32, sw.s);
+    // a homogeneous Hamming group with q <= 5 runs the register-round
+    // kernel; in 16-bit mode it packs two layers per LDS dword, and a group
+    // of binary or quaternary axes is capped at one 16-word cube per thread
+    // (4,096 words per tile) to keep its round count low
+    const int q = E.ax[j];
+    const bool round_ok = !E.generic_only && E.prob == CC_HAMMING && q <= 5;
     long long P = 1;
     int k = j;
     while (k < E.n) {
-      long long P2 = P * E.ax[k];
-      size_t need = (size_t)layers_after(k + 1) * P2 * TL0 * 4;
+      const bool homog = E.ax[k] == q;
+      const long long P2 = P * E.ax[k];
+      const int lay = layers_after(k + 1);
+      const bool packed = round_ok && homog && E.l16_field;
+      const size_t need = (size_t)(packed ? (lay + 1) / 2 : lay) * P2 * TL0 * 4;
+      const bool cap = packed && (q == 2 || q == 4) && P2 * TL0 > 4096;
       if (k > j && (need > (size_t)E.lds_budget || P2 > (1LL << 20) ||
-                    k - j >= CC_MAXSW))
+                    k - j >= CC_MAXSW || cap))
         break;
       P = P2;
       k++;
@@ -220,20 +230,20 @@ void build_plan() {
     sw.TL = TL;
     sw.ncolb = (sw.s + sw.TL - 1) / sw.TL;
     sw.ntiles = (E.X / (sw.s * P)) * sw.ncolb;
-    // round kernel: a homogeneous Hamming group of at least one full round
     sw.rq = 0;
     bool homog = true;
-    for (int i = j; i < k; i++) homog = homog && E.ax[i] == E.ax[j];
-    const int q = E.ax[j];
+    for (int i = j; i < k; i++) homog = homog && E.ax[i] == q;
     const size_t lds_r = (size_t)E.L * (padded(P * sw.TL) + 1) * 4;
-    if (!E.generic_only && E.prob == CC_HAMMING && homog && q <= 5 &&
-        sw.nax >= round_axes(q) && lds_r <= (size_t)E.lds_max)
+    const size_t lds_r16 = (size_t)((E.L + 1) / 2) * (padded(P * sw.TL) + 1) * 4;
+    // (in 16-bit mode every transform of the cell runs 16-bit, so only the
+    // packed footprint has to fit)
+    if (round_ok && homog && sw.nax >= round_axes(q) &&
+        (E.l16_field ? lds_r16 : lds_r) <= (size_t)E.lds_max)
       sw.rq = q;
     if (sw.rq) {
-      sw.lds = (sw.nax <= round_axes(q)) ? 0 : lds_r;
-      sw.lds16 = (sw.nax <= round_axes(q))
-                     ? 0
-                     : (size_t)((E.L + 1) / 2) * (padded(P * sw.TL) + 1) * 4;
+      const bool one = sw.nax <= round_axes(q);
+      sw.lds = one ? 0 : lds_r;
+      sw.lds16 = one ? 0 : lds_r16;
     } else {
       sw.lds = sw.lds16 = (size_t)sw.lmax * P * sw.TL * 4;
     }
