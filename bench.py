@@ -112,6 +112,32 @@ def measured_traffic(plan):
     return int(t.group(1))
 
 
+def run_config1(GpuBackend, with_cpu):
+    """K_3(8,2): seeded 40-word start, fields, lazy greedy, peel, 1,000 LNS
+    rounds of notch descent, on the engine (and the numpy oracle)."""
+    from coldcore_amd.search import config1_run
+    start = np.random.default_rng(0).choice(3 ** 8, size=40, replace=False)
+    out = {}
+    g = GpuBackend([3] * 8, 2)
+    try:
+        t = time.perf_counter()
+        r = config1_run(g, start, rounds=1000)
+        out["gpu_wall_s"] = round(time.perf_counter() - t, 3)
+    finally:
+        g.close()
+    out.update(greedy_size=r["greedy"], final_size=int(len(r["final"])),
+               final_uncovered=r["uncov"], rounds=r["rounds"],
+               best_feasible=r["sizes"][-1][1])
+    if with_cpu:
+        from oracle.dp import NumpyDP
+        t = time.perf_counter()
+        c = config1_run(NumpyDP([3] * 8, 2), start, rounds=1000)
+        out["cpu_oracle_wall_s"] = round(time.perf_counter() - t, 3)
+        out["trajectory_identical"] = bool(
+            c["trace"] == r["trace"] and (c["final"] == r["final"]).all())
+    return out
+
+
 def copy_calibration(mib):
     """hipMemcpy device-to-device of a known size: its dispatch
     (__amd_rocclr_copyBuffer) reads and writes exactly mib MiB."""
@@ -346,6 +372,7 @@ def main():
     rec.update(extra)
     if not args.no_extra:
         rec["other_cells"] = time_cells(GpuBackend, 10)
+        rec["config1"] = run_config1(GpuBackend, not args.no_cpu)
     if not args.no_cpu:
         rec["cpu_baseline"] = run_cpu_baseline(code, args.cpu_passes)
         rec["speedup_vs_cpu"] = round(passes_s / rec["cpu_baseline"]["value"], 1)

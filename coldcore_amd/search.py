@@ -448,3 +448,47 @@ class Searcher:
             self.b.recount(self.code)
             self.uncov = self.b.uncovered()
         return self.uncov
+
+
+def descend_rounds(searcher, rounds, kmin=8, select="fixed"):
+    """Clock-free notch descent: from a feasible code of size M, drop one
+    incumbent (exact-loss ruin) and run LNS at M - 1 until it is feasible
+    again, then notch further; stops after ``rounds`` LNS rounds in total.
+    Returns (rounds used, [(rounds so far, feasible size), ...])."""
+    s = searcher
+    used = 0
+    sizes = [(0, len(s.code))]
+    while used < rounds and s.uncov == 0 and len(s.code) > 1:
+        M = len(s.code) - 1
+        before = s.stats["rounds"]
+        s.ruin(1, "low")
+        s.lns(M, kmin=kmin, select=select, max_rounds=rounds - used)
+        used += s.stats["rounds"] - before
+        if s.uncov == 0:
+            sizes.append((used, len(s.code)))
+    return used, sizes
+
+
+def config1_run(backend, start, rounds=1000, seed=1):
+    """K_3(8,2)-style run: load the start code, read the three fields, lazy
+    greedy to a cover, peel, then ``rounds`` LNS rounds of notch descent.
+    Returns the move trace (per-round mode / uncovered / size) and the final
+    best code."""
+    trace = []
+    s = Searcher(backend, seed=seed,
+                 on_round=lambda s, r: trace.append((r["mode"], r["post"],
+                                                     len(s.code))))
+    s.load(np.asarray(start, dtype=np.int64))
+    backend.gain_map()
+    g_max = backend.map_max()
+    backend.loss_map()
+    l_max = backend.map_max()
+    s.greedy_fill(10 ** 9)
+    greedy_size = len(s.code)
+    s.peel()
+    used, sizes = descend_rounds(s, rounds)
+    best = s.best_code if s.uncov != 0 and s.best_code is not None else s.code
+    return dict(fields=(int(g_max), int(l_max)), greedy=greedy_size,
+                peeled=sizes[0][1], sizes=sizes, rounds=used, trace=trace,
+                final=np.asarray(sorted(int(x) for x in s.code)),
+                uncov=int(s.uncov))

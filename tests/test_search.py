@@ -109,3 +109,28 @@ def test_lns_rounds_identical_gpu_and_cpu(gpu_cell, select):
     g = gpu_cell([3] * 6, 1)
     assert lns_trace(g, 7, 80, select) == lns_trace(NumpyDP([3] * 6, 1), 7,
                                                     80, select)
+
+
+@pytest.mark.gpu
+def test_config1_k3_8_2_trajectory_identical_gpu_and_cpu(gpu_cell):
+    """K_3(8,2): 40-word seeded start, fields, greedy, peel and 1,000 LNS
+    rounds of notch descent give the same per-round trace and final code
+    on the engine and on the numpy oracle."""
+    from coldcore_amd.search import config1_run
+    start = np.random.default_rng(0).choice(3 ** 8, size=40, replace=False)
+    g = gpu_cell([3] * 8, 2)
+    a = config1_run(g, start, rounds=1000)
+    b = config1_run(NumpyDP([3] * 8, 2), start, rounds=1000)
+    assert a["rounds"] == b["rounds"] == 1000
+    assert a["fields"] == b["fields"] and a["greedy"] == b["greedy"]
+    assert a["sizes"] == b["sizes"] and a["trace"] == b["trace"]
+    assert (a["final"] == b["final"]).all() and a["uncov"] == b["uncov"]
+
+
+def test_config1_prefix_identical_cpu_and_ref():
+    """The first rounds of the same run on brute force and the axis DP."""
+    from coldcore_amd.search import config1_run
+    start = np.random.default_rng(0).choice(3 ** 6, size=12, replace=False)
+    a = config1_run(BruteForce([3] * 6, 2), start, rounds=40)
+    b = config1_run(NumpyDP([3] * 6, 2), start, rounds=40)
+    assert a["trace"] == b["trace"] and (a["final"] == b["final"]).all()
