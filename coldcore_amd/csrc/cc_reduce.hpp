@@ -226,3 +226,17 @@ __global__ void k_read_at(const int32_t *A, const long long *idx, int k,
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < k) out[i] = A[idx[i]];
 }
+
+// out[i * L + d] = layer d at word idx[i] (after a layers transform); LT is
+// the plane width the transform used
+template <typename LT>
+__global__ void k_layers_read_at(const SweepArgs a, int L, const long long *idx,
+                                 int k, int32_t *out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= k) return;
+  const long long w = idx[i];
+  for (int d = 0; d < L; d++) {
+    if constexpr (sizeof(LT) == 2) out[(long long)i * L + d] = (int32_t)a.H[d][w];
+    else out[(long long)i * L + d] = a.A[d][w];
+  }
+}

@@ -66,6 +66,8 @@ struct Engine {
   long long pitch = 0;        // plane pitch in words (multiple of 64)
   int l16_field = 0;          // gain / loss run 16-bit inter-sweep planes
   int l16_count = 0;          // recount runs 16-bit inter-sweep planes
+  int layers_live = 0;        // a layers transform left exact layers (1: int32
+                              // planes A, 2: uint16 planes H)
   uint16_t *cnt = nullptr;
   long long cnt_len = 0;
   uint64_t *pat = nullptr, *pat2 = nullptr;
@@ -272,7 +274,11 @@ hipError_t launch_sweep(const SweepArgs &a, int blocks, size_t lds, bool l16) {
 // result goes.  pmode is the predicate for CC_IN_PRED.
 int run_transform(int in_kind, int out_kind, int pmode) {
   const int ns = (int)E.plan.size();
-  const bool l16 = out_kind == CC_OUT_CNT ? E.l16_count : E.l16_field;
+  const bool l16 = out_kind == CC_OUT_CNT ? E.l16_count
+                   : out_kind == CC_OUT_LAYERS && in_kind == CC_IN_CODE
+                       ? E.l16_count
+                       : E.l16_field;
+  E.layers_live = out_kind == CC_OUT_LAYERS ? (l16 ? 2 : 1) : 0;
   E.nswept = 0;
   for (int i = 0; i < ns; i++) {
     const Sweep &sw = E.plan[i];
@@ -765,6 +771,7 @@ int dct_transform(int init_mode, int out_mode) {
   } else if (init_mode == 2) {
     pmode = CC_CRIT;
   }
+  E.layers_live = 0;
   if (int rc = run_transform(in_kind, out_mode == 1 ? CC_OUT_MAP : CC_OUT_CNT,
                              pmode))
     return rc;
@@ -1007,6 +1014,68 @@ int dct_use_fmt(int fmt) {
   if (fmt == 1 && !E.pat2) return -1;
   if (fmt == 1 && E.prob == CC_GRID) return -1;  // the grid walk needs v1
   E.fmt = fmt;
+  return 0;
+}
+
+// ---- additive (not in the reference ABI): exact distance layers ---------
+// dct_transform_layers(init_mode): the transform of dct_transform(init_mode,
+// 1) with the last sweep keeping the layers instead of summing them, so
+// layer d at x is the exact number of seed words at distance d from x:
+// init 0 codewords (multiset of set_code; the represented solution and cnt
+// are unchanged), 1 deficient words, 2 critical words.  The map plane is
+// overwritten.  Layered families only (Hamming, Lee).
+int dct_transform_layers(int init_mode) {
+  if (!E.live || E.L < 2 || init_mode < 0 || init_mode > 2) return -1;
+  int in_kind = CC_IN_PRED, pmode = CC_DEF;
+  if (init_mode == 0) {
+    in_kind = CC_IN_CODE;
+    hipLaunchKernelGGL(k_fill32, dim3(pass_blocks(E.X)), dim3(CC_BLOCK), 0,
+                       E.st, E.A[0], 0, E.X);
+    CK(hipGetLastError());
+    if (E.codeM > 0) {
+      hipLaunchKernelGGL(k_scatter_code, dim3((E.codeM + 255) / 256), dim3(256),
+                         0, E.st, E.A[0], E.d_code, E.codeM);
+      CK(hipGetLastError());
+    }
+  } else if (init_mode == 2) {
+    pmode = CC_CRIT;
+  }
+  if (int rc = run_transform(in_kind, CC_OUT_LAYERS, pmode)) return rc;
+  CK(hipStreamSynchronize(E.st));
+  return 0;
+}
+
+// out[i * (R+1) + d] = layer d at idx[i] after dct_transform_layers
+int dct_layers_read_at(const long long *idx, int k, int32_t *out) {
+  if (!E.live || !E.layers_live) return -1;
+  if (k <= 0) return 0;
+  if (!idx || !out) return -1;
+  for (int i = 0; i < k; i++)
+    if (idx[i] < 0 || idx[i] >= E.X) return -1;
+  SweepArgs a;
+  memset(&a, 0, sizeof(a));
+  for (int d = 0; d < CC_MAXL; d++) { a.A[d] = E.A[d]; a.H[d] = E.H[d]; }
+  const int L = E.L;
+  const int chunk = CODECAP / L;
+  for (int off = 0; off < k; off += chunk) {
+    const int nb = std::min(k - off, chunk);
+    if (int rc = stage_words(idx + off, nb)) return rc;
+    if (E.layers_live == 2)
+      hipLaunchKernelGGL(k_layers_read_at<uint16_t>, dim3((nb + 255) / 256),
+                         dim3(256), 0, E.st, a, L, E.d_words, nb, E.d_out);
+    else
+      hipLaunchKernelGGL(k_layers_read_at<int32_t>, dim3((nb + 255) / 256),
+                         dim3(256), 0, E.st, a, L, E.d_words, nb, E.d_out);
+    CK(hipGetLastError());
+    CK(hipMemcpyAsync(E.h_out, E.d_out, (size_t)nb * L * 4,
+                      hipMemcpyDeviceToHost, E.st));
+    CK(hipStreamSynchronize(E.st));
+    memcpy(out + (size_t)off * L, E.h_out, (size_t)nb * L * 4);
+  }
+  // distances beyond the space's diameter are never written: they are 0
+  const int Lv = layers_after(E.n);
+  for (int i = 0; i < k && Lv < L; i++)
+    for (int d = Lv; d < L; d++) out[(size_t)i * L + d] = 0;
   return 0;
 }
 

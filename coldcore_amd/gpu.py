@@ -32,7 +32,7 @@ OPTIONAL = ("dct_map_hist", "dct_set_mu", "dct_get_mu", "dct_ball_gather_mu",
             "dct_set_tbl2", "dct_use_fmt", "dct_ball_update_delta",
             "dct_swap_eval", "dct_stats")
 ADDITIVE = ("dct_init3w", "dct_sweep_plan", "dct_sweep_profile",
-            "dct_sweep_times")
+            "dct_sweep_times", "dct_transform_layers", "dct_layers_read_at")
 
 
 def _rc_msg(rc):
@@ -66,7 +66,8 @@ def _bind(path):
               "dct_read_cnt", "dct_map_hist", "dct_set_mu", "dct_get_mu",
               "dct_ball_gather_mu", "dct_loss_owner", "dct_set_tbl2",
               "dct_use_fmt", "dct_stats", "dct_sweep_plan",
-              "dct_sweep_profile", "dct_sweep_times"):
+              "dct_sweep_profile", "dct_sweep_times", "dct_transform_layers",
+              "dct_layers_read_at"):
         getattr(lib, f).restype = ctypes.c_int
     for f in ("dct_count_eq", "dct_count_deficit", "dct_map_extract",
               "dct_map_extract64", "dct_host_bytes"):
@@ -344,6 +345,22 @@ class GpuBackend:
         out = np.empty(length, dtype=np.uint16)
         _check("read_cnt", self._lib.dct_read_cnt(
             int(off), int(length), _ptr(out, ctypes.POINTER(ctypes.c_uint16))))
+        return out
+
+    # ---------------- additive: exact distance layers ---------------------
+    def layers_at(self, init_mode, idx, code=None):
+        """(k, R+1) exact distance layers at idx: entry [i, d] is the number
+        of seed words at distance exactly d from idx[i]; init_mode 0 seeds
+        with ``code`` (default: nothing), 1 the deficient words, 2 the
+        critical words.  cnt is unchanged."""
+        idx = np.ascontiguousarray(np.atleast_1d(idx), dtype=np.int64)
+        if init_mode == 0:
+            self._set_code(np.zeros(0, dtype=np.int64) if code is None else code)
+        _check("transform_layers", self._lib.dct_transform_layers(int(init_mode)))
+        L = self.R + 1
+        out = np.empty((len(idx), L), dtype=np.int32)
+        _check("layers_read_at", self._lib.dct_layers_read_at(
+            _ptr(idx, _LLP), len(idx), _ptr(out, _I32P)))
         return out
 
     # ---------------- additive: schedule and per-sweep timing -------------

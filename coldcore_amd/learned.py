@@ -1,0 +1,284 @@
+"""Learned ruin proposal: an MLP that scores which region LNS should ruin.
+
+A ruin region is an anchor incumbent and its k-1 nearest incumbents (digit
+distance, random tie-break -- the "cluster" shape of the reference's ruin
+operators, reference src/coldcore/search.py:576-584).  Its features are five
+windows of exact distance-layer counts, each R+1 wide (5 (R+1) <= 16):
+
+  1. mean over the region of the critical layers L_d(w)
+  2. mean over the region of the deficient layers G_d(w)
+  3. max  over the region of G_d(w)
+  4. mean over the region of the codeword layers C_d(w)
+  5. min  over the region of L_d(w)
+
+where X_d(w) is the exact number of words of that kind at distance exactly d
+from w (dct_transform_layers: the transform's per-distance layers), divided
+by the distance-d shell size so every feature lies in [0, 1].
+
+The network (coldcore_amd/csrc/mlp.hip) is trained on the GPU by its own
+forward and backward kernels on records the plain LNS logs (region features,
+outcome: 1 improved, 0.5 sideways, 0 worse).  At search time it is one more
+operator under the adaptive operator bandit, which stays the fallback: a
+region the network proposes is still repaired by the exact lazy greedy and
+judged by the exact fields, so no count is ever approximated.
+"""
+import ctypes
+import time
+from math import comb
+
+import numpy as np
+
+from .gpu import load_plugin
+from .search import Searcher
+
+IN, HID = 16, 256
+NPARAM = HID * IN + HID + HID + 1
+_FP = ctypes.POINTER(ctypes.c_float)
+
+
+class Net:
+    """The GPU network (one per process)."""
+
+    def __init__(self, max_batch=4096, seed=0, in_dim=IN):
+        self.lib = lib = load_plugin()
+        for f in ("cc_mlp_init", "cc_mlp_free", "cc_mlp_set_params",
+                  "cc_mlp_get_params", "cc_mlp_forward", "cc_mlp_grad",
+                  "cc_mlp_adam", "cc_mlp_param_count"):
+            getattr(lib, f).restype = ctypes.c_int
+        lib.cc_mlp_adam.argtypes = [ctypes.c_float] * 4
+        assert lib.cc_mlp_param_count() == NPARAM
+        self.max_batch = max_batch
+        rc = lib.cc_mlp_init(max_batch)
+        if rc:
+            raise RuntimeError(f"cc_mlp_init rc={rc}")
+        self.set_params(init_params(seed, in_dim))
+
+    def close(self):
+        self.lib.cc_mlp_free()
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == NPARAM
+        rc = self.lib.cc_mlp_set_params(p.ctypes.data_as(_FP))
+        if rc:
+            raise RuntimeError(f"cc_mlp_set_params rc={rc}")
+
+    def get_params(self):
+        p = np.empty(NPARAM, dtype=np.float32)
+        rc = self.lib.cc_mlp_get_params(p.ctypes.data_as(_FP))
+        if rc:
+            raise RuntimeError(f"cc_mlp_get_params rc={rc}")
+        return p
+
+    @staticmethod
+    def _x(X):
+        X = np.asarray(X, dtype=np.float32)
+        out = np.zeros((len(X), IN), dtype=np.float32)
+        out[:, :X.shape[1]] = X
+        return out
+
+    def forward(self, X):
+        X = self._x(X)
+        y = np.empty(len(X), dtype=np.float32)
+        for lo in range(0, len(X), self.max_batch):
+            xb = np.ascontiguousarray(X[lo:lo + self.max_batch])
+            yb = np.empty(len(xb), dtype=np.float32)
+            rc = self.lib.cc_mlp_forward(xb.ctypes.data_as(_FP), len(xb),
+                                         yb.ctypes.data_as(_FP))
+            if rc:
+                raise RuntimeError(f"cc_mlp_forward rc={rc}")
+            y[lo:lo + len(xb)] = yb
+        return y
+
+    def grad(self, X, t):
+        X = self._x(X)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        g = np.empty(NPARAM, dtype=np.float32)
+        loss = np.zeros(1, dtype=np.float32)
+        rc = self.lib.cc_mlp_grad(X.ctypes.data_as(_FP), t.ctypes.data_as(_FP),
+                                  len(X), g.ctypes.data_as(_FP),
+                                  loss.ctypes.data_as(_FP))
+        if rc:
+            raise RuntimeError(f"cc_mlp_grad rc={rc}")
+        return float(loss[0]), g
+
+    def train_step(self, X, t, lr=1e-3):
+        X = self._x(X)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        loss = np.zeros(1, dtype=np.float32)
+        rc = self.lib.cc_mlp_grad(X.ctypes.data_as(_FP), t.ctypes.data_as(_FP),
+                                  len(X), None, loss.ctypes.data_as(_FP))
+        if rc == 0:
+            rc = self.lib.cc_mlp_adam(lr, 0.9, 0.999, 1e-8)
+        if rc:
+            raise RuntimeError(f"mlp train step rc={rc}")
+        return float(loss[0])
+
+
+def init_params(seed, in_dim=IN):
+    rng = np.random.default_rng(seed)
+    p = np.zeros(NPARAM, dtype=np.float32)
+    W1 = np.zeros((HID, IN), dtype=np.float32)
+    W1[:, :in_dim] = rng.normal(0, np.sqrt(2.0 / in_dim), (HID, in_dim))
+    p[:HID * IN] = W1.ravel()
+    p[HID * IN + HID:HID * IN + 2 * HID] = rng.normal(0, np.sqrt(1.0 / HID), HID)
+    return p
+
+
+def shell_sizes(axes, R, problem="hamming"):
+    """Words at exact distance d from any word, d = 0..R (Hamming: the
+    elementary symmetric polynomials of a_j - 1)."""
+    if problem != "hamming":
+        raise ValueError("region features are defined for Hamming cells")
+    poly = [1] + [0] * R
+    for a in axes:
+        poly = [poly[d] + (a - 1) * (poly[d - 1] if d else 0) for d in range(R + 1)]
+    return np.maximum(np.array(poly, dtype=np.float64), 1.0)
+
+
+class RegionScorer:
+    """Exact-layer features of candidate ruin regions of one backend."""
+
+    def __init__(self, backend):
+        b = backend
+        self.b = b
+        self.R = int(b.R)
+        if 5 * (self.R + 1) > IN:
+            raise ValueError("five (R+1)-layer windows must fit 16 inputs (R <= 2)")
+        self.shell = shell_sizes(b.space.axes, self.R)
+
+    def layers(self, code):
+        """(m, R+1) deficient, critical and codeword layers at the code."""
+        b = self.b
+        G = b.layers_at(1, code) / self.shell
+        L = b.layers_at(2, code) / self.shell
+        C = b.layers_at(0, code, code=code) / self.shell
+        return G, L, C
+
+    def features(self, lay, regions):
+        G, L, C = lay
+        out = np.zeros((len(regions), 5 * (self.R + 1)), dtype=np.float32)
+        for i, W in enumerate(regions):
+            W = np.asarray(W)
+            out[i] = np.concatenate([L[W].mean(0), G[W].mean(0), G[W].max(0),
+                                     C[W].mean(0), L[W].min(0)])
+        return out
+
+
+def cluster_regions(searcher, k, anchors):
+    """Anchor + its k-1 nearest incumbents (digit distance, random ties)."""
+    s = searcher
+    m = len(s.code)
+    digs = s.b.index_word(s.code).astype(np.int16)
+    out = []
+    for a in anchors:
+        d = (digs != digs[a][None, :]).sum(axis=1)
+        out.append(np.argsort(d + s.rng.random(m))[:k])
+    return out
+
+
+class LoggingOperator:
+    """Wraps the plain operator mix to log (features, outcome) records: the
+    region the plain operator ruins is featurised before the ruin, the
+    round's outcome is attached after it."""
+
+    def __init__(self, scorer):
+        self.scorer = scorer
+        self.X, self.t = [], []
+        self._pending = None
+
+    def __call__(self, searcher, k, mode):
+        pick = searcher.ruin_pick(k, mode)
+        lay = self.scorer.layers(searcher.code)
+        self._pending = self.scorer.features(lay, [pick])[0]
+        return pick
+
+    def outcome(self, searcher, rec):
+        if self._pending is None:
+            return
+        pre, post = rec["pre"], rec["post"]
+        self.X.append(self._pending)
+        self.t.append(1.0 if post < pre else (0.5 if post == pre else 0.0))
+        self._pending = None
+
+
+class LearnedProposer:
+    """Scores ``ncand`` cluster regions with the network and ruins the best
+    (ties and exploration by a softmax at temperature ``tau``)."""
+
+    def __init__(self, net, scorer, ncand=32, tau=0.25):
+        self.net, self.scorer = net, scorer
+        self.ncand, self.tau = ncand, tau
+
+    def __call__(self, searcher, k, mode=None):
+        s = searcher
+        m = len(s.code)
+        anchors = s.rng.choice(m, size=min(self.ncand, m), replace=False)
+        regions = cluster_regions(s, k, anchors)
+        lay = self.scorer.layers(s.code)
+        y = self.net.forward(self.scorer.features(lay, regions))
+        z = (y - y.max()) / self.tau
+        p = np.exp(z) / np.exp(z).sum()
+        return regions[int(s.rng.choice(len(regions), p=p))]
+
+
+def collect_records(make_backend, seeds, rounds, kmin=2, kmax=6, log=None):
+    """Plain-LNS move logs: for every seed a notch descent whose every round
+    is logged by :class:`LoggingOperator`."""
+    X, t = [], []
+    for seed in seeds:
+        b = make_backend()
+        try:
+            sc = RegionScorer(b)
+            op = LoggingOperator(sc)
+            s = Searcher(b, seed=seed, on_round=op.outcome)
+            s.load(np.zeros(0, dtype=np.int64))
+            s.greedy_fill(10 ** 9)
+            s.peel()
+            used = 0
+            while used < rounds and s.uncov == 0 and len(s.code) > 1:
+                M = len(s.code) - 1
+                r0 = s.stats["rounds"]
+                s.ruin(1, "low")
+                s.lns(M, kmin=kmin, kmax=kmax, max_rounds=rounds - used,
+                      operator=op)
+                used += s.stats["rounds"] - r0
+            X.extend(op.X)
+            t.extend(op.t)
+            if log:
+                log(f"seed {seed}: {len(op.X)} records, size {len(s.code)}")
+        finally:
+            b.close()
+    return np.array(X, dtype=np.float32), np.array(t, dtype=np.float32)
+
+
+def train(net, X, t, epochs=20, batch=256, lr=2e-3, seed=0):
+    rng = np.random.default_rng(seed)
+    n = len(X)
+    losses = []
+    for _ in range(epochs):
+        order = rng.permutation(n)
+        tot, cnt = 0.0, 0
+        for lo in range(0, n, batch):
+            ix = order[lo:lo + batch]
+            tot += net.train_step(X[ix], t[ix], lr) * len(ix)
+            cnt += len(ix)
+        losses.append(tot / max(cnt, 1))
+    return losses
+
+
+def descend_timed(searcher, seconds, kmin=2, kmax=6, proposer=None):
+    """Notch descent under a wall-clock budget; returns [(t, feasible size)]
+    (t in seconds from the start of the descent)."""
+    s = searcher
+    t0 = time.perf_counter()
+    hist = [(0.0, len(s.code))]
+    while time.perf_counter() - t0 < seconds and s.uncov == 0 and len(s.code) > 1:
+        M = len(s.code) - 1
+        s.ruin(1, "low")
+        left = seconds - (time.perf_counter() - t0)
+        s.lns(M, round_budget_s=max(left, 0.0), kmin=kmin, kmax=kmax,
+              select="adaptive", proposer=proposer)
+        if s.uncov == 0:
+            hist.append((time.perf_counter() - t0, len(s.code)))
+    return hist

@@ -107,9 +107,6 @@ class Searcher:
         self._deficit_cache = (None, None)
         self.loop_reserve_s = 10.0
         self.build_reserve_s = 5.0
-        #: optional ruin-region proposer (callable(searcher, k) -> indices
-        #: into self.code, or None to fall back to the operator mix)
-        self.proposer = None
 
     # ------------------------------------------------ state
     def time_left(self):
@@ -361,11 +358,18 @@ class Searcher:
     # ------------------------------------------------ LNS at fixed M
     def lns(self, M, round_budget_s=np.inf, kmin=8, kmax=None,
             select="fixed", max_rounds=None, deficit=False, deficit_hint=3,
-            deficit_boost=4.0):
+            deficit_boost=4.0, proposer=None, operator=None):
         """Ruin/recreate rounds at fixed M until feasible, the budget or
         ``max_rounds`` (search.py:702-784).  Rejects a worse round with
         probability 0.9, restoring the best snapshot; re-verifies the field
-        every 50 rounds.  Returns the uncovered count."""
+        every 50 rounds.  Returns the uncovered count.
+
+        ``proposer`` (callable(searcher, k, mode) -> positions in the code)
+        adds a "learned" operator: weighted like ``hole`` in the fixed mix,
+        one more arm of the bandit under ``select="adaptive"``.
+        ``operator`` replaces the ruin's region choice of the drawn mode
+        (the move logger uses it); without either the pass draws exactly
+        as the reference does."""
         if select not in ("fixed", "adaptive"):
             raise ValueError(select)
         t0 = time.time()
@@ -376,10 +380,14 @@ class Searcher:
             self.greedy_fill(M)
         self._snapshot()
         draw = FIXED_DRAW_DEFICIT if deficit else FIXED_DRAW
+        ops, prior = (RUIN_MODES_EXT, RUIN_MIX_EXT) if deficit else (RUIN_MODES, RUIN_MIX)
+        if proposer is not None:
+            draw = draw + ("learned", "learned")
+            ops, prior = ops + ("learned",), dict(prior, learned=2.0)
         bandit = None
         if select == "adaptive":
-            bandit = (OperatorBandit(self.rng, ops=RUIN_MODES_EXT,
-                                     prior=RUIN_MIX_EXT) if deficit
+            bandit = (OperatorBandit(self.rng, ops=ops, prior=prior)
+                      if (deficit or proposer is not None)
                       else OperatorBandit(self.rng))
         hint = {"deficit": deficit_boost} if deficit else None
         r = 0
@@ -392,23 +400,24 @@ class Searcher:
             pre, pre_best = self.uncov, self.best_uncov
             k = int(self.rng.integers(kmin, kmax + 1))
             near = deficit and self.uncov <= deficit_hint
-            pick = None
-            if self.proposer is not None:
-                pick = self.proposer(self, min(k, len(self.code) - 1))
-            if pick is not None:
-                mode = "learned"
-                self.remove_at(np.asarray(pick, dtype=np.int64))
-            else:
-                if bandit is None:
-                    if near and self.rng.random() < 0.5:
-                        mode = "deficit"
-                    else:
-                        mode = draw[int(self.rng.integers(len(draw)))]
+            if bandit is None:
+                if near and self.rng.random() < 0.5:
+                    mode = "deficit"
                 else:
-                    mode = bandit.select(hint if near else None)
-                removed_before = self.code
+                    mode = draw[int(self.rng.integers(len(draw)))]
+            else:
+                mode = bandit.select(hint if near else None)
+            kk = min(k, len(self.code) - 1)
+            if mode == "learned" or operator is not None:
+                pick = np.zeros(0, dtype=np.int64)
+                if kk > 0:
+                    fn = proposer if mode == "learned" else operator
+                    pick = np.asarray(fn(self, kk, mode), dtype=np.int64)
+                    self.remove_at(pick)
+            else:
+                before = self.code
                 self.ruin(k, mode)
-                pick = np.flatnonzero(~np.isin(removed_before, self.code))
+                pick = np.flatnonzero(~np.isin(before, self.code))
             self.greedy_fill(M)
             self.stats["rounds"] += 1
             r += 1

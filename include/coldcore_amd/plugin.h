@@ -133,6 +133,15 @@ int  dct_init3w(int problem, int n, const int *ax, int R,
                 const uint64_t *widepat, long long balllen, int extract_cap,
                 long long *bytes, int layers_mode, int cnt_mode,
                 int use_owner);
+/* The transform of dct_transform(init_mode, 1) with its last sweep keeping
+ * the exact distance layers: after it, layer d at x = number of seed words
+ * at distance exactly d from x (init 0 codewords of set_code, 1 deficient,
+ * 2 critical).  cnt and the represented solution are unchanged, the map
+ * plane is overwritten.  Hamming and Lee only (-1 otherwise).  The layers
+ * feed the learned ruin proposal. */
+int  dct_transform_layers(int init_mode);
+/* out[i*(R+1) + d] = layer d at idx[i] after dct_transform_layers. */
+int  dct_layers_read_at(const long long *idx, int k, int32_t *out);
 /* Schedule of the LDS-tiled sweeps: 8 ints per sweep (first axis, axes in
  * the group, P, TL, layers in, layers out, axis size of the register-round
  * kernel or 0 for the per-axis kernel, LDS bytes); returns the sweep count.
@@ -143,6 +152,27 @@ int  dct_sweep_plan(int *out, int cap);
 int  dct_sweep_profile(int on);
 /* Per-sweep milliseconds of the last transform; returns the sweep count. */
 int  dct_sweep_times(float *ms, int cap);
+
+/* ---- additive: the learned ruin proposal's network -------------------
+ * No counterpart in the reference (its search has no learned operator).
+ * A 2-layer MLP, 16 fp32 inputs -> 256 relu -> 1 logit, trained with
+ * binary cross-entropy; W1 and w2 are used as bf16 (MFMA), fp32 masters
+ * and Adam state stay on the device.  One network per process; host
+ * buffers in and out.  Parameter layout: W1 (256 x 16 row-major), b1
+ * (256), w2 (256), b2 -- cc_mlp_param_count() floats. */
+int  cc_mlp_init(int max_batch);
+int  cc_mlp_free(void);
+int  cc_mlp_param_count(void);
+/* sets the fp32 masters and their bf16 copies; resets Adam */
+int  cc_mlp_set_params(const float *p);
+int  cc_mlp_get_params(float *p);
+/* y[i] = logit of row i of X (B x 16), B <= max_batch */
+int  cc_mlp_forward(const float *X, int B, float *y);
+/* gradient of the mean BCE of B rows with targets t in [0, 1]; g and loss
+ * are optional outputs (NULL skips the copy) */
+int  cc_mlp_grad(const float *X, const float *t, int B, float *g, float *loss);
+/* one Adam step on the gradient of the last cc_mlp_grad */
+int  cc_mlp_adam(float lr, float beta1, float beta2, float eps);
 
 #ifdef __cplusplus
 }

@@ -231,3 +231,24 @@ class NumpyDP(DenseQueries):
                 hit &= good
             out[lo:lo + step] = hit.sum(axis=1)
         return out
+
+    def layers_at(self, init_mode, idx, code=None):
+        """(k, R+1) exact distance layers at idx, seeded with the codewords
+        (init 0, ``code``), the deficient words (1) or the critical words
+        (2); cnt and the map are unchanged."""
+        if self.nlayers < 2:
+            raise ValueError("layers exist for the layered families only")
+        saved = self._A.copy()
+        if init_mode == 0:
+            self._A[0] = 0
+            if code is not None and len(code):
+                np.add.at(self._A[0], np.asarray(code, dtype=np.int64), 1)
+        else:
+            self._A[0] = self.role_mask(1 if init_mode == 2 else 0)
+        self._A[1:] = 0
+        step = {"hamming": self._pass_hamming, "lee": self._pass_lee}[self.problem]
+        for j in range(self.n):
+            step(j)
+        out = self._A[:, np.asarray(idx, dtype=np.int64)].T.astype(np.int32)
+        self._A[:] = saved
+        return out

@@ -93,3 +93,28 @@ class BruteForce(DenseQueries):
         idx = np.atleast_1d(np.asarray(idx, dtype=np.int64))
         return np.array([int(mask[self.ball_indices(w)].sum()) for w in idx],
                         dtype=np.int32)
+
+
+def distance(problem, axes, a, b):
+    """Exact metric distance for the layered families (brute force)."""
+    a = np.asarray(a, dtype=np.int64)
+    b = np.asarray(b, dtype=np.int64)
+    if problem == "hamming":
+        return (a != b).sum(axis=-1)
+    d = np.abs(a - b)
+    d = np.minimum(d, np.asarray(axes, dtype=np.int64) - d)
+    return d.sum(axis=-1)
+
+
+def layers_brute(problem, axes, R, seed_words, idx):
+    """layer d at x = #{s in seed (multiset) : distance(x, s) == d}."""
+    from .space import Space
+    sp = Space(axes)
+    X = sp.digits(np.asarray(idx, dtype=np.int64))
+    S = sp.digits(np.asarray(seed_words, dtype=np.int64))
+    out = np.zeros((len(X), R + 1), dtype=np.int64)
+    if len(S):
+        d = distance(problem, axes, X[:, None, :], S[None, :, :])
+        for r in range(R + 1):
+            out[:, r] = (d == r).sum(axis=1)
+    return out
