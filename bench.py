@@ -30,31 +30,31 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0        # MI355X HBM3E, published
 
 
-def sweep_bytes(plan, kind):
-    """Algorithmic bytes per word of each sweep of one pass.
+def sweep_bytes(plan):
+    """Algorithmic bytes per word of each sweep of one pass, from the
+    engine's own schedule (dct_sweep_plan2): what the sweep must read (its
+    seed -- the uint16 cnt predicate or the int32 code indicator -- or the
+    previous sweep's compact layer words) plus what it must write (compact
+    layer words, or the int32 map / uint16 cnt)."""
+    return [sw["bytes_in"] + sw["bytes_out"] for sw in plan]
 
-    kind 'field' (gain / loss: reads the uint16 cnt predicate, writes the
-    int32 map) or 'recount' (reads the int32 code indicator, writes uint16
-    cnt).  Inter-sweep layers are lb = 2 (uint16 planes, when every layer
-    value is provably < 2^16) or 4 bytes.  The first sweep reads its seed
-    and writes every live layer (layer 0 too, except an int32 recount whose
-    layer 0 stays in the code-indicator plane); middle sweeps read their
-    live layers and write those above 0; the last reads its live layers and
-    writes the result."""
-    out = []
-    ns = len(plan)
-    lb = plan[0]["layer_bytes_field" if kind == "field" else "layer_bytes_recount"]
-    for i, sw in enumerate(plan):
-        lin, lout = sw["layers_in"], sw["layers_out"]
-        rd = (2 if kind == "field" else 4) if i == 0 else lb * lin
-        if i == ns - 1:
-            wr = 4 if kind == "field" else 2
-        elif i == 0:
-            wr = lb * lout if (kind == "field" or lb == 2) else lb * (lout - 1)
-        else:
-            wr = lb * (lout - 1)
-        out.append(rd + wr)
-    return out
+
+def kernel_name(plan, i, q):
+    """The kernel a sweep launches (template instantiations are distinct
+    kernels to the profiler)."""
+    sw = plan[i]
+    if sw["W"] == 32:
+        return "k_sweep / k_sweep_r (unpacked layers)"
+    last = i == len(plan) - 1
+    return (f"k_psweep<Q={q},RR={sw['RR']},W={sw['W']},ND={sw['ND']},"
+            f"{'sum' if last else 'compact'}>")
+
+
+def describe(plan):
+    return [f"axes {s['first_axis']}..{s['first_axis'] + s['axes'] - 1}"
+            f" P={s['P']} TL={s['TL']} RR={s['RR']} W={s['W']} ND={s['ND']}"
+            f" {s['bytes_in']}+{s['bytes_out']} B/word LDS={s['lds_bytes']}"
+            for s in plan]
 
 
 def cpu_model():
@@ -91,8 +91,7 @@ def run_cpu_baseline(code, passes):
 
 
 def schedule_key(plan):
-    return " | ".join(f"axes {s['first_axis']}..{s['first_axis'] + s['axes'] - 1}"
-                      f" P={s['P']} TL={s['TL']}" for s in plan)
+    return " | ".join(describe(plan))
 
 
 def measured_traffic(plan):
@@ -179,13 +178,13 @@ def time_cells(GpuBackend, steps):
         g = GpuBackend(axes, R)
         try:
             g.recount(code.astype(np.int64))
-            plan = g.sweep_plan()
+            plan = g.sweep_plan2(0)
             g.gain_map()
             t = time.perf_counter()
             for _ in range(steps):
                 g.gain_map()
             ms = (time.perf_counter() - t) * 1e3 / steps
-            bpw = sum(sweep_bytes(plan, "field"))
+            bpw = sum(sweep_bytes(plan))
             out.append({"cell": name, "words": size, "code_words": int(len(code)),
                         "gain_pass_ms": round(ms, 4),
                         "passes_per_s": round(1e3 / ms, 1),
@@ -227,7 +226,8 @@ def main():
     g = GpuBackend([2] * n, R)
     try:
         g.recount(code)
-        plan = g.sweep_plan()
+        plan = g.sweep_plan2(0)
+        rplan = g.sweep_plan2(1)
         for _ in range(args.warmup):
             g.gain_map()
         t0 = time.perf_counter()
@@ -247,7 +247,7 @@ def main():
         g.sweep_profile(False)
         per = np.array(per)                       # (steps, sweeps) ms
         sweep_ms = per.mean(axis=0)
-        bpw = sweep_bytes(plan, "field")
+        bpw = sweep_bytes(plan)
         pass_bytes = sum(bpw) * size
         kernel_ms = float(per.sum(axis=1).mean())
         # the dominant kernel: sweeps sharing one kernel instantiation are
@@ -255,8 +255,7 @@ def main():
         # the largest time; per launch = its bytes and time over its launches
         groups = {}
         for i, sw in enumerate(plan):
-            key = (f"k_sweep_r Q={sw['round_q']}" if sw["round_q"]
-                   else "k_sweep")
+            key = kernel_name(plan, i, 2)
             gb = groups.setdefault(key, [0.0, 0, 0])
             gb[0] += float(sweep_ms[i])
             gb[1] += bpw[i] * size
@@ -277,8 +276,7 @@ def main():
                     fn()
                 extra[f"{name}_pass_ms"] = round(
                     (time.perf_counter() - t) * 100, 4)
-            rb = sweep_bytes(plan, "recount")
-            extra["recount_bytes_per_word"] = sum(rb)
+            extra["recount_bytes_per_word"] = sum(sweep_bytes(rplan))
             # 10^5 single-word updates, alternating +1 (random non-code word)
             # and -1 (random current codeword)
             rng = np.random.default_rng(1)
@@ -345,11 +343,7 @@ def main():
         "config": {
             "workload": "K_2(24,3) exact gain-field pass, 2^24 words, R=3",
             "axes": "[2]*24", "R": 3, "code_words": int(len(code)),
-            "schedule": [f"axes {s['first_axis']}..{s['first_axis'] + s['axes'] - 1}"
-                         f" P={s['P']} TL={s['TL']}"
-                         f" {'round-kernel Q=' + str(s['round_q']) if s['round_q'] else 'per-axis kernel'}"
-                         f" LDS={s['lds_bytes']}" for s in plan],
-            "layer_bytes": plan[0]["layer_bytes_field"],
+            "schedule": describe(plan),
         },
         "pass_bytes_algorithmic": int(pass_bytes),
         "pass_bytes_per_word": int(sum(bpw)),

@@ -44,3 +44,15 @@ hipError_t cc_sweep_linf(const SweepArgs &a, int prob, int blocks, size_t lds,
 // register-round kernel for homogeneous Hamming groups, axis size q <= 5
 hipError_t cc_sweep_round(const SweepArgs &a, int L, int q, int blocks,
                           size_t lds, bool l16, const LaunchCtx &lc);
+
+// packed-layer Hamming sweeps (cc_psweep.hpp), one unit per axis size Q:
+// RR axes per register round, W-bit fields in ND dwords per word
+struct PSweepArgs;
+hipError_t cc_psweep_q2(const PSweepArgs &a, int RR, int W, int ND, int blocks,
+                        int threads, size_t lds, hipStream_t st);
+hipError_t cc_psweep_q3(const PSweepArgs &a, int RR, int W, int ND, int blocks,
+                        int threads, size_t lds, hipStream_t st);
+hipError_t cc_psweep_q4(const PSweepArgs &a, int RR, int W, int ND, int blocks,
+                        int threads, size_t lds, hipStream_t st);
+hipError_t cc_psweep_q5(const PSweepArgs &a, int RR, int W, int ND, int blocks,
+                        int threads, size_t lds, hipStream_t st);

@@ -26,6 +26,7 @@
 #include "cc_reduce.hpp"
 #include "cc_launch.hpp"
 #include "cc_walk.hpp"
+#include "cc_psweep.hpp"
 
 #define CK(x)                                                               \
   do {                                                                      \
@@ -52,6 +53,25 @@ struct Sweep {
   int rq;       // round kernel: axis size Q of the homogeneous group, else 0
   size_t lds;   // LDS bytes with int32 layers
   size_t lds16; // LDS bytes in 16-bit mode (round kernel packs layer pairs)
+};
+
+// One sweep of the packed-layer path (cc_psweep.hpp).
+struct PSweep {
+  int j0, nax, RR, W, ND, P, TL, threads;
+  long long s, ncolb, ntiles;
+  int nrounds;
+  int ca[CC_PK_MAXRD], amask[CC_PK_MAXRD], qa[CC_PK_MAXRD];
+  int in_nd, out_nd;  // compact planes read / written (0: seed in, sum out)
+  int idw[CC_MAXL], ioff[CC_MAXL], ibits[CC_MAXL];
+  int odw[CC_MAXL], ooff[CC_MAXL], obits[CC_MAXL];
+  size_t lds;
+  int in_bytes, out_bytes;  // algorithmic bytes per word
+};
+
+struct PPlan {
+  bool ok = false;
+  std::vector<PSweep> sw;
+  long long bytes = 0;  // algorithmic bytes per word of one transform
 };
 
 struct Engine {
@@ -102,6 +122,8 @@ struct Engine {
   int ncu = 256;             // compute units
   int grid = -1;             // CC_SWEEP_GRID: workgroups per CU (0 = per tile)
   int generic_only = 0;      // CC_SWEEP_KERNEL=generic: no round kernel
+  int packed = 1;            // CC_PACKED=0: never take the packed path
+  PPlan pfield;              // packed plan of gain / loss (predicate seeds)
   // optional per-sweep timing of the last transform (additive ABI)
   int prof = 0;
   hipEvent_t ev[2 * CC_MAXN] = {};
@@ -188,6 +210,8 @@ int layers_after(int c) {
 int round_axes(int q) { return q == 2 ? 4 : (q <= 4 ? 2 : 1); }
 
 int padded(int i) { return i + (i >> 4); }
+// host copy of the packed kernels' LDS padding (pk_pad)
+int pk_pad_h(int i) { return i + (i >> 6); }
 
 void build_plan() {
   E.plan.clear();
@@ -321,6 +345,264 @@ int run_transform(int in_kind, int out_kind, int pmode) {
   }
   E.nswept = ns;
   return 0;
+}
+
+// ---- packed-layer schedule (Hamming, homogeneous axes, cc_psweep.hpp) ----
+// Bounds of the layer values for a seed whose words are <= s0 (1 for the
+// gain / loss predicates, the largest multiplicity for a code) and whose
+// total is <= cap (the code size; unbounded for a predicate): after c axes
+// layer d counts seed words at partial distance exactly d, at most
+// s0 * C(c,d) * (Q-1)^d of them, and a fiber sum along the next axis is at
+// most Q times that.  Every count is also <= cap.
+struct PBounds {
+  int Q, R;
+  double s0, cap;
+  double lay(int c, int d) const {
+    if (d > c) return 0;
+    double b = s0;
+    for (int i = 0; i < d; i++) b = b * (c - i) / (i + 1) * (Q - 1);
+    return std::min(b, cap);
+  }
+  double tsum(int c, int d) const { return std::min(lay(c, d) * Q, cap); }
+  // do W-bit fields hold every layer through axes [j, k) of the transform?
+  bool fits(int W, int k) const {
+    const double lim = std::ldexp(1.0, W) - 1;
+    for (int d = 0; d <= R; d++)
+      if (lay(k, d) > lim) return false;
+    if (Q > 2)
+      for (int d = 0; d < R; d++)
+        if (tsum(k - 1, d) > lim) return false;
+    return true;
+  }
+};
+
+int bitlen(double x) {
+  int b = 0;
+  while (b < 53 && std::ldexp(1.0, b) <= x) b++;
+  return b;
+}
+
+// compact HBM format of the layers after c axes: field d takes bitlen of its
+// bound, packed greedily into at most two dword planes
+bool compact_fmt(const PBounds &B, int c, int L, int *dw, int *off, int *bits,
+                 int *nd) {
+  int w = 0, used = 0;
+  for (int d = 0; d < L; d++) {
+    const int b = bitlen(B.lay(c, d));
+    if (b > 32) return false;
+    if (used + b > 32) { w++; used = 0; }
+    if (w > 1) return false;
+    dw[d] = w; off[d] = used; bits[d] = b;
+    used += b;
+  }
+  *nd = w + 1;
+  return true;
+}
+
+// largest register round of the kernel table for axis size q (cube of
+// q^RR words of nd dwords in at most 128 registers)
+int pk_rrmax(int q, int nd) {
+  const int table = q == 2 ? 6 : (q == 3 ? 4 : 3);
+  int rr = 0;
+  long long c = 1;
+  while (rr < table && c * q * nd <= 128) { c *= q; rr++; }
+  return rr;
+}
+
+// Rounds of one sweep: round 0 is the cube of the group's lowest RR axes
+// (contiguous in the first sweep: vector loads), middle rounds take the
+// axes above in ascending chunks, the last round the top RR axes (whose
+// cube elements are P / C words apart: coalesced stores in the first sweep).
+void pk_rounds(PSweep &w, int q) {
+  const int m = w.nax, RR = w.RR;
+  int nr = 0;
+  auto add = [&](int ca, int mask) {
+    int qa = 1;
+    for (int i = 0; i < ca; i++) qa *= q;
+    w.ca[nr] = ca; w.amask[nr] = mask; w.qa[nr] = qa;
+    nr++;
+  };
+  add(0, (1 << RR) - 1);
+  if (m > RR) {
+    const int top = m - RR;
+    int done = RR;
+    for (int lo = RR; lo < top; lo += RR) {
+      const int hi = std::min(lo + RR, top);
+      add(lo, (1 << (hi - lo)) - 1);
+      done = hi;
+    }
+    int mask = 0;
+    for (int i = 0; i < RR; i++)
+      if (top + i >= done) mask |= 1 << i;
+    add(top, mask);
+  }
+  w.nrounds = nr;
+}
+
+// One sweep over axes [j, k); prev is the sweep before it (its compact
+// output is this sweep's input).
+bool pk_sweep(const PBounds &B, int j, int k, bool pred_seed, bool cnt_out,
+              const PSweep *prev, PSweep &w) {
+  const int q = B.Q, L = E.L, m = k - j;
+  memset(&w, 0, sizeof(w));
+  w.j0 = j; w.nax = m;
+  w.s = E.pw[j];
+  const long long P = E.pw[k] / E.pw[j];
+  if (P > (1LL << 20) || P * w.s > (1LL << 30)) return false;
+  w.P = (int)P;
+  if (B.fits(8, k)) { w.W = 8; w.ND = (L + 3) / 4; }
+  else if (B.fits(16, k)) { w.W = 16; w.ND = (L + 1) / 2; }
+  else return false;
+  if (w.ND > 4 || (w.W == 8 && w.ND > 2)) return false;
+  const int rrmax = pk_rrmax(q, w.ND);
+  if (rrmax < 1) return false;
+  w.RR = std::min(rrmax, m);
+  int C = 1;
+  for (int i = 0; i < w.RR; i++) C *= q;
+  if (j == 0) {
+    w.TL = 1;
+    w.lds = m > w.RR ? (size_t)w.ND * (pk_pad_h(w.P) + 1) * 4 : 0;
+    if (w.lds > (size_t)E.lds_budget) return false;
+    const int ncube = w.P / C;
+    w.threads = std::min(256, (ncube + 63) / 64 * 64);
+  } else if (m == w.RR) {
+    w.TL = (int)std::min<long long>(256, w.s);
+    w.lds = 0;
+    w.threads = std::min(256, (w.TL + 63) / 64 * 64);
+  } else {
+    w.TL = 0;
+    for (int tl : {64, 32, 16}) {
+      if (tl > w.s) continue;
+      const size_t lds = (size_t)w.ND * (pk_pad_h(w.P * tl) + 1) * 4;
+      if (lds <= (size_t)E.lds_budget) { w.TL = tl; w.lds = lds; break; }
+    }
+    if (!w.TL) return false;
+    w.threads = std::min(256, (w.P / C * w.TL + 63) / 64 * 64);
+  }
+  w.ncolb = (w.s + w.TL - 1) / w.TL;
+  w.ntiles = (E.X / (w.s * P)) * w.ncolb;
+  if (prev) {
+    w.in_nd = prev->out_nd;
+    for (int d = 0; d < L; d++) {
+      w.idw[d] = prev->odw[d]; w.ioff[d] = prev->ooff[d]; w.ibits[d] = prev->obits[d];
+    }
+    w.in_bytes = 4 * w.in_nd;
+  } else {
+    w.in_nd = 0;
+    w.in_bytes = pred_seed ? 2 : 4;
+  }
+  if (k == E.n) {
+    w.out_nd = 0;
+    w.out_bytes = cnt_out ? 2 : 4;
+  } else {
+    if (!compact_fmt(B, k, L, w.odw, w.ooff, w.obits, &w.out_nd)) return false;
+    if (w.out_nd > L - 1) return false;   // compact planes live in `lay`
+    w.out_bytes = 4 * w.out_nd;
+  }
+  pk_rounds(w, q);
+  return true;
+}
+
+// Schedule with the fewest algorithmic bytes: every first-sweep size, then
+// the strided sweeps greedily as long as possible.
+PPlan pk_plan(double s0, double cap, bool pred_seed, bool cnt_out) {
+  PPlan best;
+  if (!E.packed || E.prob != CC_HAMMING || E.L < 2) return best;
+  const int q = E.ax[0];
+  for (int i = 1; i < E.n; i++)
+    if (E.ax[i] != q) return best;
+  if (q < 2 || q > 5) return best;
+  const PBounds B{q, E.R, s0, cap};
+  for (int m0 = std::min(E.n, CC_MAXSW); m0 >= 1; m0--) {
+    PPlan pl;
+    PSweep w;
+    if (!pk_sweep(B, 0, m0, pred_seed, cnt_out, nullptr, w)) continue;
+    pl.sw.push_back(w);
+    int j = m0;
+    bool ok = true;
+    while (j < E.n && ok) {
+      ok = false;
+      for (int k = std::min(E.n, j + CC_MAXSW); k > j; k--) {
+        if (pk_sweep(B, j, k, pred_seed, cnt_out, &pl.sw.back(), w)) {
+          pl.sw.push_back(w);
+          j = k;
+          ok = true;
+          break;
+        }
+      }
+    }
+    if (!ok) continue;
+    pl.ok = true;
+    for (const PSweep &x : pl.sw) pl.bytes += x.in_bytes + x.out_bytes;
+    if (!best.ok || pl.bytes < best.bytes ||
+        (pl.bytes == best.bytes && pl.sw.size() < best.sw.size()))
+      best = pl;
+  }
+  return best;
+}
+
+hipError_t pk_launch(const PSweep &w, const PSweepArgs &a) {
+  const int blocks = (int)std::min<long long>(w.ntiles, 1LL << 30);
+  switch (E.ax[0]) {
+    case 2: return cc_psweep_q2(a, w.RR, w.W, w.ND, blocks, w.threads, w.lds, E.st);
+    case 3: return cc_psweep_q3(a, w.RR, w.W, w.ND, blocks, w.threads, w.lds, E.st);
+    case 4: return cc_psweep_q4(a, w.RR, w.W, w.ND, blocks, w.threads, w.lds, E.st);
+    case 5: return cc_psweep_q5(a, w.RR, w.W, w.ND, blocks, w.threads, w.lds, E.st);
+  }
+  return hipErrorInvalidValue;
+}
+
+// Run a packed transform: in_kind CC_IN_PRED (predicate [plo <= cnt <= phi])
+// or CC_IN_CODE (A0), out_kind CC_OUT_MAP or CC_OUT_CNT.
+int run_ptransform(const PPlan &pl, int in_kind, int out_kind, int plo,
+                   int phi) {
+  const int ns = (int)pl.sw.size();
+  E.layers_live = 0;
+  E.nswept = 0;
+  for (int i = 0; i < ns; i++) {
+    const PSweep &w = pl.sw[i];
+    PSweepArgs a;
+    memset(&a, 0, sizeof(a));
+    a.cnt = E.cnt;
+    a.code = E.A[0];
+    a.pk = (uint32_t *)E.lay;
+    a.map = E.A[0];
+    a.cnt_out = E.cnt;
+    a.pitch = E.pitch;
+    a.s = w.s; a.ncolb = w.ncolb; a.ntiles = w.ntiles;
+    a.P = w.P; a.TL = w.TL; a.nax = w.nax; a.L = E.L;
+    a.in_kind = i == 0 ? in_kind : CC_IN_PACKED;
+    a.out_kind = i == ns - 1 ? out_kind : CC_OUT_PACKED;
+    a.plo = plo; a.phi = phi;
+    a.in_nd = w.in_nd; a.out_nd = w.out_nd;
+    a.nrounds = w.nrounds;
+    for (int r = 0; r < w.nrounds; r++) {
+      a.ca[r] = w.ca[r]; a.amask[r] = w.amask[r]; a.qa[r] = w.qa[r];
+    }
+    const int S = 32 / w.W;
+    for (int k = 0; k < w.ND; k++) {
+      uint32_t fm = 0;
+      for (int sl = 0; sl < S; sl++)
+        if (k * S + sl < E.L) fm |= (uint32_t)((1ull << w.W) - 1) << (w.W * sl);
+      a.fmask[k] = fm;
+    }
+    for (int d = 0; d < E.L; d++) {
+      a.idw[d] = w.idw[d]; a.ioff[d] = w.ioff[d]; a.ibits[d] = w.ibits[d];
+      a.odw[d] = w.odw[d]; a.ooff[d] = w.ooff[d]; a.obits[d] = w.obits[d];
+    }
+    if (E.prof) CK(hipEventRecord(E.ev[2 * i], E.st));
+    CK(pk_launch(w, a));
+    if (E.prof) CK(hipEventRecord(E.ev[2 * i + 1], E.st));
+  }
+  E.nswept = ns;
+  return 0;
+}
+
+// packed plan of a code seed: multiplicities of the uploaded multiset
+PPlan pk_code_plan(bool cnt_out) {
+  int mmax = 0;
+  for (const auto &kv : E.uploaded) mmax = std::max(mmax, kv.second);
+  return pk_plan(std::max(mmax, 1), std::max(E.codeM, 1), false, cnt_out);
 }
 
 int sync_stream() {
@@ -494,6 +776,7 @@ void release_all() {
   E.mu = 1;
   E.fmt = 0;
   E.nswept = 0;
+  E.pfield = PPlan();
 }
 
 }  // namespace
@@ -669,6 +952,11 @@ static int init_cell(int problem, int n, const int *ax, int R,
     E.l16_count = on;          // layer values never exceed M <= 65,535
   }
   build_plan();
+  {
+    const char *pe = getenv("CC_PACKED");
+    E.packed = !(pe && !strcmp(pe, "0")) && !E.generic_only;
+    E.pfield = pk_plan(1.0, 1e300, true, false);
+  }
   E.uploaded.clear();
   E.active.clear();
   E.active_size = 0;
@@ -772,9 +1060,17 @@ int dct_transform(int init_mode, int out_mode) {
     pmode = CC_CRIT;
   }
   E.layers_live = 0;
-  if (int rc = run_transform(in_kind, out_mode == 1 ? CC_OUT_MAP : CC_OUT_CNT,
-                             pmode))
+  const int out_kind = out_mode == 1 ? CC_OUT_MAP : CC_OUT_CNT;
+  PPlan code_plan;
+  if (in_kind == CC_IN_CODE) code_plan = pk_code_plan(out_kind == CC_OUT_CNT);
+  const PPlan &pp = in_kind == CC_IN_PRED ? E.pfield : code_plan;
+  if (pp.ok) {
+    const int plo = pmode == CC_CRIT ? 1 : 0;
+    const int phi = pmode == CC_CRIT ? E.mu : E.mu - 1;
+    if (int rc = run_ptransform(pp, in_kind, out_kind, plo, phi)) return rc;
+  } else if (int rc = run_transform(in_kind, out_kind, pmode)) {
     return rc;
+  }
   CK(hipStreamSynchronize(E.st));
   if (E.prof)
     for (int i = 0; i < E.nswept; i++)
@@ -1101,6 +1397,51 @@ int dct_sweep_plan(int *out, int cap) {
   if (ns < cap) {
     out[8 * ns + 0] = E.l16_field ? 2 : 4;
     out[8 * ns + 1] = E.l16_count ? 2 : 4;
+  }
+  return ns;
+}
+
+// dct_sweep_plan2(kind, out, cap): the schedule a transform of this kind
+// runs now (kind 0: gain / loss, 1: recount of the pending code), 12 ints
+// per sweep: first axis, axes, P, TL, register round RR (0: unpacked
+// kernels), field width W (32: unpacked layer planes), dwords per word in
+// registers, algorithmic bytes per word read and written, LDS bytes per
+// workgroup, threads per workgroup, rounds.  Returns the sweep count.
+int dct_sweep_plan2(int kind, int *out, int cap) {
+  if (!E.live || !out || kind < 0 || kind > 1) return -1;
+  PPlan cp;
+  if (kind == 1) cp = pk_code_plan(true);
+  const PPlan &pp = kind == 0 ? E.pfield : cp;
+  if (pp.ok) {
+    const int ns = (int)pp.sw.size();
+    for (int i = 0; i < ns && i < cap; i++) {
+      const PSweep &w = pp.sw[i];
+      int *o = out + 12 * i;
+      o[0] = w.j0; o[1] = w.nax; o[2] = w.P; o[3] = w.TL; o[4] = w.RR;
+      o[5] = w.W; o[6] = w.ND; o[7] = w.in_bytes; o[8] = w.out_bytes;
+      o[9] = (int)w.lds; o[10] = w.threads; o[11] = w.nrounds;
+    }
+    return ns;
+  }
+  const int ns = (int)E.plan.size();
+  const bool field = kind == 0;
+  const int lb = (field ? E.l16_field : E.l16_count) ? 2 : 4;
+  for (int i = 0; i < ns && i < cap; i++) {
+    const Sweep &sw = E.plan[i];
+    const int lin = layers_after(sw.c0), lout = sw.lmax;
+    int rd = i == 0 ? (field ? 2 : 4) : lb * lin;
+    int wr;
+    if (i == ns - 1) wr = field ? 4 : 2;
+    else if (i == 0) wr = (field || lb == 2) ? lb * lout : lb * (lout - 1);
+    else wr = lb * (lout - 1);
+    if (E.L == 1) {           // single plane: int32 in and out
+      rd = i == 0 ? (field ? 2 : 4) : 4;
+      wr = i == ns - 1 ? (field ? 4 : 2) : 4;
+    }
+    int *o = out + 12 * i;
+    o[0] = sw.j0; o[1] = sw.nax; o[2] = sw.P; o[3] = sw.TL; o[4] = 0;
+    o[5] = 32; o[6] = E.L; o[7] = rd; o[8] = wr;
+    o[9] = (int)(E.l16_field ? sw.lds16 : sw.lds); o[10] = CC_BLOCK; o[11] = 0;
   }
   return ns;
 }

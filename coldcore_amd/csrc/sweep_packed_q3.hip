@@ -1,0 +1,7 @@
+// sweep_packed_q3.hip -- packed-layer Hamming sweeps for axis size 3.
+#include "cc_psweep_inst.hpp"
+
+hipError_t cc_psweep_q3(const PSweepArgs &a, int RR, int W, int ND,
+                        int blocks, int threads, size_t lds, hipStream_t st) {
+  return pk_by_rr<3, 4>(a, RR, W, ND, blocks, threads, lds, st);
+}

@@ -31,8 +31,9 @@ OPTIONAL = ("dct_map_hist", "dct_set_mu", "dct_get_mu", "dct_ball_gather_mu",
             "dct_count_deficit", "dct_host_bytes", "dct_loss_owner",
             "dct_set_tbl2", "dct_use_fmt", "dct_ball_update_delta",
             "dct_swap_eval", "dct_stats")
-ADDITIVE = ("dct_init3w", "dct_sweep_plan", "dct_sweep_profile",
-            "dct_sweep_times", "dct_transform_layers", "dct_layers_read_at")
+ADDITIVE = ("dct_init3w", "dct_sweep_plan", "dct_sweep_plan2",
+            "dct_sweep_profile", "dct_sweep_times", "dct_transform_layers",
+            "dct_layers_read_at")
 
 
 def _rc_msg(rc):
@@ -66,7 +67,7 @@ def _bind(path):
               "dct_read_cnt", "dct_map_hist", "dct_set_mu", "dct_get_mu",
               "dct_ball_gather_mu", "dct_loss_owner", "dct_set_tbl2",
               "dct_use_fmt", "dct_stats", "dct_sweep_plan",
-              "dct_sweep_profile", "dct_sweep_times", "dct_transform_layers",
+              "dct_sweep_plan2", "dct_sweep_profile", "dct_sweep_times", "dct_transform_layers",
               "dct_layers_read_at"):
         getattr(lib, f).restype = ctypes.c_int
     for f in ("dct_count_eq", "dct_count_deficit", "dct_map_extract",
@@ -376,6 +377,18 @@ class GpuBackend:
             sw["layer_bytes_field"] = buf[8 * ns]
             sw["layer_bytes_recount"] = buf[8 * ns + 1]
         return out
+
+    def sweep_plan2(self, kind=0):
+        """Schedule a transform runs now: kind 0 gain / loss, 1 recount of
+        the pending code.  Per sweep: axes, tile, register round and field
+        format (W = 32: unpacked planes), algorithmic bytes per word."""
+        buf = (ctypes.c_int * (12 * 41))()
+        ns = self._lib.dct_sweep_plan2(kind, buf, 40)
+        if ns < 0:
+            _check("sweep_plan2", ns)
+        keys = ("first_axis", "axes", "P", "TL", "RR", "W", "ND", "bytes_in",
+                "bytes_out", "lds_bytes", "threads", "rounds")
+        return [dict(zip(keys, buf[12 * i:12 * i + 12])) for i in range(ns)]
 
     def sweep_profile(self, on=True):
         _check("sweep_profile", self._lib.dct_sweep_profile(1 if on else 0))

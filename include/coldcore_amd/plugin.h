@@ -147,6 +147,16 @@ int  dct_layers_read_at(const long long *idx, int k, int32_t *out);
  * kernel or 0 for the per-axis kernel, LDS bytes); returns the sweep count.
  * For benchmarks and the roofline accounting of each sweep's traffic. */
 int  dct_sweep_plan(int *out, int cap);
+/* The schedule a transform runs now (kind 0: gain / loss; 1: recount of the
+ * pending code): 12 ints per sweep -- first axis, axes, P, TL, register
+ * round RR (0: unpacked per-layer kernels), field width W of the packed
+ * layers (32: unpacked planes), dwords per word in registers, algorithmic
+ * bytes per word read and written, LDS bytes, threads per workgroup,
+ * rounds.  Returns the sweep count.  Homogeneous Hamming cells with axis
+ * size <= 5 run packed-layer sweeps: all R+1 layers of a word in bit
+ * fields of one or two dwords, with field widths proven from closed-form
+ * layer bounds, so inter-sweep traffic is 4-8 bytes per word. */
+int  dct_sweep_plan2(int kind, int *out, int cap);
 /* Record HIP events around every sweep of later transforms (on the
  * engine's stream; also enabled by DCT_TIMING=1 at init). */
 int  dct_sweep_profile(int on);

@@ -1,0 +1,92 @@
+"""Packed-layer sweeps: every field bit-exact against the CPU oracle on the
+cell shapes that select each register format, compact inter-sweep format
+and tile shape.
+
+The packed path keeps all R+1 distance layers of a word in bit fields of
+one to four dwords (8-bit fields when the closed-form layer bounds allow,
+16-bit otherwise), so a wrong bound, a field that overflows into its
+neighbour or a wrong shift shows up as a wrong count somewhere in the
+field.  Shapes: binary cells up to R = 6 (one and two dword formats, 8- and
+16-bit fields), q = 3, 4, 5 (fiber sums T in the fields), multiset codes
+(seed values above 1), mu = 2 predicates, and small LDS budgets that force
+short first sweeps and multi-round strided tiles.
+"""
+import numpy as np
+import pytest
+
+from oracle.dp import NumpyDP
+
+pytestmark = pytest.mark.gpu
+
+CELLS = [([2] * 16, 3), ([2] * 14, 4), ([2] * 12, 6), ([2] * 13, 1),
+         ([3] * 9, 3), ([3] * 8, 1), ([4] * 7, 3), ([5] * 6, 3), ([5] * 6, 2)]
+IDS = [f"{a[0]}^{len(a)}-R{r}" for a, r in CELLS]
+BUDGETS = {"default": None, "lds16k": "16384", "lds4k": "4096"}
+
+
+def _check_cell(g, c, rng, mults):
+    size = c.space.size
+    every = np.arange(size, dtype=np.int64)
+    for m in (max(2, size // 400), max(3, size // 60)):
+        base = rng.integers(0, size, size=m).astype(np.int64)
+        code = np.repeat(base, rng.integers(1, mults + 1, size=m))
+        g.recount(code)
+        c.recount(code)
+        assert (g.read_cnt() == c.read_cnt()).all(), ("cnt", m)
+        for which in ("gain_map", "loss_map"):
+            getattr(g, which)()
+            getattr(c, which)()
+            assert (g.map_read_at(every) == c.map_read_at(every)).all(), (which, m)
+
+
+@pytest.mark.parametrize("budget", list(BUDGETS))
+@pytest.mark.parametrize("axes,R", CELLS, ids=IDS)
+def test_packed_fields_match_oracle(gpu_cell, monkeypatch, axes, R, budget):
+    if BUDGETS[budget]:
+        monkeypatch.setenv("CC_SWEEP_LDS", BUDGETS[budget])
+    rng = np.random.default_rng(len(axes) * 10 + R)
+    g = gpu_cell(axes, R)
+    plan = g.sweep_plan2(0)
+    assert all(sw["W"] in (8, 16) for sw in plan), plan   # packed path taken
+    assert sum(sw["axes"] for sw in plan) == len(axes)
+    _check_cell(g, NumpyDP(axes, R), rng, mults=3)
+
+
+@pytest.mark.parametrize("axes,R", [([2] * 12, 3), ([3] * 7, 2)],
+                         ids=["2^12-R3", "3^7-R2"])
+def test_packed_mu2_predicates(gpu_cell, axes, R):
+    rng = np.random.default_rng(7)
+    g = gpu_cell(axes, R, mu=2)
+    _check_cell(g, NumpyDP(axes, R, mu=2), rng, mults=2)
+
+
+def test_packed_equals_unpacked_engine_k2_20_3(monkeypatch):
+    """The two GPU paths agree on a 2^20-word cell (gain, loss, recount)."""
+    from coldcore_amd.gpu import GpuBackend
+    axes, R = [2] * 20, 3
+    code = np.flatnonzero(np.random.default_rng(3).random(1 << 20) < 3e-4)
+    out = {}
+    for mode in ("1", "0"):
+        monkeypatch.setenv("CC_PACKED", mode)
+        with GpuBackend(axes, R) as g:
+            packed = any(sw["W"] != 32 for sw in g.sweep_plan2(0))
+            assert packed == (mode == "1")
+            g.recount(code)
+            cnt = g.read_cnt().copy()
+            g.gain_map()
+            gain = g.map_all().copy()
+            g.loss_map()
+            out[mode] = (cnt, gain, g.map_all().copy())
+    for a, b in zip(out["1"], out["0"]):
+        assert (a == b).all()
+
+
+def test_k2_24_3_schedule_is_three_packed_sweeps(gpu_cell):
+    """K_2(24,3): axes 0-11 with 8-bit fields in one dword, then two
+    six-axis strided sweeps with 16-bit fields; compact inter-sweep words of
+    4 bytes, so a gain pass moves 2+4 + 4+4 + 4+4 = 22 bytes per word."""
+    g = gpu_cell([2] * 24, 3)
+    plan = g.sweep_plan2(0)
+    assert [(s["first_axis"], s["axes"]) for s in plan] == [(0, 12), (12, 6), (18, 6)]
+    assert [s["W"] for s in plan] == [8, 16, 16]
+    assert sum(s["bytes_in"] + s["bytes_out"] for s in plan) == 22

@@ -14,10 +14,13 @@ from oracle.dp import NumpyDP
 pytestmark = pytest.mark.gpu
 
 
-# schedule variants: the default plan (register-round kernel on homogeneous
-# Hamming groups), the per-axis kernel only, and a tiny LDS budget that
-# forces one- and two-axis sweeps (the multi-sweep paths on small cells)
-VARIANTS = {"default": {}, "generic": {"CC_SWEEP_KERNEL": "generic"},
+# schedule variants: the default plan (packed-layer sweeps on homogeneous
+# Hamming cells with q <= 5, the register-round kernel elsewhere), the
+# unpacked register-round kernels, the per-axis kernel only, and a tiny LDS
+# budget that forces short sweeps and multi-round strided tiles (the
+# multi-sweep paths on small cells)
+VARIANTS = {"default": {}, "unpacked": {"CC_PACKED": "0"},
+            "generic": {"CC_SWEEP_KERNEL": "generic"},
             "small-tiles": {"CC_SWEEP_LDS": "4096"}}
 
 
