@@ -147,12 +147,18 @@ def copy_calibration(mib):
     assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(n)) == 0
     assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(n)) == 0
     assert hip.hipMemset(a, 1, ctypes.c_size_t(n)) == 0
-    for _ in range(3):
-        assert hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3) == 0   # D2D
     assert hip.hipDeviceSynchronize() == 0
+    times = []
+    for _ in range(3):
+        t = time.perf_counter()
+        assert hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3) == 0   # D2D
+        assert hip.hipDeviceSynchronize() == 0
+        times.append(time.perf_counter() - t)
     hip.hipFree(a)
     hip.hipFree(b)
-    print(json.dumps({"calib_copy_bytes": n, "copies": 3}), flush=True)
+    print(json.dumps({"calib_copy_bytes": n, "copies": 3,
+                      "copy_GBps_read_plus_write": round(2 * n / min(times) / 1e9, 1)}),
+          flush=True)
 
 
 #: further configured cells, timed as gain passes (name, axes, R, code recipe)

@@ -1,39 +1,42 @@
 // cc_psweep.hpp -- packed-layer sweeps of the Hamming transform.
 //
 // The R+1 distance layers of one word travel together in ND dwords of W-bit
-// fields (W = 8 or 16): layer d sits in dword d / (32/W), slot d % (32/W);
-// slots above R are don't-care.  The host proves from closed-form bounds
-// that every layer, and every fiber sum T_d with d < R, stays below 2^W for
-// the whole sweep, so whole-dword integer adds never carry between layers.
-// The Hamming fiber recurrence of one axis
+// fields (W = 8 or 16), interleaved: layer d sits in dword d % ND, slot
+// d / ND.  Slots whose layer index is above R are don't-care.  The host
+// proves from closed-form bounds that every layer, and every fiber sum T_d
+// with d < R, stays below 2^W for the whole sweep, so whole-dword integer
+// adds never carry between layers.  The Hamming fiber recurrence of one axis
 //     newA_d[a] = A_d[a] + T_{d-1} - A_{d-1}[a],   T_d = sum_a A_d[a]
 // (reference src/plugins/covering/dctcore_hamming.cuh:4-8) becomes, per word
 // of the fiber,
 //     v[a] += up(T - v[a])        (T = sum_a v[a], fieldwise)
-// where up() moves every field one slot up (layer d -> d + 1): a shift of the
-// top dword and a funnel shift (alignbit) of the lower ones.  For Q = 2,
-// T - v[a] is the other word of the pair, so a binary axis is one add of
-// the partner shifted up.  Field R of T may overflow: its carry and borrow
-// only reach don't-care slots or leave the top dword, and up() drops it.
+// where up() moves every layer d to d + 1.  With the interleaved layout that
+// is one add per dword: dword k (k > 0) receives dword k-1 at the same
+// slots, dword 0 receives dword ND-1 one slot up.  For Q = 2, T - v[a] is the
+// other word of the pair, so a binary axis costs ND adds per word.  Field R
+// of T may overflow: its carry and borrow only reach don't-care slots (the
+// slots above a real layer in the same dword hold higher layers) or leave
+// the dword, and up() drops layer R.
 //
-// Between sweeps the layers go to HBM in a compact format: field d takes
-// exactly the bits its bound needs (layer d after c axes of a seed <= s0 is
-// at most s0 * C(c,d) * (Q-1)^d), packed into one or two dword planes
-// (plane-major, so every access is a coalesced dword).  For K_2(24,3) the
-// layers after 12 and 18 axes need 20 and 24 bits: 4 bytes per word instead
-// of 8 or 16.
+// Between sweeps the layers go to HBM in a compact format: either one byte
+// per layer in one dword (when R <= 3 and every bound is <= 255), or field d
+// taking exactly the bits its bound needs (layer d after c axes of a seed
+// <= s0 is at most s0 * C(c,d) * (Q-1)^d), packed into one or two dword
+// planes (plane-major, so every access is a coalesced dword).  For K_2(24,3)
+// the layers after 12 and 18 axes need 20 and 24 bits: 4 bytes per word.
+// The byte format converts to and from the register formats with byte
+// permutes (v_perm_b32).
 //
-// Tile and rounds: a sweep applies the passes of nax consecutive axes.  As
-// in the unpacked round kernel, a thread holds a cube of RR group axes
-// (C = Q^RR words) in registers and applies those passes there; rounds
-// exchange through LDS.  Round 0 takes the top RR axes, whose cube elements
-// are P / C words apart, so consecutive threads read consecutive words even
-// in the contiguous first sweep; the later rounds take the lower axes in
-// ascending chunks.  A contiguous first sweep with several rounds writes its
-// result back to LDS and stores it in the round-0 shape (coalesced).  A
-// strided sweep stages TL consecutive columns per row, so a wave's accesses
-// are coalesced in every round; when the whole group fits one cube the
-// sweep needs no LDS at all (one fiber cube per thread).
+// Tile and rounds: a sweep applies the passes of nax consecutive axes.  A
+// thread holds a cube of RR group axes (C = Q^RR words) in registers and
+// applies those passes there; rounds exchange through LDS.  Round 0 takes
+// the group's lowest RR axes (contiguous words in the first sweep: vector
+// loads), the middle rounds the axes above in ascending chunks, the last
+// round the top RR axes, whose cube elements are P / C words apart, so
+// consecutive threads store consecutive words even in the contiguous first
+// sweep.  A strided sweep stages TL consecutive columns per row, so a
+// wave's accesses are coalesced in every round; when the whole group fits
+// one cube the sweep needs no LDS at all (one fiber cube per thread).
 #pragma once
 #include "cc_common.hpp"
 
@@ -48,6 +51,7 @@ struct PSweepArgs {
   int32_t *map;          // CC_OUT_MAP
   uint16_t *cnt_out;     // CC_OUT_CNT
   long long pitch;       // dwords between compact planes
+  long long X;           // words of every plane (buffer bounds)
   long long s;           // stride (words) of the group's first axis
   long long ncolb;       // column blocks per outer index = ceil(s / TL)
   long long ntiles;
@@ -55,10 +59,14 @@ struct PSweepArgs {
   int in_kind, out_kind;
   int plo, phi;          // CC_IN_PRED: [plo <= cnt <= phi]
   int in_nd, out_nd;     // compact planes read / written
+  int in_w8, out_w8;     // compact format is one byte per layer
+  uint32_t out_bmask;    // byte format: bytes of real layers
   int nrounds;
   int ca[CC_PK_MAXRD];     // lowest group axis of each round's cube
   int amask[CC_PK_MAXRD];  // cube axes the round applies (bit i: axis ca + i)
   int qa[CC_PK_MAXRD];     // Q^ca: in-tile stride of the round's cube
+  int lin[CC_PK_MAXRD];    // round's LDS addresses are e0 + c * kp
+  int kp[CC_PK_MAXRD];
   uint32_t fmask[4];       // real-layer slots of each register dword
   int idw[CC_MAXL], ioff[CC_MAXL], ibits[CC_MAXL];
   int odw[CC_MAXL], ooff[CC_MAXL], obits[CC_MAXL];
@@ -76,14 +84,13 @@ __device__ __forceinline__ int pk_pad(int i) { return i + (i >> 6); }
     if (((c) & 15) == 15) __builtin_amdgcn_sched_barrier(0);            \
   } while (0)
 
-// x += up(y): every field of y moved one slot up, added fieldwise
+// x += up(y): every layer of y moved to the next layer index, added
 template <int W, int ND>
 __device__ __forceinline__ void pk_add_up(uint32_t (&x)[ND],
                                           const uint32_t (&y)[ND]) {
 #pragma unroll
-  for (int k = ND - 1; k > 0; k--)
-    x[k] += __builtin_amdgcn_alignbit(y[k], y[k - 1], 32 - W);
-  x[0] += y[0] << W;
+  for (int k = ND - 1; k > 0; k--) x[k] += y[k - 1];
+  x[0] += y[ND - 1] << W;
 }
 
 // one axis of the cube: fibers of Q words ST apart
@@ -128,28 +135,63 @@ __device__ __forceinline__ void pk_apply(uint32_t (&v)[C][ND], int mask) {
   }
 }
 
-// Global addressing: the word of cube element c is ub + lane + c * usq,
-// where ub (the tile's first word) is workgroup-uniform and lane + c * usq
-// is a 32-bit element offset (the host checks that every offset inside a
-// tile, in bytes, stays below 2^32).  The tile base is read once into
-// scalar registers (readfirstlane makes it opaque, so the compiler keeps
-// it there) and every access is the scalar-base + 32-bit VGPR-offset form.
-#define CC_GLOBAL __attribute__((address_space(1)))
+// Global addressing: raw buffer operations.  The word of cube element c is
+// ub + lane + c * usq: the tile base ub is workgroup-uniform and becomes a
+// buffer descriptor (scalar registers), the per-thread part `lane` is one
+// 32-bit byte offset shared by every element, and c * usq is a scalar
+// offset -- no per-element vector address arithmetic.  The descriptor's
+// record count is the plane's remaining length, so the hardware drops any
+// access outside the plane (loads return 0).  The host checks that every
+// offset inside a tile, in bytes, stays below 2^32.
 typedef uint32_t pk_u32x4 __attribute__((ext_vector_type(4)));
-template <typename T>
-__device__ __forceinline__ CC_GLOBAL T *pk_tile(const T *plane, long long ub) {
-  const uintptr_t x = (uintptr_t)(plane + ub);
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t pk_rsrc(const void *plane,
+                                                          long long ub,
+                                                          long long len,
+                                                          int esize) {
+  const uintptr_t x = (uintptr_t)plane + (uintptr_t)(ub * esize);
   const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)x);
   const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(x >> 32));
-  return (CC_GLOBAL T *)(((uintptr_t)hi << 32) | lo);
+  const long long rem = (len - ub) * esize;
+  const uint32_t n = rem <= 0 ? 0u : (rem >= 0xFFFFFFFFLL ? 0xFFFFFFFFu : (uint32_t)rem);
+  return __builtin_amdgcn_make_buffer_rsrc(
+      (void *)(((uintptr_t)hi << 32) | lo), (short)0,
+      (int)__builtin_amdgcn_readfirstlane(n), 0x00020000);
 }
-template <typename T>
-__device__ __forceinline__ T pk_ld(const CC_GLOBAL T *t, uint32_t e) {
-  return *(const CC_GLOBAL T *)((const CC_GLOBAL char *)t + e * (uint32_t)sizeof(T));
+#define PK_LD32(r, c) \
+  __builtin_amdgcn_raw_buffer_load_b32((r), (int)(lane * 4u), (int)((c) * usq * 4u), 0)
+#define PK_LD16(r, c) \
+  ((uint32_t)__builtin_amdgcn_raw_buffer_load_b16((r), (int)(lane * 2u), (int)((c) * usq * 2u), 0))
+#define PK_ST32(r, c, val) \
+  __builtin_amdgcn_raw_buffer_store_b32((uint32_t)(val), (r), (int)(lane * 4u), (int)((c) * usq * 4u), 0)
+#define PK_ST16(r, c, val) \
+  __builtin_amdgcn_raw_buffer_store_b16((unsigned short)(val), (r), (int)(lane * 2u), (int)((c) * usq * 2u), 0)
+
+// byte-format dword (byte d = layer d, R <= 3) -> register format
+template <int W, int ND>
+__device__ __forceinline__ void pk_from_bytes(uint32_t x, uint32_t (&r)[ND]) {
+#pragma unroll
+  for (int k = 0; k < ND; k++) r[k] = 0u;
+  if constexpr (W == 8) {
+    r[0] = x;                                           // same layout
+  } else if constexpr (ND == 1) {
+    r[0] = __builtin_amdgcn_perm(0u, x, 0x0C010C00u);   // [A0, A1]
+  } else {
+    r[0] = __builtin_amdgcn_perm(0u, x, 0x0C020C00u);   // [A0, A2]
+    r[1] = __builtin_amdgcn_perm(0u, x, 0x0C030C01u);   // [A1, A3]
+  }
 }
-template <typename T>
-__device__ __forceinline__ void pk_st(CC_GLOBAL T *t, uint32_t e, T v) {
-  *(CC_GLOBAL T *)((CC_GLOBAL char *)t + e * (uint32_t)sizeof(T)) = v;
+
+// register format -> byte-format dword (real layers only)
+template <int W, int ND>
+__device__ __forceinline__ uint32_t pk_to_bytes(const uint32_t (&r)[ND],
+                                                uint32_t bmask) {
+  if constexpr (W == 8) {
+    return r[0] & bmask;
+  } else if constexpr (ND == 1) {
+    return __builtin_amdgcn_perm(0u, r[0], 0x0C0C0200u) & bmask;
+  } else {
+    return __builtin_amdgcn_perm(r[1], r[0], 0x06020400u) & bmask;
+  }
 }
 
 // seed or compact layers of C words into register format.  Compact words
@@ -157,24 +199,26 @@ __device__ __forceinline__ void pk_st(CC_GLOBAL T *t, uint32_t e, T v) {
 // word, never more than ND) and converted in place.  The predicate
 // [plo <= cnt <= phi] covers EQ (plo = phi = t), DEF (0 .. mu-1) and CRIT
 // (1 .. mu) without branches.
-template <int C, int W, int ND>
+// SEED (compile-time): the first sweep reads a seed (predicate or code
+// plane, branched on at run time); later sweeps read compact layers.
+template <int C, int W, int ND, bool SEED, bool IN2>
 __device__ __forceinline__ void pk_load(const PSweepArgs &a, long long ub,
                                         uint32_t usq, uint32_t lane,
                                         bool live, uint32_t (&v)[C][ND]) {
-  constexpr int S = 32 / W;
 #pragma unroll
   for (int c = 0; c < C; c++)
 #pragma unroll
     for (int k = 0; k < ND; k++) v[c][k] = 0u;
   if (!live) return;
-  if (a.in_kind == CC_IN_PRED) {
-    const CC_GLOBAL uint16_t *t = pk_tile(a.cnt, ub);
+  if (SEED && a.in_kind == CC_IN_PRED) {
+    const __amdgpu_buffer_rsrc_t r = pk_rsrc(a.cnt, ub, a.X, 2);
     const uint32_t lo = (uint32_t)a.plo, span = (uint32_t)(a.phi - a.plo);
     if (C % 8 == 0 && usq == 1) {
       // contiguous cube (16-byte aligned): 8 counters per load
 #pragma unroll
       for (int c = 0; c < C; c += 8) {
-        const pk_u32x4 q = *(const CC_GLOBAL pk_u32x4 *)(t + lane + c);
+        const pk_u32x4 q = __builtin_bit_cast(pk_u32x4,
+            __builtin_amdgcn_raw_buffer_load_b128(r, (int)(lane * 2u), c * 2, 0));
         const uint32_t wv[4] = {q.x, q.y, q.z, q.w};
 #pragma unroll
         for (int h = 0; h < 8; h++)
@@ -183,131 +227,144 @@ __device__ __forceinline__ void pk_load(const PSweepArgs &a, long long ub,
     } else {
 #pragma unroll
       for (int c = 0; c < C; c++) {
-        v[c][0] = (uint32_t)pk_ld(t, lane + c * usq);
+        v[c][0] = PK_LD16(r, c);
         PK_FENCE(c);
       }
     }
 #pragma unroll
     for (int c = 0; c < C; c++) v[c][0] = (v[c][0] - lo) <= span ? 1u : 0u;
-  } else if (a.in_kind == CC_IN_CODE) {
-    const CC_GLOBAL int32_t *t = pk_tile(a.code, ub);
+  } else if (SEED) {
+    const __amdgpu_buffer_rsrc_t r = pk_rsrc(a.code, ub, a.X, 4);
     if (C % 4 == 0 && usq == 1) {
 #pragma unroll
       for (int c = 0; c < C; c += 4) {
-        const pk_u32x4 q = *(const CC_GLOBAL pk_u32x4 *)(t + lane + c);
+        const pk_u32x4 q = __builtin_bit_cast(pk_u32x4,
+            __builtin_amdgcn_raw_buffer_load_b128(r, (int)(lane * 4u), c * 4, 0));
         v[c][0] = q.x; v[c + 1][0] = q.y; v[c + 2][0] = q.z; v[c + 3][0] = q.w;
       }
     } else {
 #pragma unroll
       for (int c = 0; c < C; c++) {
-        v[c][0] = (uint32_t)pk_ld(t, lane + c * usq);
+        v[c][0] = PK_LD32(r, c);
         PK_FENCE(c);
       }
     }
   } else {
-    const CC_GLOBAL uint32_t *t = pk_tile((const uint32_t *)a.pk, ub);
+    const __amdgpu_buffer_rsrc_t r = pk_rsrc(a.pk, ub, a.X, 4);
 #pragma unroll
     for (int c = 0; c < C; c++) {
-      v[c][0] = pk_ld(t, lane + c * usq);
+      v[c][0] = PK_LD32(r, c);
       PK_FENCE(c);
     }
-    if constexpr (ND > 1) {
-      if (a.in_nd > 1) {
-        const CC_GLOBAL uint32_t *t1 = pk_tile((const uint32_t *)a.pk, ub + a.pitch);
+    if constexpr (ND > 1 && IN2) {
+      const __amdgpu_buffer_rsrc_t r1 = pk_rsrc(a.pk + a.pitch, ub, a.X, 4);
 #pragma unroll
-        for (int c = 0; c < C; c++) {
-          v[c][1] = pk_ld(t1, lane + c * usq);
-          PK_FENCE(c);
-        }
+      for (int c = 0; c < C; c++) {
+        v[c][1] = PK_LD32(r1, c);
+        PK_FENCE(c);
       }
     }
-    // per-layer extraction, straight-line: layers >= L have width 0
-    int off[ND * S], bits[ND * S];
-    bool hi[ND * S];
+    if (!IN2 && a.in_w8) {
 #pragma unroll
-    for (int d = 0; d < ND * S; d++) {
-      const bool on = d < a.L;
-      off[d] = on ? a.ioff[d] : 0;
-      bits[d] = on ? a.ibits[d] : 0;
-      hi[d] = on && a.idw[d] != 0;
-    }
+      for (int c = 0; c < C; c++) {
+        uint32_t r[ND];
+        pk_from_bytes<W, ND>(v[c][0], r);
 #pragma unroll
-    for (int c = 0; c < C; c++) {
-      const uint32_t x0 = v[c][0];
-      const uint32_t x1 = ND > 1 ? v[c][ND > 1 ? 1 : 0] : 0u;
-      uint32_t r[ND];
+        for (int k = 0; k < ND; k++) v[c][k] = r[k];
+      }
+    } else {
+      // general compact fields: layer d is bits [off, off + bits) of
+      // dword plane idw[d]; layers >= L have width 0
+      constexpr int NS = ND * (32 / W);
+      int off[NS], bits[NS];
+      bool hi[NS];
 #pragma unroll
-      for (int k = 0; k < ND; k++) r[k] = 0u;
+      for (int d = 0; d < NS; d++) {
+        const bool on = d < a.L;
+        off[d] = on ? a.ioff[d] : 0;
+        bits[d] = on ? a.ibits[d] : 0;
+        hi[d] = on && a.idw[d] != 0;
+      }
 #pragma unroll
-      for (int d = 0; d < ND * S; d++)
-        r[d / S] |= __builtin_amdgcn_ubfe(hi[d] ? x1 : x0, off[d], bits[d])
-                    << (W * (d % S));
+      for (int c = 0; c < C; c++) {
+        const uint32_t x0 = v[c][0];
+        const uint32_t x1 = (ND > 1 && IN2) ? v[c][ND > 1 ? 1 : 0] : 0u;
+        uint32_t r[ND];
 #pragma unroll
-      for (int k = 0; k < ND; k++) v[c][k] = r[k];
+        for (int k = 0; k < ND; k++) r[k] = 0u;
+#pragma unroll
+        for (int d = 0; d < NS; d++) {
+          const uint32_t src = (ND > 1 && IN2) ? (hi[d] ? x1 : x0) : x0;
+          r[d % ND] |= __builtin_amdgcn_ubfe(src, off[d], bits[d]) << (W * (d / ND));
+        }
+#pragma unroll
+        for (int k = 0; k < ND; k++) v[c][k] = r[k];
+      }
     }
   }
 }
 
-// sum of the real layer fields of one word
+// sum of the real layer fields of one word (byte sums as a dot product)
 template <int W, int ND>
 __device__ __forceinline__ uint32_t pk_sum(const PSweepArgs &a,
                                            const uint32_t (&r)[ND]) {
-  constexpr int S = 32 / W;
-  constexpr uint32_t M = (1u << W) - 1u;
   uint32_t sum = 0;
 #pragma unroll
   for (int k = 0; k < ND; k++) {
     const uint32_t x = r[k] & a.fmask[k];
-#pragma unroll
-    for (int i = 0; i < S; i++) sum += (x >> (W * i)) & M;
+    if constexpr (W == 8) sum = __builtin_amdgcn_udot4(x, 0x01010101u, sum, false);
+    else {
+      typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+      const u16x2 ones = {1, 1};
+      sum = __builtin_amdgcn_udot2(__builtin_bit_cast(u16x2, x), ones, sum, false);
+    }
   }
   return sum;
 }
 
 // C words out (addressed as in pk_load): compact layers, the layer sum into
 // the map, or the clamped sum into cnt
-template <int C, int W, int ND, bool SUM>
+template <int C, int W, int ND, bool SUM, bool OUT2>
 __device__ __forceinline__ void pk_store(const PSweepArgs &a, long long ub,
                                          uint32_t usq, uint32_t lane,
                                          const uint32_t (&v)[C][ND]) {
-  constexpr int S = 32 / W;
-  constexpr uint32_t M = (1u << W) - 1u;
   if constexpr (!SUM) {
-    // compact fields, straight-line: layer d goes to bit ooff[d] of plane
-    // odw[d]; layers >= L are masked off
-    uint32_t en[ND * S], lo[ND * S];
-    int sh[ND * S];
-#pragma unroll
-    for (int d = 0; d < ND * S; d++) {
-      const bool on = d < a.L;
-      en[d] = on ? M : 0u;
-      lo[d] = on && a.odw[d] == 0 ? 0xFFFFFFFFu : 0u;
-      sh[d] = on ? a.ooff[d] : 0;
-    }
-    CC_GLOBAL uint32_t *t = pk_tile(a.pk, ub);
-    if (ND > 1 && a.out_nd > 1) {
-      CC_GLOBAL uint32_t *t1 = pk_tile(a.pk, ub + a.pitch);
+    const __amdgpu_buffer_rsrc_t r = pk_rsrc(a.pk, ub, a.X, 4);
+    if (a.out_w8) {
 #pragma unroll
       for (int c = 0; c < C; c++) {
-        uint32_t o0 = 0u, o1 = 0u;
-#pragma unroll
-        for (int d = 0; d < ND * S; d++) {
-          const uint32_t f = ((v[c][d / S] >> (W * (d % S))) & en[d]) << sh[d];
-          o0 |= f & lo[d];
-          o1 |= f & ~lo[d];
-        }
-        pk_st(t, lane + c * usq, o0);
-        pk_st(t1, lane + c * usq, o1);
+        PK_ST32(r, c, (pk_to_bytes<W, ND>(v[c], a.out_bmask)));
         PK_FENCE(c);
       }
     } else {
+      // general compact fields: layer d goes to bit ooff[d] of plane
+      // odw[d] (OUT2: two planes); don't-care slots are masked to zero
+      // first, so every slot is packed unconditionally
+      constexpr int NS = ND * (32 / W);
+      constexpr uint32_t M = (1u << W) - 1u;
+      int sh[NS];
+#pragma unroll
+      for (int d = 0; d < NS; d++)
+        sh[d] = d < a.L ? a.ooff[d] + (OUT2 && a.odw[d] ? 32 : 0) : 0;
+      const __amdgpu_buffer_rsrc_t r1 = pk_rsrc(a.pk + (OUT2 ? a.pitch : 0), ub, a.X, 4);
 #pragma unroll
       for (int c = 0; c < C; c++) {
-        uint32_t o0 = 0u;
+        uint32_t x[ND];
 #pragma unroll
-        for (int d = 0; d < ND * S; d++)
-          o0 |= ((v[c][d / S] >> (W * (d % S))) & en[d]) << sh[d];
-        pk_st(t, lane + c * usq, o0);
+        for (int k = 0; k < ND; k++) x[k] = v[c][k] & a.fmask[k];
+        if constexpr (OUT2) {
+          uint64_t o = 0;
+#pragma unroll
+          for (int d = 0; d < NS; d++)
+            o |= (uint64_t)((x[d % ND] >> (W * (d / ND))) & M) << sh[d];
+          PK_ST32(r, c, (uint32_t)o);
+          PK_ST32(r1, c, (uint32_t)(o >> 32));
+        } else {
+          uint32_t o = 0;
+#pragma unroll
+          for (int d = 0; d < NS; d++) o |= ((x[d % ND] >> (W * (d / ND))) & M) << sh[d];
+          PK_ST32(r, c, o);
+        }
         PK_FENCE(c);
       }
     }
@@ -315,47 +372,76 @@ __device__ __forceinline__ void pk_store(const PSweepArgs &a, long long ub,
     // one loop for both sums, so the per-word sums are formed next to
     // their stores (not hoisted ahead of the branch for all C words)
     const bool to_map = a.out_kind == CC_OUT_MAP;
-    CC_GLOBAL int32_t *tm = pk_tile(a.map, ub);
-    CC_GLOBAL uint16_t *tc = pk_tile(a.cnt_out, ub);
+    const __amdgpu_buffer_rsrc_t rm = pk_rsrc(a.map, ub, a.X, 4);
+    const __amdgpu_buffer_rsrc_t rc = pk_rsrc(a.cnt_out, ub, a.X, 2);
 #pragma unroll
     for (int c = 0; c < C; c++) {
       const uint32_t s = pk_sum<W, ND>(a, v[c]);
-      if (to_map) pk_st(tm, lane + c * usq, (int32_t)s);
-      else pk_st(tc, lane + c * usq, s > 65535u ? (uint16_t)65535 : (uint16_t)s);
+      if (to_map) PK_ST32(rm, c, s);
+      else PK_ST16(rc, c, (s > 65535u ? 65535u : s));
+      PK_FENCE(c);
+    }
+  }
+}
+
+// LDS exchange: element (row p, column tl) of the tile lives at
+// pad(p * TL + tl) of each dword plane; the cube's element c is row
+// pb + c * qa.  When the round's addresses are linear in c (lin: the host
+// checked that no padding step falls inside a cube) they are e0 + c * kp.
+template <int C, int ND>
+__device__ __forceinline__ void pk_lds_st(uint32_t *__restrict__ S, int PTP,
+                                          int pb, int qa, int TL, int tl,
+                                          bool lin, int kp,
+                                          const uint32_t (&v)[C][ND]) {
+  if (lin) {
+    const int e0 = pk_pad(pb * TL + tl);
+#pragma unroll
+    for (int c = 0; c < C; c++) {
+#pragma unroll
+      for (int k = 0; k < ND; k++) S[k * PTP + e0 + c * kp] = v[c][k];
+      PK_FENCE(c);
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < C; c++) {
+      const int e = pk_pad((pb + c * qa) * TL + tl);
+#pragma unroll
+      for (int k = 0; k < ND; k++) S[k * PTP + e] = v[c][k];
       PK_FENCE(c);
     }
   }
 }
 
 template <int C, int ND>
-__device__ __forceinline__ void pk_lds_st(uint32_t *__restrict__ S, int PTP,
-                                          int pb, int qa, int TL, int tl,
-                                          const uint32_t (&v)[C][ND]) {
-#pragma unroll
-  for (int c = 0; c < C; c++) {
-    const int e = pk_pad((pb + c * qa) * TL + tl);
-#pragma unroll
-    for (int k = 0; k < ND; k++) S[k * PTP + e] = v[c][k];
-    PK_FENCE(c);
-  }
-}
-
-template <int C, int ND>
 __device__ __forceinline__ void pk_lds_ld(const uint32_t *__restrict__ S,
                                           int PTP, int pb, int qa, int TL,
-                                          int tl, uint32_t (&v)[C][ND]) {
+                                          int tl, bool lin, int kp,
+                                          uint32_t (&v)[C][ND]) {
+  if (lin) {
+    const int e0 = pk_pad(pb * TL + tl);
 #pragma unroll
-  for (int c = 0; c < C; c++) {
-    const int e = pk_pad((pb + c * qa) * TL + tl);
+    for (int c = 0; c < C; c++) {
 #pragma unroll
-    for (int k = 0; k < ND; k++) v[c][k] = S[k * PTP + e];
-    PK_FENCE(c);
+      for (int k = 0; k < ND; k++) v[c][k] = S[k * PTP + e0 + c * kp];
+      PK_FENCE(c);
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < C; c++) {
+      const int e = pk_pad((pb + c * qa) * TL + tl);
+#pragma unroll
+      for (int k = 0; k < ND; k++) v[c][k] = S[k * PTP + e];
+      PK_FENCE(c);
+    }
   }
 }
 
+// SEED: the sweep starts the transform (reads the predicate or code plane).
 // SUM: the sweep ends the transform (layer sum into the map or cnt);
-// otherwise it writes compact layers
-template <int Q, int RR, int W, int ND, bool SUM>
+// otherwise it writes compact layers.  IN2 / OUT2: its compact input /
+// output has two dword planes (compile-time: a runtime plane count made the
+// compiler keep both register images of the cube)
+template <int Q, int RR, int W, int ND, bool SEED, bool SUM, bool IN2, bool OUT2>
 __global__ void __launch_bounds__(CC_BLOCK) k_psweep(PSweepArgs a) {
   extern __shared__ uint32_t Sp[];
   constexpr int C = cc_ipow(Q, RR);
@@ -371,6 +457,8 @@ __global__ void __launch_bounds__(CC_BLOCK) k_psweep(PSweepArgs a) {
     for (int rd = 0; rd < a.nrounds; rd++) {
       const int qa = a.qa[rd];
       const bool first = rd == 0, last = rd == a.nrounds - 1;
+      const bool lin = a.lin[rd] != 0;
+      const int kp = a.kp[rd];
       for (int cu = threadIdx.x; cu < ncube; cu += nt) {
         const int tl = cu % a.TL, u = cu / a.TL;
         const int pb = (u / qa) * qa * C + (u % qa);
@@ -378,17 +466,65 @@ __global__ void __launch_bounds__(CC_BLOCK) k_psweep(PSweepArgs a) {
         const uint32_t usq = (uint32_t)((long long)qa * a.s);
         uint32_t v[C][ND];
         if (first)
-          pk_load<C, W, ND>(a, base, usq, lane, tl < ncol, v);
+          pk_load<C, W, ND, SEED, IN2>(a, base, usq, lane, tl < ncol, v);
         else
-          pk_lds_ld<C, ND>(Sp, PTP, pb, qa, a.TL, tl, v);
+          pk_lds_ld<C, ND>(Sp, PTP, pb, qa, a.TL, tl, lin, kp, v);
         pk_apply<Q, C, 0, RR, W, ND>(v, a.amask[rd]);
         if (last) {
-          if (tl < ncol) pk_store<C, W, ND, SUM>(a, base, usq, lane, v);
+          if (tl < ncol) pk_store<C, W, ND, SUM, OUT2>(a, base, usq, lane, v);
         } else {
-          pk_lds_st<C, ND>(Sp, PTP, pb, qa, a.TL, tl, v);
+          pk_lds_st<C, ND>(Sp, PTP, pb, qa, a.TL, tl, lin, kp, v);
         }
       }
       __syncthreads();
     }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Strided sweep whose whole group is one register cube: every thread (or,
+// with SPLIT = 2 on binary axes, every pair of adjacent lanes) owns the Q^RR
+// words of one fiber cube of one column, rows s apart.  Consecutive lanes
+// take consecutive columns, so each row access of a wave is coalesced; a
+// workgroup covers 256 / SPLIT consecutive columns of one outer block
+// (the host requires s to be a multiple of that), so the row address is a
+// workgroup-uniform scalar base plus one per-lane 32-bit offset shared by
+// every element: no per-element address arithmetic.  With SPLIT = 2 a lane
+// holds half the cube (the rows whose top cube digit equals its lane
+// parity) and the top axis is exchanged with the partner lane by a DPP
+// quad permute, which halves the registers per lane.
+template <int W, int ND>
+__device__ __forceinline__ void pk_add_up_xlane(uint32_t (&x)[ND]) {
+  uint32_t p[ND];
+#pragma unroll
+  for (int k = 0; k < ND; k++)
+    p[k] = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x[k], 0xB1, 0xF, 0xF, false);
+  pk_add_up<W, ND>(x, p);
+}
+
+template <int Q, int RR, int W, int ND, bool SUM, bool IN2, bool OUT2, int SPLIT>
+__global__ void __launch_bounds__(CC_BLOCK) k_pcol(PSweepArgs a) {
+  constexpr int C = cc_ipow(Q, RR);
+  constexpr int CH = C / SPLIT;              // words per lane
+  constexpr int RI = SPLIT == 2 ? RR - 1 : RR;  // axes inside the lane
+  constexpr int CPB = CC_BLOCK / SPLIT;      // columns per workgroup
+  const long long ncubes = a.ntiles;         // one cube per column
+  const long long nblk = ncubes / CPB;
+  const int half = SPLIT == 2 ? (int)(threadIdx.x & 1) : 0;
+  const int colw = (int)threadIdx.x / SPLIT;
+  const uint32_t lane = (uint32_t)(colw + (long long)half * CH * a.s);
+  const uint32_t usq = (uint32_t)a.s;
+  for (long long b = blockIdx.x; b < nblk; b += gridDim.x) {
+    const long long cube0 = b * CPB;         // the block's first column
+    const long long h = cube0 / a.s;
+    const long long base = h * a.s * a.P + (cube0 - h * a.s);
+    uint32_t v[CH][ND];
+    pk_load<CH, W, ND, false, IN2>(a, base, usq, lane, true, v);
+    pk_apply<Q, CH, 0, RI, W, ND>(v, (1 << RI) - 1);
+    if constexpr (SPLIT == 2) {
+#pragma unroll
+      for (int c = 0; c < CH; c++) pk_add_up_xlane<W, ND>(v[c]);
+    }
+    pk_store<CH, W, ND, SUM, OUT2>(a, base, usq, lane, v);
   }
 }

@@ -8,12 +8,59 @@
 
 constexpr bool pk_fits(int q, int rr, int nd) { return cc_ipow(q, rr) * nd <= 128; }
 
+// The kernel instantiation for one (Q, RR, W, ND): SUM (last sweep), IN2 /
+// OUT2 (two-plane compact input / output, only with ND > 1), and for the
+// column kernel the lanes per cube.
+template <int Q, int RR, int W, int ND, bool COL, int SPLIT, bool SEED, bool SUM,
+          bool IN2, bool OUT2>
+constexpr auto pk_kernel() {
+  if constexpr (COL) return &k_pcol<Q, RR, W, ND, SUM, IN2, OUT2, SPLIT>;
+  else return &k_psweep<Q, RR, W, ND, SEED, SUM, IN2, OUT2>;
+}
+
+template <int Q, int RR, int W, int ND, bool COL, int SPLIT, bool SEED>
+void (*pk_select(bool sum, bool in2, bool out2))(PSweepArgs) {
+  if constexpr (ND == 1 || SEED) {
+    // one compact input plane (a seed has none)
+    if (sum) return pk_kernel<Q, RR, W, ND, COL, SPLIT, SEED, true, false, false>();
+    if constexpr (ND > 1)
+      if (out2) return pk_kernel<Q, RR, W, ND, COL, SPLIT, SEED, false, false, true>();
+    return pk_kernel<Q, RR, W, ND, COL, SPLIT, SEED, false, false, false>();
+  } else {
+    if (sum)
+      return in2 ? pk_kernel<Q, RR, W, ND, COL, SPLIT, SEED, true, true, false>()
+                 : pk_kernel<Q, RR, W, ND, COL, SPLIT, SEED, true, false, false>();
+    if (in2)
+      return out2 ? pk_kernel<Q, RR, W, ND, COL, SPLIT, SEED, false, true, true>()
+                  : pk_kernel<Q, RR, W, ND, COL, SPLIT, SEED, false, true, false>();
+    return out2 ? pk_kernel<Q, RR, W, ND, COL, SPLIT, SEED, false, false, true>()
+                : pk_kernel<Q, RR, W, ND, COL, SPLIT, SEED, false, false, false>();
+  }
+}
+
 template <int Q, int RR, int W, int ND>
 hipError_t pk_go(const PSweepArgs &a, int blocks, int threads, size_t lds,
                  hipStream_t st) {
+  const bool sum = a.out_kind != CC_OUT_PACKED;
+  const bool in2 = ND > 1 && a.in_kind == CC_IN_PACKED && a.in_nd > 1;
+  const bool out2 = ND > 1 && !sum && a.out_nd > 1;
+  void (*kf)(PSweepArgs) = nullptr;
+  if (a.TL < 0) {
+    // column kernel (k_pcol): -TL = lanes per cube, whole workgroups
+    if constexpr (Q == 2 && RR >= 4 && pk_fits(Q, RR - 1, ND)) {
+      if (a.TL == -2) kf = pk_select<Q, RR, W, ND, true, 2, false>(sum, in2, out2);
+    }
+    if constexpr (RR >= 2 && pk_fits(Q, RR, ND)) {
+      if (a.TL == -1) kf = pk_select<Q, RR, W, ND, true, 1, false>(sum, in2, out2);
+    }
+    if (!kf) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(kf, dim3(blocks), dim3(CC_BLOCK), 0, st, a);
+    return hipGetLastError();
+  }
   if constexpr (pk_fits(Q, RR, ND)) {
-    auto kf = a.out_kind == CC_OUT_PACKED ? k_psweep<Q, RR, W, ND, false>
-                                          : k_psweep<Q, RR, W, ND, true>;
+    kf = a.in_kind == CC_IN_PACKED
+             ? pk_select<Q, RR, W, ND, false, 1, false>(sum, in2, out2)
+             : pk_select<Q, RR, W, ND, false, 1, true>(sum, false, out2);
     if (lds > 64 * 1024) {
       hipError_t e = hipFuncSetAttribute((const void *)kf,
                                          hipFuncAttributeMaxDynamicSharedMemorySize,

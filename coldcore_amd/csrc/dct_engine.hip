@@ -62,6 +62,8 @@ struct PSweep {
   int nrounds;
   int ca[CC_PK_MAXRD], amask[CC_PK_MAXRD], qa[CC_PK_MAXRD];
   int in_nd, out_nd;  // compact planes read / written (0: seed in, sum out)
+  int in_w8, out_w8;  // compact format is one byte per layer
+  int lin[CC_PK_MAXRD], kp[CC_PK_MAXRD];  // linear LDS addressing per round
   int idw[CC_MAXL], ioff[CC_MAXL], ibits[CC_MAXL];
   int odw[CC_MAXL], ooff[CC_MAXL], obits[CC_MAXL];
   size_t lds;
@@ -123,6 +125,7 @@ struct Engine {
   int grid = -1;             // CC_SWEEP_GRID: workgroups per CU (0 = per tile)
   int generic_only = 0;      // CC_SWEEP_KERNEL=generic: no round kernel
   int packed = 1;            // CC_PACKED=0: never take the packed path
+  int pk_col = 2;            // CC_PK_COL: column kernel 0 off, 1 whole, 2 split
   PPlan pfield;              // packed plan of gain / loss (predicate seeds)
   // optional per-sweep timing of the last transform (additive ABI)
   int prof = 0;
@@ -385,7 +388,19 @@ int bitlen(double x) {
 // compact HBM format of the layers after c axes: field d takes bitlen of its
 // bound, packed greedily into at most two dword planes
 bool compact_fmt(const PBounds &B, int c, int L, int *dw, int *off, int *bits,
-                 int *nd) {
+                 int *nd, int *w8) {
+  *w8 = 0;
+  if (L <= 4) {
+    bool bytes = true;
+    for (int d = 0; d < L; d++) bytes = bytes && B.lay(c, d) <= 255.0;
+    if (bytes) {
+      // one byte per layer: converts to the register formats with permutes
+      for (int d = 0; d < L; d++) { dw[d] = 0; off[d] = 8 * d; bits[d] = 8; }
+      *nd = 1;
+      *w8 = 1;
+      return true;
+    }
+  }
   int w = 0, used = 0;
   for (int d = 0; d < L; d++) {
     const int b = bitlen(B.lay(c, d));
@@ -420,6 +435,14 @@ void pk_rounds(PSweep &w, int q) {
     int qa = 1;
     for (int i = 0; i < ca; i++) qa *= q;
     w.ca[nr] = ca; w.amask[nr] = mask; w.qa[nr] = qa;
+    // LDS addresses pad((pb + c qa) TL + tl) are linear in c when no
+    // padding step (every 64 elements) falls inside a cube
+    const long long K = (long long)qa * w.TL;
+    int C = 1;
+    for (int i = 0; i < w.RR; i++) C *= q;
+    if (K % 64 == 0) { w.lin[nr] = 1; w.kp[nr] = (int)(K + K / 64); }
+    else if (K == 1 && 64 % C == 0) { w.lin[nr] = 1; w.kp[nr] = 1; }
+    else { w.lin[nr] = 0; w.kp[nr] = 0; }
     nr++;
   };
   add(0, (1 << RR) - 1);
@@ -466,9 +489,19 @@ bool pk_sweep(const PBounds &B, int j, int k, bool pred_seed, bool cnt_out,
     const int ncube = w.P / C;
     w.threads = std::min(256, (ncube + 63) / 64 * 64);
   } else if (m == w.RR) {
-    w.TL = (int)std::min<long long>(256, w.s);
     w.lds = 0;
-    w.threads = std::min(256, (w.TL + 63) / 64 * 64);
+    // column kernel: one cube per lane (or per lane pair, binary axes
+    // with big cubes), whole workgroups of consecutive columns
+    const int col = E.pk_col;
+    const bool split = col >= 2 && q == 2 && w.RR >= 4 && w.s % 128 == 0;
+    const bool whole = col >= 1 && w.RR >= 2 && w.s % 256 == 0;
+    if (split || whole) {
+      w.TL = split ? -2 : -1;
+      w.threads = CC_BLOCK;
+    } else {
+      w.TL = (int)std::min<long long>(256, w.s);
+      w.threads = std::min(256, (w.TL + 63) / 64 * 64);
+    }
   } else {
     w.TL = 0;
     for (int tl : {64, 32, 16}) {
@@ -479,10 +512,16 @@ bool pk_sweep(const PBounds &B, int j, int k, bool pred_seed, bool cnt_out,
     if (!w.TL) return false;
     w.threads = std::min(256, (w.P / C * w.TL + 63) / 64 * 64);
   }
-  w.ncolb = (w.s + w.TL - 1) / w.TL;
-  w.ntiles = (E.X / (w.s * P)) * w.ncolb;
+  if (w.TL < 0) {
+    w.ncolb = 1;
+    w.ntiles = (E.X / (w.s * P)) * w.s;   // cubes (one per column)
+  } else {
+    w.ncolb = (w.s + w.TL - 1) / w.TL;
+    w.ntiles = (E.X / (w.s * P)) * w.ncolb;
+  }
   if (prev) {
     w.in_nd = prev->out_nd;
+    w.in_w8 = prev->out_w8;
     for (int d = 0; d < L; d++) {
       w.idw[d] = prev->odw[d]; w.ioff[d] = prev->ooff[d]; w.ibits[d] = prev->obits[d];
     }
@@ -495,7 +534,8 @@ bool pk_sweep(const PBounds &B, int j, int k, bool pred_seed, bool cnt_out,
     w.out_nd = 0;
     w.out_bytes = cnt_out ? 2 : 4;
   } else {
-    if (!compact_fmt(B, k, L, w.odw, w.ooff, w.obits, &w.out_nd)) return false;
+    if (!compact_fmt(B, k, L, w.odw, w.ooff, w.obits, &w.out_nd, &w.out_w8))
+      return false;
     if (w.out_nd > L - 1) return false;   // compact planes live in `lay`
     w.out_bytes = 4 * w.out_nd;
   }
@@ -503,8 +543,10 @@ bool pk_sweep(const PBounds &B, int j, int k, bool pred_seed, bool cnt_out,
   return true;
 }
 
-// Schedule with the fewest algorithmic bytes: every first-sweep size, then
-// the strided sweeps greedily as long as possible.
+// Schedule with the fewest algorithmic bytes, then the fewest sweeps, then
+// the fewest LDS-staged strided tiles: a shortest path over sweep
+// boundaries (the compact format at a boundary depends only on how many
+// axes precede it, so a sweep's cost depends only on its two ends).
 PPlan pk_plan(double s0, double cap, bool pred_seed, bool cnt_out) {
   PPlan best;
   if (!E.packed || E.prob != CC_HAMMING || E.L < 2) return best;
@@ -513,36 +555,46 @@ PPlan pk_plan(double s0, double cap, bool pred_seed, bool cnt_out) {
     if (E.ax[i] != q) return best;
   if (q < 2 || q > 5) return best;
   const PBounds B{q, E.R, s0, cap};
-  for (int m0 = std::min(E.n, CC_MAXSW); m0 >= 1; m0--) {
-    PPlan pl;
-    PSweep w;
-    if (!pk_sweep(B, 0, m0, pred_seed, cnt_out, nullptr, w)) continue;
-    pl.sw.push_back(w);
-    int j = m0;
-    bool ok = true;
-    while (j < E.n && ok) {
-      ok = false;
-      for (int k = std::min(E.n, j + CC_MAXSW); k > j; k--) {
-        if (pk_sweep(B, j, k, pred_seed, cnt_out, &pl.sw.back(), w)) {
-          pl.sw.push_back(w);
-          j = k;
-          ok = true;
-          break;
-        }
-      }
+  const int n = E.n;
+  struct Cost { long long bytes; int sweeps, penalty; bool ok; };
+  auto better = [](const Cost &x, const Cost &y) {
+    if (!y.ok) return x.ok;
+    if (!x.ok) return false;
+    if (x.bytes != y.bytes) return x.bytes < y.bytes;
+    if (x.sweeps != y.sweeps) return x.sweeps < y.sweeps;
+    return x.penalty < y.penalty;
+  };
+  std::vector<Cost> dp(n + 1, Cost{0, 0, 0, false});
+  std::vector<PSweep> via(n + 1);
+  std::vector<int> from(n + 1, -1);
+  dp[0] = Cost{0, 0, 0, true};
+  for (int j = 0; j < n; j++) {
+    if (!dp[j].ok) continue;
+    PSweep prev;
+    memset(&prev, 0, sizeof(prev));
+    if (j > 0 && !compact_fmt(B, j, E.L, prev.odw, prev.ooff, prev.obits,
+                              &prev.out_nd, &prev.out_w8))
+      continue;
+    for (int k = j + 1; k <= std::min(n, j + CC_MAXSW); k++) {
+      PSweep w;
+      if (!pk_sweep(B, j, k, pred_seed, cnt_out, j ? &prev : nullptr, w)) continue;
+      const bool staged = j > 0 && w.nrounds > 1;
+      const int pen = (staged ? 1 : 0) + ((j > 0 && w.TL < 64 && w.s >= 64) ? 2 : 0);
+      const Cost c{dp[j].bytes + w.in_bytes + w.out_bytes, dp[j].sweeps + 1,
+                   dp[j].penalty + pen, true};
+      if (better(c, dp[k])) { dp[k] = c; via[k] = w; from[k] = j; }
     }
-    if (!ok) continue;
-    pl.ok = true;
-    for (const PSweep &x : pl.sw) pl.bytes += x.in_bytes + x.out_bytes;
-    if (!best.ok || pl.bytes < best.bytes ||
-        (pl.bytes == best.bytes && pl.sw.size() < best.sw.size()))
-      best = pl;
   }
+  if (!dp[n].ok) return best;
+  for (int k = n; k > 0; k = from[k]) best.sw.insert(best.sw.begin(), via[k]);
+  best.ok = true;
+  best.bytes = dp[n].bytes;
   return best;
 }
 
 hipError_t pk_launch(const PSweep &w, const PSweepArgs &a) {
-  const int blocks = (int)std::min<long long>(w.ntiles, 1LL << 30);
+  const long long units = w.TL < 0 ? w.ntiles / (CC_BLOCK / -w.TL) : w.ntiles;
+  const int blocks = (int)std::min<long long>(units, 1LL << 30);
   switch (E.ax[0]) {
     case 2: return cc_psweep_q2(a, w.RR, w.W, w.ND, blocks, w.threads, w.lds, E.st);
     case 3: return cc_psweep_q3(a, w.RR, w.W, w.ND, blocks, w.threads, w.lds, E.st);
@@ -569,6 +621,7 @@ int run_ptransform(const PPlan &pl, int in_kind, int out_kind, int plo,
     a.map = E.A[0];
     a.cnt_out = E.cnt;
     a.pitch = E.pitch;
+    a.X = E.X;
     a.s = w.s; a.ncolb = w.ncolb; a.ntiles = w.ntiles;
     a.P = w.P; a.TL = w.TL; a.nax = w.nax; a.L = E.L;
     a.in_kind = i == 0 ? in_kind : CC_IN_PACKED;
@@ -578,12 +631,17 @@ int run_ptransform(const PPlan &pl, int in_kind, int out_kind, int plo,
     a.nrounds = w.nrounds;
     for (int r = 0; r < w.nrounds; r++) {
       a.ca[r] = w.ca[r]; a.amask[r] = w.amask[r]; a.qa[r] = w.qa[r];
+      a.lin[r] = w.lin[r]; a.kp[r] = w.kp[r];
     }
+    a.in_w8 = w.in_w8;
+    a.out_w8 = w.out_w8;
+    a.out_bmask = E.L >= 4 ? 0xFFFFFFFFu : (uint32_t)((1u << (8 * E.L)) - 1u);
+    // real-layer slots: layer d = slot * ND + dword (interleaved layout)
     const int S = 32 / w.W;
     for (int k = 0; k < w.ND; k++) {
       uint32_t fm = 0;
       for (int sl = 0; sl < S; sl++)
-        if (k * S + sl < E.L) fm |= (uint32_t)((1ull << w.W) - 1) << (w.W * sl);
+        if (sl * w.ND + k < E.L) fm |= (uint32_t)((1ull << w.W) - 1) << (w.W * sl);
       a.fmask[k] = fm;
     }
     for (int d = 0; d < E.L; d++) {
@@ -955,6 +1013,8 @@ static int init_cell(int problem, int n, const int *ax, int R,
   {
     const char *pe = getenv("CC_PACKED");
     E.packed = !(pe && !strcmp(pe, "0")) && !E.generic_only;
+    const char *pc = getenv("CC_PK_COL");
+    E.pk_col = pc ? atoi(pc) : 2;
     E.pfield = pk_plan(1.0, 1e300, true, false);
   }
   E.uploaded.clear();

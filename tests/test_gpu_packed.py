@@ -21,7 +21,11 @@ pytestmark = pytest.mark.gpu
 CELLS = [([2] * 16, 3), ([2] * 14, 4), ([2] * 12, 6), ([2] * 13, 1),
          ([3] * 9, 3), ([3] * 8, 1), ([4] * 7, 3), ([5] * 6, 3), ([5] * 6, 2)]
 IDS = [f"{a[0]}^{len(a)}-R{r}" for a, r in CELLS]
-BUDGETS = {"default": None, "lds16k": "16384", "lds4k": "4096"}
+# schedule variants: LDS budgets that force short first sweeps and
+# multi-round strided tiles; the column kernel off, or whole-cube only
+VARIANTS = {"default": {}, "lds16k": {"CC_SWEEP_LDS": "16384"},
+            "lds4k": {"CC_SWEEP_LDS": "4096"}, "nocol": {"CC_PK_COL": "0"},
+            "col1": {"CC_PK_COL": "1"}}
 
 
 def _check_cell(g, c, rng, mults):
@@ -39,11 +43,11 @@ def _check_cell(g, c, rng, mults):
             assert (g.map_read_at(every) == c.map_read_at(every)).all(), (which, m)
 
 
-@pytest.mark.parametrize("budget", list(BUDGETS))
+@pytest.mark.parametrize("variant", list(VARIANTS))
 @pytest.mark.parametrize("axes,R", CELLS, ids=IDS)
-def test_packed_fields_match_oracle(gpu_cell, monkeypatch, axes, R, budget):
-    if BUDGETS[budget]:
-        monkeypatch.setenv("CC_SWEEP_LDS", BUDGETS[budget])
+def test_packed_fields_match_oracle(gpu_cell, monkeypatch, axes, R, variant):
+    for k, v in VARIANTS[variant].items():
+        monkeypatch.setenv(k, v)
     rng = np.random.default_rng(len(axes) * 10 + R)
     g = gpu_cell(axes, R)
     plan = g.sweep_plan2(0)
