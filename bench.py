@@ -46,13 +46,16 @@ def kernel_name(plan, i, q):
     if sw["W"] == 32:
         return "k_sweep / k_sweep_r (unpacked layers)"
     last = i == len(plan) - 1
-    return (f"k_psweep<Q={q},RR={sw['RR']},W={sw['W']},ND={sw['ND']},"
+    name = {1: "k_psweep", 2: "k_pcol", 3: "k_pcol", 4: "k_pfirst",
+            5: "k_pfirstp"}[sw["kernel"]]
+    return (f"{name}<Q={q},RR={sw['RR']},W={sw['W']},ND={sw['ND']},"
             f"{'sum' if last else 'compact'}>")
 
 
 def describe(plan):
     return [f"axes {s['first_axis']}..{s['first_axis'] + s['axes'] - 1}"
             f" P={s['P']} TL={s['TL']} RR={s['RR']} W={s['W']} ND={s['ND']}"
+            f" kernel={s['kernel']}"
             f" {s['bytes_in']}+{s['bytes_out']} B/word LDS={s['lds_bytes']}"
             for s in plan]
 

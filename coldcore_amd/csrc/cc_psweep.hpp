@@ -56,6 +56,8 @@ struct PSweepArgs {
   long long ncolb;       // column blocks per outer index = ceil(s / TL)
   long long ntiles;
   int P, TL, nax, L;
+  int kern;              // 0 k_psweep, 1 / 2 k_pcol (lanes per cube), 3 k_pfirst,
+                         // 4 k_pfirstp
   int in_kind, out_kind;
   int plo, phi;          // CC_IN_PRED: [plo <= cnt <= phi]
   int in_nd, out_nd;     // compact planes read / written
@@ -324,47 +326,85 @@ __device__ __forceinline__ uint32_t pk_sum(const PSweepArgs &a,
 
 // C words out (addressed as in pk_load): compact layers, the layer sum into
 // the map, or the clamped sum into cnt
-template <int C, int W, int ND, bool SUM, bool OUT2>
+// compact output word(s) of one word in register format
+template <int W, int ND, bool OUT2>
+__device__ __forceinline__ uint64_t pk_compact(const PSweepArgs &a,
+                                               const uint32_t (&v)[ND],
+                                               const int (&sh)[ND * (32 / W)]) {
+  constexpr int NS = ND * (32 / W);
+  constexpr uint32_t M = (1u << W) - 1u;
+  uint32_t x[ND];
+#pragma unroll
+  for (int k = 0; k < ND; k++) x[k] = v[k] & a.fmask[k];
+  if constexpr (OUT2) {
+    uint64_t o = 0;
+#pragma unroll
+    for (int d = 0; d < NS; d++)
+      o |= (uint64_t)((x[d % ND] >> (W * (d / ND))) & M) << sh[d];
+    return o;
+  } else {
+    uint32_t o = 0;
+#pragma unroll
+    for (int d = 0; d < NS; d++) o |= ((x[d % ND] >> (W * (d / ND))) & M) << sh[d];
+    return o;
+  }
+}
+
+// C words out (addressed as in pk_load): compact layers, the layer sum into
+// the map, or the clamped sum into cnt.  VEC: the cube is C consecutive
+// words (usq == 1, 16-byte aligned): 16-byte vector stores.
+template <int C, int W, int ND, bool SUM, bool OUT2, bool VEC = false>
 __device__ __forceinline__ void pk_store(const PSweepArgs &a, long long ub,
                                          uint32_t usq, uint32_t lane,
                                          const uint32_t (&v)[C][ND]) {
+  constexpr int NS = ND * (32 / W);
   if constexpr (!SUM) {
     const __amdgpu_buffer_rsrc_t r = pk_rsrc(a.pk, ub, a.X, 4);
-    if (a.out_w8) {
+    // layer d goes to bit ooff[d] of plane odw[d] (OUT2: two planes);
+    // don't-care slots are masked to zero first, so every slot is packed
+    // unconditionally
+    int sh[NS];
+#pragma unroll
+    for (int d = 0; d < NS; d++)
+      sh[d] = d < a.L ? a.ooff[d] + (OUT2 && a.odw[d] ? 32 : 0) : 0;
+    const __amdgpu_buffer_rsrc_t r1 = pk_rsrc(a.pk + (OUT2 ? a.pitch : 0), ub, a.X, 4);
+    const bool w8 = a.out_w8 != 0;
+    if constexpr (VEC && C % 4 == 0) {
+      if (w8) {
+#pragma unroll
+        for (int c = 0; c < C; c += 4) {
+          pk_u32x4 o;
+#pragma unroll
+          for (int i = 0; i < 4; i++) o[i] = pk_to_bytes<W, ND>(v[c + i], a.out_bmask);
+          __builtin_amdgcn_raw_buffer_store_b128(o, r, (int)(lane * 4u), c * 4, 0);
+        }
+      } else {
+#pragma unroll
+        for (int c = 0; c < C; c += 4) {
+          pk_u32x4 o, o1;
+#pragma unroll
+          for (int i = 0; i < 4; i++) {
+            const uint64_t x = pk_compact<W, ND, OUT2>(a, v[c + i], sh);
+            o[i] = (uint32_t)x;
+            o1[i] = (uint32_t)(x >> 32);
+          }
+          __builtin_amdgcn_raw_buffer_store_b128(o, r, (int)(lane * 4u), c * 4, 0);
+          if (OUT2)
+            __builtin_amdgcn_raw_buffer_store_b128(o1, r1, (int)(lane * 4u), c * 4, 0);
+        }
+      }
+    } else if (w8) {
 #pragma unroll
       for (int c = 0; c < C; c++) {
         PK_ST32(r, c, (pk_to_bytes<W, ND>(v[c], a.out_bmask)));
         PK_FENCE(c);
       }
     } else {
-      // general compact fields: layer d goes to bit ooff[d] of plane
-      // odw[d] (OUT2: two planes); don't-care slots are masked to zero
-      // first, so every slot is packed unconditionally
-      constexpr int NS = ND * (32 / W);
-      constexpr uint32_t M = (1u << W) - 1u;
-      int sh[NS];
-#pragma unroll
-      for (int d = 0; d < NS; d++)
-        sh[d] = d < a.L ? a.ooff[d] + (OUT2 && a.odw[d] ? 32 : 0) : 0;
-      const __amdgpu_buffer_rsrc_t r1 = pk_rsrc(a.pk + (OUT2 ? a.pitch : 0), ub, a.X, 4);
 #pragma unroll
       for (int c = 0; c < C; c++) {
-        uint32_t x[ND];
-#pragma unroll
-        for (int k = 0; k < ND; k++) x[k] = v[c][k] & a.fmask[k];
-        if constexpr (OUT2) {
-          uint64_t o = 0;
-#pragma unroll
-          for (int d = 0; d < NS; d++)
-            o |= (uint64_t)((x[d % ND] >> (W * (d / ND))) & M) << sh[d];
-          PK_ST32(r, c, (uint32_t)o);
-          PK_ST32(r1, c, (uint32_t)(o >> 32));
-        } else {
-          uint32_t o = 0;
-#pragma unroll
-          for (int d = 0; d < NS; d++) o |= ((x[d % ND] >> (W * (d / ND))) & M) << sh[d];
-          PK_ST32(r, c, o);
-        }
+        const uint64_t o = pk_compact<W, ND, OUT2>(a, v[c], sh);
+        PK_ST32(r, c, (uint32_t)o);
+        if (OUT2) PK_ST32(r1, c, (uint32_t)(o >> 32));
         PK_FENCE(c);
       }
     }
@@ -374,12 +414,33 @@ __device__ __forceinline__ void pk_store(const PSweepArgs &a, long long ub,
     const bool to_map = a.out_kind == CC_OUT_MAP;
     const __amdgpu_buffer_rsrc_t rm = pk_rsrc(a.map, ub, a.X, 4);
     const __amdgpu_buffer_rsrc_t rc = pk_rsrc(a.cnt_out, ub, a.X, 2);
+    if constexpr (VEC && C % 8 == 0) {
 #pragma unroll
-    for (int c = 0; c < C; c++) {
-      const uint32_t s = pk_sum<W, ND>(a, v[c]);
-      if (to_map) PK_ST32(rm, c, s);
-      else PK_ST16(rc, c, (s > 65535u ? 65535u : s));
-      PK_FENCE(c);
+      for (int c = 0; c < C; c += 8) {
+        uint32_t sm[8];
+#pragma unroll
+        for (int i = 0; i < 8; i++) sm[i] = pk_sum<W, ND>(a, v[c + i]);
+        if (to_map) {
+          const pk_u32x4 o0 = {sm[0], sm[1], sm[2], sm[3]};
+          const pk_u32x4 o1 = {sm[4], sm[5], sm[6], sm[7]};
+          __builtin_amdgcn_raw_buffer_store_b128(o0, rm, (int)(lane * 4u), c * 4, 0);
+          __builtin_amdgcn_raw_buffer_store_b128(o1, rm, (int)(lane * 4u), c * 4 + 16, 0);
+        } else {
+          pk_u32x4 o;
+#pragma unroll
+          for (int i = 0; i < 4; i++)
+            o[i] = min(sm[2 * i], 65535u) | (min(sm[2 * i + 1], 65535u) << 16);
+          __builtin_amdgcn_raw_buffer_store_b128(o, rc, (int)(lane * 2u), c * 2, 0);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < C; c++) {
+        const uint32_t s = pk_sum<W, ND>(a, v[c]);
+        if (to_map) PK_ST32(rm, c, s);
+        else PK_ST16(rc, c, (s > 65535u ? 65535u : s));
+        PK_FENCE(c);
+      }
     }
   }
 }
@@ -526,5 +587,156 @@ __global__ void __launch_bounds__(CC_BLOCK) k_pcol(PSweepArgs a) {
       for (int c = 0; c < CH; c++) pk_add_up_xlane<W, ND>(v[c]);
     }
     pk_store<CH, W, ND, SUM, OUT2>(a, base, usq, lane, v);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// First sweep of a binary cell without LDS: a wave owns a tile of 2^(RR+6)
+// consecutive words, lane l the C = 2^RR contiguous words l*C .. l*C+C-1
+// (vector loads and stores).  The RR low axes are in-thread; the six axes
+// above are the lane index bits and are applied across lanes: xor 1, 2, 4, 8
+// with DPP row permutes (quad_perm [1,0,3,2], [2,3,0,1], row_half_mirror
+// then quad_perm [3,2,1,0], row_ror:8), xor 16 and 32 with permlane16 /
+// permlane32 swaps of register pairs, which put each lane's cross-row
+// partner into its own registers (the swap is its own inverse, so a second
+// swap restores the layout).  No LDS, so occupancy is bounded by registers
+// alone.
+template <int CTRL>
+__device__ __forceinline__ uint32_t pk_dpp(uint32_t x) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, CTRL, 0xF, 0xF, false);
+}
+
+template <int W, int ND, int XOR>
+__device__ __forceinline__ void pk_xlane_row(uint32_t (&x)[ND]) {
+  uint32_t p[ND];
+#pragma unroll
+  for (int k = 0; k < ND; k++) {
+    if constexpr (XOR == 1) p[k] = pk_dpp<0xB1>(x[k]);
+    else if constexpr (XOR == 2) p[k] = pk_dpp<0x4E>(x[k]);
+    else if constexpr (XOR == 4) p[k] = pk_dpp<0x1B>(pk_dpp<0x141>(x[k]));
+    else p[k] = pk_dpp<0x128>(x[k]);
+  }
+  pk_add_up<W, ND>(x, p);
+}
+
+// registers a and b (one dword of two words): swap halves so that each lane
+// holds a cross-row pair, apply the axis to the pair, swap back
+template <int W, int ND, int XOR>
+__device__ __forceinline__ void pk_xlane_swap(uint32_t (&a)[ND], uint32_t (&b)[ND]) {
+  uint32_t s0[ND], s1[ND];
+#pragma unroll
+  for (int k = 0; k < ND; k++) {
+    auto r = XOR == 16 ? __builtin_amdgcn_permlane16_swap(a[k], b[k], false, false)
+                       : __builtin_amdgcn_permlane32_swap(a[k], b[k], false, false);
+    s0[k] = r[0];
+    s1[k] = r[1];
+  }
+  uint32_t t0[ND], t1[ND];
+#pragma unroll
+  for (int k = 0; k < ND; k++) { t0[k] = s0[k]; t1[k] = s1[k]; }
+  pk_add_up<W, ND>(s0, t1);
+  pk_add_up<W, ND>(s1, t0);
+#pragma unroll
+  for (int k = 0; k < ND; k++) {
+    auto r = XOR == 16 ? __builtin_amdgcn_permlane16_swap(s0[k], s1[k], false, false)
+                       : __builtin_amdgcn_permlane32_swap(s0[k], s1[k], false, false);
+    a[k] = r[0];
+    b[k] = r[1];
+  }
+}
+
+template <int RR, int W, int ND, bool SUM, bool OUT2>
+__global__ void __launch_bounds__(CC_BLOCK) k_pfirst(PSweepArgs a) {
+  constexpr int C = 1 << RR;
+  constexpr int TW = C * 64;                  // words per wave tile
+  const int wave = (int)threadIdx.x >> 6;
+  const uint32_t ln = threadIdx.x & 63;
+  const uint32_t lane = ln * C;               // first word of this lane
+  const long long ntiles = a.X / TW;
+  for (long long t0 = (long long)blockIdx.x * (CC_BLOCK / 64); t0 < ntiles;
+       t0 += (long long)gridDim.x * (CC_BLOCK / 64)) {
+    const long long tile = t0 + wave;
+    if (tile >= ntiles) break;                // wave-uniform
+    const long long base = tile * TW;
+    uint32_t v[C][ND];
+    pk_load<C, W, ND, true, false>(a, base, 1u, lane, true, v);
+    pk_apply<2, C, 0, RR, W, ND>(v, (1 << RR) - 1);
+    // (scheduling fences every 8 words keep the compiler from interleaving
+    // the cross-lane chains of the whole cube, which costs a temporary per
+    // word)
+#pragma unroll
+    for (int c = 0; c < C; c++) {
+      pk_xlane_row<W, ND, 1>(v[c]);
+      pk_xlane_row<W, ND, 2>(v[c]);
+      pk_xlane_row<W, ND, 4>(v[c]);
+      pk_xlane_row<W, ND, 8>(v[c]);
+      if ((c & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+    }
+#pragma unroll
+    for (int c = 0; c < C / 2; c++) {
+      pk_xlane_swap<W, ND, 16>(v[c], v[c + C / 2]);
+      pk_xlane_swap<W, ND, 32>(v[c], v[c + C / 2]);
+      if ((c & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+    }
+    pk_store<C, W, ND, SUM, OUT2, true>(a, base, 1u, lane, v);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// First sweep with an LDS exchange on a persistent grid, predicate seed:
+// one cube per thread, and while a tile runs its rounds each thread already
+// has the next tile's counters in flight (raw 16-byte loads into registers),
+// so the memory pipe never waits for a tile's compute.  Round 0 is the
+// contiguous cube (vector loads), the last round the top cube (coalesced
+// stores), as in k_psweep.
+template <int C>
+__device__ __forceinline__ void pk_cnt_raw(const PSweepArgs &a, long long ub,
+                                           uint32_t lane, uint32_t (&pf)[C / 2]) {
+  const __amdgpu_buffer_rsrc_t r = pk_rsrc(a.cnt, ub, a.X, 2);
+#pragma unroll
+  for (int c = 0; c < C; c += 8) {
+    const pk_u32x4 q = __builtin_bit_cast(pk_u32x4,
+        __builtin_amdgcn_raw_buffer_load_b128(r, (int)(lane * 2u), c * 2, 0));
+    pf[c / 2] = q.x; pf[c / 2 + 1] = q.y; pf[c / 2 + 2] = q.z; pf[c / 2 + 3] = q.w;
+  }
+}
+
+template <int Q, int RR, int W, int ND, bool SUM, bool OUT2>
+__global__ void __launch_bounds__(CC_BLOCK) k_pfirstp(PSweepArgs a) {
+  extern __shared__ uint32_t Sp[];
+  constexpr int C = cc_ipow(Q, RR);
+  static_assert(C % 8 == 0, "vector counter loads");
+  const int PTP = pk_pad(a.P) + 1;
+  const int u = threadIdx.x;                 // this thread's round-0 cube
+  const uint32_t lane0 = (uint32_t)u * C;
+  const uint32_t lo = (uint32_t)a.plo, span = (uint32_t)(a.phi - a.plo);
+  uint32_t pf[C / 2];
+  long long tile = blockIdx.x;
+  if (tile < a.ntiles) pk_cnt_raw<C>(a, tile * a.P, lane0, pf);
+  for (; tile < a.ntiles; tile += gridDim.x) {
+    const long long base = tile * a.P;
+    uint32_t v[C][ND];
+#pragma unroll
+    for (int c = 0; c < C; c++) {
+#pragma unroll
+      for (int k = 0; k < ND; k++) v[c][k] = 0u;
+      const uint32_t cv = (pf[c >> 1] >> (16 * (c & 1))) & 0xFFFFu;
+      v[c][0] = (cv - lo) <= span ? 1u : 0u;
+    }
+    const long long next = tile + gridDim.x;
+    if (next < a.ntiles) pk_cnt_raw<C>(a, next * a.P, lane0, pf);
+    for (int rd = 0; rd < a.nrounds; rd++) {
+      const int qa = a.qa[rd];
+      const int pb = (u / qa) * qa * C + (u % qa);
+      const bool lin = a.lin[rd] != 0;
+      const int kp = a.kp[rd];
+      if (rd > 0) pk_lds_ld<C, ND>(Sp, PTP, pb, qa, 1, 0, lin, kp, v);
+      pk_apply<Q, C, 0, RR, W, ND>(v, a.amask[rd]);
+      if (rd == a.nrounds - 1)
+        pk_store<C, W, ND, SUM, OUT2>(a, base, (uint32_t)qa, (uint32_t)pb, v);
+      else
+        pk_lds_st<C, ND>(Sp, PTP, pb, qa, 1, 0, lin, kp, v);
+      __syncthreads();
+    }
   }
 }

@@ -45,13 +45,54 @@ hipError_t pk_go(const PSweepArgs &a, int blocks, int threads, size_t lds,
   const bool in2 = ND > 1 && a.in_kind == CC_IN_PACKED && a.in_nd > 1;
   const bool out2 = ND > 1 && !sum && a.out_nd > 1;
   void (*kf)(PSweepArgs) = nullptr;
-  if (a.TL < 0) {
-    // column kernel (k_pcol): -TL = lanes per cube, whole workgroups
+  if (a.kern == 1 || a.kern == 2) {
+    // column kernel (k_pcol): kern = lanes per cube, whole workgroups
     if constexpr (Q == 2 && RR >= 4 && pk_fits(Q, RR - 1, ND)) {
-      if (a.TL == -2) kf = pk_select<Q, RR, W, ND, true, 2, false>(sum, in2, out2);
+      if (a.kern == 2) kf = pk_select<Q, RR, W, ND, true, 2, false>(sum, in2, out2);
     }
     if constexpr (RR >= 2 && pk_fits(Q, RR, ND)) {
-      if (a.TL == -1) kf = pk_select<Q, RR, W, ND, true, 1, false>(sum, in2, out2);
+      if (a.kern == 1) kf = pk_select<Q, RR, W, ND, true, 1, false>(sum, in2, out2);
+    }
+    if (!kf) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(kf, dim3(blocks), dim3(CC_BLOCK), 0, st, a);
+    return hipGetLastError();
+  }
+  if (a.kern == 4) {
+    // first sweep, LDS exchange, persistent grid with next-tile prefetch
+    if constexpr (cc_ipow(Q, RR) % 8 == 0 && pk_fits(Q, RR, ND)) {
+      if (sum) kf = &k_pfirstp<Q, RR, W, ND, true, false>;
+      else if constexpr (ND > 1) kf = out2 ? &k_pfirstp<Q, RR, W, ND, false, true>
+                                           : &k_pfirstp<Q, RR, W, ND, false, false>;
+      else kf = &k_pfirstp<Q, RR, W, ND, false, false>;
+    }
+    if (!kf) return hipErrorInvalidValue;
+    if (lds > 64 * 1024) {
+      hipError_t e = hipFuncSetAttribute((const void *)kf,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         (int)lds);
+      if (e != hipSuccess) return e;
+    }
+    int per_cu = 0, dev = 0, ncu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)kf,
+                                                     threads, lds) != hipSuccess ||
+        per_cu < 1)
+      per_cu = 1;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) !=
+            hipSuccess || ncu < 1)
+      ncu = 256;
+    const long long resident = (long long)per_cu * ncu;
+    const int g = (int)(blocks < resident ? blocks : resident);
+    hipLaunchKernelGGL(kf, dim3(g), dim3(threads), lds, st, a);
+    return hipGetLastError();
+  }
+  if (a.kern == 3) {
+    // LDS-free first sweep (k_pfirst): RR in-thread axes + 6 lane axes
+    if constexpr (Q == 2 && RR >= 4 && (1 << RR) * ND <= 64) {
+      if (sum) kf = &k_pfirst<RR, W, ND, true, false>;
+      else if constexpr (ND > 1) kf = out2 ? &k_pfirst<RR, W, ND, false, true>
+                                           : &k_pfirst<RR, W, ND, false, false>;
+      else kf = &k_pfirst<RR, W, ND, false, false>;
     }
     if (!kf) return hipErrorInvalidValue;
     hipLaunchKernelGGL(kf, dim3(blocks), dim3(CC_BLOCK), 0, st, a);

@@ -58,6 +58,7 @@ struct Sweep {
 // One sweep of the packed-layer path (cc_psweep.hpp).
 struct PSweep {
   int j0, nax, RR, W, ND, P, TL, threads;
+  int kern;  // 0 k_psweep, 1 / 2 k_pcol (lanes per cube), 3 k_pfirst
   long long s, ncolb, ntiles;
   int nrounds;
   int ca[CC_PK_MAXRD], amask[CC_PK_MAXRD], qa[CC_PK_MAXRD];
@@ -126,6 +127,8 @@ struct Engine {
   int generic_only = 0;      // CC_SWEEP_KERNEL=generic: no round kernel
   int packed = 1;            // CC_PACKED=0: never take the packed path
   int pk_col = 2;            // CC_PK_COL: column kernel 0 off, 1 whole, 2 split
+  int pk_first = 1;          // CC_PK_FIRST: first sweep 0 k_psweep, 1 persistent
+                             // with prefetch, 2 LDS-free (cross-lane)
   PPlan pfield;              // packed plan of gain / loss (predicate seeds)
   // optional per-sweep timing of the last transform (additive ABI)
   int prof = 0;
@@ -482,12 +485,26 @@ bool pk_sweep(const PBounds &B, int j, int k, bool pred_seed, bool cnt_out,
   w.RR = std::min(rrmax, m);
   int C = 1;
   for (int i = 0; i < w.RR; i++) C *= q;
-  if (j == 0) {
+  w.kern = 0;
+  if (j == 0 && E.pk_first == 2 && q == 2 && m >= 10 && m <= 12 &&
+      (1 << (m - 6)) * w.ND <= 64) {
+    // LDS-free first sweep: m - 6 axes in-thread, 6 across the lanes
+    w.kern = 3;
+    w.RR = m - 6;
+    w.TL = 1;
+    w.lds = 0;
+    w.threads = CC_BLOCK;
+  } else if (j == 0) {
     w.TL = 1;
     w.lds = m > w.RR ? (size_t)w.ND * (pk_pad_h(w.P) + 1) * 4 : 0;
     if (w.lds > (size_t)E.lds_budget) return false;
     const int ncube = w.P / C;
     w.threads = std::min(256, (ncube + 63) / 64 * 64);
+    // persistent grid with next-tile prefetch: one cube per thread, a
+    // whole number of waves, vector counter loads, at least two rounds
+    if (E.pk_first == 1 && pred_seed && m > w.RR && C % 8 == 0 &&
+        ncube <= CC_BLOCK && ncube % 64 == 0)
+      w.kern = 4;
   } else if (m == w.RR) {
     w.lds = 0;
     // column kernel: one cube per lane (or per lane pair, binary axes
@@ -496,7 +513,8 @@ bool pk_sweep(const PBounds &B, int j, int k, bool pred_seed, bool cnt_out,
     const bool split = col >= 2 && q == 2 && w.RR >= 4 && w.s % 128 == 0;
     const bool whole = col >= 1 && w.RR >= 2 && w.s % 256 == 0;
     if (split || whole) {
-      w.TL = split ? -2 : -1;
+      w.kern = split ? 2 : 1;
+      w.TL = 1;
       w.threads = CC_BLOCK;
     } else {
       w.TL = (int)std::min<long long>(256, w.s);
@@ -512,7 +530,10 @@ bool pk_sweep(const PBounds &B, int j, int k, bool pred_seed, bool cnt_out,
     if (!w.TL) return false;
     w.threads = std::min(256, (w.P / C * w.TL + 63) / 64 * 64);
   }
-  if (w.TL < 0) {
+  if (w.kern == 3) {
+    w.ncolb = 1;
+    w.ntiles = E.X / ((long long)64 << w.RR);   // wave tiles
+  } else if (w.kern) {
     w.ncolb = 1;
     w.ntiles = (E.X / (w.s * P)) * w.s;   // cubes (one per column)
   } else {
@@ -593,7 +614,9 @@ PPlan pk_plan(double s0, double cap, bool pred_seed, bool cnt_out) {
 }
 
 hipError_t pk_launch(const PSweep &w, const PSweepArgs &a) {
-  const long long units = w.TL < 0 ? w.ntiles / (CC_BLOCK / -w.TL) : w.ntiles;
+  const long long units = w.kern == 3 ? (w.ntiles + CC_BLOCK / 64 - 1) / (CC_BLOCK / 64)
+                         : (w.kern == 1 || w.kern == 2) ? w.ntiles / (CC_BLOCK / w.kern)
+                                                        : w.ntiles;
   const int blocks = (int)std::min<long long>(units, 1LL << 30);
   switch (E.ax[0]) {
     case 2: return cc_psweep_q2(a, w.RR, w.W, w.ND, blocks, w.threads, w.lds, E.st);
@@ -624,6 +647,7 @@ int run_ptransform(const PPlan &pl, int in_kind, int out_kind, int plo,
     a.X = E.X;
     a.s = w.s; a.ncolb = w.ncolb; a.ntiles = w.ntiles;
     a.P = w.P; a.TL = w.TL; a.nax = w.nax; a.L = E.L;
+    a.kern = w.kern;
     a.in_kind = i == 0 ? in_kind : CC_IN_PACKED;
     a.out_kind = i == ns - 1 ? out_kind : CC_OUT_PACKED;
     a.plo = plo; a.phi = phi;
@@ -1015,6 +1039,8 @@ static int init_cell(int problem, int n, const int *ax, int R,
     E.packed = !(pe && !strcmp(pe, "0")) && !E.generic_only;
     const char *pc = getenv("CC_PK_COL");
     E.pk_col = pc ? atoi(pc) : 2;
+    const char *pf = getenv("CC_PK_FIRST");
+    E.pk_first = pf ? atoi(pf) : 1;
     E.pfield = pk_plan(1.0, 1e300, true, false);
   }
   E.uploaded.clear();
@@ -1462,11 +1488,13 @@ int dct_sweep_plan(int *out, int cap) {
 }
 
 // dct_sweep_plan2(kind, out, cap): the schedule a transform of this kind
-// runs now (kind 0: gain / loss, 1: recount of the pending code), 12 ints
+// runs now (kind 0: gain / loss, 1: recount of the pending code), 13 ints
 // per sweep: first axis, axes, P, TL, register round RR (0: unpacked
 // kernels), field width W (32: unpacked layer planes), dwords per word in
 // registers, algorithmic bytes per word read and written, LDS bytes per
-// workgroup, threads per workgroup, rounds.  Returns the sweep count.
+// workgroup, threads per workgroup, rounds, kernel (0 unpacked, 1 LDS-tiled
+// packed, 2 / 3 column kernel with 1 / 2 lanes per cube, 4 LDS-free first
+// sweep, 5 persistent first sweep with prefetch).  Returns the sweep count.
 int dct_sweep_plan2(int kind, int *out, int cap) {
   if (!E.live || !out || kind < 0 || kind > 1) return -1;
   PPlan cp;
@@ -1476,10 +1504,11 @@ int dct_sweep_plan2(int kind, int *out, int cap) {
     const int ns = (int)pp.sw.size();
     for (int i = 0; i < ns && i < cap; i++) {
       const PSweep &w = pp.sw[i];
-      int *o = out + 12 * i;
+      int *o = out + 13 * i;
       o[0] = w.j0; o[1] = w.nax; o[2] = w.P; o[3] = w.TL; o[4] = w.RR;
       o[5] = w.W; o[6] = w.ND; o[7] = w.in_bytes; o[8] = w.out_bytes;
       o[9] = (int)w.lds; o[10] = w.threads; o[11] = w.nrounds;
+      o[12] = 1 + w.kern;
     }
     return ns;
   }
@@ -1498,10 +1527,11 @@ int dct_sweep_plan2(int kind, int *out, int cap) {
       rd = i == 0 ? (field ? 2 : 4) : 4;
       wr = i == ns - 1 ? (field ? 4 : 2) : 4;
     }
-    int *o = out + 12 * i;
+    int *o = out + 13 * i;
     o[0] = sw.j0; o[1] = sw.nax; o[2] = sw.P; o[3] = sw.TL; o[4] = 0;
     o[5] = 32; o[6] = E.L; o[7] = rd; o[8] = wr;
     o[9] = (int)(E.l16_field ? sw.lds16 : sw.lds); o[10] = CC_BLOCK; o[11] = 0;
+    o[12] = 0;
   }
   return ns;
 }

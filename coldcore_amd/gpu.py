@@ -382,13 +382,13 @@ class GpuBackend:
         """Schedule a transform runs now: kind 0 gain / loss, 1 recount of
         the pending code.  Per sweep: axes, tile, register round and field
         format (W = 32: unpacked planes), algorithmic bytes per word."""
-        buf = (ctypes.c_int * (12 * 41))()
+        buf = (ctypes.c_int * (13 * 41))()
         ns = self._lib.dct_sweep_plan2(kind, buf, 40)
         if ns < 0:
             _check("sweep_plan2", ns)
         keys = ("first_axis", "axes", "P", "TL", "RR", "W", "ND", "bytes_in",
-                "bytes_out", "lds_bytes", "threads", "rounds")
-        return [dict(zip(keys, buf[12 * i:12 * i + 12])) for i in range(ns)]
+                "bytes_out", "lds_bytes", "threads", "rounds", "kernel")
+        return [dict(zip(keys, buf[13 * i:13 * i + 13])) for i in range(ns)]
 
     def sweep_profile(self, on=True):
         _check("sweep_profile", self._lib.dct_sweep_profile(1 if on else 0))

@@ -148,11 +148,13 @@ int  dct_layers_read_at(const long long *idx, int k, int32_t *out);
  * For benchmarks and the roofline accounting of each sweep's traffic. */
 int  dct_sweep_plan(int *out, int cap);
 /* The schedule a transform runs now (kind 0: gain / loss; 1: recount of the
- * pending code): 12 ints per sweep -- first axis, axes, P, TL, register
+ * pending code): 13 ints per sweep -- first axis, axes, P, TL, register
  * round RR (0: unpacked per-layer kernels), field width W of the packed
  * layers (32: unpacked planes), dwords per word in registers, algorithmic
  * bytes per word read and written, LDS bytes, threads per workgroup,
- * rounds.  Returns the sweep count.  Homogeneous Hamming cells with axis
+ * rounds, kernel (0 unpacked, 1 LDS-tiled packed, 2 / 3 column kernel with
+ * 1 / 2 lanes per cube, 4 LDS-free first sweep, 5 persistent first sweep
+ * with prefetch).  Returns the sweep count.  Homogeneous Hamming cells with axis
  * size <= 5 run packed-layer sweeps: all R+1 layers of a word in bit
  * fields of one or two dwords, with field widths proven from closed-form
  * layer bounds, so inter-sweep traffic is 4-8 bytes per word. */

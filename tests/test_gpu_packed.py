@@ -19,13 +19,15 @@ from oracle.dp import NumpyDP
 pytestmark = pytest.mark.gpu
 
 CELLS = [([2] * 16, 3), ([2] * 14, 4), ([2] * 12, 6), ([2] * 13, 1),
+         ([2] * 11, 2), ([2] * 10, 3), ([2] * 18, 3),
          ([3] * 9, 3), ([3] * 8, 1), ([4] * 7, 3), ([5] * 6, 3), ([5] * 6, 2)]
 IDS = [f"{a[0]}^{len(a)}-R{r}" for a, r in CELLS]
 # schedule variants: LDS budgets that force short first sweeps and
 # multi-round strided tiles; the column kernel off, or whole-cube only
 VARIANTS = {"default": {}, "lds16k": {"CC_SWEEP_LDS": "16384"},
             "lds4k": {"CC_SWEEP_LDS": "4096"}, "nocol": {"CC_PK_COL": "0"},
-            "col1": {"CC_PK_COL": "1"}}
+            "col1": {"CC_PK_COL": "1"}, "nofirst": {"CC_PK_FIRST": "0"},
+            "xlanefirst": {"CC_PK_FIRST": "2"}}
 
 
 def _check_cell(g, c, rng, mults):
