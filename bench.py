@@ -97,21 +97,18 @@ def schedule_key(plan):
     return " | ".join(describe(plan))
 
 
-def measured_traffic(plan):
-    """HBM bytes per launch of the dominant kernel, as measured by the
-    rocprofv3 FETCH_SIZE / WRITE_SIZE runs summarised in profiles/ (copy-
-    calibrated); None when the summary was taken on another schedule."""
-    path = os.path.join(ROOT, "profiles", "k_sweep_r_K2_24_3.md")
+def measured_traffic(kernel):
+    """HBM bytes per launch of a sweep kernel, as measured by the rocprofv3
+    FETCH_SIZE / WRITE_SIZE runs summarised in profiles/ (copy-calibrated);
+    None when no summary covers that kernel."""
+    path = os.path.join(ROOT, "profiles", "k_pcol_K2_24_3.md")
     try:
         text = open(path).read()
     except OSError:
         return None
     import re
-    t = re.search(r"^traffic_bytes_per_launch: (\d+)$", text, re.M)
-    sch = re.search(r"^schedule: (.*)$", text, re.M)
-    if not t or not sch or sch.group(1).strip() != schedule_key(plan):
-        return None
-    return int(t.group(1))
+    t = re.search(r"^traffic\[" + re.escape(kernel) + r"\]: (\d+)$", text, re.M)
+    return int(t.group(1)) if t else None
 
 
 def run_config1(GpuBackend, with_cpu):
@@ -272,7 +269,7 @@ def main():
         dom_key = max(groups, key=lambda k: groups[k][0])
         dom_ms, dom_bytes, dom_n = groups[dom_key]
         dom_gbs = dom_bytes / (dom_ms * 1e-3) / 1e9
-        traffic = measured_traffic(plan)
+        traffic = measured_traffic(dom_key)
 
         extra = {}
         if not args.no_extra:

@@ -8,6 +8,8 @@
 
 constexpr bool pk_fits(int q, int rr, int nd) { return cc_ipow(q, rr) * nd <= 128; }
 
+// blocks == 0: only report whether the kernel instantiation exists (the
+// planner checks every sweep it schedules).
 // The kernel instantiation for one (Q, RR, W, ND): SUM (last sweep), IN2 /
 // OUT2 (two-plane compact input / output, only with ND > 1), and for the
 // column kernel the lanes per cube.
@@ -54,6 +56,7 @@ hipError_t pk_go(const PSweepArgs &a, int blocks, int threads, size_t lds,
       if (a.kern == 1) kf = pk_select<Q, RR, W, ND, true, 1, false>(sum, in2, out2);
     }
     if (!kf) return hipErrorInvalidValue;
+    if (blocks == 0) return hipSuccess;
     hipLaunchKernelGGL(kf, dim3(blocks), dim3(CC_BLOCK), 0, st, a);
     return hipGetLastError();
   }
@@ -66,6 +69,7 @@ hipError_t pk_go(const PSweepArgs &a, int blocks, int threads, size_t lds,
       else kf = &k_pfirstp<Q, RR, W, ND, false, false>;
     }
     if (!kf) return hipErrorInvalidValue;
+    if (blocks == 0) return hipSuccess;
     if (lds > 64 * 1024) {
       hipError_t e = hipFuncSetAttribute((const void *)kf,
                                          hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -95,6 +99,7 @@ hipError_t pk_go(const PSweepArgs &a, int blocks, int threads, size_t lds,
       else kf = &k_pfirst<RR, W, ND, false, false>;
     }
     if (!kf) return hipErrorInvalidValue;
+    if (blocks == 0) return hipSuccess;
     hipLaunchKernelGGL(kf, dim3(blocks), dim3(CC_BLOCK), 0, st, a);
     return hipGetLastError();
   }
@@ -102,6 +107,7 @@ hipError_t pk_go(const PSweepArgs &a, int blocks, int threads, size_t lds,
     kf = a.in_kind == CC_IN_PACKED
              ? pk_select<Q, RR, W, ND, false, 1, false>(sum, in2, out2)
              : pk_select<Q, RR, W, ND, false, 1, true>(sum, false, out2);
+    if (blocks == 0) return hipSuccess;
     if (lds > 64 * 1024) {
       hipError_t e = hipFuncSetAttribute((const void *)kf,
                                          hipFuncAttributeMaxDynamicSharedMemorySize,

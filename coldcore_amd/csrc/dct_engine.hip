@@ -19,6 +19,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <string>
 #include <unordered_map>
 #include <vector>
 
@@ -37,6 +38,8 @@
       return -(100 + (int)e_);                                              \
     }                                                                       \
   } while (0)
+
+extern "C" int dct_autotune(int reps, float *best_us);
 
 namespace {
 
@@ -69,6 +72,7 @@ struct PSweep {
   int odw[CC_MAXL], ooff[CC_MAXL], obits[CC_MAXL];
   size_t lds;
   int in_bytes, out_bytes;  // algorithmic bytes per word
+  double cost;              // estimated relative time per word
 };
 
 struct PPlan {
@@ -127,6 +131,8 @@ struct Engine {
   int generic_only = 0;      // CC_SWEEP_KERNEL=generic: no round kernel
   int packed = 1;            // CC_PACKED=0: never take the packed path
   int pk_col = 2;            // CC_PK_COL: column kernel 0 off, 1 whole, 2 split
+  int autotune = 1;          // CC_AUTOTUNE=0: never time schedules by itself
+  int tuned = 0;             // the schedules of this cell were timed
   int pk_first = 1;          // CC_PK_FIRST: first sweep 0 k_psweep, 1 persistent
                              // with prefetch, 2 LDS-free (cross-lane)
   PPlan pfield;              // packed plan of gain / loss (predicate seeds)
@@ -465,6 +471,18 @@ void pk_rounds(PSweep &w, int q) {
   w.nrounds = nr;
 }
 
+// Launch a packed sweep kernel, or with blocks == 0 only check that its
+// instantiation exists.
+hipError_t pk_dispatch(const PSweep &w, const PSweepArgs &a, int blocks) {
+  switch (E.ax[0]) {
+    case 2: return cc_psweep_q2(a, w.RR, w.W, w.ND, blocks, w.threads, w.lds, E.st);
+    case 3: return cc_psweep_q3(a, w.RR, w.W, w.ND, blocks, w.threads, w.lds, E.st);
+    case 4: return cc_psweep_q4(a, w.RR, w.W, w.ND, blocks, w.threads, w.lds, E.st);
+    case 5: return cc_psweep_q5(a, w.RR, w.W, w.ND, blocks, w.threads, w.lds, E.st);
+  }
+  return hipErrorInvalidValue;
+}
+
 // One sweep over axes [j, k); prev is the sweep before it (its compact
 // output is this sweep's input).
 bool pk_sweep(const PBounds &B, int j, int k, bool pred_seed, bool cnt_out,
@@ -483,8 +501,24 @@ bool pk_sweep(const PBounds &B, int j, int k, bool pred_seed, bool cnt_out,
   const int rrmax = pk_rrmax(q, w.ND);
   if (rrmax < 1) return false;
   w.RR = std::min(rrmax, m);
-  int C = 1;
-  for (int i = 0; i < w.RR; i++) C *= q;
+  auto qpow = [q](int e) { int c = 1; for (int i = 0; i < e; i++) c *= q; return c; };
+  if (j == 0 && m > 1) {
+    // first sweep: a cube small enough that the tile has whole waves of
+    // cubes (64..256 threads, one cube each) when possible
+    int best = 0;
+    for (int r = std::min(rrmax, m); r >= 1 && !best; r--) {
+      const long long nc = P / qpow(r);
+      if (nc >= 64 && nc <= CC_BLOCK && nc % 64 == 0) best = r;
+    }
+    for (int r = std::min(rrmax, m); r >= 1 && !best; r--)
+      if (P / qpow(r) >= 64) best = r;
+    if (best) w.RR = best;
+  } else if (j > 0 && E.pk_col >= 2 && q == 2 && m == rrmax + 1 && m >= 4 &&
+             w.s % 128 == 0 && qpow(m - 1) * w.ND <= 128) {
+    // a lane pair holds the cube: one more axis than a lane alone
+    w.RR = m;
+  }
+  int C = qpow(w.RR);
   w.kern = 0;
   if (j == 0 && E.pk_first == 2 && q == 2 && m >= 10 && m <= 12 &&
       (1 << (m - 6)) * w.ND <= 64) {
@@ -511,7 +545,8 @@ bool pk_sweep(const PBounds &B, int j, int k, bool pred_seed, bool cnt_out,
     // with big cubes), whole workgroups of consecutive columns
     const int col = E.pk_col;
     const bool split = col >= 2 && q == 2 && w.RR >= 4 && w.s % 128 == 0;
-    const bool whole = col >= 1 && w.RR >= 2 && w.s % 256 == 0;
+    const bool whole = col >= 1 && w.RR >= 2 && w.s % 256 == 0 &&
+                       (long long)C * w.ND <= 128;
     if (split || whole) {
       w.kern = split ? 2 : 1;
       w.TL = 1;
@@ -561,13 +596,49 @@ bool pk_sweep(const PBounds &B, int j, int k, bool pred_seed, bool cnt_out,
     w.out_bytes = 4 * w.out_nd;
   }
   pk_rounds(w, q);
+  {
+    // only schedule sweeps whose kernel instantiation exists
+    PSweepArgs a;
+    memset(&a, 0, sizeof(a));
+    a.kern = w.kern;
+    a.in_kind = prev ? CC_IN_PACKED : (pred_seed ? CC_IN_PRED : CC_IN_CODE);
+    a.out_kind = k == E.n ? (cnt_out ? CC_OUT_CNT : CC_OUT_MAP) : CC_OUT_PACKED;
+    a.in_nd = w.in_nd;
+    a.out_nd = w.out_nd;
+    if (pk_dispatch(w, a, 0) != hipSuccess) return false;
+  }
+  // relative time: algorithmic bytes over the bandwidth this kernel shape
+  // reaches (MI355X measurements, K_2(24,3), K_2(30,4), K_5(10,3) sweeps,
+  // normalised to the split column kernel at ~4.8 TB/s)
+  double f;
+  if (w.kern == 2) f = 1.0;
+  else if (w.kern == 1) f = 1.35;
+  else if (w.kern == 4) f = 1.25;
+  else if (w.kern == 3) f = 1.8;
+  else if (j == 0) {
+    const long long nc = P / C;
+    f = 2.0 * (nc < 64 ? 64.0 / (double)std::max(1LL, nc) : 1.0);
+  } else if (w.nrounds > 1) {
+    f = w.TL >= 64 ? 1.8 : (w.TL >= 32 ? 2.2 : 2.6);
+  } else {
+    f = w.TL >= 64 ? 1.8 : 2.6;
+  }
+  if (j == 0 && w.nrounds > 2) f *= 1.0 + 0.25 * (w.nrounds - 2);
+  if (j == 0 && w.lds > 0) {
+    // workgroups a CU holds at once, by LDS: few resident waves leave the
+    // memory pipe idle between a tile's load and its stores
+    const double waves = (double)(160 * 1024 / (long long)w.lds) * (w.threads / 64);
+    if (waves < 8) f *= 8.0 / std::max(1.0, waves);
+  }
+  w.cost = (w.in_bytes + w.out_bytes) * f;
   return true;
 }
 
-// Schedule with the fewest algorithmic bytes, then the fewest sweeps, then
-// the fewest LDS-staged strided tiles: a shortest path over sweep
-// boundaries (the compact format at a boundary depends only on how many
-// axes precede it, so a sweep's cost depends only on its two ends).
+// Schedule with the least estimated time (algorithmic bytes weighted by the
+// kernel shape's measured bandwidth), then the fewest bytes and sweeps: a
+// shortest path over sweep boundaries (the compact format at a boundary
+// depends only on how many axes precede it, so a sweep's cost depends only
+// on its two ends).
 PPlan pk_plan(double s0, double cap, bool pred_seed, bool cnt_out) {
   PPlan best;
   if (!E.packed || E.prob != CC_HAMMING || E.L < 2) return best;
@@ -577,13 +648,13 @@ PPlan pk_plan(double s0, double cap, bool pred_seed, bool cnt_out) {
   if (q < 2 || q > 5) return best;
   const PBounds B{q, E.R, s0, cap};
   const int n = E.n;
-  struct Cost { long long bytes; int sweeps, penalty; bool ok; };
+  struct Cost { double t; long long bytes; int sweeps; bool ok; };
   auto better = [](const Cost &x, const Cost &y) {
     if (!y.ok) return x.ok;
     if (!x.ok) return false;
+    if (x.t != y.t) return x.t < y.t;
     if (x.bytes != y.bytes) return x.bytes < y.bytes;
-    if (x.sweeps != y.sweeps) return x.sweeps < y.sweeps;
-    return x.penalty < y.penalty;
+    return x.sweeps < y.sweeps;
   };
   std::vector<Cost> dp(n + 1, Cost{0, 0, 0, false});
   std::vector<PSweep> via(n + 1);
@@ -598,11 +669,13 @@ PPlan pk_plan(double s0, double cap, bool pred_seed, bool cnt_out) {
       continue;
     for (int k = j + 1; k <= std::min(n, j + CC_MAXSW); k++) {
       PSweep w;
-      if (!pk_sweep(B, j, k, pred_seed, cnt_out, j ? &prev : nullptr, w)) continue;
-      const bool staged = j > 0 && w.nrounds > 1;
-      const int pen = (staged ? 1 : 0) + ((j > 0 && w.TL < 64 && w.s >= 64) ? 2 : 0);
-      const Cost c{dp[j].bytes + w.in_bytes + w.out_bytes, dp[j].sweeps + 1,
-                   dp[j].penalty + pen, true};
+      const bool okw = pk_sweep(B, j, k, pred_seed, cnt_out, j ? &prev : nullptr, w);
+      if (getenv("CC_PK_DEBUG"))
+        fprintf(stderr, "pk_plan [%d,%d) ok=%d kern=%d RR=%d W=%d ND=%d TL=%d cost=%.2f\n",
+                j, k, (int)okw, w.kern, w.RR, w.W, w.ND, w.TL, okw ? w.cost : -1.0);
+      if (!okw) continue;
+      const Cost c{dp[j].t + w.cost, dp[j].bytes + w.in_bytes + w.out_bytes,
+                   dp[j].sweeps + 1, true};
       if (better(c, dp[k])) { dp[k] = c; via[k] = w; from[k] = j; }
     }
   }
@@ -617,14 +690,8 @@ hipError_t pk_launch(const PSweep &w, const PSweepArgs &a) {
   const long long units = w.kern == 3 ? (w.ntiles + CC_BLOCK / 64 - 1) / (CC_BLOCK / 64)
                          : (w.kern == 1 || w.kern == 2) ? w.ntiles / (CC_BLOCK / w.kern)
                                                         : w.ntiles;
-  const int blocks = (int)std::min<long long>(units, 1LL << 30);
-  switch (E.ax[0]) {
-    case 2: return cc_psweep_q2(a, w.RR, w.W, w.ND, blocks, w.threads, w.lds, E.st);
-    case 3: return cc_psweep_q3(a, w.RR, w.W, w.ND, blocks, w.threads, w.lds, E.st);
-    case 4: return cc_psweep_q4(a, w.RR, w.W, w.ND, blocks, w.threads, w.lds, E.st);
-    case 5: return cc_psweep_q5(a, w.RR, w.W, w.ND, blocks, w.threads, w.lds, E.st);
-  }
-  return hipErrorInvalidValue;
+  const int blocks = (int)std::max(1LL, std::min<long long>(units, 1LL << 30));
+  return pk_dispatch(w, a, blocks);
 }
 
 // Run a packed transform: in_kind CC_IN_PRED (predicate [plo <= cnt <= phi])
@@ -1041,6 +1108,13 @@ static int init_cell(int problem, int n, const int *ax, int R,
     E.pk_col = pc ? atoi(pc) : 2;
     const char *pf = getenv("CC_PK_FIRST");
     E.pk_first = pf ? atoi(pf) : 1;
+    // an explicit schedule setting turns the automatic timing off
+    const char *pa = getenv("CC_AUTOTUNE");
+    E.autotune = pa ? atoi(pa) : 1;
+    for (const char *k : {"CC_PACKED", "CC_SWEEP_KERNEL", "CC_SWEEP_LDS",
+                          "CC_PK_COL", "CC_PK_FIRST", "CC_LAYER16"})
+      if (getenv(k) && !pa) E.autotune = 0;
+    E.tuned = 0;
     E.pfield = pk_plan(1.0, 1e300, true, false);
   }
   E.uploaded.clear();
@@ -1146,6 +1220,12 @@ int dct_transform(int init_mode, int out_mode) {
     pmode = CC_CRIT;
   }
   E.layers_live = 0;
+  if (in_kind == CC_IN_PRED && E.autotune && !E.tuned && E.X >= (1LL << 22)) {
+    // big cell, first gain / loss transform: time the candidate schedules
+    // once (the transform below overwrites the map they leave)
+    E.tuned = 1;
+    if (int rc = dct_autotune(2, nullptr); rc < 0) return rc;
+  }
   const int out_kind = out_mode == 1 ? CC_OUT_MAP : CC_OUT_CNT;
   PPlan code_plan;
   if (in_kind == CC_IN_CODE) code_plan = pk_code_plan(out_kind == CC_OUT_CNT);
@@ -1459,6 +1539,125 @@ int dct_layers_read_at(const long long *idx, int k, int32_t *out) {
   for (int i = 0; i < k && Lv < L; i++)
     for (int d = Lv; d < L; d++) out[(size_t)i * L + d] = 0;
   return 0;
+}
+
+// ---- additive (not in the reference ABI): planner probe --------------------
+// dct_plan_probe: the packed schedule the engine would plan for a Hamming
+// cell (homogeneous axes), without a device and without a live cell: kind
+// 0 gain / loss, kind 1 recount of a code with multiplicities <= s0 and M
+// words.  Rows as dct_sweep_plan2.  Returns the sweep count (0: the cell
+// runs the unpacked kernels), -1 on bad arguments or while a cell is live.
+int dct_plan_probe(int n, const int *ax, int R, int kind, int s0, int M,
+                   int *out, int cap) {
+  if (E.live || n < 1 || n > CC_MAXN || !ax || R < 1 || R > CC_MAXR || !out ||
+      kind < 0 || kind > 1 || s0 < 1 || M < 1)
+    return -1;
+  Engine saved = E;
+  E.prob = CC_HAMMING; E.n = n; E.R = R; E.L = R + 1;
+  E.pw[0] = 1;
+  for (int i = 0; i < n; i++) {
+    if (ax[i] < 2 || ax[i] > 10) { E = saved; return -1; }
+    E.ax[i] = ax[i];
+    E.pw[i + 1] = E.pw[i] * ax[i];
+    if (E.pw[i + 1] > (1LL << 40)) { E = saved; return -1; }
+  }
+  E.X = E.pw[n];
+  E.lds_max = 160 * 1024;
+  const char *lb = getenv("CC_SWEEP_LDS");
+  E.lds_budget = std::min(lb ? std::max(4096, atoi(lb)) : 64 * 1024, E.lds_max);
+  const char *pe = getenv("CC_PACKED");
+  const char *sk = getenv("CC_SWEEP_KERNEL");
+  E.packed = !(pe && !strcmp(pe, "0")) && !(sk && !strcmp(sk, "generic"));
+  const char *pc = getenv("CC_PK_COL");
+  E.pk_col = pc ? atoi(pc) : 2;
+  const char *pf = getenv("CC_PK_FIRST");
+  E.pk_first = pf ? atoi(pf) : 1;
+  const PPlan pl = kind == 0 ? pk_plan(1.0, 1e300, true, false)
+                             : pk_plan((double)s0, (double)M, false, true);
+  int ns = 0;
+  if (pl.ok) {
+    ns = (int)pl.sw.size();
+    for (int i = 0; i < ns && i < cap; i++) {
+      const PSweep &w = pl.sw[i];
+      int *o = out + 13 * i;
+      o[0] = w.j0; o[1] = w.nax; o[2] = w.P; o[3] = w.TL; o[4] = w.RR;
+      o[5] = w.W; o[6] = w.ND; o[7] = w.in_bytes; o[8] = w.out_bytes;
+      o[9] = (int)w.lds; o[10] = w.threads; o[11] = w.nrounds;
+      o[12] = 1 + w.kern;
+    }
+  }
+  E = saved;
+  return ns;
+}
+
+// ---- additive (not in the reference ABI): schedule autotuner -------------
+// dct_autotune(reps, best_us): time the candidate schedules of the gain /
+// loss transform on the live cell and keep the fastest.  Candidates are the
+// planner's schedule under every combination of LDS budget (16-128 KB),
+// column-kernel mode (off / whole cube / lane pairs) and first-sweep kernel
+// (LDS-tiled / persistent with prefetch), plus the unpacked per-layer
+// kernels; duplicates are timed once.  Each is run once untimed and then
+// reps times between HIP events on the engine's stream.  The winning
+// options also steer the recount schedule.  Like a transform, it
+// overwrites the map plane; cnt and the represented solution are unchanged.
+// Returns the number of schedules timed (best_us: the winner's
+// microseconds per transform), or < 0.
+int dct_autotune(int reps, float *best_us) {
+  if (!E.live || reps < 1) return -1;
+  E.tuned = 1;
+  struct Opt { int lds, col, first, packed; };
+  std::vector<Opt> opts;
+  for (int lds : {16384, 32768, 65536, 98304, 131072}) {
+    if (lds > E.lds_max) continue;
+    for (int col : {2, 1, 0})
+      for (int first : {1, 0}) opts.push_back({lds, col, first, 1});
+  }
+  opts.push_back({E.lds_budget, E.pk_col, E.pk_first, 0});
+  const Opt saved{E.lds_budget, E.pk_col, E.pk_first, E.packed};
+  std::vector<std::string> seen;
+  float best = 1e30f;
+  Opt win = saved;
+  PPlan wplan = E.pfield;
+  int timed = 0;
+  hipEvent_t e0 = E.ev[0], e1 = E.ev[1];
+  for (const Opt &o : opts) {
+    E.lds_budget = o.lds; E.pk_col = o.col; E.pk_first = o.first; E.packed = o.packed;
+    PPlan pl = o.packed ? pk_plan(1.0, 1e300, true, false) : PPlan();
+    std::string sig = "unpacked";
+    if (pl.ok) {
+      sig.clear();
+      for (const PSweep &w : pl.sw) {
+        char b[96];
+        snprintf(b, sizeof(b), "%d.%d.%d.%d.%d.%d.%d.%zu|", w.j0, w.nax, w.RR, w.W,
+                 w.ND, w.TL, w.kern, w.lds);
+        sig += b;
+      }
+    } else if (o.packed) {
+      continue;                      // no packed schedule for this cell
+    }
+    if (std::find(seen.begin(), seen.end(), sig) != seen.end()) continue;
+    seen.push_back(sig);
+    const int prof = E.prof;
+    E.prof = 0;
+    float ms = 0;
+    for (int r = 0; r <= reps; r++) {
+      if (r == 1) CK(hipEventRecord(e0, E.st));
+      int rc = pl.ok ? run_ptransform(pl, CC_IN_PRED, CC_OUT_MAP, 0, E.mu - 1)
+                     : run_transform(CC_IN_PRED, CC_OUT_MAP, CC_DEF);
+      if (rc) { E.prof = prof; return rc; }
+    }
+    CK(hipEventRecord(e1, E.st));
+    CK(hipEventSynchronize(e1));
+    CK(hipEventElapsedTime(&ms, e0, e1));
+    E.prof = prof;
+    timed++;
+    if (ms < best) { best = ms; win = o; wplan = pl; }
+  }
+  E.lds_budget = win.lds; E.pk_col = win.col; E.pk_first = win.first;
+  E.packed = win.packed;
+  E.pfield = wplan;
+  if (best_us) *best_us = best * 1000.0f / reps;
+  return timed;
 }
 
 // ---- additive (not in the reference ABI): schedule and sweep timing ------

@@ -32,8 +32,9 @@ OPTIONAL = ("dct_map_hist", "dct_set_mu", "dct_get_mu", "dct_ball_gather_mu",
             "dct_set_tbl2", "dct_use_fmt", "dct_ball_update_delta",
             "dct_swap_eval", "dct_stats")
 ADDITIVE = ("dct_init3w", "dct_sweep_plan", "dct_sweep_plan2",
-            "dct_sweep_profile", "dct_sweep_times", "dct_transform_layers",
-            "dct_layers_read_at")
+            "dct_plan_probe", "dct_autotune", "dct_sweep_profile",
+            "dct_sweep_times",
+            "dct_transform_layers", "dct_layers_read_at")
 
 
 def _rc_msg(rc):
@@ -67,7 +68,9 @@ def _bind(path):
               "dct_read_cnt", "dct_map_hist", "dct_set_mu", "dct_get_mu",
               "dct_ball_gather_mu", "dct_loss_owner", "dct_set_tbl2",
               "dct_use_fmt", "dct_stats", "dct_sweep_plan",
-              "dct_sweep_plan2", "dct_sweep_profile", "dct_sweep_times", "dct_transform_layers",
+              "dct_sweep_plan2", "dct_plan_probe", "dct_autotune",
+              "dct_sweep_profile",
+              "dct_sweep_times", "dct_transform_layers",
               "dct_layers_read_at"):
         getattr(lib, f).restype = ctypes.c_int
     for f in ("dct_count_eq", "dct_count_deficit", "dct_map_extract",
@@ -390,6 +393,16 @@ class GpuBackend:
                 "bytes_out", "lds_bytes", "threads", "rounds", "kernel")
         return [dict(zip(keys, buf[13 * i:13 * i + 13])) for i in range(ns)]
 
+    def autotune(self, reps=3):
+        """Time the candidate transform schedules on this cell and keep the
+        fastest (overwrites the map plane).  Returns (schedules timed,
+        microseconds per transform of the winner)."""
+        us = ctypes.c_float(0.0)
+        n = self._lib.dct_autotune(int(reps), ctypes.byref(us))
+        if n < 0:
+            _check("autotune", n)
+        return n, float(us.value)
+
     def sweep_profile(self, on=True):
         _check("sweep_profile", self._lib.dct_sweep_profile(1 if on else 0))
 
@@ -399,6 +412,24 @@ class GpuBackend:
         if ns < 0:
             _check("sweep_times", ns)
         return [float(buf[i]) for i in range(ns)]
+
+
+PLAN_KEYS = ("first_axis", "axes", "P", "TL", "RR", "W", "ND", "bytes_in",
+             "bytes_out", "lds_bytes", "threads", "rounds", "kernel")
+
+
+def plan_probe(axes, R, kind=0, s0=1, M=1, path=None):
+    """The packed schedule the engine would plan for a homogeneous Hamming
+    cell, computed on the host (no device, no live cell).  [] when the cell
+    runs the unpacked kernels."""
+    lib = load_plugin(path)
+    ax = np.ascontiguousarray(axes, dtype=np.int32)
+    buf = (ctypes.c_int * (13 * 41))()
+    ns = lib.dct_plan_probe(len(ax), ax.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                            int(R), int(kind), int(s0), int(M), buf, 40)
+    if ns < 0:
+        raise RuntimeError(f"plan_probe rc={ns}")
+    return [dict(zip(PLAN_KEYS, buf[13 * i:13 * i + 13])) for i in range(ns)]
 
 
 class GpuCoveringBackend(GpuBackend):

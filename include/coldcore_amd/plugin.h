@@ -159,6 +159,23 @@ int  dct_sweep_plan(int *out, int cap);
  * fields of one or two dwords, with field widths proven from closed-form
  * layer bounds, so inter-sweep traffic is 4-8 bytes per word. */
 int  dct_sweep_plan2(int kind, int *out, int cap);
+/* The packed schedule the engine would plan for a homogeneous Hamming cell
+ * (axes ax[0..n-1], radius R), computed on the host with no device and no
+ * live cell: kind 0 gain / loss, kind 1 recount of a code of M words with
+ * multiplicities <= s0.  Rows as dct_sweep_plan2; returns the sweep count
+ * (0: unpacked kernels), -1 on bad arguments or while a cell is live.  Every
+ * scheduled sweep's kernel instantiation exists (the planner checks). */
+int  dct_plan_probe(int n, const int *ax, int R, int kind, int s0, int M,
+                    int *out, int cap);
+/* Schedule autotuner: time the candidate schedules of the gain / loss
+ * transform on the live cell (every combination of LDS budget, column-kernel
+ * mode and first-sweep kernel the planner offers, plus the unpacked
+ * per-layer kernels; reps timed runs each) and keep the fastest; the
+ * winning options also steer the recount schedule.  Overwrites the map
+ * plane like a transform; cnt and the represented solution are unchanged.
+ * Returns the number of schedules timed (*best_us: microseconds per
+ * transform of the winner), or < 0. */
+int  dct_autotune(int reps, float *best_us);
 /* Record HIP events around every sweep of later transforms (on the
  * engine's stream; also enabled by DCT_TIMING=1 at init). */
 int  dct_sweep_profile(int on);

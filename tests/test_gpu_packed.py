@@ -96,3 +96,27 @@ def test_k2_24_3_schedule_is_three_packed_sweeps(gpu_cell):
     assert [(s["first_axis"], s["axes"]) for s in plan] == [(0, 12), (12, 6), (18, 6)]
     assert [s["W"] for s in plan] == [8, 16, 16]
     assert sum(s["bytes_in"] + s["bytes_out"] for s in plan) == 22
+
+
+def test_autotune_keeps_fields_exact(monkeypatch):
+    """The autotuner only changes which schedule runs: after it, gain, loss
+    and recount equal the planner's default schedule bit for bit, on a
+    2^22-word cell (where the engine also tunes by itself)."""
+    from coldcore_amd.gpu import GpuBackend
+    monkeypatch.setenv("CC_AUTOTUNE", "0")
+    axes, R = [2] * 22, 3
+    code = np.flatnonzero(np.random.default_rng(4).random(1 << 22) < 2e-4)
+    with GpuBackend(axes, R) as g:
+        def fields():
+            g.recount(code)
+            cnt = g.read_cnt().copy()
+            g.gain_map()
+            gain = g.map_all().copy()
+            g.loss_map()
+            return cnt, gain, g.map_all().copy()
+        before = fields()
+        n, us = g.autotune(2)
+        assert n >= 2 and us > 0
+        after = fields()
+    for a, b in zip(before, after):
+        assert (a == b).all()

@@ -123,3 +123,50 @@ def test_library_exports_every_declared_symbol():
     lib = ctypes.CDLL(LIB)
     for name in _declared_symbols():
         assert hasattr(lib, name), name
+
+
+# ---------------------------------------------------------------- planner
+def test_plan_probe_k2_24_3_schedule():
+    """K_2(24,3): axes 0-11 in one persistent LDS-tiled sweep with 8-bit
+    fields, then two six-axis column sweeps with 16-bit fields; 4-byte
+    compact words between sweeps: 2+4 + 4+4 + 4+4 = 22 bytes per word."""
+    from coldcore_amd.gpu import plan_probe
+    plan = plan_probe([2] * 24, 3)
+    assert [(s["first_axis"], s["axes"]) for s in plan] == [(0, 12), (12, 6), (18, 6)]
+    assert [s["W"] for s in plan] == [8, 16, 16]
+    assert sum(s["bytes_in"] + s["bytes_out"] for s in plan) == 22
+
+
+@pytest.mark.parametrize("q,ns,Rs", [(2, range(4, 41, 3), range(1, 7)),
+                                     (3, range(3, 26, 2), range(1, 5)),
+                                     (4, range(3, 21, 2), range(1, 5)),
+                                     (5, range(2, 18, 2), range(1, 4))])
+def test_plan_probe_schedules_are_consistent(q, ns, Rs):
+    """Every packed schedule the planner makes covers the axes once, in
+    order; each sweep reads what the previous one wrote; seeds are read
+    only by the first sweep and the map / cnt written only by the last; and
+    (checked inside the planner) every sweep's kernel instantiation exists."""
+    from coldcore_amd.gpu import plan_probe
+    for n in ns:
+        if q ** n > 2 ** 40:
+            continue
+        for R in Rs:
+            for kind, s0, M in ((0, 1, 1), (1, 1, 500), (1, 3, 65535)):
+                plan = plan_probe([q] * n, R, kind, s0, M)
+                if not plan:
+                    continue
+                j = 0
+                for i, sw in enumerate(plan):
+                    assert sw["first_axis"] == j and sw["axes"] >= 1
+                    j += sw["axes"]
+                    assert sw["W"] in (8, 16) and 1 <= sw["ND"] <= 4
+                    assert sw["kernel"] >= 1
+                    if i == 0:
+                        assert sw["bytes_in"] == (2 if kind == 0 else 4)
+                    else:
+                        assert sw["bytes_in"] == plan[i - 1]["bytes_out"]
+                    if i == len(plan) - 1:
+                        assert sw["bytes_out"] == (4 if kind == 0 else 2)
+                    else:
+                        assert sw["bytes_out"] in (4, 8)
+                assert j == n, (q, n, R, kind)
