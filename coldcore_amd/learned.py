@@ -206,9 +206,10 @@ class LearnedProposer:
     """Scores ``ncand`` cluster regions with the network and ruins the best
     (ties and exploration by a softmax at temperature ``tau``)."""
 
-    def __init__(self, net, scorer, ncand=32, tau=0.25):
+    def __init__(self, net, scorer, ncand=32, tau=0.25, norm=None):
         self.net, self.scorer = net, scorer
         self.ncand, self.tau = ncand, tau
+        self.norm = norm          # (mean, std) the network was trained with
 
     def __call__(self, searcher, k, mode=None):
         s = searcher
@@ -216,7 +217,10 @@ class LearnedProposer:
         anchors = s.rng.choice(m, size=min(self.ncand, m), replace=False)
         regions = cluster_regions(s, k, anchors)
         lay = self.scorer.layers(s.code)
-        y = self.net.forward(self.scorer.features(lay, regions))
+        F = self.scorer.features(lay, regions)
+        if self.norm is not None:
+            F = (F - self.norm[0]) / self.norm[1]
+        y = self.net.forward(F)
         z = (y - y.max()) / self.tau
         p = np.exp(z) / np.exp(z).sum()
         return regions[int(s.rng.choice(len(regions), p=p))]
@@ -282,3 +286,112 @@ def descend_timed(searcher, seconds, kmin=2, kmax=6, proposer=None):
         if s.uncov == 0:
             hist.append((time.perf_counter() - t0, len(s.code)))
     return hist
+
+
+def start_code(make_backend, seed):
+    """The common start of both arms: lazy greedy from empty, then peel
+    (Searcher seed ``seed``); returns the code."""
+    b = make_backend()
+    try:
+        s = Searcher(b, seed=seed)
+        s.load(np.zeros(0, dtype=np.int64))
+        s.greedy_fill(10 ** 9)
+        s.peel()
+        return s.code.copy()
+    finally:
+        b.close()
+
+
+def time_to_size(hist, size):
+    """First time (s) a run's feasible size reached ``size``; None if never."""
+    for t, m in hist:
+        if m <= size:
+            return t
+    return None
+
+
+def learned_ab(make_backend, train_cells, seeds, seconds, train_rounds=2000,
+               epochs=40, ncand=32, tau=0.25, log=None):
+    """Time-to-size of the learned ruin proposal against plain adaptive LNS.
+
+    1. Move logs: plain LNS (fixed operator mix) on every cell of
+       ``train_cells`` (callables making backends), ``train_rounds`` rounds
+       per cell, every round logged (region features, outcome).
+    2. The network is trained on them on the GPU (own kernels).
+    3. For every seed, both arms start from the same greedy + peel cover
+       and run notch descent for ``seconds`` on ``make_backend``: plain =
+       adaptive operator bandit; learned = the same bandit with the network's
+       proposal as one more arm (the exact fields still judge every move).
+    Per seed: F = the smallest size plain reached, T_plain = when plain first
+    reached F, T_learned = when the learned arm first reached F; the ratio
+    T_learned / T_plain (< 1: the learned arm got there sooner; None when it
+    never did)."""
+    t0 = time.perf_counter()
+    Xs, ts = [], []
+    for i, mk in enumerate(train_cells):
+        X, t = collect_records(mk, seeds=[100 + i], rounds=train_rounds)
+        Xs.append(X)
+        ts.append(t)
+    X = np.concatenate(Xs)
+    t = np.concatenate(ts)
+    collect_s = time.perf_counter() - t0
+    # standardised features; target = the round improved (binary); a
+    # quarter held out to measure how well the network ranks regions
+    mu = X.mean(axis=0)
+    sd = np.maximum(X.std(axis=0), 1e-6)
+    Xn = ((X - mu) / sd).astype(np.float32)
+    y = (t == 1.0).astype(np.float32)
+    perm = np.random.default_rng(0).permutation(len(X))
+    cut = int(0.75 * len(X))
+    tr, te = perm[:cut], perm[cut:]
+    net = Net(max_batch=max(1024, ncand), seed=7)
+    try:
+        t1 = time.perf_counter()
+        losses = train(net, Xn[tr], y[tr], epochs=epochs, batch=256, lr=3e-3)
+        train_s = time.perf_counter() - t1
+        score = net.forward(Xn[te])
+        pos, neg = score[y[te] == 1], score[y[te] == 0]
+        auc = (float((pos[:, None] > neg[None, :]).mean())
+               if len(pos) and len(neg) else None)
+        runs = []
+        for seed in seeds:
+            code = start_code(make_backend, seed)
+            res = {"seed": int(seed), "start_size": int(len(code))}
+            for arm in ("plain", "learned"):
+                b = make_backend()
+                try:
+                    s = Searcher(b, seed=seed)
+                    s.load(code)
+                    prop = (LearnedProposer(net, RegionScorer(b), ncand=ncand,
+                                            tau=tau, norm=(mu, sd))
+                            if arm == "learned" else None)
+                    hist = descend_timed(s, seconds, proposer=prop)
+                    res[arm] = {"final_size": int(min(m for _, m in hist)),
+                                "rounds": int(s.stats["rounds"]),
+                                "hist": [(round(a, 3), int(m)) for a, m in hist]}
+                finally:
+                    b.close()
+            F = res["plain"]["final_size"]
+            tp = time_to_size(res["plain"]["hist"], F)
+            tl = time_to_size(res["learned"]["hist"], F)
+            res["target_size"] = F
+            res["t_plain_s"] = tp
+            res["t_learned_s"] = tl
+            res["ratio"] = (None if tl is None or not tp else round(tl / tp, 3))
+            runs.append(res)
+            if log:
+                log(f"seed {seed}: plain {F} at {tp}s, learned "
+                    f"{res['learned']['final_size']} (reached {F} at {tl}s)")
+        ratios = [r["ratio"] for r in runs]
+        done = [x for x in ratios if x is not None]
+        return {"records": int(len(X)), "improving": int(y.sum()),
+                "heldout_auc": None if auc is None else round(auc, 3),
+                "collect_s": round(collect_s, 2),
+                "train_s": round(train_s, 2),
+                "train_loss": [round(float(losses[0]), 4), round(float(losses[-1]), 4)],
+                "seconds_per_arm": seconds, "runs": runs,
+                "median_ratio": (float(np.median(done)) if len(done) == len(ratios)
+                                 else None),
+                "reached": f"{len(done)}/{len(ratios)}"}
+    finally:
+        net.close()
