@@ -137,6 +137,107 @@ def run_config1(GpuBackend, with_cpu):
     return out
 
 
+def batch_cells64():
+    """The 64 uneven cells of the batched workload: Hamming K_3(n,R) for
+    n = 8..12 and R = 1..3, Lee 9x9x9 R = 2, king tori k x k for
+    k = 9..31 (R = 1), and 25 open-grid boxes with axes drawn from
+    {2,3,4,6,8,9,12,16}, n <= 8, at most 2^20 words (seed 64, R = 1)."""
+    cells = [("hamming", [3] * n, R) for n in range(8, 13) for R in (1, 2, 3)]
+    cells.append(("lee", [9, 9, 9], 2))
+    cells += [("torus_linf", [k, k], 1) for k in range(9, 32)]
+    rng = np.random.default_rng(64)
+    sizes = [2, 3, 4, 6, 8, 9, 12, 16]
+    while len(cells) < 64:
+        n = int(rng.integers(2, 9))
+        axes = [int(x) for x in rng.choice(sizes, n)]
+        while int(np.prod(axes)) > 1 << 20:
+            axes.remove(max(axes))
+        cells.append(("grid_linf", axes, 1))
+    return cells
+
+
+def run_batch64(GpuBackend, rounds=200, k=2):
+    """64 uneven cells in one batch: greedy sizes, seeded random starts at
+    1.5 x greedy (seed = cell index), the batched transform (one persistent
+    launch) against the same cells' singleton transforms run one after the
+    other, and LNS rounds/s with and without the captured HIP graph."""
+    from coldcore_amd.batch import GpuBatch
+    cells = batch_cells64()
+    out = {"cells": len(cells),
+           "words": int(sum(int(np.prod(a)) for _, a, _ in cells))}
+    with GpuBatch(cells, cap=65535) as b:
+        out.update({f"schedule_{k_}": v for k_, v in b.info().items()})
+        empty = np.zeros(0, dtype=np.int64)
+        for i in range(len(cells)):
+            b.set_code(i, empty)
+        b.recount()
+        t = time.perf_counter()
+        out["greedy_steps"] = b.greedy_cover(256)
+        out["greedy_s"] = round(time.perf_counter() - t, 3)
+        greedy = [b.stats(i)["M"] for i in range(len(cells))]
+        out["greedy_words_total"] = int(sum(greedy))
+        starts = []
+        for i, (_, a, _) in enumerate(cells):
+            size = int(np.prod(a))
+            m = min(size, int(round(1.5 * greedy[i])))
+            starts.append(np.random.default_rng(i).choice(size, m, replace=False))
+
+        def reset():
+            for i, c in enumerate(starts):
+                b.set_code(i, c)
+            b.recount()
+            b.lns_init(0)
+
+        reset()
+        b.gain_maps()
+        t = time.perf_counter()
+        for _ in range(20):
+            b.gain_maps()
+        out["batched_gain_ms"] = round((time.perf_counter() - t) * 50, 4)
+        # LNS: direct launches, then the captured graph from the same start
+        reset()
+        b.lns_run(1, k=k, graph=False)
+        reset()
+        ms = b.lns_run(rounds, k=k, graph=False)
+        out["lns_rounds"] = rounds
+        out["lns_k"] = k
+        out["lns_direct_rounds_per_s"] = round(rounds / (ms * 1e-3), 1)
+        direct = [b.stats(i) for i in range(len(cells))]
+        reset()
+        b.lns_run(1, k=k, graph=True)             # capture
+        reset()
+        ms = b.lns_run(rounds, k=k, graph=True)
+        out["lns_graph_rounds_per_s"] = round(rounds / (ms * 1e-3), 1)
+        graph = [b.stats(i) for i in range(len(cells))]
+        out["graph_equals_direct"] = bool(direct == graph)
+        out["cells_covered_at_end"] = int(sum(s["uncovered"] == 0 for s in graph))
+        out["best_words_total_covered"] = int(sum(
+            s["best"] for s in graph if s["best"] > 0))
+    # the same cells one at a time through the singleton ABI (the cells
+    # whose axes its v1 tables address: axes <= 16; the tori with k > 16
+    # are left out, which only understates the batch's advantage)
+    tot = 0.0
+    nsingle = 0
+    for i, (p, a, R) in enumerate(cells):
+        if max(a) > 16:
+            continue
+        nsingle += 1
+        g = GpuBackend(a, R, problem=p)
+        try:
+            g.recount(starts[i].astype(np.int64))
+            g.gain_map()
+            t = time.perf_counter()
+            for _ in range(5):
+                g.gain_map()
+            tot += (time.perf_counter() - t) / 5
+        finally:
+            g.close()
+    out["singleton_cells"] = nsingle
+    out["singleton_gain_ms_sum"] = round(tot * 1e3, 4)
+    out["batched_speedup"] = round(tot * 1e3 / out["batched_gain_ms"], 2)
+    return out
+
+
 def copy_calibration(mib):
     """hipMemcpy device-to-device of a known size: its dispatch
     (__amd_rocclr_copyBuffer) reads and writes exactly mib MiB."""
@@ -373,6 +474,10 @@ def main():
     if not args.no_extra:
         rec["other_cells"] = time_cells(GpuBackend, 10)
         rec["config1"] = run_config1(GpuBackend, not args.no_cpu)
+        try:
+            rec["batch64"] = run_batch64(GpuBackend)
+        except RuntimeError as e:          # reported, never hidden
+            rec["batch64"] = {"error": str(e)}
     if not args.no_cpu:
         rec["cpu_baseline"] = run_cpu_baseline(code, args.cpu_passes)
         rec["speedup_vs_cpu"] = round(passes_s / rec["cpu_baseline"]["value"], 1)

@@ -1,0 +1,499 @@
+// batch.hip -- the batched multi-cell engine (additive C ABI, `ccb_*`).
+//
+// Holds many uneven cells at once (Hamming, Lee, king torus, open grid; any
+// mix of axis sizes and radii R <= 3), all in one device arena beside the
+// singleton `dct_*` cell.  A transform of every cell is one persistent
+// launch per sweep level (cc_batch.hpp); a batched LNS round (ruin k random
+// words per cell,
+// greedy recreate by exact gain, accept or revert) is a fixed sequence of
+// launches, captured once into a HIP graph and replayed.  No counterpart in
+// the reference (one cell per process there, reference docs/PLUGIN.md:73-76).
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "../../include/coldcore_amd/batch.h"
+#include "cc_batch.hpp"
+
+#define CKB(x)                                                              \
+  do {                                                                      \
+    hipError_t e_ = (x);                                                    \
+    if (e_ != hipSuccess) {                                                 \
+      fprintf(stderr, "HIP error %s at %s:%d\n", hipGetErrorString(e_),     \
+              __FILE__, __LINE__);                                          \
+      return -(100 + (int)e_);                                              \
+    }                                                                       \
+  } while (0)
+
+namespace {
+
+constexpr int CCB_LDS = 32 * 1024;   // LDS budget of a sweep tile
+
+struct Batch {
+  bool live = false;
+  int nc = 0;
+  std::vector<BCell> h;               // host copies (geometry and pointers)
+  BCell *d = nullptr;                 // device cell table
+  std::vector<void *> allocs;
+  std::vector<SweepArgs> args[3];     // 0 gain, 1 loss, 2 recount
+  SweepArgs *dargs[3] = {};
+  std::vector<int> argcell, need;
+  std::vector<int2> items;
+  std::vector<int> level_off;         // items of level l: [off[l], off[l+1])
+  int *d_argcell = nullptr;
+  int2 *d_items = nullptr;
+  unsigned *d_counters = nullptr;     // one queue head per level
+  int nargs = 0, nitems = 0, nlevels = 0;
+  size_t lds = 0;
+  int per_cu = 1, ncu = 1;
+  long long maxX = 0;
+  BItem *d_ruin = nullptr, *d_add = nullptr, *d_rev = nullptr;
+  hipStream_t st = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int graph_k = -1;
+};
+
+Batch B;
+
+template <typename T>
+int balloc(T **p, size_t bytes) {
+  if (hipMalloc((void **)p, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    *p = nullptr;
+    return -1;
+  }
+  B.allocs.push_back((void *)*p);
+  return 0;
+}
+
+void bfree_all() {
+  if (B.st) (void)hipStreamSynchronize(B.st);
+  if (B.exec) { (void)hipGraphExecDestroy(B.exec); B.exec = nullptr; }
+  if (B.graph) { (void)hipGraphDestroy(B.graph); B.graph = nullptr; }
+  for (void *p : B.allocs) (void)hipFree(p);
+  B.allocs.clear();
+  if (B.st) { (void)hipStreamDestroy(B.st); B.st = nullptr; }
+  B = Batch();
+}
+
+// largest partial distance one axis adds (Hamming 1, Lee floor(a/2))
+int live_after(const BCell &c, int upto) {
+  if (c.L == 1) return 1;
+  long long w = 0;
+  for (int j = 0; j < upto; j++) w += c.prob == CC_LEE ? c.ax[j] / 2 : 1;
+  return (int)std::min<long long>(w, c.R) + 1;
+}
+
+// the cell's sweeps: groups of consecutive axes whose tile (L live planes
+// x P words x TL columns, int32) fits the LDS budget; the first group is
+// contiguous (TL = 1), later ones stage up to 32 columns
+void plan_cell(int ci, const BCell &c) {
+  int j = 0;
+  while (j < c.n) {
+    const long long s = c.pw[j];
+    const int TL = j == 0 ? 1 : (int)std::min<long long>(32, s);
+    long long P = 1;
+    int k = j;
+    while (k < c.n) {
+      const long long P2 = P * c.ax[k];
+      if (k > j && ((long long)c.L * P2 * TL * 4 > CCB_LDS || k - j >= CC_MAXSW))
+        break;
+      P = P2;
+      k++;
+    }
+    SweepArgs a;
+    memset(&a, 0, sizeof(a));
+    for (int d = 0; d < c.L; d++) a.A[d] = c.A[d];
+    a.cnt = c.cnt;
+    a.s = s;
+    a.P = (int)P;
+    a.TL = TL;
+    a.ncolb = (s + TL - 1) / TL;
+    a.ntiles = (c.X / (s * P)) * a.ncolb;
+    a.nax = k - j;
+    a.c0 = j;
+    a.R = c.R;
+    a.mu = c.mu;
+    long long sk = 1;
+    for (int i = 0; i < a.nax; i++) {
+      a.Q[i] = c.ax[j + i];
+      a.sk[i] = (int)sk;
+      sk *= a.Q[i];
+    }
+    for (int i = 0; i <= a.nax; i++) a.live[i] = live_after(c, j + i);
+    const bool first = j == 0, last = k == c.n;
+    for (int kind = 0; kind < 3; kind++) {
+      SweepArgs b = a;
+      b.in_kind = first ? (kind == 2 ? CC_IN_CODE : CC_IN_PRED) : CC_IN_LAYERS;
+      b.out_kind = last ? (kind == 2 ? CC_OUT_CNT : CC_OUT_MAP) : CC_OUT_LAYERS;
+      b.pmode = kind == 1 ? CC_CRIT : CC_DEF;
+      b.write_l0 = c.L == 1 || (first && kind != 2);
+      B.args[kind].push_back(b);
+    }
+    B.argcell.push_back(ci);
+    B.need.push_back((int)a.ntiles);
+    B.lds = std::max(B.lds, (size_t)(c.L * P * TL * 4));
+    j = k;
+  }
+}
+
+int transform_launch(int kind) {
+  // the per-level queue heads are cleared by a kernel (graph-capturable; no
+  // memset nodes)
+  hipLaunchKernelGGL(k_ccb_fill, dim3(1), dim3(256), 0, B.st, (int *)B.d_counters,
+                     0, B.nlevels);
+  CKB(hipGetLastError());
+  if (kind == 2) {
+    const int gx = (int)std::min<long long>((B.maxX + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_ccb_zero_a0, dim3(gx, B.nc), dim3(256), 0, B.st, B.d);
+    CKB(hipGetLastError());
+    hipLaunchKernelGGL(k_ccb_scatter, dim3(64, B.nc), dim3(256), 0, B.st, B.d);
+    CKB(hipGetLastError());
+  }
+  // one persistent launch per sweep level: at most what the CUs hold at
+  // once, at most one workgroup per item
+  for (int l = 0; l < B.nlevels; l++) {
+    const int n = B.level_off[l + 1] - B.level_off[l];
+    const int grid = std::max(1, std::min(B.per_cu * B.ncu, n));
+    hipLaunchKernelGGL(k_ccb_sweeps, dim3(grid), dim3(CC_BLOCK), B.lds, B.st,
+                       B.dargs[kind], B.d_argcell, B.d, B.d_items + B.level_off[l],
+                       n, B.d_counters + l);
+    CKB(hipGetLastError());
+  }
+  return 0;
+}
+
+int reduce_launch(int with_map) {
+  hipLaunchKernelGGL(k_ccb_reset_red, dim3((B.nc + 63) / 64), dim3(64), 0, B.st,
+                     B.d, B.nc);
+  CKB(hipGetLastError());
+  const int gx = (int)std::max<long long>(1, std::min<long long>((B.maxX + 2047) / 2048, 512));
+  hipLaunchKernelGGL(k_ccb_reduce, dim3(gx, B.nc), dim3(CC_BLOCK), 0, B.st, B.d,
+                     with_map);
+  CKB(hipGetLastError());
+  return 0;
+}
+
+// one batched LNS round (fixed launch sequence)
+int round_launch(int k) {
+  const dim3 cg((B.nc + 63) / 64), cb(64);
+  hipLaunchKernelGGL(k_ccb_ruin, cg, cb, 0, B.st, B.d, B.nc, k, B.d_ruin);
+  CKB(hipGetLastError());
+  hipLaunchKernelGGL(k_ccb_walk, dim3(B.nc * CCB_KMAX), dim3(CC_BLOCK), 0, B.st,
+                     B.d, B.d_ruin);
+  CKB(hipGetLastError());
+  for (int j = 0; j < k; j++) {
+    if (int rc = transform_launch(0)) return rc;
+    if (int rc = reduce_launch(1)) return rc;
+    hipLaunchKernelGGL(k_ccb_add, cg, cb, 0, B.st, B.d, B.nc, j, B.d_add);
+    CKB(hipGetLastError());
+    hipLaunchKernelGGL(k_ccb_walk, dim3(B.nc), dim3(CC_BLOCK), 0, B.st, B.d,
+                       B.d_add);
+    CKB(hipGetLastError());
+  }
+  if (int rc = reduce_launch(0)) return rc;
+  hipLaunchKernelGGL(k_ccb_decide, cg, cb, 0, B.st, B.d, B.nc, B.d_rev);
+  CKB(hipGetLastError());
+  hipLaunchKernelGGL(k_ccb_walk, dim3(B.nc * 2 * CCB_KMAX), dim3(CC_BLOCK), 0,
+                     B.st, B.d, B.d_rev);
+  CKB(hipGetLastError());
+  return 0;
+}
+
+int sync_batch() {
+  CKB(hipStreamSynchronize(B.st));
+  return 0;
+}
+
+}  // namespace
+
+#pragma GCC visibility push(default)
+extern "C" {
+
+// ccb_init: ncells cells; cell i has problem[i] (plugin ids: 0 Hamming,
+// 1 torus L-inf, 2 Lee, 3 open grid), n[i] axes (axes concatenated), radius
+// R[i] <= 3, and for Hamming / Lee its support table (npats[i] patterns of
+// format fmt[i]: 0 v1, 1 wide; concatenated in pats).  cap: code words per
+// cell.  Returns 0, -1 (invalid), -3 (allocation).
+int ccb_init(int ncells, const int *problem, const int *n, const int *axes,
+             const int *R, const uint64_t *pats, const long long *npats,
+             const int *fmt, int cap) {
+  if (ncells < 1 || ncells > 4096 || !problem || !n || !axes || !R || !npats ||
+      !fmt || cap < 2 || cap > 65535)
+    return -1;
+  if (B.live) bfree_all();
+  B.nc = ncells;
+  B.h.assign(ncells, BCell());
+  int aoff = 0;
+  long long poff = 0;
+  for (int i = 0; i < ncells; i++) {
+    BCell &c = B.h[i];
+    memset(&c, 0, sizeof(c));
+    c.prob = problem[i];
+    c.n = n[i];
+    c.R = R[i];
+    c.mu = 1;
+    c.fmt = fmt[i];
+    if (c.n < 1 || c.n > CC_MAXN || c.R < 1 || c.R > 3 || c.prob < 0 || c.prob > 3)
+      { bfree_all(); return -1; }
+    const bool linf = c.prob == CC_TORUS || c.prob == CC_GRID;
+    c.L = linf ? 1 : c.R + 1;
+    c.pw[0] = 1;
+    for (int j = 0; j < c.n; j++) {
+      c.ax[j] = axes[aoff + j];
+      const int lim = linf ? 32 : (c.prob == CC_HAMMING ? 10 : 16);
+      if (c.ax[j] < 2 || c.ax[j] > lim || (c.prob == CC_TORUS && c.ax[j] < 2 * c.R + 1))
+        { bfree_all(); return -1; }
+      c.pw[j + 1] = c.pw[j] * c.ax[j];
+      if (c.pw[j + 1] > (1LL << 26)) { bfree_all(); return -1; }
+    }
+    if ((linf && c.n > 8) || (c.fmt == CC_FMT_V1 && c.n > CC_MAXN_V1) ||
+        (c.fmt != CC_FMT_V1 && c.fmt != CC_FMT_WIDE))
+      { bfree_all(); return -1; }
+    aoff += c.n;
+    c.X = c.pw[c.n];
+    c.blen = linf ? 0 : npats[i];
+    if (!linf && (c.blen < 1 || !pats)) { bfree_all(); return -1; }
+    // device planes
+    for (int d = 0; d < c.L; d++)
+      if (balloc(&c.A[d], (size_t)c.X * 4)) { bfree_all(); return -3; }
+    if (balloc(&c.cnt, (size_t)((c.X + 1) & ~1LL) * 2) ||
+        balloc(&c.code, (size_t)cap * 8) || balloc(&c.best, (size_t)cap * 8))
+      { bfree_all(); return -3; }
+    c.cap = cap;
+    if (!linf) {
+      uint64_t *dp = nullptr;
+      if (balloc(&dp, (size_t)c.blen * 8)) { bfree_all(); return -3; }
+      if (hipMemcpy(dp, pats + poff, (size_t)c.blen * 8, hipMemcpyHostToDevice) !=
+          hipSuccess) { bfree_all(); return -3; }
+      c.pat = dp;
+      poff += c.blen;
+    }
+    if (hipMemset(c.cnt, 0, (size_t)((c.X + 1) & ~1LL) * 2) != hipSuccess)
+      { bfree_all(); return -3; }
+    B.maxX = std::max(B.maxX, c.X);
+  }
+  if (hipStreamCreateWithFlags(&B.st, hipStreamNonBlocking) != hipSuccess)
+    { bfree_all(); return -3; }
+  // sweep plans, work items by level
+  std::vector<std::vector<int>> cell_args(ncells);
+  for (int i = 0; i < ncells; i++) {
+    const int before = (int)B.argcell.size();
+    plan_cell(i, B.h[i]);
+    for (int a = before; a < (int)B.argcell.size(); a++) cell_args[i].push_back(a);
+  }
+  B.nargs = (int)B.argcell.size();
+  for (int level = 0;; level++) {
+    bool any = false;
+    const int off = (int)B.items.size();
+    for (int i = 0; i < ncells; i++) {
+      if (level >= (int)cell_args[i].size()) continue;
+      any = true;
+      const int a = cell_args[i][level];
+      for (int t = 0; t < B.need[a]; t++) B.items.push_back(make_int2(a, t));
+    }
+    if (!any) break;
+    B.level_off.push_back(off);
+  }
+  B.nlevels = (int)B.level_off.size();
+  B.nitems = (int)B.items.size();
+  B.level_off.push_back(B.nitems);
+  if (balloc(&B.d, sizeof(BCell) * ncells) ||
+      balloc(&B.d_argcell, 4 * B.nargs) || balloc(&B.d_items, 8 * B.nitems) ||
+      balloc(&B.d_counters, 4 * B.nlevels) ||
+      balloc(&B.d_ruin, sizeof(BItem) * ncells * CCB_KMAX) ||
+      balloc(&B.d_add, sizeof(BItem) * ncells) ||
+      balloc(&B.d_rev, sizeof(BItem) * ncells * 2 * CCB_KMAX))
+    { bfree_all(); return -3; }
+  for (int k = 0; k < 3; k++)
+    if (balloc(&B.dargs[k], sizeof(SweepArgs) * B.nargs) ||
+        hipMemcpy(B.dargs[k], B.args[k].data(), sizeof(SweepArgs) * B.nargs,
+                  hipMemcpyHostToDevice) != hipSuccess)
+      { bfree_all(); return -3; }
+  if (hipMemcpy(B.d, B.h.data(), sizeof(BCell) * ncells, hipMemcpyHostToDevice) ||
+      hipMemcpy(B.d_argcell, B.argcell.data(), 4 * B.nargs, hipMemcpyHostToDevice) ||
+      hipMemcpy(B.d_items, B.items.data(), 8 * B.nitems, hipMemcpyHostToDevice))
+    { bfree_all(); return -3; }
+  // persistent grids: what the CUs hold at once
+  int per_cu = 1, dev = 0, ncu = 256;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_ccb_sweeps,
+                                                   CC_BLOCK, B.lds) != hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  B.per_cu = per_cu;
+  B.ncu = std::max(ncu, 1);
+  // the set-up copies and clears ran outside the batch's (non-blocking)
+  // stream: finish them before its first launch
+  if (hipDeviceSynchronize() != hipSuccess) { bfree_all(); return -3; }
+  B.live = true;
+  return 0;
+}
+
+int ccb_free(void) {
+  bfree_all();
+  return 0;
+}
+
+// the code of one cell (a set or multiset of word indices, M <= cap)
+int ccb_set_code(int cell, const long long *idx, int M) {
+  if (!B.live || cell < 0 || cell >= B.nc || M < 0 || M > B.h[cell].cap ||
+      (M && !idx))
+    return -1;
+  for (int i = 0; i < M; i++)
+    if (idx[i] < 0 || idx[i] >= B.h[cell].X) return -1;
+  CKB(hipStreamSynchronize(B.st));
+  if (M) CKB(hipMemcpy(B.h[cell].code, idx, (size_t)M * 8, hipMemcpyHostToDevice));
+  CKB(hipMemcpy(&B.d[cell].M, &M, 4, hipMemcpyHostToDevice));
+  return 0;
+}
+
+// the transform of every cell in one persistent launch: init_mode / out_mode
+// as dct_transform ((1,1) gain, (2,1) loss, (0,2) recount)
+int ccb_transform(int init_mode, int out_mode) {
+  if (!B.live) return -1;
+  int kind;
+  if (init_mode == 1 && out_mode == 1) kind = 0;
+  else if (init_mode == 2 && out_mode == 1) kind = 1;
+  else if (init_mode == 0 && out_mode == 2) kind = 2;
+  else return -1;
+  if (int rc = transform_launch(kind)) return rc;
+  return sync_batch();
+}
+
+int ccb_read_cnt(int cell, uint16_t *out) {
+  if (!B.live || cell < 0 || cell >= B.nc || !out) return -1;
+  CKB(hipStreamSynchronize(B.st));
+  CKB(hipMemcpy(out, B.h[cell].cnt, (size_t)B.h[cell].X * 2, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int ccb_read_map(int cell, int32_t *out) {
+  if (!B.live || cell < 0 || cell >= B.nc || !out) return -1;
+  CKB(hipStreamSynchronize(B.st));
+  CKB(hipMemcpy(out, B.h[cell].A[0], (size_t)B.h[cell].X * 4, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+// LNS state from the current cnt of every cell (call after a recount):
+// uncovered counts, per-cell random streams from seed, best = start if
+// feasible
+int ccb_lns_init(unsigned long long seed) {
+  if (!B.live) return -1;
+  if (int rc = reduce_launch(0)) return rc;
+  hipLaunchKernelGGL(k_ccb_lns_init, dim3((B.nc + 63) / 64), dim3(64), 0, B.st,
+                     B.d, B.nc, seed);
+  CKB(hipGetLastError());
+  CKB(hipStreamSynchronize(B.st));
+  return 0;
+}
+
+// rounds batched LNS rounds, k words ruined per cell per round (k <=
+// CCB_KMAX); graph != 0 replays one captured round, else launches the same
+// sequence directly.  *ms: wall milliseconds of the rounds.
+int ccb_lns_run(int rounds, int k, int graph, float *ms) {
+  if (!B.live || rounds < 0 || k < 1 || k > CCB_KMAX) return -1;
+  hipEvent_t e0, e1;
+  CKB(hipEventCreate(&e0));
+  CKB(hipEventCreate(&e1));
+  if (graph && (!B.exec || B.graph_k != k)) {
+    if (B.exec) { CKB(hipGraphExecDestroy(B.exec)); B.exec = nullptr; }
+    if (B.graph) { CKB(hipGraphDestroy(B.graph)); B.graph = nullptr; }
+    CKB(hipStreamBeginCapture(B.st, hipStreamCaptureModeThreadLocal));
+    const int rc = round_launch(k);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(B.st, &g);
+    if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+    CKB(ec);
+    B.graph = g;
+    CKB(hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0));
+    B.graph_k = k;
+  }
+  CKB(hipEventRecord(e0, B.st));
+  for (int r = 0; r < rounds; r++) {
+    if (graph) CKB(hipGraphLaunch(B.exec, B.st));
+    else if (int rc = round_launch(k)) return rc;
+  }
+  CKB(hipEventRecord(e1, B.st));
+  CKB(hipEventSynchronize(e1));
+  float t = 0;
+  CKB(hipEventElapsedTime(&t, e0, e1));
+  if (ms) *ms = t;
+  CKB(hipEventDestroy(e0));
+  CKB(hipEventDestroy(e1));
+  return 0;
+}
+
+// steps greedy steps on every cell: gain transform, argmax, add (cells with
+// no uncovered word left are unchanged), cnt kept exact by walks.  Start
+// from a recounted code (ccb_transform(0, 2)); returns the number of cells
+// still uncovered after the steps (>= 0).
+int ccb_greedy(int steps) {
+  if (!B.live || steps < 0) return -1;
+  for (int s = 0; s < steps; s++) {
+    if (int rc = transform_launch(0)) return rc;
+    if (int rc = reduce_launch(1)) return rc;
+    hipLaunchKernelGGL(k_ccb_add, dim3((B.nc + 63) / 64), dim3(64), 0, B.st, B.d,
+                       B.nc, -1, B.d_add);
+    CKB(hipGetLastError());
+    hipLaunchKernelGGL(k_ccb_walk, dim3(B.nc), dim3(CC_BLOCK), 0, B.st, B.d,
+                       B.d_add);
+    CKB(hipGetLastError());
+  }
+  if (int rc = reduce_launch(0)) return rc;
+  if (int rc = sync_batch()) return rc;
+  std::vector<BCell> h(B.nc);
+  CKB(hipMemcpy(h.data(), B.d, sizeof(BCell) * B.nc, hipMemcpyDeviceToHost));
+  int open = 0;
+  for (const BCell &c : h) open += c.nzero != 0ull;
+  return open;
+}
+
+// which 0: the current code, 1: the best feasible code; returns its size
+int ccb_get_code(int cell, int which, long long *out, int cap) {
+  if (!B.live || cell < 0 || cell >= B.nc || which < 0 || which > 1) return -1;
+  CKB(hipStreamSynchronize(B.st));
+  BCell c;
+  CKB(hipMemcpy(&c, &B.d[cell], sizeof(BCell), hipMemcpyDeviceToHost));
+  const int m = which ? (c.bestM == 0x7FFFFFFF ? 0 : c.bestM) : c.M;
+  const int mc = std::min(m, std::max(cap, 0));
+  if (mc && !out) return -1;
+  if (mc) CKB(hipMemcpy(out, which ? c.best : c.code, (size_t)mc * 8,
+                        hipMemcpyDeviceToHost));
+  return m;
+}
+
+// out[0..5] = M, uncovered, best feasible size (-1: none), rounds,
+// accepted, reverted
+int ccb_stats(int cell, long long *out) {
+  if (!B.live || cell < 0 || cell >= B.nc || !out) return -1;
+  CKB(hipStreamSynchronize(B.st));
+  BCell c;
+  CKB(hipMemcpy(&c, &B.d[cell], sizeof(BCell), hipMemcpyDeviceToHost));
+  out[0] = c.M;
+  out[1] = c.uncov;
+  out[2] = c.bestM == 0x7FFFFFFF ? -1 : c.bestM;
+  out[3] = c.rounds;
+  out[4] = c.accepted;
+  out[5] = c.reverted;
+  return 0;
+}
+
+// schedule of the batch: out[0..4] = cells, sweeps, work items, sweep
+// levels (persistent launches per transform), workgroups per launch at most
+int ccb_info(long long *out) {
+  if (!B.live || !out) return -1;
+  out[0] = B.nc;
+  out[1] = B.nargs;
+  out[2] = B.nitems;
+  out[3] = B.nlevels;
+  out[4] = (long long)B.per_cu * B.ncu;
+  return 0;
+}
+
+}  // extern "C"
+#pragma GCC visibility pop

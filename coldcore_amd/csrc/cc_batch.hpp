@@ -1,0 +1,402 @@
+// cc_batch.hpp -- many uneven cells on one GPU: the transform of every cell
+// by one persistent launch per sweep level, batched walks and reductions,
+// and the device side of a batched LNS round (ruin k, greedy recreate,
+// accept / revert) whose fixed launch sequence is captured once into a HIP
+// graph.
+//
+// Batched transform: a cell's sweeps run in order (sweep s + 1 reads what
+// sweep s wrote), so the work is split by sweep level: level l holds the
+// tiles of every cell's l-th sweep.  One persistent launch per level takes
+// (sweep, tile) items from an atomic counter; items of one level never
+// depend on each other, so no workgroup ever waits on another (the kernel
+// boundary orders the levels, and the L2 write-back / invalidate between
+// XCDs happens there once, not per tile).
+#pragma once
+#include "cc_common.hpp"
+#include "cc_sweep.hpp"
+
+#define CCB_KMAX 8       // ruin size per LNS round
+
+struct BCell {
+  long long pw[CC_MAXN + 1];
+  int ax[CC_MAXN];
+  int prob, n, R, L, mu, fmt;
+  long long X;
+  int32_t *A[CC_MAXL];   // A[0]: map / code indicator; A[1..L-1]: layers
+  uint16_t *cnt;         // padded to an even length
+  const uint64_t *pat;   // Hamming / Lee: support table (v1 or wide)
+  long long blen;
+  long long *code;       // current code, M words (capacity cap)
+  long long *best;       // best feasible code found by the LNS (bestM words)
+  int cap;
+  // LNS state (control kernels only)
+  int M, uncov, bestM, add_budget, pre_uncov, nrem, nadd;
+  long long rem[CCB_KMAX], add[CCB_KMAX];
+  unsigned long long rng;
+  long long rounds, accepted, reverted;
+  // reductions of the current step
+  unsigned long long amax;   // (gain << 32) | (0xFFFFFFFF - index)
+  unsigned long long nzero;  // words with cnt == 0
+};
+
+struct BItem {
+  int cell, delta;   // delta 0: no-op slot
+  long long w;
+};
+
+// ---------------------------------------------------------------- transform
+// runtime-Q window sums for the L-inf families with axes longer than the
+// unrolled table (king tori up to 32 x 32): one fiber per thread, in place,
+// with a shift register of the 2R+1 original values under the window and
+// the first R originals kept for the circular wrap (no indexed arrays, so
+// nothing spills to scratch)
+template <int R, bool TORUS>
+__device__ __forceinline__ void ccb_step_linf_rt(int32_t *__restrict__ S, int TL,
+                                                 int nf, int sk, int Q) {
+  for (int fi = threadIdx.x; fi < nf; fi += blockDim.x) {
+    const int f = fi / TL, tl = fi - f * TL;
+    const int hi = f / sk, lo = f - hi * sk;
+    const int e0 = (hi * sk * Q + lo) * TL + tl, es = sk * TL;
+    uint32_t head[R], win[2 * R + 1];
+#pragma unroll
+    for (int i = 0; i < R; i++) head[i] = (uint32_t)S[e0 + i * es];
+#pragma unroll
+    for (int i = 0; i <= 2 * R; i++) {
+      const int b = i - R;   // -R .. R  (Q >= 2R+1 for the torus)
+      if (b < 0) win[i] = TORUS ? (uint32_t)S[e0 + (b + Q) * es] : 0u;
+      else win[i] = b < Q ? (uint32_t)S[e0 + b * es] : 0u;
+    }
+    for (int a = 0; a < Q; a++) {
+      uint32_t s = 0;
+#pragma unroll
+      for (int i = 0; i <= 2 * R; i++) s += win[i];
+      S[e0 + a * es] = (int32_t)s;
+      // slide: the next window ends at a + 1 + R
+      const int b = a + 1 + R;
+      uint32_t nx = 0;
+      if (b < Q) {
+        nx = (uint32_t)S[e0 + b * es];
+      } else if (TORUS) {
+#pragma unroll
+        for (int i = 0; i < R; i++) nx = (b - Q == i) ? head[i] : nx;
+      }
+#pragma unroll
+      for (int i = 0; i < 2 * R; i++) win[i] = win[i + 1];
+      win[2 * R] = nx;
+    }
+  }
+}
+
+template <int PROB>
+__device__ __forceinline__ void ccb_tile_linf(const SweepArgs &a, long long tile,
+                                              int32_t *__restrict__ S) {
+  bool big = false;
+  for (int k = 0; k < a.nax; k++) big = big || a.Q[k] > 16;
+  if (!big) {
+    cc_sweep_tile<PROB, 1, int32_t>(a, tile, S);
+    return;
+  }
+  // same stage / write-back as cc_sweep_tile, axes by the runtime-Q path
+  const int PT = a.P * a.TL;
+  const long long h = tile / a.ncolb;
+  const long long l0 = (tile - h * a.ncolb) * a.TL;
+  const int ncol = (int)min((long long)a.TL, a.s - l0);
+  const long long base = h * a.s * a.P + l0;
+  for (int e = threadIdx.x; e < PT; e += blockDim.x) {
+    const int p = e / a.TL, tl = e - p * a.TL;
+    const bool live = tl < ncol;
+    const long long w = base + (long long)p * a.s + tl;
+    if (a.in_kind == CC_IN_PRED)
+      S[e] = (live && cc_pred(a.pmode, (int)a.cnt[w], 0, a.mu)) ? 1 : 0;
+    else
+      S[e] = live ? a.A[0][w] : 0;
+  }
+  __syncthreads();
+  for (int k = 0; k < a.nax; k++) {
+    const int Q = a.Q[k];
+    const int nf = (a.P / Q) * a.TL;
+    constexpr bool T = PROB == CC_TORUS;
+    if (a.R == 1) ccb_step_linf_rt<1, T>(S, a.TL, nf, a.sk[k], Q);
+    else if (a.R == 2) ccb_step_linf_rt<2, T>(S, a.TL, nf, a.sk[k], Q);
+    else ccb_step_linf_rt<3, T>(S, a.TL, nf, a.sk[k], Q);
+    __syncthreads();
+  }
+  for (int e = threadIdx.x; e < PT; e += blockDim.x) {
+    const int p = e / a.TL, tl = e - p * a.TL;
+    if (tl >= ncol) continue;
+    const long long w = base + (long long)p * a.s + tl;
+    if (a.out_kind == CC_OUT_LAYERS) a.A[0][w] = S[e];
+    else if (a.out_kind == CC_OUT_MAP) a.A[0][w] = S[e];
+    else a.cnt[w] = S[e] > 65535 ? (uint16_t)65535 : (uint16_t)S[e];
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ void ccb_tile(const SweepArgs &a, int prob, int L,
+                                         long long tile, int32_t *S) {
+  if (prob == CC_HAMMING) {
+    if (L == 2) cc_sweep_tile<CC_HAMMING, 2, int32_t>(a, tile, S);
+    else if (L == 3) cc_sweep_tile<CC_HAMMING, 3, int32_t>(a, tile, S);
+    else cc_sweep_tile<CC_HAMMING, 4, int32_t>(a, tile, S);
+  } else if (prob == CC_LEE) {
+    if (L == 2) cc_sweep_tile<CC_LEE, 2, int32_t>(a, tile, S);
+    else if (L == 3) cc_sweep_tile<CC_LEE, 3, int32_t>(a, tile, S);
+    else cc_sweep_tile<CC_LEE, 4, int32_t>(a, tile, S);
+  } else if (prob == CC_TORUS) {
+    ccb_tile_linf<CC_TORUS>(a, tile, S);
+  } else {
+    ccb_tile_linf<CC_GRID>(a, tile, S);
+  }
+}
+
+// items[i] = (sweep argument index, tile), all of one sweep level
+__global__ void __launch_bounds__(CC_BLOCK)
+    k_ccb_sweeps(const SweepArgs *__restrict__ args, const int *__restrict__ argcell,
+                 const BCell *__restrict__ cells, const int2 *__restrict__ items,
+                 int nitems, unsigned *counter) {
+  extern __shared__ int32_t S[];
+  __shared__ int cur;
+  for (;;) {
+    if (threadIdx.x == 0) cur = (int)atomicAdd(counter, 1u);
+    __syncthreads();
+    const int it = cur;
+    __syncthreads();
+    if (it >= nitems) break;
+    // uniform item: its arguments load through the scalar unit
+    const int ai = __builtin_amdgcn_readfirstlane(items[it].x);
+    const long long tile = (long long)__builtin_amdgcn_readfirstlane(items[it].y);
+    const SweepArgs &a = args[ai];
+    const BCell &c = cells[argcell[ai]];
+    ccb_tile(a, c.prob, c.L, tile, S);
+  }
+}
+
+// ---------------------------------------------------------------- seeds
+// the per-level queue heads are cleared by a kernel, so the captured round
+// holds no memset node
+__global__ void k_ccb_fill(int *p, int v, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// A0 := 0 then A0[code] += 1, every cell (blockIdx.y = cell)
+__global__ void k_ccb_zero_a0(BCell *cells) {
+  const BCell &c = cells[blockIdx.y];
+  const long long gs = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < c.X; i += gs)
+    c.A[0][i] = 0;
+}
+
+__global__ void k_ccb_scatter(BCell *cells) {
+  const BCell &c = cells[blockIdx.y];
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < c.M; i += gridDim.x * blockDim.x)
+    atomicAdd(&c.A[0][c.code[i]], 1);
+}
+
+// ---------------------------------------------------------------- walks
+// one workgroup per item: cnt += delta over the ball of w in its cell.
+// Hamming / Lee walk their support table; torus / grid enumerate the
+// (2R+1)^n L-inf offsets (the open grid drops the ones off the board).
+__global__ void __launch_bounds__(CC_BLOCK)
+    k_ccb_walk(BCell *cells, const BItem *__restrict__ items) {
+  const BItem itm = items[blockIdx.x];
+  if (itm.delta == 0) return;
+  const BCell &c = cells[itm.cell];
+  const long long w = itm.w;
+  __shared__ long long disp[CC_MAXN * CC_CSTRIDE];
+  __shared__ int dig[CC_MAXN];
+  for (int p = threadIdx.x; p < c.n; p += blockDim.x)
+    dig[p] = (int)((w / c.pw[p]) % c.ax[p]);
+  __syncthreads();
+  auto bump = [&](long long x) {
+    uint32_t *cell32 = (uint32_t *)(c.cnt + (x & ~1LL));
+    const int sh = (int)(x & 1) * 16;
+    if (itm.delta > 0) atomicAdd(cell32, 1u << sh);
+    else atomicSub(cell32, 1u << sh);
+  };
+  if (c.prob == CC_TORUS || c.prob == CC_GRID) {
+    const int side = 2 * c.R + 1;
+    long long tot = 1;
+    for (int p = 0; p < c.n; p++) tot *= side;
+    for (long long t = threadIdx.x; t < tot; t += blockDim.x) {
+      long long x = 0, r = t;
+      bool ok = true;
+      for (int p = 0; p < c.n; p++) {
+        const int e = (int)(r % side) - c.R;
+        r /= side;
+        int d = dig[p] + e;
+        if (c.prob == CC_TORUS) d = (d % c.ax[p] + c.ax[p]) % c.ax[p];
+        else if (d < 0 || d >= c.ax[p]) { ok = false; break; }
+        x += (long long)d * c.pw[p];
+      }
+      if (ok) bump(x);
+    }
+    return;
+  }
+  for (int t = threadIdx.x; t < CC_MAXN * CC_CSTRIDE; t += blockDim.x) {
+    const int p = t / CC_CSTRIDE, dl = t - p * CC_CSTRIDE + 1;
+    long long off = 0;
+    if (p < c.n && dl < c.ax[p]) {
+      const int a = c.ax[p];
+      const int sh = (dig[p] + dl < a) ? dl : dl - a;
+      off = (long long)sh * c.pw[p];
+    }
+    disp[t] = off;
+  }
+  __syncthreads();
+  const int FB = c.fmt == CC_FMT_V1 ? 8 : 10;
+  const int NF = c.fmt == CC_FMT_V1 ? CC_MAXPAT : CC_MAXPAT_W;
+  const unsigned FM = (1u << FB) - 1u;
+  for (long long t = threadIdx.x; t < c.blen; t += blockDim.x) {
+    const uint64_t pt = c.pat[t];
+    long long x = w;
+    for (int k = 0; k < NF; k++) {
+      const unsigned b = (unsigned)(pt >> (FB * k)) & FM;
+      if (!b) break;
+      x += disp[(int)(b >> 4) * CC_CSTRIDE + (int)(b & 15u) - 1];
+    }
+    bump(x);
+  }
+}
+
+// ---------------------------------------------------------------- reductions
+__global__ void k_ccb_reset_red(BCell *cells, int ncells) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < ncells) { cells[i].amax = 0ull; cells[i].nzero = 0ull; }
+}
+
+// per cell (blockIdx.y): number of uncovered words; with MAP also the
+// largest map value and its smallest index
+__global__ void __launch_bounds__(CC_BLOCK) k_ccb_reduce(BCell *cells, int with_map) {
+  BCell &c = cells[blockIdx.y];
+  const long long gs = (long long)gridDim.x * CC_BLOCK;
+  unsigned long long z = 0, best = 0;
+  for (long long i = (long long)blockIdx.x * CC_BLOCK + threadIdx.x; i < c.X; i += gs) {
+    z += c.cnt[i] == 0;
+    if (with_map) {
+      const int32_t g = c.A[0][i];
+      if (g > 0) {
+        const unsigned long long key = ((unsigned long long)(uint32_t)g << 32) |
+                                       (0xFFFFFFFFull - (unsigned long long)i);
+        best = key > best ? key : best;
+      }
+    }
+  }
+  const long long zs = cc_block_sum((long long)z);
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long b2 = __shfl_xor(best, o, 64);
+    best = b2 > best ? b2 : best;
+  }
+  if ((threadIdx.x & 63) == 0 && best) atomicMax(&c.amax, best);
+  if (threadIdx.x == 0 && zs) atomicAdd(&c.nzero, (unsigned long long)zs);
+}
+
+// ---------------------------------------------------------------- LNS control
+__device__ __forceinline__ unsigned long long ccb_mix(unsigned long long x) {
+  // splitmix64 finaliser
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+__device__ __forceinline__ unsigned long long ccb_next(BCell &c) {
+  c.rng = ccb_mix(c.rng);
+  return c.rng;
+}
+
+// ruin: remove k distinct random words (a partial Fisher-Yates from the
+// tail); a feasible cell will re-add one fewer (notch descent)
+__global__ void k_ccb_ruin(BCell *cells, int ncells, int k, BItem *items) {
+  const int ci = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ci >= ncells) return;
+  BCell &c = cells[ci];
+  c.pre_uncov = c.uncov;
+  const int kk = min(k, c.M - 1);
+  c.nrem = kk > 0 ? kk : 0;
+  c.nadd = 0;
+  c.add_budget = kk <= 0 ? 0 : (c.uncov == 0 ? kk - 1 : kk);
+  for (int j = 0; j < CCB_KMAX; j++) {
+    BItem it{ci, 0, 0};
+    if (j < c.nrem) {
+      const int top = c.M - 1 - j;
+      const int r = (int)(ccb_next(c) % (unsigned long long)(top + 1));
+      const long long t = c.code[r];
+      c.code[r] = c.code[top];
+      c.code[top] = t;
+      c.rem[j] = t;
+      it.delta = -1;
+      it.w = t;
+    }
+    items[ci * CCB_KMAX + j] = it;
+  }
+  c.M -= c.nrem;
+}
+
+// recreate step j: add the word of largest gain (smallest index on ties).
+// j < 0: a greedy step instead (add while the cell has uncovered words; the
+// added words are not recorded for a revert)
+__global__ void k_ccb_add(BCell *cells, int ncells, int j, BItem *items) {
+  const int ci = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ci >= ncells) return;
+  BCell &c = cells[ci];
+  BItem it{ci, 0, 0};
+  const bool go = j < 0 ? c.nzero != 0ull : j < c.add_budget;
+  if (go && c.amax != 0ull && c.M < c.cap) {
+    const long long w = (long long)(0xFFFFFFFFull - (c.amax & 0xFFFFFFFFull));
+    c.code[c.M++] = w;
+    if (j >= 0) c.add[c.nadd++] = w;
+    it.delta = 1;
+    it.w = w;
+  }
+  items[ci] = it;
+}
+
+// accept / revert: a round that uncovers more words is undone with
+// probability 0.9 (added words out, removed words back)
+__global__ void k_ccb_decide(BCell *cells, int ncells, BItem *items) {
+  const int ci = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ci >= ncells) return;
+  BCell &c = cells[ci];
+  const int now = (int)c.nzero;
+  bool revert = false;
+  if (c.nrem > 0 && now > c.pre_uncov) {
+    const double u = (double)(ccb_next(c) >> 11) * (1.0 / 9007199254740992.0);
+    revert = u > 0.10;
+  }
+  for (int j = 0; j < 2 * CCB_KMAX; j++) items[ci * 2 * CCB_KMAX + j] = BItem{ci, 0, 0};
+  if (revert) {
+    for (int j = 0; j < c.nadd; j++)
+      items[ci * 2 * CCB_KMAX + j] = BItem{ci, -1, c.add[j]};
+    c.M -= c.nadd;
+    for (int j = 0; j < c.nrem; j++) {
+      items[ci * 2 * CCB_KMAX + CCB_KMAX + j] = BItem{ci, 1, c.rem[j]};
+      c.code[c.M++] = c.rem[j];
+    }
+    c.uncov = c.pre_uncov;
+    c.reverted++;
+  } else {
+    c.uncov = now;
+    if (c.nrem > 0) c.accepted++;
+    if (now == 0 && c.M < c.bestM) {
+      c.bestM = c.M;
+      for (int i = 0; i < c.M; i++) c.best[i] = c.code[i];
+    }
+  }
+  c.rounds++;
+}
+
+// initial state after a recount: uncovered count, best = the start if feasible
+__global__ void k_ccb_lns_init(BCell *cells, int ncells, unsigned long long seed) {
+  const int ci = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ci >= ncells) return;
+  BCell &c = cells[ci];
+  c.uncov = (int)c.nzero;
+  c.rng = ccb_mix(seed * 0x100000001B3ull + (unsigned long long)ci);
+  c.rounds = c.accepted = c.reverted = 0;
+  c.bestM = 0x7FFFFFFF;
+  if (c.uncov == 0) {
+    c.bestM = c.M;
+    for (int i = 0; i < c.M; i++) c.best[i] = c.code[i];
+  }
+}

@@ -127,69 +127,76 @@ __device__ __forceinline__ void cc_lay_st(const SweepArgs &a, int d,
   else a.A[d][w] = (int32_t)v;
 }
 
+// One tile of a sweep: stage, run every axis of the group in LDS, write
+// back (the batched transform calls it for tiles of many cells).
 template <int PROB, int R1, typename LT>
-__global__ void __launch_bounds__(CC_BLOCK) k_sweep(SweepArgs a) {
-  extern __shared__ int32_t S[];
+__device__ __forceinline__ void cc_sweep_tile(const SweepArgs &a, long long tile,
+                                              int32_t *__restrict__ S) {
   constexpr int QMIN = CcQRange<PROB>::lo, QMAX = CcQRange<PROB>::hi;
   const int PT = a.P * a.TL;
   const int Lmax = a.live[a.nax];
   const int Lin = a.live[0];
-  for (long long tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
-    const long long h = tile / a.ncolb;
-    const long long l0 = (tile - h * a.ncolb) * a.TL;
-    const int ncol = (int)min((long long)a.TL, a.s - l0);
-    const long long base = h * a.s * a.P + l0;
-    // ---- stage the tile: every live layer of every word, coalesced along tl
-    for (int e = threadIdx.x; e < PT; e += CC_BLOCK) {
-      const int p = e / a.TL, tl = e - p * a.TL;
-      const bool live = tl < ncol;
-      const long long w = base + (long long)p * a.s + tl;
-      if (a.in_kind == CC_IN_PRED)
-        S[e] = (live && cc_pred(a.pmode, (int)a.cnt[w], 0, a.mu)) ? 1 : 0;
-      else if (a.in_kind == CC_IN_CODE)
-        S[e] = live ? a.A[0][w] : 0;
-      else
-        S[e] = live ? (int32_t)cc_lay_ld<LT>(a, 0, w) : 0;
-      for (int d = 1; d < Lmax; d++)
-        S[d * PT + e] = (live && d < Lin && a.in_kind == CC_IN_LAYERS)
-                            ? (int32_t)cc_lay_ld<LT>(a, d, w) : 0;
-    }
-    __syncthreads();
-    // ---- every axis of the group, in LDS
-    for (int k = 0; k < a.nax; k++) {
-      const int lv = a.live[k], lw = a.live[k + 1];
-      const int Q = a.Q[k], sk = a.sk[k];
-      const int nf = (a.P / Q) * a.TL;
-      switch (Q) {
-        CC_STEP_CASE(2) CC_STEP_CASE(3) CC_STEP_CASE(4) CC_STEP_CASE(5)
-        CC_STEP_CASE(6) CC_STEP_CASE(7) CC_STEP_CASE(8) CC_STEP_CASE(9)
-        CC_STEP_CASE(10) CC_STEP_CASE(11) CC_STEP_CASE(12) CC_STEP_CASE(13)
-        CC_STEP_CASE(14) CC_STEP_CASE(15) CC_STEP_CASE(16)
-        default: break;
-      }
-      __syncthreads();
-    }
-    // ---- write back: layers, or the layer sum into the map / cnt plane
-    for (int e = threadIdx.x; e < PT; e += CC_BLOCK) {
-      const int p = e / a.TL, tl = e - p * a.TL;
-      if (tl >= ncol) continue;
-      const long long w = base + (long long)p * a.s + tl;
-      if (a.out_kind == CC_OUT_LAYERS) {
-        for (int d = a.write_l0 ? 0 : 1; d < Lmax; d++)
-          cc_lay_st<LT>(a, d, w, (uint32_t)S[d * PT + e]);
-      } else {
-        uint32_t sum = 0;
-        for (int d = 0; d < Lmax; d++) sum += (uint32_t)S[d * PT + e];
-        if (a.out_kind == CC_OUT_MAP) {
-          a.A[0][w] = (int32_t)sum;
-        } else {
-          const int32_t si = (int32_t)sum;
-          a.cnt[w] = si > 65535 ? (uint16_t)65535 : (uint16_t)si;
-        }
-      }
+  const long long h = tile / a.ncolb;
+  const long long l0 = (tile - h * a.ncolb) * a.TL;
+  const int ncol = (int)min((long long)a.TL, a.s - l0);
+  const long long base = h * a.s * a.P + l0;
+  // ---- stage the tile: every live layer of every word, coalesced along tl
+  for (int e = threadIdx.x; e < PT; e += blockDim.x) {
+    const int p = e / a.TL, tl = e - p * a.TL;
+    const bool live = tl < ncol;
+    const long long w = base + (long long)p * a.s + tl;
+    if (a.in_kind == CC_IN_PRED)
+      S[e] = (live && cc_pred(a.pmode, (int)a.cnt[w], 0, a.mu)) ? 1 : 0;
+    else if (a.in_kind == CC_IN_CODE)
+      S[e] = live ? a.A[0][w] : 0;
+    else
+      S[e] = live ? (int32_t)cc_lay_ld<LT>(a, 0, w) : 0;
+    for (int d = 1; d < Lmax; d++)
+      S[d * PT + e] = (live && d < Lin && a.in_kind == CC_IN_LAYERS)
+                          ? (int32_t)cc_lay_ld<LT>(a, d, w) : 0;
+  }
+  __syncthreads();
+  // ---- every axis of the group, in LDS
+  for (int k = 0; k < a.nax; k++) {
+    const int lv = a.live[k], lw = a.live[k + 1];
+    const int Q = a.Q[k], sk = a.sk[k];
+    const int nf = (a.P / Q) * a.TL;
+    switch (Q) {
+      CC_STEP_CASE(2) CC_STEP_CASE(3) CC_STEP_CASE(4) CC_STEP_CASE(5)
+      CC_STEP_CASE(6) CC_STEP_CASE(7) CC_STEP_CASE(8) CC_STEP_CASE(9)
+      CC_STEP_CASE(10) CC_STEP_CASE(11) CC_STEP_CASE(12) CC_STEP_CASE(13)
+      CC_STEP_CASE(14) CC_STEP_CASE(15) CC_STEP_CASE(16)
+      default: break;
     }
     __syncthreads();
   }
+  // ---- write back: layers, or the layer sum into the map / cnt plane
+  for (int e = threadIdx.x; e < PT; e += blockDim.x) {
+    const int p = e / a.TL, tl = e - p * a.TL;
+    if (tl >= ncol) continue;
+    const long long w = base + (long long)p * a.s + tl;
+    if (a.out_kind == CC_OUT_LAYERS) {
+      for (int d = a.write_l0 ? 0 : 1; d < Lmax; d++)
+        cc_lay_st<LT>(a, d, w, (uint32_t)S[d * PT + e]);
+    } else {
+      uint32_t sum = 0;
+      for (int d = 0; d < Lmax; d++) sum += (uint32_t)S[d * PT + e];
+      if (a.out_kind == CC_OUT_MAP) {
+        a.A[0][w] = (int32_t)sum;
+      } else {
+        const int32_t si = (int32_t)sum;
+        a.cnt[w] = si > 65535 ? (uint16_t)65535 : (uint16_t)si;
+      }
+    }
+  }
+  __syncthreads();
+}
+
+template <int PROB, int R1, typename LT>
+__global__ void __launch_bounds__(CC_BLOCK) k_sweep(SweepArgs a) {
+  extern __shared__ int32_t S[];
+  for (long long tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x)
+    cc_sweep_tile<PROB, R1, LT>(a, tile, S);
 }
 #undef CC_STEP_CASE
 

@@ -102,10 +102,13 @@ def test_locality_order_is_permutation():
         assert out[0] == 0 and (np.sort(out) == np.sort(pats)).all()
 
 
-def _declared_symbols():
-    with open(HEADER) as f:
+BATCH_HEADER = os.path.join(ROOT, "include", "coldcore_amd", "batch.h")
+
+
+def _declared_symbols(header=HEADER, prefix="dct_"):
+    with open(header) as f:
         text = f.read()
-    return sorted(set(re.findall(r"\b(dct_[a-z0-9_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(" + prefix + r"[a-z0-9_]+)\s*\(", text)))
 
 
 def test_header_declares_the_v1_surface():
@@ -121,8 +124,13 @@ def test_library_exports_every_declared_symbol():
         pytest.fail("libcoldcore_covering.so is not built "
                     "(python -m coldcore_amd.build)")
     lib = ctypes.CDLL(LIB)
-    for name in _declared_symbols():
+    for name in _declared_symbols() + _declared_symbols(BATCH_HEADER, "ccb_"):
         assert hasattr(lib, name), name
+
+
+def test_batch_header_matches_binding():
+    from coldcore_amd.batch import BATCH_SYMBOLS
+    assert set(_declared_symbols(BATCH_HEADER, "ccb_")) == set(BATCH_SYMBOLS)
 
 
 # ---------------------------------------------------------------- planner
