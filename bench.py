@@ -238,6 +238,66 @@ def run_batch64(GpuBackend, rounds=200, k=2):
     return out
 
 
+def run_near_cover(GpuBackend):
+    """Set-side gathers and the swap-evaluation fast path against the walk
+    (CC_SET_POLICY=0) on near-cover states: K_2(24,3) with 60,000 random
+    code words (seed 3; a few thousand words stay uncovered), 65,536 gain
+    queries (seed 4); K_5(8,3) with 1,000 random code words (seed 5), one
+    polish move = swap_eval of 1 drop and 110 candidates (seed 6)."""
+    out = {}
+    size = 1 << 24
+    code = np.random.default_rng(3).choice(size, 60000, replace=False)
+    code = code.astype(np.int64)
+    cands = np.random.default_rng(4).integers(0, size, 65536).astype(np.int64)
+    res = {}
+    try:
+        for pol in ("1", "0"):
+            os.environ["CC_SET_POLICY"] = pol
+            g = GpuBackend([2] * 24, 3)
+            try:
+                g.recount(code)
+                out["k2_24_3_uncovered"] = int(g.count_eq(0))
+                t = time.perf_counter()
+                r = g.ball_gather(cands, 0)            # lists Z on the set side
+                first = (time.perf_counter() - t) * 1e3
+                t = time.perf_counter()
+                for _ in range(10):
+                    r = g.ball_gather(cands, 0)
+                res[pol] = (first, (time.perf_counter() - t) * 100, r)
+            finally:
+                g.close()
+        out["gather65536_set_side_first_ms"] = round(res["1"][0], 3)
+        out["gather65536_set_side_ms"] = round(res["1"][1], 3)
+        out["gather65536_walk_ms"] = round(res["0"][1], 3)
+        out["gather_identical"] = bool((res["1"][2] == res["0"][2]).all())
+        s5 = 5 ** 8
+        code5 = np.random.default_rng(5).choice(s5, 1000, replace=False)
+        code5 = code5.astype(np.int64)
+        rng = np.random.default_rng(6)
+        drops = code5[:1]
+        cand5 = [rng.integers(0, s5, 110).astype(np.int64)]
+        for pol in ("1", "0"):
+            os.environ["CC_SET_POLICY"] = pol
+            g = GpuBackend([5] * 8, 3)
+            try:
+                g.recount(code5)
+                ref = g.swap_eval(drops, cand5)
+                t = time.perf_counter()
+                for _ in range(50):
+                    got = g.swap_eval(drops, cand5)
+                us = (time.perf_counter() - t) * 1e6 / 50
+                key = "fast" if pol == "1" else "fallback"
+                out[f"swap_eval_k5_8_3_{key}_us"] = round(us, 1)
+                res["swap" + pol] = (got[0].tolist(), [x.tolist() for x in got[1]])
+                assert res["swap" + pol][0] == ref[0].tolist()
+            finally:
+                g.close()
+        out["swap_identical"] = res["swap1"] == res["swap0"]
+    finally:
+        os.environ.pop("CC_SET_POLICY", None)
+    return out
+
+
 def copy_calibration(mib):
     """hipMemcpy device-to-device of a known size: its dispatch
     (__amd_rocclr_copyBuffer) reads and writes exactly mib MiB."""
@@ -315,7 +375,14 @@ def main():
                     help="only run a device-to-device copy of MIB MiB "
                          "(3 times): the reference kernel for scaling the "
                          "FETCH_SIZE / WRITE_SIZE counters")
+    ap.add_argument("--only", choices=("near_cover", "batch64"),
+                    help="run one side section and print its JSON only")
     args = ap.parse_args()
+    if args.only:
+        from coldcore_amd.gpu import GpuBackend
+        fn = run_near_cover if args.only == "near_cover" else run_batch64
+        print(json.dumps({args.only: fn(GpuBackend)}), flush=True)
+        return
     if args.calib_copy:
         copy_calibration(args.calib_copy)
         return
@@ -474,6 +541,7 @@ def main():
     if not args.no_extra:
         rec["other_cells"] = time_cells(GpuBackend, 10)
         rec["config1"] = run_config1(GpuBackend, not args.no_cpu)
+        rec["near_cover"] = run_near_cover(GpuBackend)
         try:
             rec["batch64"] = run_batch64(GpuBackend)
         except RuntimeError as e:          # reported, never hidden

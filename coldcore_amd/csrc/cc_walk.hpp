@@ -30,17 +30,12 @@ struct Geom {
 #define CC_WALK_TRANS 1   // ... and count deficit transitions (update_delta)
 #define CC_WALK_GATHER 2  // count predicate(cnt) over the ball
 
-template <int KIND, bool GRID, int FMT>
-__global__ void __launch_bounds__(CC_BLOCK)
-    k_walk(Geom g, WordsArg wa, const long long *__restrict__ wptr, int split,
-           int delta, uint16_t *cnt, const uint64_t *__restrict__ pat,
-           long long blen, int pmode, int target, int32_t *out,
-           unsigned long long *trans) {
-  __shared__ long long disp[CC_MAXN * CC_CSTRIDE];
-  __shared__ unsigned char okf[CC_MAXN * CC_CSTRIDE];
-  const int wi = blockIdx.x / split, seg = blockIdx.x - wi * split;
-  const long long w = wptr ? wptr[wi] : wa.w[wi];
-  for (int t = threadIdx.x; t < CC_MAXN * CC_CSTRIDE; t += CC_BLOCK) {
+// the displacement (and, for the open grid, in-board flag) of every
+// (position, delta) slot for centre w; the caller synchronises
+__device__ __forceinline__ void cc_walk_disp(const Geom &g, long long w,
+                                             long long *disp,
+                                             unsigned char *okf) {
+  for (int t = threadIdx.x; t < CC_MAXN * CC_CSTRIDE; t += blockDim.x) {
     const int p = t / CC_CSTRIDE, dl = t - p * CC_CSTRIDE + 1;
     long long off = 0;
     unsigned char ok = 0;
@@ -54,25 +49,50 @@ __global__ void __launch_bounds__(CC_BLOCK)
     disp[t] = off;
     okf[t] = ok;
   }
+}
+
+// the word pattern pt reaches from centre w; *ok = false when an open-grid
+// move leaves the board
+template <bool GRID, int FMT>
+__device__ __forceinline__ long long cc_walk_word(uint64_t pt, long long w,
+                                                  const long long *disp,
+                                                  const unsigned char *okf,
+                                                  bool *ok) {
+  long long x = w;
+  bool good = true;
+  constexpr int FB = FMT == CC_FMT_V1 ? 8 : 10;
+  constexpr int NF = FMT == CC_FMT_V1 ? CC_MAXPAT : CC_MAXPAT_W;
+  constexpr unsigned FM = (1u << FB) - 1u;
+#pragma unroll
+  for (int k = 0; k < NF; k++) {
+    const unsigned b = (unsigned)(pt >> (FB * k)) & FM;
+    if (!b) break;
+    const int e = (int)(b >> 4) * CC_CSTRIDE + (int)(b & 15u) - 1;
+    x += disp[e];
+    if (GRID) good = good && okf[e];
+  }
+  *ok = good;
+  return x;
+}
+
+template <int KIND, bool GRID, int FMT>
+__global__ void __launch_bounds__(CC_BLOCK)
+    k_walk(Geom g, WordsArg wa, const long long *__restrict__ wptr, int split,
+           int delta, uint16_t *cnt, const uint64_t *__restrict__ pat,
+           long long blen, int pmode, int target, int32_t *out,
+           unsigned long long *trans) {
+  __shared__ long long disp[CC_MAXN * CC_CSTRIDE];
+  __shared__ unsigned char okf[CC_MAXN * CC_CSTRIDE];
+  const int wi = blockIdx.x / split, seg = blockIdx.x - wi * split;
+  const long long w = wptr ? wptr[wi] : wa.w[wi];
+  cc_walk_disp(g, w, disp, okf);
   __syncthreads();
   long long acc = 0;
   const long long step = (long long)split * CC_BLOCK;
   for (long long t = (long long)seg * CC_BLOCK + threadIdx.x; t < blen;
        t += step) {
-    const uint64_t pt = pat[t];
-    long long x = w;
-    bool ok = true;
-    constexpr int FB = FMT == CC_FMT_V1 ? 8 : 10;
-    constexpr int NF = FMT == CC_FMT_V1 ? CC_MAXPAT : CC_MAXPAT_W;
-    constexpr unsigned FM = (1u << FB) - 1u;
-#pragma unroll
-    for (int k = 0; k < NF; k++) {
-      const unsigned b = (unsigned)(pt >> (FB * k)) & FM;
-      if (!b) break;
-      const int e = (int)(b >> 4) * CC_CSTRIDE + (int)(b & 15u) - 1;
-      x += disp[e];
-      if (GRID) ok = ok && okf[e];
-    }
+    bool ok;
+    const long long x = cc_walk_word<GRID, FMT>(pat[t], w, disp, okf, &ok);
     if (GRID && !ok) continue;
     if (KIND == CC_WALK_GATHER) {
       acc += cc_pred(pmode, (int)cnt[x], target, g.mu) ? 1 : 0;

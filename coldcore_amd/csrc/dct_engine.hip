@@ -28,6 +28,7 @@
 #include "cc_launch.hpp"
 #include "cc_walk.hpp"
 #include "cc_psweep.hpp"
+#include "cc_setside.hpp"
 
 #define CK(x)                                                               \
   do {                                                                      \
@@ -48,6 +49,10 @@ constexpr int CODECAP = 1 << 16;       // staging words per chunk
 constexpr int ASYNC_UPDATE_MAX = 1024; // larger update batches synchronise
 constexpr int NV_WORDS = 128;          // words per by-value launch
 constexpr long long EXTRACT_CH = 1 << 14;  // words per extraction chunk
+// swap evaluation staging: candidates and drops (<= CODECAP each) with the
+// candidate -> drop map in, gains, losses and the list count out
+constexpr size_t SWAP_IN_BYTES = (size_t)CODECAP * (8 + 8 + 4);
+constexpr size_t SWAP_RES_BYTES = (size_t)CODECAP * (4 + 4) + 16;
 
 struct Sweep {
   int j0, nax, c0, P, TL;
@@ -136,6 +141,29 @@ struct Engine {
   int pk_first = 1;          // CC_PK_FIRST: first sweep 0 k_psweep, 1 persistent
                              // with prefetch, 2 LDS-free (cross-lane)
   PPlan pfield;              // packed plan of gain / loss (predicate seeds)
+  // set-side gathers: exact when the walked table is exactly the metric
+  // ball (proven at init / dct_set_tbl2); Z cached per cnt state
+  int set_policy = 1;        // CC_SET_POLICY: 0 never, 1 auto, 2 always
+  int set_ratio = 2;         // CC_SET_RATIO: list Z only while |Z| <= ratio |B|
+  long long set_min = -1;    // CC_SET_MIN: walk reads from which the set side
+                             // is tried (-1: max(|X| / 8, 2^21))
+  int set_ok1 = 0, set_ok2 = 0;  // init table / ox table proven to be the ball
+  int dbits = 4;             // bits per packed digit
+  uint64_t *zbuf = nullptr;
+  long long zcap = 0, zn = 0;
+  int z_mode = -1, z_target = 0, z_mu = 0;
+  unsigned long long z_ver = 0;
+  int o_mode = -1, o_target = 0, o_mu = 0;   // last listing that overflowed
+  long long o_F = 0, o_plus = 0;
+  unsigned long long cnt_ver = 1;            // bumped by every cnt change
+  long long plus_words = 0;                  // words placed by +1 updates
+  // swap evaluation fast path: drop lists (packed digits, drop index)
+  uint64_t *lbuf = nullptr;
+  int32_t *ldrop = nullptr;
+  long long lcap = 0;
+  // one packed upload [cand nc x 8][drops k x 8][cand -> drop nc x 4] and
+  // one result region [gain nc x 4][loss k x 4][pad][list count 8]
+  char *d_swap = nullptr, *h_swap = nullptr, *d_sres = nullptr, *h_sres = nullptr;
   // optional per-sweep timing of the last transform (additive ABI)
   int prof = 0;
   hipEvent_t ev[2 * CC_MAXN] = {};
@@ -144,6 +172,21 @@ struct Engine {
 };
 
 Engine E;
+
+bool set_ok() { return E.fmt == 1 ? E.set_ok2 : E.set_ok1; }
+
+Dig make_dig() {
+  Dig d;
+  memset(&d, 0, sizeof(d));
+  for (int i = 0; i < E.n; i++) d.ax[i] = E.ax[i];
+  d.n = E.n;
+  d.b = E.dbits;
+  d.R = E.R;
+  d.mu = E.mu;
+  d.lsb = 0;
+  for (int i = 0; i < E.n; i++) d.lsb |= 1ull << (E.dbits * i);
+  return d;
+}
 
 Geom make_geom() {
   Geom g;
@@ -202,6 +245,66 @@ bool table_ok(const uint64_t *pat, long long len, int fmt) {
     }
   }
   return true;
+}
+
+// Is a (well-formed) table exactly the metric ball of radius R?  Every
+// pattern within the radius (Hamming: one per moved position; Lee: the
+// circular distances sum; torus / grid: every circular distance <= R), no
+// two patterns equal (fields compared in position order), and the count
+// equals the ball's size (the coefficients 0..R of the per-axis weight
+// enumerator product; a single coefficient for the L-inf families).  Only
+// then does a set-side count equal the walk's.
+bool table_is_ball(const uint64_t *pat, long long len, int fmt) {
+  const int fb = fmt == CC_FMT_V1 ? 8 : 10;
+  const int nf = fmt == CC_FMT_V1 ? CC_MAXPAT : CC_MAXPAT_W;
+  const uint64_t fm = (1ull << fb) - 1ull;
+  const bool linf = E.prob == CC_TORUS || E.prob == CC_GRID;
+  auto wt = [&](int pos, int d) -> int {   // -1: outside the radius
+    const int a = E.ax[pos], c = std::min(d, a - d);
+    if (E.prob == CC_HAMMING) return 1;
+    if (E.prob == CC_LEE) return c <= E.R ? c : -1;
+    return c <= E.R ? 0 : -1;
+  };
+  std::vector<uint64_t> keys((size_t)len);
+  for (long long t = 0; t < len; t++) {
+    const uint64_t p = pat[t];
+    uint64_t f[CC_MAXPAT] = {};
+    int k = 0, w = 0;
+    for (int i = 0; i < nf; i++) {
+      const unsigned b = (unsigned)((p >> (fb * i)) & fm);
+      if (!b) break;
+      const int e = wt((int)(b >> 4), (int)(b & 15u));
+      if (e < 0) return false;
+      w += e;
+      f[k++] = b;
+    }
+    if (w > E.R) return false;
+    std::sort(f, f + k);   // position-major: pos << 4 | delta
+    uint64_t key = 0;
+    for (int i = 0; i < k; i++) key |= f[i] << (fb * i);
+    keys[(size_t)t] = key;
+  }
+  std::sort(keys.begin(), keys.end());
+  if (std::adjacent_find(keys.begin(), keys.end()) != keys.end()) return false;
+  const int W = linf ? 0 : E.R;
+  std::vector<unsigned long long> poly(W + 1, 0);
+  poly[0] = 1;
+  const unsigned long long cap = 1ull << 62;
+  for (int i = 0; i < E.n; i++) {
+    std::vector<unsigned long long> nx(W + 1, 0);
+    for (int w = 0; w <= W; w++) {
+      if (!poly[w]) continue;
+      nx[w] = std::min(cap, nx[w] + poly[w]);
+      for (int d = 1; d < E.ax[i]; d++) {
+        const int e = wt(i, d);
+        if (e >= 0 && w + e <= W) nx[w + e] = std::min(cap, nx[w + e] + poly[w]);
+      }
+    }
+    poly = nx;
+  }
+  unsigned long long tot = 0;
+  for (int w = 0; w <= W; w++) tot = std::min(cap, tot + poly[w]);
+  return tot == (unsigned long long)len;
 }
 
 // ---- sweep schedule ------------------------------------------------------
@@ -812,6 +915,8 @@ hipError_t launch_walk(const WordsArg &wa, const long long *wptr, int nb,
 template <int KIND>
 int update_inner(const long long *words, int nwords, int delta) {
   if (nwords <= 0) return 0;
+  E.cnt_ver++;
+  if (delta > 0) E.plus_words += nwords;
   for (int off = 0; off < nwords; off += CODECAP) {
     const int nb = std::min(nwords - off, CODECAP);
     if (nb <= ASYNC_UPDATE_MAX) {
@@ -835,6 +940,108 @@ int update_inner(const long long *words, int nwords, int delta) {
   return 0;
 }
 
+// Z = {x : pred(cnt[x])} listed for the current cnt (reference dct_hc.cu:
+// 607-646 for the caching contract).  Returns 1 when the list is valid, 0
+// when Z has more than lim entries (the caller walks), < 0 on error.  The
+// predicates listed shrink under placements, so a listing that overflowed
+// by F words still overflows while F - (words placed since) x |B| > lim:
+// that case skips the scan.
+int ensure_zlist(int pmode, int target, long long lim) {
+  if (E.z_mode == pmode && E.z_target == target && E.z_mu == E.mu &&
+      E.z_ver == E.cnt_ver)
+    return 1;
+  if (E.o_mode == pmode && E.o_target == target && E.o_mu == E.mu &&
+      E.o_F - (E.plus_words - E.o_plus) * E.blen > lim)
+    return 0;
+  lim = std::min(lim, E.zcap);
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, E.st, E.d_scal + 2, 1);
+  CK(hipGetLastError());
+  const Dig dg = make_dig();
+  if (pmode == CC_DEF)
+    hipLaunchKernelGGL(k_zlist<CC_DEF>, dim3(pass_blocks(E.X)), dim3(CC_BLOCK), 0,
+                       E.st, E.cnt, E.X, target, dg, E.zbuf, lim, E.d_scal + 2);
+  else
+    hipLaunchKernelGGL(k_zlist<CC_EQ>, dim3(pass_blocks(E.X)), dim3(CC_BLOCK), 0,
+                       E.st, E.cnt, E.X, target, dg, E.zbuf, lim, E.d_scal + 2);
+  CK(hipGetLastError());
+  unsigned long long found = 0;
+  if (int rc = fetch_scalar(2, &found)) return rc;
+  E.stats[2]++;
+  if ((long long)found <= lim) {
+    E.z_mode = pmode; E.z_target = target; E.z_mu = E.mu;
+    E.z_ver = E.cnt_ver; E.zn = (long long)found;
+    return 1;
+  }
+  E.stats[3]++;
+  E.o_mode = pmode; E.o_target = target; E.o_mu = E.mu;
+  E.o_F = (long long)found; E.o_plus = E.plus_words;
+  E.z_mode = -1;
+  return 0;
+}
+
+// set side for this chunk?  Only shrinking predicates (deficient, cnt == 0),
+// which are the small sets near a cover; by default when the walk would
+// read at least max(|X| / 8, 2^21) cnt entries (reference dct_hc.cu:648-659)
+bool want_set(int pmode, int target, int nb) {
+  if (!set_ok() || !E.zbuf || E.set_policy == 0) return false;
+  if (!(pmode == CC_DEF || (pmode == CC_EQ && target == 0))) return false;
+  if (E.set_policy == 2) return true;
+  const long long lim = E.set_min >= 0 ? E.set_min
+                                       : std::max(E.X / 8, 1LL << 21);
+  return (long long)nb * E.blen >= lim;
+}
+
+template <int PROB>
+void launch_gather_set(const long long *dw, int nb, int32_t *dout) {
+  const int gy = (nb + CCS_CB - 1) / CCS_CB;
+  long long gx = std::max(1LL, (E.zn + 2047) / 2048);
+  gx = std::min(gx, std::max(1LL, 4096LL / gy));
+  const long long zper = (E.zn + gx - 1) / gx;
+  hipLaunchKernelGGL(k_gather_set<PROB>, dim3((unsigned)gx, (unsigned)gy),
+                     dim3(CC_BLOCK), 0, E.st, dw, nb, E.zbuf, E.zn, zper,
+                     make_dig(), dout);
+}
+
+// the gather of nb device words into dout, set side or support walk, no
+// final synchronisation; zeroed: dout already holds zeros
+int gather_launch(int pmode, int target, int nb, const long long *dw,
+                  int32_t *dout, bool zeroed) {
+  const long long lim = std::min(E.blen * E.set_ratio, E.zcap);
+  const int zs = want_set(pmode, target, nb) ? ensure_zlist(pmode, target, lim)
+                                             : 0;
+  if (zs < 0) return zs;
+  if (zs == 1) {
+    E.stats[0]++;
+    if (E.zn > 32768) E.stats[7]++;
+    if (!zeroed) {
+      hipLaunchKernelGGL(k_fill32, dim3((nb + 255) / 256), dim3(256), 0, E.st,
+                         dout, 0, (long long)nb);
+      CK(hipGetLastError());
+    }
+    if (E.zn > 0) {
+      switch (E.prob) {
+        case CC_HAMMING: launch_gather_set<CC_HAMMING>(dw, nb, dout); break;
+        case CC_LEE: launch_gather_set<CC_LEE>(dw, nb, dout); break;
+        case CC_TORUS: launch_gather_set<CC_TORUS>(dw, nb, dout); break;
+        default: launch_gather_set<CC_GRID>(dw, nb, dout); break;
+      }
+      CK(hipGetLastError());
+    }
+    return 0;
+  }
+  E.stats[1]++;
+  const int split = pick_split(nb);
+  if (split > 1 && !zeroed) {
+    hipLaunchKernelGGL(k_fill32, dim3((nb + 255) / 256), dim3(256), 0, E.st,
+                       dout, 0, (long long)nb);
+    CK(hipGetLastError());
+  }
+  WordsArg wa;
+  memset(&wa, 0, sizeof(wa));
+  CK((launch_walk<CC_WALK_GATHER>(wa, dw, nb, split, 0, pmode, target, dout)));
+  return 0;
+}
+
 int gather_impl(const long long *words, int nwords, int pmode, int target,
                 int32_t *out) {
   if (nwords <= 0) return 0;
@@ -842,21 +1049,12 @@ int gather_impl(const long long *words, int nwords, int pmode, int target,
   for (int off = 0; off < nwords; off += CODECAP) {
     const int nb = std::min(nwords - off, CODECAP);
     if (int rc = stage_words(words + off, nb)) return rc;
-    const int split = pick_split(nb);
-    if (split > 1) {
-      hipLaunchKernelGGL(k_fill32, dim3((nb + 255) / 256), dim3(256), 0, E.st,
-                         E.d_out, 0, (long long)nb);
-      CK(hipGetLastError());
-    }
-    WordsArg wa;
-    memset(&wa, 0, sizeof(wa));
-    CK((launch_walk<CC_WALK_GATHER>(wa, E.d_words, nb, split, 0, pmode, target,
-                                    E.d_out)));
+    if (int rc = gather_launch(pmode, target, nb, E.d_words, E.d_out, false))
+      return rc;
     CK(hipMemcpyAsync(E.h_out, E.d_out, (size_t)nb * 4, hipMemcpyDeviceToHost,
                       E.st));
     CK(hipStreamSynchronize(E.st));
     memcpy(out + off, E.h_out, (size_t)nb * 4);
-    E.stats[1]++;
   }
   return 0;
 }
@@ -904,13 +1102,21 @@ void release_all() {
   for (int d = 0; d < CC_MAXL; d++) { E.A[d] = nullptr; E.H[d] = nullptr; }
   void *bufs[] = {E.cnt, E.pat, E.pat2, E.d_code, E.d_words, E.d_out,
                   E.d_scal, E.d_hist, E.d_eidx64, E.d_eidx32, E.d_eval,
-                  E.d_bcnt, E.d_boff};
+                  E.d_bcnt, E.d_boff, E.zbuf, E.lbuf, E.ldrop, E.d_swap,
+                  E.d_sres};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   E.cnt = nullptr; E.pat = E.pat2 = nullptr; E.d_code = E.d_words = nullptr;
   E.d_out = nullptr; E.d_scal = nullptr; E.d_hist = nullptr;
   E.d_eidx64 = nullptr; E.d_eidx32 = nullptr; E.d_eval = nullptr;
   E.d_bcnt = E.d_boff = nullptr;
+  E.zbuf = nullptr; E.lbuf = nullptr; E.ldrop = nullptr;
+  E.d_swap = nullptr; E.d_sres = nullptr;
+  E.zcap = E.lcap = 0;
+  E.set_ok1 = E.set_ok2 = 0;
+  E.z_mode = E.o_mode = -1;
+  if (E.h_swap) { (void)hipHostFree(E.h_swap); E.h_swap = nullptr; }
+  if (E.h_sres) { (void)hipHostFree(E.h_sres); E.h_sres = nullptr; }
   if (E.h_words) { (void)hipHostFree(E.h_words); E.h_words = nullptr; }
   if (E.h_out) { (void)hipHostFree(E.h_out); E.h_out = nullptr; }
   if (E.h_scal) { (void)hipHostFree(E.h_scal); E.h_scal = nullptr; }
@@ -1068,6 +1274,8 @@ static int init_cell(int problem, int n, const int *ax, int R,
     return -3;
   }
   CK(hipHostMalloc((void **)&E.h_words, (size_t)CODECAP * 8, hipHostMallocDefault));
+  CK(hipHostMalloc((void **)&E.h_swap, SWAP_IN_BYTES, hipHostMallocDefault));
+  CK(hipHostMalloc((void **)&E.h_sres, SWAP_RES_BYTES, hipHostMallocDefault));
   CK(hipHostMalloc((void **)&E.h_out, (size_t)std::max(CODECAP, 4096) * 4,
                    hipHostMallocDefault));
   CK(hipHostMalloc((void **)&E.h_scal, 64, hipHostMallocDefault));
@@ -1076,6 +1284,40 @@ static int init_cell(int problem, int n, const int *ax, int R,
   E.prof = pe ? atoi(pe) : 0;
   if (int rc = zero_scalars(8)) return rc;
   E.trans_last = 0;
+  {
+    // set-side gathers and the swap fast path (exact only on a proven ball)
+    const char *sp = getenv("CC_SET_POLICY");
+    E.set_policy = sp ? atoi(sp) : 1;
+    const char *sr = getenv("CC_SET_RATIO");
+    E.set_ratio = sr ? std::max(1, atoi(sr)) : 2;
+    const char *sm = getenv("CC_SET_MIN");
+    E.set_min = sm ? atoll(sm) : -1;
+    int amax = 2;
+    for (int i = 0; i < n; i++) amax = std::max(amax, ax[i]);
+    E.dbits = 1;
+    while ((1 << E.dbits) < amax) E.dbits++;
+    E.set_ok1 = E.set_policy != 0 && n * E.dbits <= 64 &&
+                table_is_ball(ballpat, balllen, tfmt);
+    E.set_ok2 = 0;
+    E.cnt_ver = 1;
+    E.plus_words = 0;
+    E.z_mode = E.o_mode = -1;
+    if (E.set_ok1) {
+      E.zcap = std::min(X, std::min(1LL << 27,
+                                    std::max(4096LL, balllen * E.set_ratio)));
+      E.lcap = std::min(1LL << 22, std::max(4096LL, 64 * balllen));
+      if (dalloc(&E.zbuf, (size_t)E.zcap * 8) ||
+          dalloc(&E.lbuf, (size_t)E.lcap * 8) ||
+          dalloc(&E.ldrop, (size_t)E.lcap * 4)) {
+        release_all();
+        return -3;
+      }
+    }
+    if (dalloc(&E.d_swap, SWAP_IN_BYTES) || dalloc(&E.d_sres, SWAP_RES_BYTES)) {
+      release_all();
+      return -3;
+    }
+  }
   {
     // exact ball size: coefficients 0..R of prod_j (per-axis enumerator)
     std::vector<double> poly(R + 1, 0.0);
@@ -1244,6 +1486,8 @@ int dct_transform(int init_mode, int out_mode) {
   if (out_mode == 2) {
     E.active = E.uploaded;
     E.active_size = E.codeM;
+    E.cnt_ver++;
+    E.o_mode = -1;
   }
   return 0;
 }
@@ -1312,6 +1556,72 @@ int dct_swap_eval(const long long *drops, int k, const long long *cand,
     if (E.active.find(drops[i]) == E.active.end()) return -1;
   const int lmode = E.mu == 1 ? CC_EQ : CC_CRIT, lt = E.mu == 1 ? 1 : 0;
   const int gmode = E.mu == 1 ? CC_EQ : CC_DEF;
+  if (set_ok() && E.lbuf) {
+    // fast path: loss and drop lists by one walk per drop, the candidates'
+    // gains on the current cnt, then the per-drop corrections; cnt is never
+    // modified.  One upload, one cleared result region, one download.
+    const size_t in_drop = (size_t)nc * 8, in_cdrop = (size_t)(nc + k) * 8;
+    const size_t in_bytes = in_cdrop + (size_t)nc * 4;
+    const size_t r_loss = (size_t)nc * 4;
+    const size_t r_cnt = ((size_t)(nc + k) * 4 + 7) & ~(size_t)7;
+    const size_t r_bytes = r_cnt + 8;
+    if (nc) memcpy(E.h_swap, cand, (size_t)nc * 8);
+    memcpy(E.h_swap + in_drop, drops, (size_t)k * 8);
+    int32_t *hc = (int32_t *)(E.h_swap + in_cdrop);
+    for (int i = 0; i < k; i++)
+      for (long long j = off[i]; j < off[i + 1]; j++) hc[j] = i;
+    const long long *dcand = (const long long *)E.d_swap;
+    const long long *ddrop = (const long long *)(E.d_swap + in_drop);
+    const int32_t *dcdrop = (const int32_t *)(E.d_swap + in_cdrop);
+    int32_t *dgain = (int32_t *)E.d_sres;
+    int32_t *dloss = (int32_t *)(E.d_sres + r_loss);
+    unsigned long long *dlcnt = (unsigned long long *)(E.d_sres + r_cnt);
+    CK(hipMemcpyAsync(E.d_swap, E.h_swap, in_bytes, hipMemcpyHostToDevice, E.st));
+    hipLaunchKernelGGL(k_fill32, dim3((unsigned)((r_bytes / 4 + 255) / 256)),
+                       dim3(256), 0, E.st, (int32_t *)E.d_sres, 0,
+                       (long long)(r_bytes / 4));
+    CK(hipGetLastError());
+    const Geom g = make_geom();
+    const Dig dg = make_dig();
+    const int split = pick_split(k);
+    const dim3 grid((unsigned)k * split);
+    const bool wide = E.fmt == 0 && E.tfmt == CC_FMT_WIDE;
+#define CC_DROP(GR, FM)                                                       \
+    hipLaunchKernelGGL((k_drop_walk<GR, FM>), grid, dim3(CC_BLOCK), 0, E.st, g, \
+                       dg, ddrop, split, E.cnt, walk_table(), E.blen, dloss,    \
+                       E.lbuf, E.ldrop, E.lcap, dlcnt)
+    if (E.prob == CC_GRID) CC_DROP(true, CC_FMT_V1);
+    else if (wide) CC_DROP(false, CC_FMT_WIDE);
+    else CC_DROP(false, CC_FMT_V1);
+#undef CC_DROP
+    CK(hipGetLastError());
+    if (nc) {
+      if (int rc = gather_launch(gmode, 0, (int)nc, dcand, dgain, true))
+        return rc;
+      const int cs = (int)std::max(1LL, std::min(64LL, 8192 / nc));
+      const dim3 g2((unsigned)(nc * cs));
+#define CC_CORR(P)                                                             \
+      hipLaunchKernelGGL(k_swap_corr<P>, g2, dim3(CC_BLOCK), 0, E.st, dcand,    \
+                         dcdrop, cs, E.lbuf, E.ldrop, E.lcap, dlcnt, dg, dgain)
+      switch (E.prob) {
+        case CC_HAMMING: CC_CORR(CC_HAMMING); break;
+        case CC_LEE: CC_CORR(CC_LEE); break;
+        case CC_TORUS: CC_CORR(CC_TORUS); break;
+        default: CC_CORR(CC_GRID); break;
+      }
+#undef CC_CORR
+      CK(hipGetLastError());
+    }
+    CK(hipMemcpyAsync(E.h_sres, E.d_sres, r_bytes, hipMemcpyDeviceToHost, E.st));
+    CK(hipStreamSynchronize(E.st));
+    const unsigned long long lfound = *(const unsigned long long *)(E.h_sres + r_cnt);
+    if ((long long)lfound <= E.lcap) {
+      memcpy(loss, E.h_sres + r_loss, (size_t)k * 4);
+      if (nc) memcpy(gain, E.h_sres, (size_t)nc * 4);
+      E.stats[4]++;
+      return 0;
+    }
+  }
   for (int i = 0; i < k; i++) {
     if (int rc = gather_impl(drops + i, 1, lmode, lt, loss + i)) return rc;
     const long long ni = off[i + 1] - off[i];
@@ -1344,6 +1654,8 @@ int dct_loss_owner(const long long *words, int M, int32_t *out) {
   hipLaunchKernelGGL(k_fill16, dim3(pass_blocks(E.cnt_len)), dim3(CC_BLOCK), 0,
                      E.st, E.cnt, (uint16_t)0, E.cnt_len);
   CK(hipGetLastError());
+  E.cnt_ver++;
+  E.o_mode = -1;
   if (int rc = update_inner<CC_WALK_UPDATE>(words, M, +1)) return rc;
   E.active.clear();
   for (int i = 0; i < M; i++) ++E.active[words[i]];
@@ -1466,6 +1778,7 @@ int dct_set_tbl2(const uint16_t *mask, const uint64_t *offs, long long len) {
     v1[(size_t)t] = p;
   }
   if (!table_ok(v1.data(), len, CC_FMT_V1)) return -1;
+  E.set_ok2 = E.set_ok1 && table_is_ball(v1.data(), len, CC_FMT_V1);
   if (!E.pat2 && dalloc(&E.pat2, (size_t)len * 8)) return -3;
   CK(hipMemcpy(E.pat2, v1.data(), (size_t)len * 8, hipMemcpyHostToDevice));
   return 0;

@@ -94,7 +94,12 @@ int  dct_ball_update_delta(const long long *words, int nwords, int delta,
 /* reference plugin.h:112-114, dct_hc.cu:882-977 */
 int  dct_swap_eval(const long long *drops, int k, const long long *cand,
                    const long long *off, int32_t *gain, int32_t *loss);
-/* reference plugin.h:115, dct_hc.cu:779-783 */
+/* reference plugin.h:115, dct_hc.cu:779-783.  out[0..7]: set-side gathers,
+ * walk gathers, predicate listings, listing overflows, fast swap_eval
+ * calls, fallback swap_eval calls, (unused), set-side gathers over more
+ * than 32,768 listed words.  The set side is taken only when the walked
+ * table is proven to be exactly the metric ball; CC_SET_POLICY (0 never,
+ * 1 auto, 2 always), CC_SET_RATIO and CC_SET_MIN are read at init. */
 int  dct_stats(long long *out, int n);
 
 /* ---- reductions / map queries ------------------------------------------- */
