@@ -12,7 +12,8 @@ the time is the real per-call wall time through the C ABI.
 Also reported: loss and recount passes, 10^5 single-word updates
 (alternating +1 / -1, seed 1), 10^6 exact gain queries (seed 2, chunks of
 65,536), the per-sweep schedule with its algorithmic bytes, the roofline of
-the dominant sweep kernel (HIP events on the engine's stream), and the CPU
+the dominant sweep kernel (HIP start / stop events stamped by the kernel
+launch on the engine's stream), and the CPU
 baseline: the numpy axis-DP oracle on one core over a bounded sample.
 """
 import argparse
@@ -156,11 +157,13 @@ def batch_cells64():
     return cells
 
 
-def run_batch64(GpuBackend, rounds=200, k=2):
+def run_batch64(GpuBackend, rounds=200, k=2, quick=False):
     """64 uneven cells in one batch: greedy sizes, seeded random starts at
     1.5 x greedy (seed = cell index), the batched transform (one persistent
-    launch) against the same cells' singleton transforms run one after the
-    other, and LNS rounds/s with and without the captured HIP graph."""
+    launch per sweep level) against the same cells' singleton transforms run
+    one after the other, and LNS rounds/s with and without the captured HIP
+    graph.  quick: no greedy (starts of |X|/64 random words), 20 rounds and
+    no singleton comparison -- a short run for the profiler."""
     from coldcore_amd.batch import GpuBatch
     cells = batch_cells64()
     out = {"cells": len(cells),
@@ -171,15 +174,19 @@ def run_batch64(GpuBackend, rounds=200, k=2):
         for i in range(len(cells)):
             b.set_code(i, empty)
         b.recount()
-        t = time.perf_counter()
-        out["greedy_steps"] = b.greedy_cover(256)
-        out["greedy_s"] = round(time.perf_counter() - t, 3)
-        greedy = [b.stats(i)["M"] for i in range(len(cells))]
-        out["greedy_words_total"] = int(sum(greedy))
+        if quick:
+            rounds = 20
+            greedy = [max(2, int(np.prod(a)) // 96) for _, a, _ in cells]
+        else:
+            t = time.perf_counter()
+            out["greedy_steps"] = b.greedy_cover(256)
+            out["greedy_s"] = round(time.perf_counter() - t, 3)
+            greedy = [b.stats(i)["M"] for i in range(len(cells))]
+            out["greedy_words_total"] = int(sum(greedy))
         starts = []
         for i, (_, a, _) in enumerate(cells):
             size = int(np.prod(a))
-            m = min(size, int(round(1.5 * greedy[i])))
+            m = min(size, 65535, int(round(1.5 * greedy[i])))
             starts.append(np.random.default_rng(i).choice(size, m, replace=False))
 
         def reset():
@@ -213,6 +220,8 @@ def run_batch64(GpuBackend, rounds=200, k=2):
         out["cells_covered_at_end"] = int(sum(s["uncovered"] == 0 for s in graph))
         out["best_words_total_covered"] = int(sum(
             s["best"] for s in graph if s["best"] > 0))
+    if quick:
+        return out
     # the same cells one at a time through the singleton ABI (the cells
     # whose axes its v1 tables address: axes <= 16; the tori with k > 16
     # are left out, which only understates the batch's advantage)
@@ -375,13 +384,16 @@ def main():
                     help="only run a device-to-device copy of MIB MiB "
                          "(3 times): the reference kernel for scaling the "
                          "FETCH_SIZE / WRITE_SIZE counters")
-    ap.add_argument("--only", choices=("near_cover", "batch64"),
+    ap.add_argument("--only", choices=("near_cover", "batch64", "batch64-quick"),
                     help="run one side section and print its JSON only")
     args = ap.parse_args()
     if args.only:
         from coldcore_amd.gpu import GpuBackend
-        fn = run_near_cover if args.only == "near_cover" else run_batch64
-        print(json.dumps({args.only: fn(GpuBackend)}), flush=True)
+        if args.only == "near_cover":
+            r = run_near_cover(GpuBackend)
+        else:
+            r = run_batch64(GpuBackend, quick=args.only.endswith("quick"))
+        print(json.dumps({args.only: r}), flush=True)
         return
     if args.calib_copy:
         copy_calibration(args.calib_copy)

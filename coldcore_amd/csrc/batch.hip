@@ -50,6 +50,9 @@ struct Batch {
   int per_cu = 1, ncu = 1;
   long long maxX = 0;
   BItem *d_ruin = nullptr, *d_add = nullptr, *d_rev = nullptr;
+  std::vector<long long> chunks;      // reduction chunks: cell << 40 | word
+  long long *d_chunks = nullptr;
+  int snap_grid = 1;                  // snapshot blocks per cell
   hipStream_t st = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -104,14 +107,22 @@ void plan_cell(int ci, const BCell &c) {
       P = P2;
       k++;
     }
+    // strided groups: widen the tile to fill the budget (a group of one
+    // short axis would otherwise make thousands of 96-word tiles)
+    int TLw = TL;
+    if (j > 0) {
+      long long tl = std::min<long long>(s, CCB_LDS / (4LL * c.L * P));
+      if (tl >= 64) tl = tl / 32 * 32;
+      TLw = (int)std::max<long long>(TL, tl);
+    }
     SweepArgs a;
     memset(&a, 0, sizeof(a));
     for (int d = 0; d < c.L; d++) a.A[d] = c.A[d];
     a.cnt = c.cnt;
     a.s = s;
     a.P = (int)P;
-    a.TL = TL;
-    a.ncolb = (s + TL - 1) / TL;
+    a.TL = TLw;
+    a.ncolb = (s + TLw - 1) / TLw;
     a.ntiles = (c.X / (s * P)) * a.ncolb;
     a.nax = k - j;
     a.c0 = j;
@@ -135,7 +146,7 @@ void plan_cell(int ci, const BCell &c) {
     }
     B.argcell.push_back(ci);
     B.need.push_back((int)a.ntiles);
-    B.lds = std::max(B.lds, (size_t)(c.L * P * TL * 4));
+    B.lds = std::max(B.lds, (size_t)(c.L * P * TLw * 4));
     j = k;
   }
 }
@@ -170,9 +181,15 @@ int reduce_launch(int with_map) {
   hipLaunchKernelGGL(k_ccb_reset_red, dim3((B.nc + 63) / 64), dim3(64), 0, B.st,
                      B.d, B.nc);
   CKB(hipGetLastError());
-  const int gx = (int)std::max<long long>(1, std::min<long long>((B.maxX + 2047) / 2048, 512));
-  hipLaunchKernelGGL(k_ccb_reduce, dim3(gx, B.nc), dim3(CC_BLOCK), 0, B.st, B.d,
-                     with_map);
+  hipLaunchKernelGGL(k_ccb_reduce, dim3((unsigned)B.chunks.size()), dim3(CC_BLOCK),
+                     0, B.st, B.d, B.d_chunks, with_map);
+  CKB(hipGetLastError());
+  return 0;
+}
+
+int snapshot_launch() {
+  hipLaunchKernelGGL(k_ccb_snapshot, dim3(B.snap_grid, B.nc), dim3(CC_BLOCK), 0,
+                     B.st, B.d);
   CKB(hipGetLastError());
   return 0;
 }
@@ -197,6 +214,7 @@ int round_launch(int k) {
   if (int rc = reduce_launch(0)) return rc;
   hipLaunchKernelGGL(k_ccb_decide, cg, cb, 0, B.st, B.d, B.nc, B.d_rev);
   CKB(hipGetLastError());
+  if (int rc = snapshot_launch()) return rc;
   hipLaunchKernelGGL(k_ccb_walk, dim3(B.nc * 2 * CCB_KMAX), dim3(CC_BLOCK), 0,
                      B.st, B.d, B.d_rev);
   CKB(hipGetLastError());
@@ -301,9 +319,14 @@ int ccb_init(int ncells, const int *problem, const int *n, const int *axes,
   B.nlevels = (int)B.level_off.size();
   B.nitems = (int)B.items.size();
   B.level_off.push_back(B.nitems);
+  for (int i = 0; i < ncells; i++)
+    for (long long w = 0; w < B.h[i].X; w += CCB_CHUNK)
+      B.chunks.push_back(((long long)i << 40) | w);
+  B.snap_grid = (cap + 4 * CC_BLOCK - 1) / (4 * CC_BLOCK);
   if (balloc(&B.d, sizeof(BCell) * ncells) ||
       balloc(&B.d_argcell, 4 * B.nargs) || balloc(&B.d_items, 8 * B.nitems) ||
       balloc(&B.d_counters, 4 * B.nlevels) ||
+      balloc(&B.d_chunks, 8 * B.chunks.size()) ||
       balloc(&B.d_ruin, sizeof(BItem) * ncells * CCB_KMAX) ||
       balloc(&B.d_add, sizeof(BItem) * ncells) ||
       balloc(&B.d_rev, sizeof(BItem) * ncells * 2 * CCB_KMAX))
@@ -315,7 +338,9 @@ int ccb_init(int ncells, const int *problem, const int *n, const int *axes,
       { bfree_all(); return -3; }
   if (hipMemcpy(B.d, B.h.data(), sizeof(BCell) * ncells, hipMemcpyHostToDevice) ||
       hipMemcpy(B.d_argcell, B.argcell.data(), 4 * B.nargs, hipMemcpyHostToDevice) ||
-      hipMemcpy(B.d_items, B.items.data(), 8 * B.nitems, hipMemcpyHostToDevice))
+      hipMemcpy(B.d_items, B.items.data(), 8 * B.nitems, hipMemcpyHostToDevice) ||
+      hipMemcpy(B.d_chunks, B.chunks.data(), 8 * B.chunks.size(),
+                hipMemcpyHostToDevice))
     { bfree_all(); return -3; }
   // persistent grids: what the CUs hold at once
   int per_cu = 1, dev = 0, ncu = 256;
@@ -388,6 +413,7 @@ int ccb_lns_init(unsigned long long seed) {
   hipLaunchKernelGGL(k_ccb_lns_init, dim3((B.nc + 63) / 64), dim3(64), 0, B.st,
                      B.d, B.nc, seed);
   CKB(hipGetLastError());
+  if (int rc = snapshot_launch()) return rc;
   CKB(hipStreamSynchronize(B.st));
   return 0;
 }

@@ -31,6 +31,7 @@ struct BCell {
   int cap;
   // LNS state (control kernels only)
   int M, uncov, bestM, add_budget, pre_uncov, nrem, nadd;
+  int snap;              // the current code is the new best: copy it
   long long rem[CCB_KMAX], add[CCB_KMAX];
   unsigned long long rng;
   long long rounds, accepted, reverted;
@@ -265,13 +266,20 @@ __global__ void k_ccb_reset_red(BCell *cells, int ncells) {
   if (i < ncells) { cells[i].amax = 0ull; cells[i].nzero = 0ull; }
 }
 
-// per cell (blockIdx.y): number of uncovered words; with MAP also the
-// largest map value and its smallest index
-__global__ void __launch_bounds__(CC_BLOCK) k_ccb_reduce(BCell *cells, int with_map) {
-  BCell &c = cells[blockIdx.y];
-  const long long gs = (long long)gridDim.x * CC_BLOCK;
+// one block per chunk (cell, first word; chunks of CCB_CHUNK words): number
+// of uncovered words; with MAP also the largest map value and its smallest
+// index.  The chunk table sizes the grid by the words, not by the largest
+// cell times the cell count.
+#define CCB_CHUNK 16384
+__global__ void __launch_bounds__(CC_BLOCK)
+    k_ccb_reduce(BCell *cells, const long long *__restrict__ chunks, int with_map) {
+  const long long ch = chunks[blockIdx.x];
+  const int ci = (int)(ch >> 40);
+  const long long w0 = ch & ((1LL << 40) - 1);
+  BCell &c = cells[ci];
+  const long long w1 = min(c.X, w0 + CCB_CHUNK);
   unsigned long long z = 0, best = 0;
-  for (long long i = (long long)blockIdx.x * CC_BLOCK + threadIdx.x; i < c.X; i += gs) {
+  for (long long i = w0 + threadIdx.x; i < w1; i += CC_BLOCK) {
     z += c.cnt[i] == 0;
     if (with_map) {
       const int32_t g = c.A[0][i];
@@ -289,6 +297,16 @@ __global__ void __launch_bounds__(CC_BLOCK) k_ccb_reduce(BCell *cells, int with_
   }
   if ((threadIdx.x & 63) == 0 && best) atomicMax(&c.amax, best);
   if (threadIdx.x == 0 && zs) atomicAdd(&c.nzero, (unsigned long long)zs);
+}
+
+// best := code for the cells flagged by the control kernels (blockIdx.y =
+// cell), all threads copying
+__global__ void k_ccb_snapshot(BCell *cells) {
+  BCell &c = cells[blockIdx.y];
+  if (!c.snap) return;
+  const int gs = gridDim.x * blockDim.x;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < c.M; i += gs)
+    c.best[i] = c.code[i];
 }
 
 // ---------------------------------------------------------------- LNS control
@@ -353,13 +371,15 @@ __global__ void k_ccb_add(BCell *cells, int ncells, int j, BItem *items) {
 }
 
 // accept / revert: a round that uncovers more words is undone with
-// probability 0.9 (added words out, removed words back)
+// probability 0.9 (added words out, removed words back); a new best
+// covering code is flagged for k_ccb_snapshot
 __global__ void k_ccb_decide(BCell *cells, int ncells, BItem *items) {
   const int ci = blockIdx.x * blockDim.x + threadIdx.x;
   if (ci >= ncells) return;
   BCell &c = cells[ci];
   const int now = (int)c.nzero;
   bool revert = false;
+  int snap = 0;
   if (c.nrem > 0 && now > c.pre_uncov) {
     const double u = (double)(ccb_next(c) >> 11) * (1.0 / 9007199254740992.0);
     revert = u > 0.10;
@@ -380,9 +400,10 @@ __global__ void k_ccb_decide(BCell *cells, int ncells, BItem *items) {
     if (c.nrem > 0) c.accepted++;
     if (now == 0 && c.M < c.bestM) {
       c.bestM = c.M;
-      for (int i = 0; i < c.M; i++) c.best[i] = c.code[i];
+      snap = 1;
     }
   }
+  c.snap = snap;
   c.rounds++;
 }
 
@@ -395,8 +416,9 @@ __global__ void k_ccb_lns_init(BCell *cells, int ncells, unsigned long long seed
   c.rng = ccb_mix(seed * 0x100000001B3ull + (unsigned long long)ci);
   c.rounds = c.accepted = c.reverted = 0;
   c.bestM = 0x7FFFFFFF;
+  c.snap = 0;
   if (c.uncov == 0) {
     c.bestM = c.M;
-    for (int i = 0; i < c.M; i++) c.best[i] = c.code[i];
+    c.snap = 1;
   }
 }

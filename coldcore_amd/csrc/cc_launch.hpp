@@ -46,13 +46,18 @@ hipError_t cc_sweep_round(const SweepArgs &a, int L, int q, int blocks,
                           size_t lds, bool l16, const LaunchCtx &lc);
 
 // packed-layer Hamming sweeps (cc_psweep.hpp), one unit per axis size Q:
-// RR axes per register round, W-bit fields in ND dwords per word
+// RR axes per register round, W-bit fields in ND dwords per word; with
+// events e0 / e1 the launch records the kernel's own start and end
 struct PSweepArgs;
 hipError_t cc_psweep_q2(const PSweepArgs &a, int RR, int W, int ND, int blocks,
-                        int threads, size_t lds, hipStream_t st);
+                        int threads, size_t lds, hipStream_t st,
+                        hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 hipError_t cc_psweep_q3(const PSweepArgs &a, int RR, int W, int ND, int blocks,
-                        int threads, size_t lds, hipStream_t st);
+                        int threads, size_t lds, hipStream_t st,
+                        hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 hipError_t cc_psweep_q4(const PSweepArgs &a, int RR, int W, int ND, int blocks,
-                        int threads, size_t lds, hipStream_t st);
+                        int threads, size_t lds, hipStream_t st,
+                        hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 hipError_t cc_psweep_q5(const PSweepArgs &a, int RR, int W, int ND, int blocks,
-                        int threads, size_t lds, hipStream_t st);
+                        int threads, size_t lds, hipStream_t st,
+                        hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);

@@ -3,6 +3,7 @@
 // words) and register format (W-bit fields in ND dwords) whose cube stays
 // within 128 registers (C * ND <= 128).
 #pragma once
+#include <hip/hip_ext.h>
 #include "cc_launch.hpp"
 #include "cc_psweep.hpp"
 
@@ -40,9 +41,21 @@ void (*pk_select(bool sum, bool in2, bool out2))(PSweepArgs) {
   }
 }
 
+// launch kf; with events, the runtime stamps the kernel's own start and end
+// (the kernel duration, not the gaps around it)
+inline void pk_run(void (*kf)(PSweepArgs), int grid, int threads, size_t lds,
+                   hipStream_t st, hipEvent_t e0, hipEvent_t e1,
+                   const PSweepArgs &a) {
+  if (e0 || e1)
+    hipExtLaunchKernelGGL(kf, dim3(grid), dim3(threads), (uint32_t)lds, st, e0, e1,
+                          0u, a);
+  else
+    hipLaunchKernelGGL(kf, dim3(grid), dim3(threads), lds, st, a);
+}
+
 template <int Q, int RR, int W, int ND>
 hipError_t pk_go(const PSweepArgs &a, int blocks, int threads, size_t lds,
-                 hipStream_t st) {
+                 hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
   const bool sum = a.out_kind != CC_OUT_PACKED;
   const bool in2 = ND > 1 && a.in_kind == CC_IN_PACKED && a.in_nd > 1;
   const bool out2 = ND > 1 && !sum && a.out_nd > 1;
@@ -57,7 +70,7 @@ hipError_t pk_go(const PSweepArgs &a, int blocks, int threads, size_t lds,
     }
     if (!kf) return hipErrorInvalidValue;
     if (blocks == 0) return hipSuccess;
-    hipLaunchKernelGGL(kf, dim3(blocks), dim3(CC_BLOCK), 0, st, a);
+    pk_run(kf, blocks, CC_BLOCK, 0, st, e0, e1, a);
     return hipGetLastError();
   }
   if (a.kern == 4) {
@@ -87,7 +100,7 @@ hipError_t pk_go(const PSweepArgs &a, int blocks, int threads, size_t lds,
       ncu = 256;
     const long long resident = (long long)per_cu * ncu;
     const int g = (int)(blocks < resident ? blocks : resident);
-    hipLaunchKernelGGL(kf, dim3(g), dim3(threads), lds, st, a);
+    pk_run(kf, g, threads, lds, st, e0, e1, a);
     return hipGetLastError();
   }
   if (a.kern == 3) {
@@ -100,7 +113,7 @@ hipError_t pk_go(const PSweepArgs &a, int blocks, int threads, size_t lds,
     }
     if (!kf) return hipErrorInvalidValue;
     if (blocks == 0) return hipSuccess;
-    hipLaunchKernelGGL(kf, dim3(blocks), dim3(CC_BLOCK), 0, st, a);
+    pk_run(kf, blocks, CC_BLOCK, 0, st, e0, e1, a);
     return hipGetLastError();
   }
   if constexpr (pk_fits(Q, RR, ND)) {
@@ -114,7 +127,7 @@ hipError_t pk_go(const PSweepArgs &a, int blocks, int threads, size_t lds,
                                          (int)lds);
       if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(kf, dim3(blocks), dim3(threads), lds, st, a);
+    pk_run(kf, blocks, threads, lds, st, e0, e1, a);
     return hipGetLastError();
   } else {
     return hipErrorInvalidValue;
@@ -123,26 +136,28 @@ hipError_t pk_go(const PSweepArgs &a, int blocks, int threads, size_t lds,
 
 template <int Q, int RR>
 hipError_t pk_by_fmt(const PSweepArgs &a, int W, int ND, int blocks,
-                     int threads, size_t lds, hipStream_t st) {
-  if (W == 8 && ND == 1) return pk_go<Q, RR, 8, 1>(a, blocks, threads, lds, st);
-  if (W == 8 && ND == 2) return pk_go<Q, RR, 8, 2>(a, blocks, threads, lds, st);
-  if (W == 16 && ND == 1) return pk_go<Q, RR, 16, 1>(a, blocks, threads, lds, st);
-  if (W == 16 && ND == 2) return pk_go<Q, RR, 16, 2>(a, blocks, threads, lds, st);
-  if (W == 16 && ND == 3) return pk_go<Q, RR, 16, 3>(a, blocks, threads, lds, st);
-  if (W == 16 && ND == 4) return pk_go<Q, RR, 16, 4>(a, blocks, threads, lds, st);
+                     int threads, size_t lds, hipStream_t st, hipEvent_t e0,
+                     hipEvent_t e1) {
+  if (W == 8 && ND == 1) return pk_go<Q, RR, 8, 1>(a, blocks, threads, lds, st, e0, e1);
+  if (W == 8 && ND == 2) return pk_go<Q, RR, 8, 2>(a, blocks, threads, lds, st, e0, e1);
+  if (W == 16 && ND == 1) return pk_go<Q, RR, 16, 1>(a, blocks, threads, lds, st, e0, e1);
+  if (W == 16 && ND == 2) return pk_go<Q, RR, 16, 2>(a, blocks, threads, lds, st, e0, e1);
+  if (W == 16 && ND == 3) return pk_go<Q, RR, 16, 3>(a, blocks, threads, lds, st, e0, e1);
+  if (W == 16 && ND == 4) return pk_go<Q, RR, 16, 4>(a, blocks, threads, lds, st, e0, e1);
   return hipErrorInvalidValue;
 }
 
 template <int Q, int MAXRR>
 hipError_t pk_by_rr(const PSweepArgs &a, int RR, int W, int ND, int blocks,
-                    int threads, size_t lds, hipStream_t st) {
+                    int threads, size_t lds, hipStream_t st, hipEvent_t e0,
+                    hipEvent_t e1) {
   switch (RR) {
-    case 1: return pk_by_fmt<Q, 1>(a, W, ND, blocks, threads, lds, st);
-    case 2: return pk_by_fmt<Q, 2>(a, W, ND, blocks, threads, lds, st);
-    case 3: if constexpr (MAXRR >= 3) return pk_by_fmt<Q, 3>(a, W, ND, blocks, threads, lds, st); break;
-    case 4: if constexpr (MAXRR >= 4) return pk_by_fmt<Q, 4>(a, W, ND, blocks, threads, lds, st); break;
-    case 5: if constexpr (MAXRR >= 5) return pk_by_fmt<Q, 5>(a, W, ND, blocks, threads, lds, st); break;
-    case 6: if constexpr (MAXRR >= 6) return pk_by_fmt<Q, 6>(a, W, ND, blocks, threads, lds, st); break;
+    case 1: return pk_by_fmt<Q, 1>(a, W, ND, blocks, threads, lds, st, e0, e1);
+    case 2: return pk_by_fmt<Q, 2>(a, W, ND, blocks, threads, lds, st, e0, e1);
+    case 3: if constexpr (MAXRR >= 3) return pk_by_fmt<Q, 3>(a, W, ND, blocks, threads, lds, st, e0, e1); break;
+    case 4: if constexpr (MAXRR >= 4) return pk_by_fmt<Q, 4>(a, W, ND, blocks, threads, lds, st, e0, e1); break;
+    case 5: if constexpr (MAXRR >= 5) return pk_by_fmt<Q, 5>(a, W, ND, blocks, threads, lds, st, e0, e1); break;
+    case 6: if constexpr (MAXRR >= 6) return pk_by_fmt<Q, 6>(a, W, ND, blocks, threads, lds, st, e0, e1); break;
   }
   return hipErrorInvalidValue;
 }

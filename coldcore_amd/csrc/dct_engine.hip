@@ -576,12 +576,13 @@ void pk_rounds(PSweep &w, int q) {
 
 // Launch a packed sweep kernel, or with blocks == 0 only check that its
 // instantiation exists.
-hipError_t pk_dispatch(const PSweep &w, const PSweepArgs &a, int blocks) {
+hipError_t pk_dispatch(const PSweep &w, const PSweepArgs &a, int blocks,
+                       hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
   switch (E.ax[0]) {
-    case 2: return cc_psweep_q2(a, w.RR, w.W, w.ND, blocks, w.threads, w.lds, E.st);
-    case 3: return cc_psweep_q3(a, w.RR, w.W, w.ND, blocks, w.threads, w.lds, E.st);
-    case 4: return cc_psweep_q4(a, w.RR, w.W, w.ND, blocks, w.threads, w.lds, E.st);
-    case 5: return cc_psweep_q5(a, w.RR, w.W, w.ND, blocks, w.threads, w.lds, E.st);
+    case 2: return cc_psweep_q2(a, w.RR, w.W, w.ND, blocks, w.threads, w.lds, E.st, e0, e1);
+    case 3: return cc_psweep_q3(a, w.RR, w.W, w.ND, blocks, w.threads, w.lds, E.st, e0, e1);
+    case 4: return cc_psweep_q4(a, w.RR, w.W, w.ND, blocks, w.threads, w.lds, E.st, e0, e1);
+    case 5: return cc_psweep_q5(a, w.RR, w.W, w.ND, blocks, w.threads, w.lds, E.st, e0, e1);
   }
   return hipErrorInvalidValue;
 }
@@ -789,12 +790,13 @@ PPlan pk_plan(double s0, double cap, bool pred_seed, bool cnt_out) {
   return best;
 }
 
-hipError_t pk_launch(const PSweep &w, const PSweepArgs &a) {
+hipError_t pk_launch(const PSweep &w, const PSweepArgs &a, hipEvent_t e0 = nullptr,
+                     hipEvent_t e1 = nullptr) {
   const long long units = w.kern == 3 ? (w.ntiles + CC_BLOCK / 64 - 1) / (CC_BLOCK / 64)
                          : (w.kern == 1 || w.kern == 2) ? w.ntiles / (CC_BLOCK / w.kern)
                                                         : w.ntiles;
   const int blocks = (int)std::max(1LL, std::min<long long>(units, 1LL << 30));
-  return pk_dispatch(w, a, blocks);
+  return pk_dispatch(w, a, blocks, e0, e1);
 }
 
 // Run a packed transform: in_kind CC_IN_PRED (predicate [plo <= cnt <= phi])
@@ -842,9 +844,9 @@ int run_ptransform(const PPlan &pl, int in_kind, int out_kind, int plo,
       a.idw[d] = w.idw[d]; a.ioff[d] = w.ioff[d]; a.ibits[d] = w.ibits[d];
       a.odw[d] = w.odw[d]; a.ooff[d] = w.ooff[d]; a.obits[d] = w.obits[d];
     }
-    if (E.prof) CK(hipEventRecord(E.ev[2 * i], E.st));
-    CK(pk_launch(w, a));
-    if (E.prof) CK(hipEventRecord(E.ev[2 * i + 1], E.st));
+    // timing: the launch itself stamps the kernel's start and end
+    if (E.prof) CK(pk_launch(w, a, E.ev[2 * i], E.ev[2 * i + 1]));
+    else CK(pk_launch(w, a));
   }
   E.nswept = ns;
   return 0;
