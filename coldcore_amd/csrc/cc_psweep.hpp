@@ -62,6 +62,10 @@ struct PSweepArgs {
   int plo, phi;          // CC_IN_PRED: [plo <= cnt <= phi]
   int in_nd, out_nd;     // compact planes read / written
   int in_w8, out_w8;     // compact format is one byte per layer
+  int in_nar, out_nar;   // narrow compact word: 1 = 16 bits (a uint16
+                         // plane), 2 = 24 bits (a uint16 and a uint8 plane)
+  long long nar_off;     // dwords from pk to the narrow planes (the uint8
+                         // plane starts 2 * pitch bytes after the uint16 one)
   uint32_t out_bmask;    // byte format: bytes of real layers
   int nrounds;
   int ca[CC_PK_MAXRD];     // lowest group axis of each round's cube
@@ -167,6 +171,18 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t pk_rsrc(const void *plane,
   __builtin_amdgcn_raw_buffer_store_b32((uint32_t)(val), (r), (int)(lane * 4u), (int)((c) * usq * 4u), 0)
 #define PK_ST16(r, c, val) \
   __builtin_amdgcn_raw_buffer_store_b16((unsigned short)(val), (r), (int)(lane * 2u), (int)((c) * usq * 2u), 0)
+#define PK_LD8(r, c) \
+  ((uint32_t)__builtin_amdgcn_raw_buffer_load_b8((r), (int)lane, (int)((c) * usq), 0))
+#define PK_ST8(r, c, val) \
+  __builtin_amdgcn_raw_buffer_store_b8((unsigned char)(val), (r), (int)lane, (int)((c) * usq), 0)
+
+// the narrow compact planes of a sweep
+__device__ __forceinline__ const uint16_t *pk_nar16(const PSweepArgs &a) {
+  return (const uint16_t *)(a.pk + a.nar_off);
+}
+__device__ __forceinline__ const uint8_t *pk_nar8(const PSweepArgs &a) {
+  return (const uint8_t *)(a.pk + a.nar_off) + 2 * a.pitch;
+}
 
 // byte-format dword (byte d = layer d, R <= 3) -> register format
 template <int W, int ND>
@@ -193,6 +209,49 @@ __device__ __forceinline__ uint32_t pk_to_bytes(const uint32_t (&r)[ND],
     return __builtin_amdgcn_perm(0u, r[0], 0x0C0C0200u) & bmask;
   } else {
     return __builtin_amdgcn_perm(r[1], r[0], 0x06020400u) & bmask;
+  }
+}
+
+// compact words (in v[c][0], and v[c][1] for a second plane) -> register
+// format, in place
+template <int C, int W, int ND, bool IN2>
+__device__ __forceinline__ void pk_unpack(const PSweepArgs &a, uint32_t (&v)[C][ND]) {
+  if (!IN2 && a.in_w8) {
+#pragma unroll
+    for (int c = 0; c < C; c++) {
+      uint32_t r[ND];
+      pk_from_bytes<W, ND>(v[c][0], r);
+#pragma unroll
+      for (int k = 0; k < ND; k++) v[c][k] = r[k];
+    }
+  } else {
+    // general compact fields: layer d is bits [off, off + bits) of
+    // dword plane idw[d]; layers >= L have width 0
+    constexpr int NS = ND * (32 / W);
+    int off[NS], bits[NS];
+    bool hi[NS];
+#pragma unroll
+    for (int d = 0; d < NS; d++) {
+      const bool on = d < a.L;
+      off[d] = on ? a.ioff[d] : 0;
+      bits[d] = on ? a.ibits[d] : 0;
+      hi[d] = on && a.idw[d] != 0;
+    }
+#pragma unroll
+    for (int c = 0; c < C; c++) {
+      const uint32_t x0 = v[c][0];
+      const uint32_t x1 = (ND > 1 && IN2) ? v[c][ND > 1 ? 1 : 0] : 0u;
+      uint32_t r[ND];
+#pragma unroll
+      for (int k = 0; k < ND; k++) r[k] = 0u;
+#pragma unroll
+      for (int d = 0; d < NS; d++) {
+        const uint32_t src = (ND > 1 && IN2) ? (hi[d] ? x1 : x0) : x0;
+        r[d % ND] |= __builtin_amdgcn_ubfe(src, off[d], bits[d]) << (W * (d / ND));
+      }
+#pragma unroll
+      for (int k = 0; k < ND; k++) v[c][k] = r[k];
+    }
   }
 }
 
@@ -251,6 +310,24 @@ __device__ __forceinline__ void pk_load(const PSweepArgs &a, long long ub,
         PK_FENCE(c);
       }
     }
+  } else if (a.in_nar) {
+    // narrow word: 16 bits from the uint16 plane, bits 16..23 from the
+    // uint8 plane; its fields are then extracted as from one dword
+    const __amdgpu_buffer_rsrc_t r16 = pk_rsrc(pk_nar16(a), ub, a.X, 2);
+#pragma unroll
+    for (int c = 0; c < C; c++) {
+      v[c][0] = PK_LD16(r16, c);
+      PK_FENCE(c);
+    }
+    if (a.in_nar == 2) {
+      const __amdgpu_buffer_rsrc_t r8 = pk_rsrc(pk_nar8(a), ub, a.X, 1);
+#pragma unroll
+      for (int c = 0; c < C; c++) {
+        v[c][0] |= PK_LD8(r8, c) << 16;
+        PK_FENCE(c);
+      }
+    }
+    pk_unpack<C, W, ND, false>(a, v);
   } else {
     const __amdgpu_buffer_rsrc_t r = pk_rsrc(a.pk, ub, a.X, 4);
 #pragma unroll
@@ -266,43 +343,7 @@ __device__ __forceinline__ void pk_load(const PSweepArgs &a, long long ub,
         PK_FENCE(c);
       }
     }
-    if (!IN2 && a.in_w8) {
-#pragma unroll
-      for (int c = 0; c < C; c++) {
-        uint32_t r[ND];
-        pk_from_bytes<W, ND>(v[c][0], r);
-#pragma unroll
-        for (int k = 0; k < ND; k++) v[c][k] = r[k];
-      }
-    } else {
-      // general compact fields: layer d is bits [off, off + bits) of
-      // dword plane idw[d]; layers >= L have width 0
-      constexpr int NS = ND * (32 / W);
-      int off[NS], bits[NS];
-      bool hi[NS];
-#pragma unroll
-      for (int d = 0; d < NS; d++) {
-        const bool on = d < a.L;
-        off[d] = on ? a.ioff[d] : 0;
-        bits[d] = on ? a.ibits[d] : 0;
-        hi[d] = on && a.idw[d] != 0;
-      }
-#pragma unroll
-      for (int c = 0; c < C; c++) {
-        const uint32_t x0 = v[c][0];
-        const uint32_t x1 = (ND > 1 && IN2) ? v[c][ND > 1 ? 1 : 0] : 0u;
-        uint32_t r[ND];
-#pragma unroll
-        for (int k = 0; k < ND; k++) r[k] = 0u;
-#pragma unroll
-        for (int d = 0; d < NS; d++) {
-          const uint32_t src = (ND > 1 && IN2) ? (hi[d] ? x1 : x0) : x0;
-          r[d % ND] |= __builtin_amdgcn_ubfe(src, off[d], bits[d]) << (W * (d / ND));
-        }
-#pragma unroll
-        for (int k = 0; k < ND; k++) v[c][k] = r[k];
-      }
-    }
+    pk_unpack<C, W, ND, IN2>(a, v);
   }
 }
 
@@ -359,6 +400,24 @@ __device__ __forceinline__ void pk_store(const PSweepArgs &a, long long ub,
                                          const uint32_t (&v)[C][ND]) {
   constexpr int NS = ND * (32 / W);
   if constexpr (!SUM) {
+    if (a.out_nar) {
+      // narrow word: low 16 bits to the uint16 plane, bits 16..23 to the
+      // uint8 plane
+      int sh[NS];
+#pragma unroll
+      for (int d = 0; d < NS; d++) sh[d] = d < a.L ? a.ooff[d] : 0;
+      const __amdgpu_buffer_rsrc_t r16 = pk_rsrc(pk_nar16(a), ub, a.X, 2);
+      const __amdgpu_buffer_rsrc_t r8 = pk_rsrc(pk_nar8(a), ub, a.X, 1);
+      const bool two = a.out_nar == 2;
+#pragma unroll
+      for (int c = 0; c < C; c++) {
+        const uint32_t o = (uint32_t)pk_compact<W, ND, false>(a, v[c], sh);
+        PK_ST16(r16, c, o);
+        if (two) PK_ST8(r8, c, o >> 16);
+        PK_FENCE(c);
+      }
+      return;
+    }
     const __amdgpu_buffer_rsrc_t r = pk_rsrc(a.pk, ub, a.X, 4);
     // layer d goes to bit ooff[d] of plane odw[d] (OUT2: two planes);
     // don't-care slots are masked to zero first, so every slot is packed

@@ -72,6 +72,7 @@ struct PSweep {
   int ca[CC_PK_MAXRD], amask[CC_PK_MAXRD], qa[CC_PK_MAXRD];
   int in_nd, out_nd;  // compact planes read / written (0: seed in, sum out)
   int in_w8, out_w8;  // compact format is one byte per layer
+  int in_nar, out_nar;  // narrow compact word (1: 16 bits, 2: 24 bits)
   int lin[CC_PK_MAXRD], kp[CC_PK_MAXRD];  // linear LDS addressing per round
   int idw[CC_MAXL], ioff[CC_MAXL], ibits[CC_MAXL];
   int odw[CC_MAXL], ooff[CC_MAXL], obits[CC_MAXL];
@@ -138,6 +139,9 @@ struct Engine {
   int pk_col = 2;            // CC_PK_COL: column kernel 0 off, 1 whole, 2 split
   int autotune = 1;          // CC_AUTOTUNE=0: never time schedules by itself
   int tuned = 0;             // the schedules of this cell were timed
+  int pk_narrow = 0;         // CC_PK_NARROW=1: 16 / 24-bit compact words (fewer
+                             // bytes, but the sub-dword accesses measured
+                             // slower on K_2(24,3): off by default)
   int pk_first = 1;          // CC_PK_FIRST: first sweep 0 k_psweep, 1 persistent
                              // with prefetch, 2 LDS-free (cross-lane)
   PPlan pfield;              // packed plan of gain / loss (predicate seeds)
@@ -500,8 +504,27 @@ int bitlen(double x) {
 // compact HBM format of the layers after c axes: field d takes bitlen of its
 // bound, packed greedily into at most two dword planes
 bool compact_fmt(const PBounds &B, int c, int L, int *dw, int *off, int *bits,
-                 int *nd, int *w8) {
+                 int *nd, int *w8, int *nar) {
   *w8 = 0;
+  *nar = 0;
+  if (L >= 4 && E.pk_narrow) {
+    // narrow word: every field packed into 16 or 24 bits, stored as a
+    // uint16 plane (and a uint8 plane) in the last layer plane of `lay`,
+    // which the dword formats (planes 0 and 1) never touch, so a sweep may
+    // switch between the two without its output overwriting its input
+    int tot = 0;
+    for (int d = 0; d < L; d++) tot += bitlen(B.lay(c, d));
+    if (tot <= 24) {
+      int used = 0;
+      for (int d = 0; d < L; d++) {
+        dw[d] = 0; off[d] = used; bits[d] = bitlen(B.lay(c, d));
+        used += bits[d];
+      }
+      *nd = 1;
+      *nar = tot <= 16 ? 1 : 2;
+      return true;
+    }
+  }
   if (L <= 4) {
     bool bytes = true;
     for (int d = 0; d < L; d++) bytes = bytes && B.lay(c, d) <= 255.0;
@@ -682,10 +705,11 @@ bool pk_sweep(const PBounds &B, int j, int k, bool pred_seed, bool cnt_out,
   if (prev) {
     w.in_nd = prev->out_nd;
     w.in_w8 = prev->out_w8;
+    w.in_nar = prev->out_nar;
     for (int d = 0; d < L; d++) {
       w.idw[d] = prev->odw[d]; w.ioff[d] = prev->ooff[d]; w.ibits[d] = prev->obits[d];
     }
-    w.in_bytes = 4 * w.in_nd;
+    w.in_bytes = prev->out_bytes;
   } else {
     w.in_nd = 0;
     w.in_bytes = pred_seed ? 2 : 4;
@@ -694,10 +718,11 @@ bool pk_sweep(const PBounds &B, int j, int k, bool pred_seed, bool cnt_out,
     w.out_nd = 0;
     w.out_bytes = cnt_out ? 2 : 4;
   } else {
-    if (!compact_fmt(B, k, L, w.odw, w.ooff, w.obits, &w.out_nd, &w.out_w8))
+    if (!compact_fmt(B, k, L, w.odw, w.ooff, w.obits, &w.out_nd, &w.out_w8,
+                     &w.out_nar))
       return false;
     if (w.out_nd > L - 1) return false;   // compact planes live in `lay`
-    w.out_bytes = 4 * w.out_nd;
+    w.out_bytes = w.out_nar ? 1 + w.out_nar : 4 * w.out_nd;
   }
   pk_rounds(w, q);
   {
@@ -769,8 +794,9 @@ PPlan pk_plan(double s0, double cap, bool pred_seed, bool cnt_out) {
     PSweep prev;
     memset(&prev, 0, sizeof(prev));
     if (j > 0 && !compact_fmt(B, j, E.L, prev.odw, prev.ooff, prev.obits,
-                              &prev.out_nd, &prev.out_w8))
+                              &prev.out_nd, &prev.out_w8, &prev.out_nar))
       continue;
+    prev.out_bytes = prev.out_nar ? 1 + prev.out_nar : 4 * prev.out_nd;
     for (int k = j + 1; k <= std::min(n, j + CC_MAXSW); k++) {
       PSweep w;
       const bool okw = pk_sweep(B, j, k, pred_seed, cnt_out, j ? &prev : nullptr, w);
@@ -831,6 +857,9 @@ int run_ptransform(const PPlan &pl, int in_kind, int out_kind, int plo,
     }
     a.in_w8 = w.in_w8;
     a.out_w8 = w.out_w8;
+    a.in_nar = w.in_nar;
+    a.out_nar = w.out_nar;
+    a.nar_off = (long long)(E.L - 2) * E.pitch;
     a.out_bmask = E.L >= 4 ? 0xFFFFFFFFu : (uint32_t)((1u << (8 * E.L)) - 1u);
     // real-layer slots: layer d = slot * ND + dword (interleaved layout)
     const int S = 32 / w.W;
@@ -1352,11 +1381,13 @@ static int init_cell(int problem, int n, const int *ax, int R,
     E.pk_col = pc ? atoi(pc) : 2;
     const char *pf = getenv("CC_PK_FIRST");
     E.pk_first = pf ? atoi(pf) : 1;
+    const char *pn = getenv("CC_PK_NARROW");
+    E.pk_narrow = pn ? atoi(pn) : 0;
     // an explicit schedule setting turns the automatic timing off
     const char *pa = getenv("CC_AUTOTUNE");
     E.autotune = pa ? atoi(pa) : 1;
     for (const char *k : {"CC_PACKED", "CC_SWEEP_KERNEL", "CC_SWEEP_LDS",
-                          "CC_PK_COL", "CC_PK_FIRST", "CC_LAYER16"})
+                          "CC_PK_COL", "CC_PK_FIRST", "CC_PK_NARROW", "CC_LAYER16"})
       if (getenv(k) && !pa) E.autotune = 0;
     E.tuned = 0;
     E.pfield = pk_plan(1.0, 1e300, true, false);
@@ -1887,6 +1918,8 @@ int dct_plan_probe(int n, const int *ax, int R, int kind, int s0, int M,
   E.pk_col = pc ? atoi(pc) : 2;
   const char *pf = getenv("CC_PK_FIRST");
   E.pk_first = pf ? atoi(pf) : 1;
+  const char *pn = getenv("CC_PK_NARROW");
+  E.pk_narrow = pn ? atoi(pn) : 0;
   const PPlan pl = kind == 0 ? pk_plan(1.0, 1e300, true, false)
                              : pk_plan((double)s0, (double)M, false, true);
   int ns = 0;
@@ -1920,15 +1953,15 @@ int dct_plan_probe(int n, const int *ax, int R, int kind, int s0, int M,
 int dct_autotune(int reps, float *best_us) {
   if (!E.live || reps < 1) return -1;
   E.tuned = 1;
-  struct Opt { int lds, col, first, packed; };
+  struct Opt { int lds, col, first, packed, narrow; };
   std::vector<Opt> opts;
   for (int lds : {16384, 32768, 65536, 98304, 131072}) {
     if (lds > E.lds_max) continue;
     for (int col : {2, 1, 0})
-      for (int first : {1, 0}) opts.push_back({lds, col, first, 1});
+      for (int first : {1, 0}) opts.push_back({lds, col, first, 1, E.pk_narrow});
   }
-  opts.push_back({E.lds_budget, E.pk_col, E.pk_first, 0});
-  const Opt saved{E.lds_budget, E.pk_col, E.pk_first, E.packed};
+  opts.push_back({E.lds_budget, E.pk_col, E.pk_first, 0, E.pk_narrow});
+  const Opt saved{E.lds_budget, E.pk_col, E.pk_first, E.packed, E.pk_narrow};
   std::vector<std::string> seen;
   float best = 1e30f;
   Opt win = saved;
@@ -1937,14 +1970,15 @@ int dct_autotune(int reps, float *best_us) {
   hipEvent_t e0 = E.ev[0], e1 = E.ev[1];
   for (const Opt &o : opts) {
     E.lds_budget = o.lds; E.pk_col = o.col; E.pk_first = o.first; E.packed = o.packed;
+    E.pk_narrow = o.narrow;
     PPlan pl = o.packed ? pk_plan(1.0, 1e300, true, false) : PPlan();
     std::string sig = "unpacked";
     if (pl.ok) {
       sig.clear();
       for (const PSweep &w : pl.sw) {
         char b[96];
-        snprintf(b, sizeof(b), "%d.%d.%d.%d.%d.%d.%d.%zu|", w.j0, w.nax, w.RR, w.W,
-                 w.ND, w.TL, w.kern, w.lds);
+        snprintf(b, sizeof(b), "%d.%d.%d.%d.%d.%d.%d.%zu.%d|", w.j0, w.nax, w.RR,
+                 w.W, w.ND, w.TL, w.kern, w.lds, w.out_nar);
         sig += b;
       }
     } else if (o.packed) {
@@ -1970,6 +2004,7 @@ int dct_autotune(int reps, float *best_us) {
   }
   E.lds_budget = win.lds; E.pk_col = win.col; E.pk_first = win.first;
   E.packed = win.packed;
+  E.pk_narrow = win.narrow;
   E.pfield = wplan;
   if (best_us) *best_us = best * 1000.0f / reps;
   return timed;

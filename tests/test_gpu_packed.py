@@ -23,11 +23,12 @@ CELLS = [([2] * 16, 3), ([2] * 14, 4), ([2] * 12, 6), ([2] * 13, 1),
          ([3] * 9, 3), ([3] * 8, 1), ([4] * 7, 3), ([5] * 6, 3), ([5] * 6, 2)]
 IDS = [f"{a[0]}^{len(a)}-R{r}" for a, r in CELLS]
 # schedule variants: LDS budgets that force short first sweeps and
-# multi-round strided tiles; the column kernel off, or whole-cube only
+# multi-round strided tiles; the column kernel off, or whole-cube only; the
+# first-sweep kernels; narrow (16 / 24-bit) inter-sweep words
 VARIANTS = {"default": {}, "lds16k": {"CC_SWEEP_LDS": "16384"},
             "lds4k": {"CC_SWEEP_LDS": "4096"}, "nocol": {"CC_PK_COL": "0"},
             "col1": {"CC_PK_COL": "1"}, "nofirst": {"CC_PK_FIRST": "0"},
-            "xlanefirst": {"CC_PK_FIRST": "2"}}
+            "xlanefirst": {"CC_PK_FIRST": "2"}, "narrow": {"CC_PK_NARROW": "1"}}
 
 
 def _check_cell(g, c, rng, mults):

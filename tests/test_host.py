@@ -134,15 +134,20 @@ def test_batch_header_matches_binding():
 
 
 # ---------------------------------------------------------------- planner
-def test_plan_probe_k2_24_3_schedule():
+def test_plan_probe_k2_24_3_schedule(monkeypatch):
     """K_2(24,3): axes 0-11 in one persistent LDS-tiled sweep with 8-bit
     fields, then two six-axis column sweeps with 16-bit fields; 4-byte
-    compact words between sweeps: 2+4 + 4+4 + 4+4 = 22 bytes per word."""
+    compact words between sweeps: 2+4 + 4+4 + 4+4 = 22 bytes per word.  With
+    narrow words on, the layers' 1+4+7+8 = 20 bits (after 12 axes) and
+    1+5+8+10 = 24 bits (after 18) travel as 3-byte words: 18 bytes."""
     from coldcore_amd.gpu import plan_probe
     plan = plan_probe([2] * 24, 3)
     assert [(s["first_axis"], s["axes"]) for s in plan] == [(0, 12), (12, 6), (18, 6)]
     assert [s["W"] for s in plan] == [8, 16, 16]
     assert sum(s["bytes_in"] + s["bytes_out"] for s in plan) == 22
+    monkeypatch.setenv("CC_PK_NARROW", "1")
+    plan = plan_probe([2] * 24, 3)
+    assert [(s["bytes_in"], s["bytes_out"]) for s in plan] == [(2, 3), (3, 3), (3, 4)]
 
 
 @pytest.mark.parametrize("q,ns,Rs", [(2, range(4, 41, 3), range(1, 7)),
@@ -176,5 +181,7 @@ def test_plan_probe_schedules_are_consistent(q, ns, Rs):
                     if i == len(plan) - 1:
                         assert sw["bytes_out"] == (4 if kind == 0 else 2)
                     else:
-                        assert sw["bytes_out"] in (4, 8)
+                        # dword planes (4 / 8 bytes) or, with R >= 3, a
+                        # narrow 16 / 24-bit word
+                        assert sw["bytes_out"] in ((2, 3, 4, 8) if R >= 3 else (4, 8))
                 assert j == n, (q, n, R, kind)
