@@ -307,6 +307,17 @@ def run_near_cover(GpuBackend):
     return out
 
 
+def run_learned(GpuBackend, seconds=15.0, seeds=(0, 1)):
+    """Time-to-size of the learned ruin proposal against plain adaptive LNS
+    on K_3(10,2): move logs of plain LNS on K_3(10,2) and K_4(8,2) (3,000
+    rounds each), the network trained on the GPU, then per seed both arms
+    from the same greedy + peel cover for ``seconds`` of notch descent."""
+    from coldcore_amd.learned import learned_ab
+    cells = [lambda: GpuBackend([3] * 10, 2), lambda: GpuBackend([4] * 8, 2)]
+    return learned_ab(lambda: GpuBackend([3] * 10, 2), cells, list(seeds),
+                      seconds, train_rounds=3000)
+
+
 def copy_calibration(mib):
     """hipMemcpy device-to-device of a known size: its dispatch
     (__amd_rocclr_copyBuffer) reads and writes exactly mib MiB."""
@@ -384,13 +395,18 @@ def main():
                     help="only run a device-to-device copy of MIB MiB "
                          "(3 times): the reference kernel for scaling the "
                          "FETCH_SIZE / WRITE_SIZE counters")
-    ap.add_argument("--only", choices=("near_cover", "batch64", "batch64-quick"),
+    ap.add_argument("--only", choices=("near_cover", "batch64", "batch64-quick",
+                                       "cells", "learned"),
                     help="run one side section and print its JSON only")
     args = ap.parse_args()
     if args.only:
         from coldcore_amd.gpu import GpuBackend
         if args.only == "near_cover":
             r = run_near_cover(GpuBackend)
+        elif args.only == "cells":
+            r = time_cells(GpuBackend, 10)
+        elif args.only == "learned":
+            r = run_learned(GpuBackend)
         else:
             r = run_batch64(GpuBackend, quick=args.only.endswith("quick"))
         print(json.dumps({args.only: r}), flush=True)

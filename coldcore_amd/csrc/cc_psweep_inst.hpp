@@ -89,16 +89,26 @@ hipError_t pk_go(const PSweepArgs &a, int blocks, int threads, size_t lds,
                                          (int)lds);
       if (e != hipSuccess) return e;
     }
-    int per_cu = 0, dev = 0, ncu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)kf,
-                                                     threads, lds) != hipSuccess ||
-        per_cu < 1)
-      per_cu = 1;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) !=
-            hipSuccess || ncu < 1)
-      ncu = 256;
-    const long long resident = (long long)per_cu * ncu;
+    // resident workgroups: queried once per (kernel, threads, LDS); the
+    // occupancy and attribute calls cost microseconds of host time per call
+    static void (*c_kf)(PSweepArgs) = nullptr;
+    static int c_threads = -1;
+    static size_t c_lds = 0;
+    static long long c_resident = 0;
+    if (kf != c_kf || threads != c_threads || lds != c_lds) {
+      int per_cu = 0, dev = 0, ncu = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)kf,
+                                                       threads, lds) != hipSuccess ||
+          per_cu < 1)
+        per_cu = 1;
+      if (hipGetDevice(&dev) != hipSuccess ||
+          hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) !=
+              hipSuccess || ncu < 1)
+        ncu = 256;
+      c_kf = kf; c_threads = threads; c_lds = lds;
+      c_resident = (long long)per_cu * ncu;
+    }
+    const long long resident = c_resident;
     const int g = (int)(blocks < resident ? blocks : resident);
     pk_run(kf, g, threads, lds, st, e0, e1, a);
     return hipGetLastError();

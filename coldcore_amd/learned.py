@@ -156,25 +156,26 @@ class RegionScorer:
         return G, L, C
 
     def features(self, lay, regions):
+        """(regions, 5 (R+1)) features of equal-size regions (positions
+        into the code), all regions at once."""
         G, L, C = lay
-        out = np.zeros((len(regions), 5 * (self.R + 1)), dtype=np.float32)
-        for i, W in enumerate(regions):
-            W = np.asarray(W)
-            out[i] = np.concatenate([L[W].mean(0), G[W].mean(0), G[W].max(0),
-                                     C[W].mean(0), L[W].min(0)])
-        return out
+        W = np.asarray(regions, dtype=np.int64)          # (regions, k)
+        Lw, Gw, Cw = L[W], G[W], C[W]                    # (regions, k, R+1)
+        return np.concatenate([Lw.mean(1), Gw.mean(1), Gw.max(1), Cw.mean(1),
+                               Lw.min(1)], axis=1).astype(np.float32)
 
 
 def cluster_regions(searcher, k, anchors):
-    """Anchor + its k-1 nearest incumbents (digit distance, random ties)."""
+    """Anchor + its k-1 nearest incumbents (digit distance, random ties),
+    for all anchors at once: a (anchors, k) array of code positions."""
     s = searcher
     m = len(s.code)
-    digs = s.b.index_word(s.code).astype(np.int16)
-    out = []
-    for a in anchors:
-        d = (digs != digs[a][None, :]).sum(axis=1)
-        out.append(np.argsort(d + s.rng.random(m))[:k])
-    return out
+    digs = s.b.index_word(s.code).astype(np.int8)
+    anchors = np.asarray(anchors, dtype=np.int64)
+    d = (digs[None, :, :] != digs[anchors][:, None, :]).sum(axis=2)
+    d = d + s.rng.random((len(anchors), m))
+    k = min(k, m)
+    return np.argpartition(d, k - 1, axis=1)[:, :k]
 
 
 class LoggingOperator:
