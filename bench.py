@@ -169,6 +169,8 @@ def run_batch64(GpuBackend, rounds=200, k=2, quick=False):
     out = {"cells": len(cells),
            "words": int(sum(int(np.prod(a)) for _, a, _ in cells))}
     with GpuBatch(cells, cap=65535) as b:
+        # the batch's own schedule autotuner (LDS budget per tile)
+        out["autotune_budget"], out["autotune_us"] = b.autotune(reps=5)
         out.update({f"schedule_{k_}": v for k_, v in b.info().items()})
         empty = np.zeros(0, dtype=np.int64)
         for i in range(len(cells)):

@@ -27,7 +27,8 @@ _U64P = ctypes.POINTER(ctypes.c_uint64)
 #: every batch symbol the library exports
 BATCH_SYMBOLS = ("ccb_init", "ccb_free", "ccb_set_code", "ccb_transform",
                  "ccb_read_cnt", "ccb_read_map", "ccb_greedy", "ccb_lns_init",
-                 "ccb_lns_run", "ccb_get_code", "ccb_stats", "ccb_info")
+                 "ccb_lns_run", "ccb_get_code", "ccb_stats", "ccb_info",
+                 "ccb_autotune")
 
 KMAX = 8
 MAX_R = 3
@@ -51,6 +52,7 @@ def _bind_batch(lib):
     lib.ccb_get_code.argtypes = [ctypes.c_int, ctypes.c_int, _LLP, ctypes.c_int]
     lib.ccb_stats.argtypes = [ctypes.c_int, _LLP]
     lib.ccb_info.argtypes = [_LLP]
+    lib.ccb_autotune.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_float)]
     lib._ccb_bound = True
     return lib
 
@@ -185,7 +187,16 @@ class GpuBatch:
         return dict(zip(keys, (int(v) for v in out)))
 
     def info(self):
-        out = (_LL * 5)()
+        out = (_LL * 6)()
         _check("ccb_info", self._lib.ccb_info(out))
-        return dict(zip(("cells", "sweeps", "items", "levels", "workgroups"),
-                        (int(v) for v in out)))
+        return dict(zip(("cells", "sweeps", "items", "levels", "workgroups",
+                         "lds_budget"), (int(v) for v in out)))
+
+    def autotune(self, reps=5):
+        """Time the batched gain transform under each LDS budget and keep
+        the fastest plan; returns (budget bytes, us per transform)."""
+        us = ctypes.c_float(0.0)
+        r = self._lib.ccb_autotune(int(reps), ctypes.byref(us))
+        if r < 0:
+            raise RuntimeError(f"ccb_autotune rc={r}")
+        return int(r), float(us.value)

@@ -29,7 +29,7 @@
 
 namespace {
 
-constexpr int CCB_LDS = 32 * 1024;   // LDS budget of a sweep tile
+constexpr int CCB_LDS = 32 * 1024;   // default LDS budget of a sweep tile
 
 struct Batch {
   bool live = false;
@@ -48,6 +48,8 @@ struct Batch {
   int nargs = 0, nitems = 0, nlevels = 0;
   size_t lds = 0;
   int per_cu = 1, ncu = 1;
+  int budget = CCB_LDS;               // LDS budget the plan was built for
+  std::vector<void *> plan_allocs;    // freed when the plan is rebuilt
   long long maxX = 0;
   BItem *d_ruin = nullptr, *d_add = nullptr, *d_rev = nullptr;
   std::vector<long long> chunks;      // reduction chunks: cell << 40 | word
@@ -78,6 +80,8 @@ void bfree_all() {
   if (B.graph) { (void)hipGraphDestroy(B.graph); B.graph = nullptr; }
   for (void *p : B.allocs) (void)hipFree(p);
   B.allocs.clear();
+  for (void *p : B.plan_allocs) (void)hipFree(p);
+  B.plan_allocs.clear();
   if (B.st) { (void)hipStreamDestroy(B.st); B.st = nullptr; }
   B = Batch();
 }
@@ -102,7 +106,7 @@ void plan_cell(int ci, const BCell &c) {
     int k = j;
     while (k < c.n) {
       const long long P2 = P * c.ax[k];
-      if (k > j && ((long long)c.L * P2 * TL * 4 > CCB_LDS || k - j >= CC_MAXSW))
+      if (k > j && ((long long)c.L * P2 * TL * 4 > B.budget || k - j >= CC_MAXSW))
         break;
       P = P2;
       k++;
@@ -111,7 +115,7 @@ void plan_cell(int ci, const BCell &c) {
     // short axis would otherwise make thousands of 96-word tiles)
     int TLw = TL;
     if (j > 0) {
-      long long tl = std::min<long long>(s, CCB_LDS / (4LL * c.L * P));
+      long long tl = std::min<long long>(s, B.budget / (4LL * c.L * P));
       if (tl >= 64) tl = tl / 32 * 32;
       TLw = (int)std::max<long long>(TL, tl);
     }
@@ -149,6 +153,76 @@ void plan_cell(int ci, const BCell &c) {
     B.lds = std::max(B.lds, (size_t)(c.L * P * TLw * 4));
     j = k;
   }
+}
+
+// The sweep plan of every cell for an LDS budget: sweeps, work items by
+// level, their device copies and the persistent grid.  Rebuilding drops a
+// captured graph (its launches hold the old plan).
+int build_plan(int budget) {
+  if (B.st) CKB(hipStreamSynchronize(B.st));
+  if (B.exec) { CKB(hipGraphExecDestroy(B.exec)); B.exec = nullptr; }
+  if (B.graph) { CKB(hipGraphDestroy(B.graph)); B.graph = nullptr; }
+  B.graph_k = -1;
+  for (void *p : B.plan_allocs) CKB(hipFree(p));
+  B.plan_allocs.clear();
+  for (auto &v : B.args) v.clear();
+  B.argcell.clear();
+  B.need.clear();
+  B.items.clear();
+  B.level_off.clear();
+  B.lds = 0;
+  B.budget = budget;
+  const int ncells = B.nc;
+  std::vector<std::vector<int>> cell_args(ncells);
+  for (int i = 0; i < ncells; i++) {
+    const int before = (int)B.argcell.size();
+    plan_cell(i, B.h[i]);
+    for (int a = before; a < (int)B.argcell.size(); a++) cell_args[i].push_back(a);
+  }
+  B.nargs = (int)B.argcell.size();
+  for (int level = 0;; level++) {
+    bool any = false;
+    const int off = (int)B.items.size();
+    for (int i = 0; i < ncells; i++) {
+      if (level >= (int)cell_args[i].size()) continue;
+      any = true;
+      const int a = cell_args[i][level];
+      for (int t = 0; t < B.need[a]; t++) B.items.push_back(make_int2(a, t));
+    }
+    if (!any) break;
+    B.level_off.push_back(off);
+  }
+  B.nlevels = (int)B.level_off.size();
+  B.nitems = (int)B.items.size();
+  B.level_off.push_back(B.nitems);
+  auto palloc = [](void **p, size_t bytes) {
+    if (hipMalloc(p, bytes) != hipSuccess) { (void)hipGetLastError(); return -1; }
+    B.plan_allocs.push_back(*p);
+    return 0;
+  };
+  for (int k = 0; k < 3; k++)
+    if (palloc((void **)&B.dargs[k], sizeof(SweepArgs) * B.nargs) ||
+        hipMemcpy(B.dargs[k], B.args[k].data(), sizeof(SweepArgs) * B.nargs,
+                  hipMemcpyHostToDevice) != hipSuccess)
+      return -3;
+  if (palloc((void **)&B.d_argcell, 4 * B.nargs) ||
+      palloc((void **)&B.d_items, 8 * B.nitems) ||
+      palloc((void **)&B.d_counters, 4 * B.nlevels) ||
+      hipMemcpy(B.d_argcell, B.argcell.data(), 4 * B.nargs, hipMemcpyHostToDevice) ||
+      hipMemcpy(B.d_items, B.items.data(), 8 * B.nitems, hipMemcpyHostToDevice))
+    return -3;
+  if (B.lds > 48 * 1024)   // beyond the default dynamic LDS of a launch
+    CKB(hipFuncSetAttribute((const void *)k_ccb_sweeps,
+                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)B.lds + 1024));
+  // persistent grids: what the CUs hold at once
+  int per_cu = 1;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_ccb_sweeps,
+                                                   CC_BLOCK, B.lds) != hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  B.per_cu = per_cu;
+  return 0;
 }
 
 int transform_launch(int kind) {
@@ -296,62 +370,27 @@ int ccb_init(int ncells, const int *problem, const int *n, const int *axes,
   }
   if (hipStreamCreateWithFlags(&B.st, hipStreamNonBlocking) != hipSuccess)
     { bfree_all(); return -3; }
-  // sweep plans, work items by level
-  std::vector<std::vector<int>> cell_args(ncells);
-  for (int i = 0; i < ncells; i++) {
-    const int before = (int)B.argcell.size();
-    plan_cell(i, B.h[i]);
-    for (int a = before; a < (int)B.argcell.size(); a++) cell_args[i].push_back(a);
-  }
-  B.nargs = (int)B.argcell.size();
-  for (int level = 0;; level++) {
-    bool any = false;
-    const int off = (int)B.items.size();
-    for (int i = 0; i < ncells; i++) {
-      if (level >= (int)cell_args[i].size()) continue;
-      any = true;
-      const int a = cell_args[i][level];
-      for (int t = 0; t < B.need[a]; t++) B.items.push_back(make_int2(a, t));
-    }
-    if (!any) break;
-    B.level_off.push_back(off);
-  }
-  B.nlevels = (int)B.level_off.size();
-  B.nitems = (int)B.items.size();
-  B.level_off.push_back(B.nitems);
   for (int i = 0; i < ncells; i++)
     for (long long w = 0; w < B.h[i].X; w += CCB_CHUNK)
       B.chunks.push_back(((long long)i << 40) | w);
   B.snap_grid = (cap + 4 * CC_BLOCK - 1) / (4 * CC_BLOCK);
   if (balloc(&B.d, sizeof(BCell) * ncells) ||
-      balloc(&B.d_argcell, 4 * B.nargs) || balloc(&B.d_items, 8 * B.nitems) ||
-      balloc(&B.d_counters, 4 * B.nlevels) ||
       balloc(&B.d_chunks, 8 * B.chunks.size()) ||
       balloc(&B.d_ruin, sizeof(BItem) * ncells * CCB_KMAX) ||
       balloc(&B.d_add, sizeof(BItem) * ncells) ||
       balloc(&B.d_rev, sizeof(BItem) * ncells * 2 * CCB_KMAX))
     { bfree_all(); return -3; }
-  for (int k = 0; k < 3; k++)
-    if (balloc(&B.dargs[k], sizeof(SweepArgs) * B.nargs) ||
-        hipMemcpy(B.dargs[k], B.args[k].data(), sizeof(SweepArgs) * B.nargs,
-                  hipMemcpyHostToDevice) != hipSuccess)
-      { bfree_all(); return -3; }
   if (hipMemcpy(B.d, B.h.data(), sizeof(BCell) * ncells, hipMemcpyHostToDevice) ||
-      hipMemcpy(B.d_argcell, B.argcell.data(), 4 * B.nargs, hipMemcpyHostToDevice) ||
-      hipMemcpy(B.d_items, B.items.data(), 8 * B.nitems, hipMemcpyHostToDevice) ||
       hipMemcpy(B.d_chunks, B.chunks.data(), 8 * B.chunks.size(),
                 hipMemcpyHostToDevice))
     { bfree_all(); return -3; }
-  // persistent grids: what the CUs hold at once
-  int per_cu = 1, dev = 0, ncu = 256;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_ccb_sweeps,
-                                                   CC_BLOCK, B.lds) != hipSuccess ||
-      per_cu < 1)
-    per_cu = 1;
-  (void)hipGetDevice(&dev);
-  (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-  B.per_cu = per_cu;
-  B.ncu = std::max(ncu, 1);
+  {
+    int dev = 0, ncu = 256;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    B.ncu = std::max(ncu, 1);
+  }
+  if (build_plan(CCB_LDS)) { bfree_all(); return -3; }
   // the set-up copies and clears ran outside the batch's (non-blocking)
   // stream: finish them before its first launch
   if (hipDeviceSynchronize() != hipSuccess) { bfree_all(); return -3; }
@@ -509,8 +548,9 @@ int ccb_stats(int cell, long long *out) {
   return 0;
 }
 
-// schedule of the batch: out[0..4] = cells, sweeps, work items, sweep
-// levels (persistent launches per transform), workgroups per launch at most
+// schedule of the batch: out[0..5] = cells, sweeps, work items, sweep
+// levels (persistent launches per transform), workgroups per launch at
+// most, LDS budget of the plan (bytes per tile)
 int ccb_info(long long *out) {
   if (!B.live || !out) return -1;
   out[0] = B.nc;
@@ -518,7 +558,39 @@ int ccb_info(long long *out) {
   out[2] = B.nitems;
   out[3] = B.nlevels;
   out[4] = (long long)B.per_cu * B.ncu;
+  out[5] = B.budget;
   return 0;
+}
+
+// Time the batched gain transform under each LDS budget (8, 16, 32, 64 KiB
+// per tile: the axis grouping, tile widths and persistent grid follow from
+// it) and keep the fastest plan.  reps timed transforms per budget after
+// one warm-up; the map planes are overwritten, cnt is not.  Returns the
+// chosen budget (bytes); *best_us = its time per transform.
+int ccb_autotune(int reps, float *best_us) {
+  if (!B.live || reps < 1) return -1;
+  hipEvent_t e0, e1;
+  CKB(hipEventCreate(&e0));
+  CKB(hipEventCreate(&e1));
+  float best = 1e30f;
+  int win = B.budget;
+  for (int budget : {8192, 16384, 32768, 65536}) {
+    if (int rc = build_plan(budget)) return rc;
+    if (int rc = transform_launch(0)) return rc;
+    CKB(hipEventRecord(e0, B.st));
+    for (int r = 0; r < reps; r++)
+      if (int rc = transform_launch(0)) return rc;
+    CKB(hipEventRecord(e1, B.st));
+    CKB(hipEventSynchronize(e1));
+    float ms = 0;
+    CKB(hipEventElapsedTime(&ms, e0, e1));
+    if (ms < best) { best = ms; win = budget; }
+  }
+  CKB(hipEventDestroy(e0));
+  CKB(hipEventDestroy(e1));
+  if (int rc = build_plan(win)) return rc;
+  if (best_us) *best_us = best * 1000.0f / reps;
+  return win;
 }
 
 }  // extern "C"

@@ -86,6 +86,36 @@ def test_batched_transform_matches_oracle():
                                           err_msg=f"loss cell {i}")
 
 
+def test_batch_autotune_keeps_fields_exact():
+    """The schedule autotuner only changes the plan: after it, every cell's
+    recount, gain and loss still equal the oracle."""
+    cells = BATCH_CELLS[::2]
+    codes = _codes(cells, seed=5)
+    with _batch(cells) as b:
+        for i, c in enumerate(codes):
+            b.set_code(i, c)
+        b.recount()
+        budget, us = b.autotune(reps=2)
+        assert budget in (8192, 16384, 32768, 65536) and us > 0
+        assert b.info()["lds_budget"] == budget
+        b.recount()
+        for which in ("recount", "gain", "loss"):
+            if which == "gain":
+                b.gain_maps()
+            elif which == "loss":
+                b.loss_maps()
+            for i, (p, axes, R) in enumerate(cells):
+                o = NumpyDP(axes, R, problem=p)
+                o.recount(codes[i])
+                if which == "recount":
+                    np.testing.assert_array_equal(b.read_cnt(i),
+                                                  o.cnt.astype(np.uint16))
+                else:
+                    getattr(o, which + "_map")()
+                    np.testing.assert_array_equal(b.read_map(i), o.map,
+                                                  err_msg=f"{which} cell {i}")
+
+
 def test_batched_transform_repeats_exactly():
     """The persistent work list resets itself: repeated launches agree."""
     cells = BATCH_CELLS[:8]
