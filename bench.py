@@ -249,6 +249,57 @@ def run_batch64(GpuBackend, rounds=200, k=2, quick=False):
     return out
 
 
+def run_walks(g, code, size, n_upd=100000, n_q=10 ** 6):
+    """Incremental updates and exact queries on a recounted K_2(24,3) cell:
+    n_upd single-word updates alternating +1 (a random non-code word, seed
+    1) and -1 (a random current codeword), 20 x (+1, -1) batches of 1,024
+    words, and n_q exact gain queries (seed 2) in chunks of 65,536."""
+    out = {}
+    rng = np.random.default_rng(1)
+    live = list(code.tolist())
+    liveset = set(live)
+    adds = rng.integers(0, size, n_upd)
+    picks = rng.random(n_upd)
+    g.count_eq(0)
+    t = time.perf_counter()
+    for i in range(n_upd):
+        if i % 2 == 0:
+            w = int(adds[i])
+            while w in liveset:
+                w = (w + 1) % size
+            g.ball_update(np.array([w], dtype=np.int64), +1)
+            live.append(w)
+            liveset.add(w)
+        else:
+            j = int(picks[i] * len(live))
+            w = live[j]
+            live[j] = live[-1]
+            live.pop()
+            liveset.discard(w)
+            g.ball_update(np.array([w], dtype=np.int64), -1)
+    g.count_eq(0)                          # drains the stream
+    du = time.perf_counter() - t
+    out["updates_per_s"] = round(n_upd / du, 1)
+    out["update_us"] = round(du / n_upd * 1e6, 3)
+    wb = rng.choice(size, 1024, replace=False).astype(np.int64)
+    t = time.perf_counter()
+    for _ in range(20):
+        g.ball_update(wb, +1)
+        g.ball_update(wb, -1)
+    g.count_eq(0)
+    out["update_words_per_s_batch1024"] = round(
+        40 * 1024 / (time.perf_counter() - t), 1)
+    q = np.random.default_rng(2).integers(0, size, n_q).astype(np.int64)
+    g.ball_gather(q[:65536], 0)
+    t = time.perf_counter()
+    for lo in range(0, len(q), 65536):
+        g.ball_gather(q[lo:lo + 65536], 0)
+    dq = time.perf_counter() - t
+    out["queries_per_s"] = round(n_q / dq, 1)
+    out["query_cnt_reads_per_s"] = round(n_q * 2325 / dq, 1)
+    return out
+
+
 def run_near_cover(GpuBackend):
     """Set-side gathers and the swap-evaluation fast path against the walk
     (CC_SET_POLICY=0) on near-cover states: K_2(24,3) with 60,000 random
@@ -398,7 +449,7 @@ def main():
                          "(3 times): the reference kernel for scaling the "
                          "FETCH_SIZE / WRITE_SIZE counters")
     ap.add_argument("--only", choices=("near_cover", "batch64", "batch64-quick",
-                                       "cells", "learned"),
+                                       "cells", "learned", "walks"),
                     help="run one side section and print its JSON only")
     args = ap.parse_args()
     if args.only:
@@ -409,6 +460,16 @@ def main():
             r = time_cells(GpuBackend, 10)
         elif args.only == "learned":
             r = run_learned(GpuBackend)
+        elif args.only == "walks":
+            # a short run of the update / query kernels (for the profiler)
+            size = 1 << 24
+            code = np.flatnonzero(np.random.default_rng(0).random(size) < 1e-4)
+            g = GpuBackend([2] * 24, 3)
+            try:
+                g.recount(code.astype(np.int64))
+                r = run_walks(g, code.astype(np.int64), size, 20000, 200000)
+            finally:
+                g.close()
         else:
             r = run_batch64(GpuBackend, quick=args.only.endswith("quick"))
         print(json.dumps({args.only: r}), flush=True)
@@ -481,53 +542,7 @@ def main():
                 extra[f"{name}_pass_ms"] = round(
                     (time.perf_counter() - t) * 100, 4)
             extra["recount_bytes_per_word"] = sum(sweep_bytes(rplan))
-            # 10^5 single-word updates, alternating +1 (random non-code word)
-            # and -1 (random current codeword)
-            rng = np.random.default_rng(1)
-            live = list(code.tolist())
-            liveset = set(live)
-            adds = rng.integers(0, size, 100000)
-            picks = rng.random(100000)
-            g.count_eq(0)
-            t = time.perf_counter()
-            for i in range(100000):
-                if i % 2 == 0:
-                    w = int(adds[i])
-                    while w in liveset:
-                        w = (w + 1) % size
-                    g.ball_update(np.array([w], dtype=np.int64), +1)
-                    live.append(w)
-                    liveset.add(w)
-                else:
-                    j = int(picks[i] * len(live))
-                    w = live[j]
-                    live[j] = live[-1]
-                    live.pop()
-                    liveset.discard(w)
-                    g.ball_update(np.array([w], dtype=np.int64), -1)
-            g.count_eq(0)                          # drains the stream
-            du = time.perf_counter() - t
-            extra["updates_per_s"] = round(100000 / du, 1)
-            extra["update_us"] = round(du / 100000 * 1e6, 3)
-            # batches of 1,024 words (+1 then -1)
-            wb = rng.choice(size, 1024, replace=False).astype(np.int64)
-            t = time.perf_counter()
-            for _ in range(20):
-                g.ball_update(wb, +1)
-                g.ball_update(wb, -1)
-            g.count_eq(0)
-            extra["update_words_per_s_batch1024"] = round(
-                40 * 1024 / (time.perf_counter() - t), 1)
-            # 10^6 exact gain queries in chunks of 65,536
-            q = np.random.default_rng(2).integers(0, size, 10 ** 6)
-            q = q.astype(np.int64)
-            g.ball_gather(q[:65536], 0)
-            t = time.perf_counter()
-            for lo in range(0, len(q), 65536):
-                g.ball_gather(q[lo:lo + 65536], 0)
-            dq = time.perf_counter() - t
-            extra["queries_per_s"] = round(1e6 / dq, 1)
-            extra["query_cnt_reads_per_s"] = round(1e6 * 2325 / dq, 1)
+            extra.update(run_walks(g, code, size))
     finally:
         g.close()
 

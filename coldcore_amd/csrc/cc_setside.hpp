@@ -183,9 +183,10 @@ __global__ void __launch_bounds__(CC_BLOCK)
                 long long lcap, unsigned long long *lcount) {
   __shared__ long long disp[CC_MAXN * CC_CSTRIDE];
   __shared__ unsigned char okf[CC_MAXN * CC_CSTRIDE];
+  __shared__ int digs[CC_MAXN];
   const int di = blockIdx.x / split, seg = blockIdx.x - di * split;
   const long long w = drops[di];
-  cc_walk_disp(g, w, disp, okf);
+  cc_walk_disp(g, w, disp, okf, digs);
   __syncthreads();
   long long acc = 0;
   const long long step = (long long)split * CC_BLOCK;

@@ -31,17 +31,31 @@ struct Geom {
 #define CC_WALK_GATHER 2  // count predicate(cnt) over the ball
 
 // the displacement (and, for the open grid, in-board flag) of every
-// (position, delta) slot for centre w; the caller synchronises
+// (position, delta) slot for centre w: the centre's digits first (one
+// division per axis, 32-bit when the index allows), then the table; the
+// caller synchronises after it
 __device__ __forceinline__ void cc_walk_disp(const Geom &g, long long w,
                                              long long *disp,
-                                             unsigned char *okf) {
+                                             unsigned char *okf, int *digs) {
+  for (int p = threadIdx.x; p < g.n; p += blockDim.x) {
+    int d;
+    if (w < (1LL << 32)) {
+      const uint32_t w32 = (uint32_t)w;
+      d = g.pw[p] > (long long)w32 ? 0
+                                   : (int)((w32 / (uint32_t)g.pw[p]) % (uint32_t)g.ax[p]);
+    } else {
+      d = (int)((w / g.pw[p]) % g.ax[p]);
+    }
+    digs[p] = d;
+  }
+  __syncthreads();
   for (int t = threadIdx.x; t < CC_MAXN * CC_CSTRIDE; t += blockDim.x) {
     const int p = t / CC_CSTRIDE, dl = t - p * CC_CSTRIDE + 1;
     long long off = 0;
     unsigned char ok = 0;
     if (p < g.n && dl < g.ax[p]) {
       const int a = g.ax[p];
-      const int dig = (int)((w / g.pw[p]) % a);
+      const int dig = digs[p];
       const int sh = (dig + dl < a) ? dl : dl - a;
       off = (long long)sh * g.pw[p];
       ok = (sh <= g.R && sh >= -g.R) ? 1 : 0;
@@ -83,9 +97,10 @@ __global__ void __launch_bounds__(CC_BLOCK)
            unsigned long long *trans) {
   __shared__ long long disp[CC_MAXN * CC_CSTRIDE];
   __shared__ unsigned char okf[CC_MAXN * CC_CSTRIDE];
+  __shared__ int digs[CC_MAXN];
   const int wi = blockIdx.x / split, seg = blockIdx.x - wi * split;
   const long long w = wptr ? wptr[wi] : wa.w[wi];
-  cc_walk_disp(g, w, disp, okf);
+  cc_walk_disp(g, w, disp, okf, digs);
   __syncthreads();
   long long acc = 0;
   const long long step = (long long)split * CC_BLOCK;

@@ -985,6 +985,29 @@ int ensure_zlist(int pmode, int target, long long lim) {
       E.o_F - (E.plus_words - E.o_plus) * E.blen > lim)
     return 0;
   lim = std::min(lim, E.zcap);
+  // size first: one vector-load counting pass (the count of cnt == target,
+  // or for the deficient predicate the deficit, an upper bound on the
+  // count) decides whether a listing can fit, so a state far from a cover
+  // never pays for a listing of millions of words
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, E.st, E.d_scal + 2, 1);
+  CK(hipGetLastError());
+  if (pmode == CC_DEF)
+    hipLaunchKernelGGL(k_cnt_reduce<CC_DEF>, dim3(pass_blocks(E.X / 8 + 1)),
+                       dim3(CC_BLOCK), 0, E.st, E.cnt, E.X, 0, E.mu, E.d_scal + 2);
+  else
+    hipLaunchKernelGGL(k_cnt_reduce<CC_EQ>, dim3(pass_blocks(E.X / 8 + 1)),
+                       dim3(CC_BLOCK), 0, E.st, E.cnt, E.X, target, E.mu,
+                       E.d_scal + 2);
+  CK(hipGetLastError());
+  unsigned long long bound = 0;
+  if (int rc = fetch_scalar(2, &bound)) return rc;
+  if ((long long)bound > lim) {
+    E.stats[3]++;
+    E.o_mode = pmode; E.o_target = target; E.o_mu = E.mu;
+    E.o_F = (long long)bound; E.o_plus = E.plus_words;
+    E.z_mode = -1;
+    return 0;
+  }
   hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, E.st, E.d_scal + 2, 1);
   CK(hipGetLastError());
   const Dig dg = make_dig();
