@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -121,7 +122,10 @@ void plan_cell(int ci, const BCell &c) {
     }
     SweepArgs a;
     memset(&a, 0, sizeof(a));
-    for (int d = 0; d < c.L; d++) a.A[d] = c.A[d];
+    for (int d = 0; d < c.L; d++) {
+      a.A[d] = c.A[d];
+      a.H[d] = c.l16 ? c.H + (size_t)d * c.X : nullptr;
+    }
     a.cnt = c.cnt;
     a.s = s;
     a.P = (int)P;
@@ -145,7 +149,9 @@ void plan_cell(int ci, const BCell &c) {
       b.in_kind = first ? (kind == 2 ? CC_IN_CODE : CC_IN_PRED) : CC_IN_LAYERS;
       b.out_kind = last ? (kind == 2 ? CC_OUT_CNT : CC_OUT_MAP) : CC_OUT_LAYERS;
       b.pmode = kind == 1 ? CC_CRIT : CC_DEF;
-      b.write_l0 = c.L == 1 || (first && kind != 2);
+      // uint16 planes keep layer 0 in H[0] too: the first sweep writes it
+      // whatever seeded it (A[0] holds the code indicator as int32)
+      b.write_l0 = c.L == 1 || (first && (kind != 2 || c.l16));
       B.args[kind].push_back(b);
     }
     B.argcell.push_back(ci);
@@ -349,9 +355,40 @@ int ccb_init(int ncells, const int *problem, const int *n, const int *axes,
     c.X = c.pw[c.n];
     c.blen = linf ? 0 : npats[i];
     if (!linf && (c.blen < 1 || !pats)) { bfree_all(); return -1; }
+    // 16-bit inter-sweep layers when every value fits: a layer value is a
+    // count of seed words at one partial distance, at most the ball size for
+    // the gain / loss indicators and the code size (<= cap) for a recount;
+    // the L-inf cells also need every axis on the unrolled window path
+    {
+      double ball = 1.0;
+      if (linf) {
+        for (int j = 0; j < c.n; j++) ball *= std::min(c.ax[j], 2 * c.R + 1);
+      } else {
+        std::vector<double> poly(c.R + 1, 0.0);
+        poly[0] = 1.0;
+        for (int j = 0; j < c.n; j++) {
+          std::vector<double> nx(c.R + 1, 0.0);
+          for (int w = 0; w <= c.R; w++) {
+            if (poly[w] == 0) continue;
+            nx[w] += poly[w];
+            for (int dl = 1; dl < c.ax[j]; dl++) {
+              const int e = c.prob == CC_LEE ? std::min(dl, c.ax[j] - dl) : 1;
+              if (w + e <= c.R) nx[w + e] += poly[w];
+            }
+          }
+          poly = nx;
+        }
+        ball = 0;
+        for (double v : poly) ball += v;
+      }
+      bool small_axes = true;
+      for (int j = 0; j < c.n; j++) small_axes = small_axes && c.ax[j] <= 16;
+      c.l16 = ball <= 65535.0 && (!linf || small_axes) && !getenv("CCB_INT32");
+    }
     // device planes
     for (int d = 0; d < c.L; d++)
       if (balloc(&c.A[d], (size_t)c.X * 4)) { bfree_all(); return -3; }
+    if (c.l16 && balloc(&c.H, (size_t)c.L * c.X * 2)) { bfree_all(); return -3; }
     if (balloc(&c.cnt, (size_t)((c.X + 1) & ~1LL) * 2) ||
         balloc(&c.code, (size_t)cap * 8) || balloc(&c.best, (size_t)cap * 8))
       { bfree_all(); return -3; }

@@ -23,6 +23,8 @@ struct BCell {
   int prob, n, R, L, mu, fmt;
   long long X;
   int32_t *A[CC_MAXL];   // A[0]: map / code indicator; A[1..L-1]: layers
+  uint16_t *H;           // l16: L uint16 layer planes of X words (inter-sweep)
+  int l16;               // every layer value fits 16 bits (ball, code <= 65,535)
   uint16_t *cnt;         // padded to an even length
   const uint64_t *pat;   // Hamming / Lee: support table (v1 or wide)
   long long blen;
@@ -133,21 +135,33 @@ __device__ __forceinline__ void ccb_tile_linf(const SweepArgs &a, long long tile
   __syncthreads();
 }
 
-__device__ __forceinline__ void ccb_tile(const SweepArgs &a, int prob, int L,
-                                         long long tile, int32_t *S) {
+// LT: inter-sweep layer planes int32 (A) or uint16 (H, cells whose layer
+// values provably fit 16 bits; never with the runtime-length window path)
+template <typename LT>
+__device__ __forceinline__ void ccb_tile_t(const SweepArgs &a, int prob, int L,
+                                           long long tile, int32_t *S) {
   if (prob == CC_HAMMING) {
-    if (L == 2) cc_sweep_tile<CC_HAMMING, 2, int32_t>(a, tile, S);
-    else if (L == 3) cc_sweep_tile<CC_HAMMING, 3, int32_t>(a, tile, S);
-    else cc_sweep_tile<CC_HAMMING, 4, int32_t>(a, tile, S);
+    if (L == 2) cc_sweep_tile<CC_HAMMING, 2, LT>(a, tile, S);
+    else if (L == 3) cc_sweep_tile<CC_HAMMING, 3, LT>(a, tile, S);
+    else cc_sweep_tile<CC_HAMMING, 4, LT>(a, tile, S);
   } else if (prob == CC_LEE) {
-    if (L == 2) cc_sweep_tile<CC_LEE, 2, int32_t>(a, tile, S);
-    else if (L == 3) cc_sweep_tile<CC_LEE, 3, int32_t>(a, tile, S);
-    else cc_sweep_tile<CC_LEE, 4, int32_t>(a, tile, S);
+    if (L == 2) cc_sweep_tile<CC_LEE, 2, LT>(a, tile, S);
+    else if (L == 3) cc_sweep_tile<CC_LEE, 3, LT>(a, tile, S);
+    else cc_sweep_tile<CC_LEE, 4, LT>(a, tile, S);
+  } else if constexpr (sizeof(LT) == 2) {
+    if (prob == CC_TORUS) cc_sweep_tile<CC_TORUS, 1, LT>(a, tile, S);
+    else cc_sweep_tile<CC_GRID, 1, LT>(a, tile, S);
   } else if (prob == CC_TORUS) {
     ccb_tile_linf<CC_TORUS>(a, tile, S);
   } else {
     ccb_tile_linf<CC_GRID>(a, tile, S);
   }
+}
+
+__device__ __forceinline__ void ccb_tile(const SweepArgs &a, const BCell &c,
+                                         long long tile, int32_t *S) {
+  if (c.l16) ccb_tile_t<uint16_t>(a, c.prob, c.L, tile, S);
+  else ccb_tile_t<int32_t>(a, c.prob, c.L, tile, S);
 }
 
 // items[i] = (sweep argument index, tile), all of one sweep level
@@ -168,7 +182,7 @@ __global__ void __launch_bounds__(CC_BLOCK)
     const long long tile = (long long)__builtin_amdgcn_readfirstlane(items[it].y);
     const SweepArgs &a = args[ai];
     const BCell &c = cells[argcell[ai]];
-    ccb_tile(a, c.prob, c.L, tile, S);
+    ccb_tile(a, c, tile, S);
   }
 }
 
