@@ -156,7 +156,7 @@ void plan_cell(int ci, const BCell &c) {
     }
     B.argcell.push_back(ci);
     B.need.push_back((int)a.ntiles);
-    B.lds = std::max(B.lds, (size_t)(c.L * P * TLw * 4));
+    B.lds = std::max(B.lds, (size_t)c.L * cc_gplane<false>((int)(P * TLw)) * 4);
     j = k;
   }
 }

@@ -56,29 +56,30 @@ struct BItem {
 template <int R, bool TORUS>
 __device__ __forceinline__ void ccb_step_linf_rt(int32_t *__restrict__ S, int TL,
                                                  int nf, int sk, int Q) {
+  const uint32_t mT = cc_magic((uint32_t)TL), mS = cc_magic((uint32_t)sk);
   for (int fi = threadIdx.x; fi < nf; fi += blockDim.x) {
-    const int f = fi / TL, tl = fi - f * TL;
-    const int hi = f / sk, lo = f - hi * sk;
+    const int f = cc_div(fi, mT), tl = fi - f * TL;
+    const int hi = cc_div(f, mS), lo = f - hi * sk;
     const int e0 = (hi * sk * Q + lo) * TL + tl, es = sk * TL;
     uint32_t head[R], win[2 * R + 1];
 #pragma unroll
-    for (int i = 0; i < R; i++) head[i] = (uint32_t)S[e0 + i * es];
+    for (int i = 0; i < R; i++) head[i] = (uint32_t)S[cc_gpad<false>(e0 + i * es)];
 #pragma unroll
     for (int i = 0; i <= 2 * R; i++) {
       const int b = i - R;   // -R .. R  (Q >= 2R+1 for the torus)
-      if (b < 0) win[i] = TORUS ? (uint32_t)S[e0 + (b + Q) * es] : 0u;
-      else win[i] = b < Q ? (uint32_t)S[e0 + b * es] : 0u;
+      if (b < 0) win[i] = TORUS ? (uint32_t)S[cc_gpad<false>(e0 + (b + Q) * es)] : 0u;
+      else win[i] = b < Q ? (uint32_t)S[cc_gpad<false>(e0 + b * es)] : 0u;
     }
     for (int a = 0; a < Q; a++) {
       uint32_t s = 0;
 #pragma unroll
       for (int i = 0; i <= 2 * R; i++) s += win[i];
-      S[e0 + a * es] = (int32_t)s;
+      S[cc_gpad<false>(e0 + a * es)] = (int32_t)s;
       // slide: the next window ends at a + 1 + R
       const int b = a + 1 + R;
       uint32_t nx = 0;
       if (b < Q) {
-        nx = (uint32_t)S[e0 + b * es];
+        nx = (uint32_t)S[cc_gpad<false>(e0 + b * es)];
       } else if (TORUS) {
 #pragma unroll
         for (int i = 0; i < R; i++) nx = (b - Q == i) ? head[i] : nx;
@@ -96,7 +97,7 @@ __device__ __forceinline__ void ccb_tile_linf(const SweepArgs &a, long long tile
   bool big = false;
   for (int k = 0; k < a.nax; k++) big = big || a.Q[k] > 16;
   if (!big) {
-    cc_sweep_tile<PROB, 1, int32_t>(a, tile, S);
+    cc_sweep_tile<PROB, 1, int32_t, false>(a, tile, S);
     return;
   }
   // same stage / write-back as cc_sweep_tile, axes by the runtime-Q path
@@ -110,9 +111,9 @@ __device__ __forceinline__ void ccb_tile_linf(const SweepArgs &a, long long tile
     const bool live = tl < ncol;
     const long long w = base + (long long)p * a.s + tl;
     if (a.in_kind == CC_IN_PRED)
-      S[e] = (live && cc_pred(a.pmode, (int)a.cnt[w], 0, a.mu)) ? 1 : 0;
+      S[cc_gpad<false>(e)] = (live && cc_pred(a.pmode, (int)a.cnt[w], 0, a.mu)) ? 1 : 0;
     else
-      S[e] = live ? a.A[0][w] : 0;
+      S[cc_gpad<false>(e)] = live ? a.A[0][w] : 0;
   }
   __syncthreads();
   for (int k = 0; k < a.nax; k++) {
@@ -128,9 +129,10 @@ __device__ __forceinline__ void ccb_tile_linf(const SweepArgs &a, long long tile
     const int p = e / a.TL, tl = e - p * a.TL;
     if (tl >= ncol) continue;
     const long long w = base + (long long)p * a.s + tl;
-    if (a.out_kind == CC_OUT_LAYERS) a.A[0][w] = S[e];
-    else if (a.out_kind == CC_OUT_MAP) a.A[0][w] = S[e];
-    else a.cnt[w] = S[e] > 65535 ? (uint16_t)65535 : (uint16_t)S[e];
+    const int32_t x = S[cc_gpad<false>(e)];
+    if (a.out_kind == CC_OUT_LAYERS) a.A[0][w] = x;
+    else if (a.out_kind == CC_OUT_MAP) a.A[0][w] = x;
+    else a.cnt[w] = x > 65535 ? (uint16_t)65535 : (uint16_t)x;
   }
   __syncthreads();
 }
@@ -141,16 +143,16 @@ template <typename LT>
 __device__ __forceinline__ void ccb_tile_t(const SweepArgs &a, int prob, int L,
                                            long long tile, int32_t *S) {
   if (prob == CC_HAMMING) {
-    if (L == 2) cc_sweep_tile<CC_HAMMING, 2, LT>(a, tile, S);
-    else if (L == 3) cc_sweep_tile<CC_HAMMING, 3, LT>(a, tile, S);
-    else cc_sweep_tile<CC_HAMMING, 4, LT>(a, tile, S);
+    if (L == 2) cc_sweep_tile<CC_HAMMING, 2, LT, false>(a, tile, S);
+    else if (L == 3) cc_sweep_tile<CC_HAMMING, 3, LT, false>(a, tile, S);
+    else cc_sweep_tile<CC_HAMMING, 4, LT, false>(a, tile, S);
   } else if (prob == CC_LEE) {
-    if (L == 2) cc_sweep_tile<CC_LEE, 2, LT>(a, tile, S);
-    else if (L == 3) cc_sweep_tile<CC_LEE, 3, LT>(a, tile, S);
-    else cc_sweep_tile<CC_LEE, 4, LT>(a, tile, S);
+    if (L == 2) cc_sweep_tile<CC_LEE, 2, LT, false>(a, tile, S);
+    else if (L == 3) cc_sweep_tile<CC_LEE, 3, LT, false>(a, tile, S);
+    else cc_sweep_tile<CC_LEE, 4, LT, false>(a, tile, S);
   } else if constexpr (sizeof(LT) == 2) {
-    if (prob == CC_TORUS) cc_sweep_tile<CC_TORUS, 1, LT>(a, tile, S);
-    else cc_sweep_tile<CC_GRID, 1, LT>(a, tile, S);
+    if (prob == CC_TORUS) cc_sweep_tile<CC_TORUS, 1, LT, false>(a, tile, S);
+    else cc_sweep_tile<CC_GRID, 1, LT, false>(a, tile, S);
   } else if (prob == CC_TORUS) {
     ccb_tile_linf<CC_TORUS>(a, tile, S);
   } else {

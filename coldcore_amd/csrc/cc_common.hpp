@@ -117,6 +117,27 @@ struct SweepArgs {
   int live[CC_MAXSW + 1];
 };
 
+// LDS layout of a generic tile: layer plane d starts at d * cc_gplane(PT),
+// tile element e sits at cc_gpad(e).  With PAD, one pad word every 32
+// spreads the stride-Q fiber accesses (Q = 2, 4, 8, ...) over the LDS banks
+// (measured faster for the singleton's mixed-radix sweeps, slower for the
+// batched tiles, which use PAD = false).
+template <bool PAD = true>
+__device__ __forceinline__ int cc_gpad(int e) { return PAD ? e + (e >> 5) : e; }
+template <bool PAD = true>
+__host__ __device__ __forceinline__ int cc_gplane(int pt) {
+  return PAD ? pt + (pt >> 5) + 1 : pt;
+}
+
+// x / d for x, d < 2^16 by one high multiply: m = floor((2^32 - 1) / d) + 1
+// (m = 0 stands for d = 1); exact because x * d < 2^32
+__device__ __forceinline__ uint32_t cc_magic(uint32_t d) {
+  return d == 1 ? 0u : 0xFFFFFFFFu / d + 1u;
+}
+__device__ __forceinline__ int cc_div(int x, uint32_t m) {
+  return m ? (int)__umulhi((uint32_t)x, m) : x;
+}
+
 // ---------------------------------------------------------------- walks
 struct WordsArg {
   long long w[128];  // small batches travel by value (no staging, no sync)

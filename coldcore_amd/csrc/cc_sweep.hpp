@@ -25,7 +25,7 @@
 #pragma once
 #include "cc_common.hpp"
 
-template <int PROB, int R1, int Q>
+template <int PROB, int R1, int Q, bool PAD>
 __device__ __forceinline__ void cc_fiber(int32_t *__restrict__ S, int PT,
                                          int e0, int es, int lv, int lw,
                                          int R) {
@@ -34,7 +34,7 @@ __device__ __forceinline__ void cc_fiber(int32_t *__restrict__ S, int PT,
   for (int d = 0; d < R1; d++)
 #pragma unroll
     for (int a = 0; a < Q; a++)
-      v[d][a] = (d < lv) ? (uint32_t)S[d * PT + e0 + a * es] : 0u;
+      v[d][a] = (d < lv) ? (uint32_t)S[d * PT + cc_gpad<PAD>(e0 + a * es)] : 0u;
   if (PROB == CC_HAMMING) {
     uint32_t T[R1];
 #pragma unroll
@@ -48,7 +48,7 @@ __device__ __forceinline__ void cc_fiber(int32_t *__restrict__ S, int PT,
       if (d < lw)
 #pragma unroll
         for (int a = 0; a < Q; a++)
-          S[d * PT + e0 + a * es] = (int32_t)(v[d][a] + T[d - 1] - v[d - 1][a]);
+          S[d * PT + cc_gpad<PAD>(e0 + a * es)] = (int32_t)(v[d][a] + T[d - 1] - v[d - 1][a]);
   } else if (PROB == CC_LEE) {
     constexpr int W = Q / 2;
 #pragma unroll
@@ -65,7 +65,7 @@ __device__ __forceinline__ void cc_fiber(int32_t *__restrict__ S, int PT,
               if (ap != am) s += v[d - w][ap];
             }
           }
-          S[d * PT + e0 + a * es] = (int32_t)s;
+          S[d * PT + cc_gpad<PAD>(e0 + a * es)] = (int32_t)s;
         }
   } else {
     // single-plane L-inf window sums (torus: circular, grid: clamped)
@@ -78,28 +78,29 @@ __device__ __forceinline__ void cc_fiber(int32_t *__restrict__ S, int PT,
         if (PROB == CC_TORUS && Q - dd < dd) dd = Q - dd;
         s += (dd <= R) ? v[0][b] : 0u;
       }
-      S[e0 + a * es] = (int32_t)s;
+      S[cc_gpad<PAD>(e0 + a * es)] = (int32_t)s;
     }
   }
 }
 
-template <int PROB, int R1, int Q>
+template <int PROB, int R1, int Q, bool PAD>
 __device__ __forceinline__ void cc_step(int32_t *__restrict__ S, int PT,
                                         int TL, int nf, int sk, int lv,
                                         int lw, int R) {
+  const uint32_t mT = cc_magic((uint32_t)TL), mS = cc_magic((uint32_t)sk);
   for (int fi = threadIdx.x; fi < nf; fi += CC_BLOCK) {
-    const int f = fi / TL, tl = fi - f * TL;
-    const int hi = f / sk, lo = f - hi * sk;
+    const int f = cc_div(fi, mT), tl = fi - f * TL;
+    const int hi = cc_div(f, mS), lo = f - hi * sk;
     const int e0 = (hi * sk * Q + lo) * TL + tl;
-    cc_fiber<PROB, R1, Q>(S, PT, e0, sk * TL, lv, lw, R);
+    cc_fiber<PROB, R1, Q, PAD>(S, PT, e0, sk * TL, lv, lw, R);
   }
 }
 
 #define CC_STEP_CASE(QQ)                                                    \
   case QQ:                                                                  \
     if (QQ >= QMIN && QQ <= QMAX)                                           \
-      cc_step<PROB, R1, (QQ >= QMIN && QQ <= QMAX) ? QQ : QMIN>(            \
-          S, PT, a.TL, nf, sk, lv, lw, a.R);                                \
+      cc_step<PROB, R1, (QQ >= QMIN && QQ <= QMAX) ? QQ : QMIN, PAD>(       \
+          S, PTP, a.TL, nf, sk, lv, lw, a.R);                               \
     break;
 
 template <int PROB>
@@ -129,11 +130,12 @@ __device__ __forceinline__ void cc_lay_st(const SweepArgs &a, int d,
 
 // One tile of a sweep: stage, run every axis of the group in LDS, write
 // back (the batched transform calls it for tiles of many cells).
-template <int PROB, int R1, typename LT>
+template <int PROB, int R1, typename LT, bool PAD = true>
 __device__ __forceinline__ void cc_sweep_tile(const SweepArgs &a, long long tile,
                                               int32_t *__restrict__ S) {
   constexpr int QMIN = CcQRange<PROB>::lo, QMAX = CcQRange<PROB>::hi;
   const int PT = a.P * a.TL;
+  const int PTP = cc_gplane<PAD>(PT);
   const int Lmax = a.live[a.nax];
   const int Lin = a.live[0];
   const long long h = tile / a.ncolb;
@@ -145,15 +147,16 @@ __device__ __forceinline__ void cc_sweep_tile(const SweepArgs &a, long long tile
     const int p = e / a.TL, tl = e - p * a.TL;
     const bool live = tl < ncol;
     const long long w = base + (long long)p * a.s + tl;
+    const int ep = cc_gpad<PAD>(e);
     if (a.in_kind == CC_IN_PRED)
-      S[e] = (live && cc_pred(a.pmode, (int)a.cnt[w], 0, a.mu)) ? 1 : 0;
+      S[ep] = (live && cc_pred(a.pmode, (int)a.cnt[w], 0, a.mu)) ? 1 : 0;
     else if (a.in_kind == CC_IN_CODE)
-      S[e] = live ? a.A[0][w] : 0;
+      S[ep] = live ? a.A[0][w] : 0;
     else
-      S[e] = live ? (int32_t)cc_lay_ld<LT>(a, 0, w) : 0;
+      S[ep] = live ? (int32_t)cc_lay_ld<LT>(a, 0, w) : 0;
     for (int d = 1; d < Lmax; d++)
-      S[d * PT + e] = (live && d < Lin && a.in_kind == CC_IN_LAYERS)
-                          ? (int32_t)cc_lay_ld<LT>(a, d, w) : 0;
+      S[d * PTP + ep] = (live && d < Lin && a.in_kind == CC_IN_LAYERS)
+                            ? (int32_t)cc_lay_ld<LT>(a, d, w) : 0;
   }
   __syncthreads();
   // ---- every axis of the group, in LDS
@@ -175,12 +178,13 @@ __device__ __forceinline__ void cc_sweep_tile(const SweepArgs &a, long long tile
     const int p = e / a.TL, tl = e - p * a.TL;
     if (tl >= ncol) continue;
     const long long w = base + (long long)p * a.s + tl;
+    const int ep = cc_gpad<PAD>(e);
     if (a.out_kind == CC_OUT_LAYERS) {
       for (int d = a.write_l0 ? 0 : 1; d < Lmax; d++)
-        cc_lay_st<LT>(a, d, w, (uint32_t)S[d * PT + e]);
+        cc_lay_st<LT>(a, d, w, (uint32_t)S[d * PTP + ep]);
     } else {
       uint32_t sum = 0;
-      for (int d = 0; d < Lmax; d++) sum += (uint32_t)S[d * PT + e];
+      for (int d = 0; d < Lmax; d++) sum += (uint32_t)S[d * PTP + ep];
       if (a.out_kind == CC_OUT_MAP) {
         a.A[0][w] = (int32_t)sum;
       } else {

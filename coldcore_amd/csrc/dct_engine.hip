@@ -356,7 +356,9 @@ void build_plan() {
       const long long P2 = P * E.ax[k];
       const int lay = layers_after(k + 1);
       const bool packed = round_ok && homog && E.l16_field;
-      const size_t need = (size_t)(packed ? (lay + 1) / 2 : lay) * P2 * TL0 * 4;
+      const size_t need = packed ? (size_t)((lay + 1) / 2) * P2 * TL0 * 4
+                                 : (size_t)lay * cc_gplane((int)std::min<long long>(
+                                       P2 * TL0, 1 << 24)) * 4;
       const bool cap = packed && (q == 2 || q == 4) && P2 * TL0 > 4096;
       if (k > j && (need > (size_t)E.lds_budget || P2 > (1LL << 20) ||
                     k - j >= CC_MAXSW || cap))
@@ -370,7 +372,7 @@ void build_plan() {
     // widen short strided tiles to ~4096 words so every thread has work
     int TL = TL0;
     while (j > 0 && TL * 2LL <= sw.s && P * TL < 4096 &&
-           (size_t)sw.lmax * P * TL * 2 * 4 <= (size_t)E.lds_budget)
+           (size_t)sw.lmax * cc_gplane((int)(P * TL * 2)) * 4 <= (size_t)E.lds_budget)
       TL *= 2;
     sw.TL = TL;
     sw.ncolb = (sw.s + sw.TL - 1) / sw.TL;
@@ -390,7 +392,7 @@ void build_plan() {
       sw.lds = one ? 0 : lds_r;
       sw.lds16 = one ? 0 : lds_r16;
     } else {
-      sw.lds = sw.lds16 = (size_t)sw.lmax * P * sw.TL * 4;
+      sw.lds = sw.lds16 = (size_t)sw.lmax * cc_gplane((int)(P * sw.TL)) * 4;
     }
     E.plan.push_back(sw);
     j = k;
