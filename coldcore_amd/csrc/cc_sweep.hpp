@@ -130,6 +130,8 @@ __device__ __forceinline__ void cc_lay_st(const SweepArgs &a, int d,
 
 // One tile of a sweep: stage, run every axis of the group in LDS, write
 // back (the batched transform calls it for tiles of many cells).
+#define CC_STAGE 8   // tile elements per thread whose loads are issued together
+
 template <int PROB, int R1, typename LT, bool PAD = true>
 __device__ __forceinline__ void cc_sweep_tile(const SweepArgs &a, long long tile,
                                               int32_t *__restrict__ S) {
@@ -142,21 +144,42 @@ __device__ __forceinline__ void cc_sweep_tile(const SweepArgs &a, long long tile
   const long long l0 = (tile - h * a.ncolb) * a.TL;
   const int ncol = (int)min((long long)a.TL, a.s - l0);
   const long long base = h * a.s * a.P + l0;
-  // ---- stage the tile: every live layer of every word, coalesced along tl
-  for (int e = threadIdx.x; e < PT; e += blockDim.x) {
-    const int p = e / a.TL, tl = e - p * a.TL;
-    const bool live = tl < ncol;
-    const long long w = base + (long long)p * a.s + tl;
-    const int ep = cc_gpad<PAD>(e);
-    if (a.in_kind == CC_IN_PRED)
-      S[ep] = (live && cc_pred(a.pmode, (int)a.cnt[w], 0, a.mu)) ? 1 : 0;
-    else if (a.in_kind == CC_IN_CODE)
-      S[ep] = live ? a.A[0][w] : 0;
-    else
-      S[ep] = live ? (int32_t)cc_lay_ld<LT>(a, 0, w) : 0;
-    for (int d = 1; d < Lmax; d++)
-      S[d * PTP + ep] = (live && d < Lin && a.in_kind == CC_IN_LAYERS)
-                            ? (int32_t)cc_lay_ld<LT>(a, d, w) : 0;
+  const uint32_t mTL = cc_magic((uint32_t)a.TL);
+  // ---- stage the tile: every live layer of every word, coalesced along tl;
+  // CC_STAGE elements per thread are loaded before any is stored to LDS, so
+  // their global loads are in flight together
+  constexpr int U = CC_STAGE;
+  for (int e0 = threadIdx.x; e0 < PT; e0 += U * blockDim.x) {
+    uint32_t val[U][R1];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int e = e0 + u * (int)blockDim.x;
+#pragma unroll
+      for (int d = 0; d < R1; d++) val[u][d] = 0u;
+      if (e >= PT) continue;
+      const int p = cc_div(e, mTL), tl = e - p * a.TL;
+      if (tl >= ncol) continue;
+      const long long w = base + (long long)p * a.s + tl;
+      if (a.in_kind == CC_IN_PRED)
+        val[u][0] = cc_pred(a.pmode, (int)a.cnt[w], 0, a.mu) ? 1u : 0u;
+      else if (a.in_kind == CC_IN_CODE)
+        val[u][0] = (uint32_t)a.A[0][w];
+      else
+        val[u][0] = cc_lay_ld<LT>(a, 0, w);
+      if (a.in_kind == CC_IN_LAYERS)
+#pragma unroll
+        for (int d = 1; d < R1; d++)
+          if (d < Lin) val[u][d] = cc_lay_ld<LT>(a, d, w);
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int e = e0 + u * (int)blockDim.x;
+      if (e >= PT) continue;
+      const int ep = cc_gpad<PAD>(e);
+#pragma unroll
+      for (int d = 0; d < R1; d++)
+        if (d < Lmax) S[d * PTP + ep] = (int32_t)val[u][d];
+    }
   }
   __syncthreads();
   // ---- every axis of the group, in LDS
@@ -175,7 +198,7 @@ __device__ __forceinline__ void cc_sweep_tile(const SweepArgs &a, long long tile
   }
   // ---- write back: layers, or the layer sum into the map / cnt plane
   for (int e = threadIdx.x; e < PT; e += blockDim.x) {
-    const int p = e / a.TL, tl = e - p * a.TL;
+    const int p = cc_div(e, mTL), tl = e - p * a.TL;
     if (tl >= ncol) continue;
     const long long w = base + (long long)p * a.s + tl;
     const int ep = cc_gpad<PAD>(e);
