@@ -279,24 +279,24 @@ int round_launch(int k) {
   const dim3 cg((B.nc + 63) / 64), cb(64);
   hipLaunchKernelGGL(k_ccb_ruin, cg, cb, 0, B.st, B.d, B.nc, k, B.d_ruin);
   CKB(hipGetLastError());
-  hipLaunchKernelGGL(k_ccb_walk, dim3(B.nc * CCB_KMAX), dim3(CC_BLOCK), 0, B.st,
-                     B.d, B.d_ruin);
+  hipLaunchKernelGGL(k_ccb_walk, dim3(B.nc * CCB_KMAX * CCB_WSPLIT), dim3(CC_BLOCK),
+                     0, B.st, B.d, B.d_ruin, CCB_WSPLIT);
   CKB(hipGetLastError());
   for (int j = 0; j < k; j++) {
     if (int rc = transform_launch(0)) return rc;
     if (int rc = reduce_launch(1)) return rc;
     hipLaunchKernelGGL(k_ccb_add, cg, cb, 0, B.st, B.d, B.nc, j, B.d_add);
     CKB(hipGetLastError());
-    hipLaunchKernelGGL(k_ccb_walk, dim3(B.nc), dim3(CC_BLOCK), 0, B.st, B.d,
-                       B.d_add);
+    hipLaunchKernelGGL(k_ccb_walk, dim3(B.nc * CCB_WSPLIT), dim3(CC_BLOCK), 0, B.st,
+                       B.d, B.d_add, CCB_WSPLIT);
     CKB(hipGetLastError());
   }
   if (int rc = reduce_launch(0)) return rc;
   hipLaunchKernelGGL(k_ccb_decide, cg, cb, 0, B.st, B.d, B.nc, B.d_rev);
   CKB(hipGetLastError());
   if (int rc = snapshot_launch()) return rc;
-  hipLaunchKernelGGL(k_ccb_walk, dim3(B.nc * 2 * CCB_KMAX), dim3(CC_BLOCK), 0,
-                     B.st, B.d, B.d_rev);
+  hipLaunchKernelGGL(k_ccb_walk, dim3(B.nc * 2 * CCB_KMAX * CCB_WSPLIT),
+                     dim3(CC_BLOCK), 0, B.st, B.d, B.d_rev, CCB_WSPLIT);
   CKB(hipGetLastError());
   return 0;
 }
@@ -354,7 +354,7 @@ int ccb_init(int ncells, const int *problem, const int *n, const int *axes,
     aoff += c.n;
     c.X = c.pw[c.n];
     c.blen = linf ? 0 : npats[i];
-    if (!linf && (c.blen < 1 || !pats)) { bfree_all(); return -1; }
+    if (!linf && (c.blen < 1 || c.blen > INT32_MAX || !pats)) { bfree_all(); return -1; }
     // 16-bit inter-sweep layers when every value fits: a layer value is a
     // count of seed words at one partial distance, at most the ball size for
     // the gain / loss indicators and the code size (<= cap) for a recount;
@@ -542,8 +542,8 @@ int ccb_greedy(int steps) {
     hipLaunchKernelGGL(k_ccb_add, dim3((B.nc + 63) / 64), dim3(64), 0, B.st, B.d,
                        B.nc, -1, B.d_add);
     CKB(hipGetLastError());
-    hipLaunchKernelGGL(k_ccb_walk, dim3(B.nc), dim3(CC_BLOCK), 0, B.st, B.d,
-                       B.d_add);
+    hipLaunchKernelGGL(k_ccb_walk, dim3(B.nc * CCB_WSPLIT), dim3(CC_BLOCK), 0, B.st,
+                       B.d, B.d_add, CCB_WSPLIT);
     CKB(hipGetLastError());
   }
   if (int rc = reduce_launch(0)) return rc;

@@ -16,6 +16,7 @@
 #include "cc_sweep.hpp"
 
 #define CCB_KMAX 8       // ruin size per LNS round
+#define CCB_WSPLIT 8     // workgroups per walked ball
 
 struct BCell {
   long long pw[CC_MAXN + 1];
@@ -211,40 +212,57 @@ __global__ void k_ccb_scatter(BCell *cells) {
 }
 
 // ---------------------------------------------------------------- walks
-// one workgroup per item: cnt += delta over the ball of w in its cell.
-// Hamming / Lee walk their support table; torus / grid enumerate the
-// (2R+1)^n L-inf offsets (the open grid drops the ones off the board).
+// split workgroups per item (blockIdx = item * split + segment), each taking
+// every split-th block of CC_BLOCK ball elements: cnt += delta over the ball
+// of w in its cell.  Hamming / Lee walk their support table; torus / grid
+// enumerate the (2R+1)^n L-inf offsets (the open grid drops the ones off the
+// board).  Cell sizes stay below 2^26 words and L-inf balls below 7^8
+// offsets, so all index arithmetic is 32-bit and the offset digits come
+// from high-multiply divisions (exact: offset * (2R + 1) < 2^32).
 __global__ void __launch_bounds__(CC_BLOCK)
-    k_ccb_walk(BCell *cells, const BItem *__restrict__ items) {
-  const BItem itm = items[blockIdx.x];
+    k_ccb_walk(const BCell *__restrict__ cells, const BItem *__restrict__ items,
+               int split) {
+  const int ii = blockIdx.x / split, seg = blockIdx.x - ii * split;
+  const BItem itm = items[ii];
   if (itm.delta == 0) return;
   const BCell &c = cells[itm.cell];
-  const long long w = itm.w;
+  const int n = c.n, R = c.R, prob = c.prob;
+  uint16_t *const cnt = c.cnt;
+  const int w = (int)itm.w;
   __shared__ long long disp[CC_MAXN * CC_CSTRIDE];
-  __shared__ int dig[CC_MAXN];
-  for (int p = threadIdx.x; p < c.n; p += blockDim.x)
-    dig[p] = (int)((w / c.pw[p]) % c.ax[p]);
+  __shared__ int dig[CC_MAXN], sax[CC_MAXN], spw[CC_MAXN];
+  for (int p = threadIdx.x; p < n; p += blockDim.x) {
+    const int pw = (int)c.pw[p], a = c.ax[p];
+    dig[p] = (w / pw) % a;
+    sax[p] = a;
+    spw[p] = pw;
+  }
   __syncthreads();
-  auto bump = [&](long long x) {
-    uint32_t *cell32 = (uint32_t *)(c.cnt + (x & ~1LL));
-    const int sh = (int)(x & 1) * 16;
-    if (itm.delta > 0) atomicAdd(cell32, 1u << sh);
-    else atomicSub(cell32, 1u << sh);
+  const bool add = itm.delta > 0;
+  auto bump = [&](int x) {
+    uint32_t *cell32 = (uint32_t *)(cnt + (x & ~1));
+    const uint32_t v = 1u << ((x & 1) * 16);
+    if (add) atomicAdd(cell32, v);
+    else atomicSub(cell32, v);
   };
-  if (c.prob == CC_TORUS || c.prob == CC_GRID) {
-    const int side = 2 * c.R + 1;
-    long long tot = 1;
-    for (int p = 0; p < c.n; p++) tot *= side;
-    for (long long t = threadIdx.x; t < tot; t += blockDim.x) {
-      long long x = 0, r = t;
+  const int t0 = seg * CC_BLOCK + (int)threadIdx.x, step = split * CC_BLOCK;
+  if (prob == CC_TORUS || prob == CC_GRID) {
+    const int side = 2 * R + 1;
+    const uint32_t ms = cc_magic((uint32_t)side);
+    int tot = 1;
+    for (int p = 0; p < n; p++) tot *= side;
+    const bool torus = prob == CC_TORUS;
+    for (int t = t0; t < tot; t += step) {
+      int x = 0, r = t;
       bool ok = true;
-      for (int p = 0; p < c.n; p++) {
-        const int e = (int)(r % side) - c.R;
-        r /= side;
-        int d = dig[p] + e;
-        if (c.prob == CC_TORUS) d = (d % c.ax[p] + c.ax[p]) % c.ax[p];
-        else if (d < 0 || d >= c.ax[p]) { ok = false; break; }
-        x += (long long)d * c.pw[p];
+      for (int p = 0; p < n; p++) {
+        const int q = cc_div(r, ms);
+        const int a = sax[p];
+        int d = dig[p] + (r - q * side) - R;
+        r = q;
+        if (torus) d += d < 0 ? a : (d >= a ? -a : 0);
+        else ok = ok && d >= 0 && d < a;
+        x += d * spw[p];
       }
       if (ok) bump(x);
     }
@@ -253,10 +271,10 @@ __global__ void __launch_bounds__(CC_BLOCK)
   for (int t = threadIdx.x; t < CC_MAXN * CC_CSTRIDE; t += blockDim.x) {
     const int p = t / CC_CSTRIDE, dl = t - p * CC_CSTRIDE + 1;
     long long off = 0;
-    if (p < c.n && dl < c.ax[p]) {
-      const int a = c.ax[p];
+    if (p < n && dl < sax[p]) {
+      const int a = sax[p];
       const int sh = (dig[p] + dl < a) ? dl : dl - a;
-      off = (long long)sh * c.pw[p];
+      off = (long long)sh * spw[p];
     }
     disp[t] = off;
   }
@@ -264,15 +282,17 @@ __global__ void __launch_bounds__(CC_BLOCK)
   const int FB = c.fmt == CC_FMT_V1 ? 8 : 10;
   const int NF = c.fmt == CC_FMT_V1 ? CC_MAXPAT : CC_MAXPAT_W;
   const unsigned FM = (1u << FB) - 1u;
-  for (long long t = threadIdx.x; t < c.blen; t += blockDim.x) {
-    const uint64_t pt = c.pat[t];
+  const uint64_t *__restrict__ pat = c.pat;
+  const int blen = (int)c.blen;
+  for (int t = t0; t < blen; t += step) {
+    const uint64_t pt = pat[t];
     long long x = w;
     for (int k = 0; k < NF; k++) {
       const unsigned b = (unsigned)(pt >> (FB * k)) & FM;
       if (!b) break;
       x += disp[(int)(b >> 4) * CC_CSTRIDE + (int)(b & 15u) - 1];
     }
-    bump(x);
+    bump((int)x);
   }
 }
 
