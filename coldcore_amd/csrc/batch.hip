@@ -257,10 +257,14 @@ int transform_launch(int kind) {
   return 0;
 }
 
-int reduce_launch(int with_map) {
-  hipLaunchKernelGGL(k_ccb_reset_red, dim3((B.nc + 63) / 64), dim3(64), 0, B.st,
-                     B.d, B.nc);
-  CKB(hipGetLastError());
+// reset = false when the previous control kernel (k_ccb_ruin, k_ccb_add)
+// has already cleared every cell's accumulators
+int reduce_launch(int with_map, bool reset = true) {
+  if (reset) {
+    hipLaunchKernelGGL(k_ccb_reset_red, dim3((B.nc + 63) / 64), dim3(64), 0, B.st,
+                       B.d, B.nc);
+    CKB(hipGetLastError());
+  }
   hipLaunchKernelGGL(k_ccb_reduce, dim3((unsigned)B.chunks.size()), dim3(CC_BLOCK),
                      0, B.st, B.d, B.d_chunks, with_map);
   CKB(hipGetLastError());
@@ -284,14 +288,14 @@ int round_launch(int k) {
   CKB(hipGetLastError());
   for (int j = 0; j < k; j++) {
     if (int rc = transform_launch(0)) return rc;
-    if (int rc = reduce_launch(1)) return rc;
+    if (int rc = reduce_launch(1, false)) return rc;
     hipLaunchKernelGGL(k_ccb_add, cg, cb, 0, B.st, B.d, B.nc, j, B.d_add);
     CKB(hipGetLastError());
     hipLaunchKernelGGL(k_ccb_walk, dim3(B.nc * CCB_WSPLIT), dim3(CC_BLOCK), 0, B.st,
                        B.d, B.d_add, CCB_WSPLIT);
     CKB(hipGetLastError());
   }
-  if (int rc = reduce_launch(0)) return rc;
+  if (int rc = reduce_launch(0, false)) return rc;
   hipLaunchKernelGGL(k_ccb_decide, cg, cb, 0, B.st, B.d, B.nc, B.d_rev);
   CKB(hipGetLastError());
   if (int rc = snapshot_launch()) return rc;

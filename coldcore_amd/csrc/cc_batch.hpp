@@ -307,6 +307,15 @@ __global__ void k_ccb_reset_red(BCell *cells, int ncells) {
 // index.  The chunk table sizes the grid by the words, not by the largest
 // cell times the cell count.
 #define CCB_CHUNK 16384
+__device__ __forceinline__ void ccb_key_max(unsigned long long &best, int32_t g,
+                                            long long i) {
+  if (g > 0) {
+    const unsigned long long key = ((unsigned long long)(uint32_t)g << 32) |
+                                   (0xFFFFFFFFull - (unsigned long long)i);
+    best = key > best ? key : best;
+  }
+}
+
 __global__ void __launch_bounds__(CC_BLOCK)
     k_ccb_reduce(BCell *cells, const long long *__restrict__ chunks, int with_map) {
   const long long ch = chunks[blockIdx.x];
@@ -314,17 +323,29 @@ __global__ void __launch_bounds__(CC_BLOCK)
   const long long w0 = ch & ((1LL << 40) - 1);
   BCell &c = cells[ci];
   const long long w1 = min(c.X, w0 + CCB_CHUNK);
+  const uint16_t *__restrict__ cnt = c.cnt;
+  const int32_t *__restrict__ map = c.A[0];
   unsigned long long z = 0, best = 0;
-  for (long long i = w0 + threadIdx.x; i < w1; i += CC_BLOCK) {
-    z += c.cnt[i] == 0;
+  // groups of 8 words: one 16-byte cnt load and two 16-byte map loads (w0 is
+  // a multiple of CCB_CHUNK, so every group is 16-byte aligned)
+  const int ng = (int)((w1 - w0) >> 3);
+  for (int g = threadIdx.x; g < ng; g += CC_BLOCK) {
+    const long long i = w0 + 8LL * g;
+    const uint4 cv = *(const uint4 *)(cnt + i);
+    const uint32_t cw[4] = {cv.x, cv.y, cv.z, cv.w};
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+      z += ((cw[k] & 0xFFFFu) == 0u) + ((cw[k] >> 16) == 0u);
     if (with_map) {
-      const int32_t g = c.A[0][i];
-      if (g > 0) {
-        const unsigned long long key = ((unsigned long long)(uint32_t)g << 32) |
-                                       (0xFFFFFFFFull - (unsigned long long)i);
-        best = key > best ? key : best;
-      }
+      const int4 a0 = *(const int4 *)(map + i), a1 = *(const int4 *)(map + i + 4);
+      const int32_t gv[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+#pragma unroll
+      for (int k = 0; k < 8; k++) ccb_key_max(best, gv[k], i + k);
     }
+  }
+  for (long long i = w0 + 8LL * ng + threadIdx.x; i < w1; i += CC_BLOCK) {
+    z += cnt[i] == 0;
+    if (with_map) ccb_key_max(best, map[i], i);
   }
   const long long zs = cc_block_sum((long long)z);
   for (int o = 32; o > 0; o >>= 1) {
@@ -360,7 +381,9 @@ __device__ __forceinline__ unsigned long long ccb_next(BCell &c) {
 }
 
 // ruin: remove k distinct random words (a partial Fisher-Yates from the
-// tail); a feasible cell will re-add one fewer (notch descent)
+// tail); a feasible cell will re-add one fewer (notch descent).  Ruin and
+// add leave amax / nzero cleared, so the reductions of a round need no
+// separate reset launch.
 __global__ void k_ccb_ruin(BCell *cells, int ncells, int k, BItem *items) {
   const int ci = blockIdx.x * blockDim.x + threadIdx.x;
   if (ci >= ncells) return;
@@ -385,6 +408,8 @@ __global__ void k_ccb_ruin(BCell *cells, int ncells, int k, BItem *items) {
     items[ci * CCB_KMAX + j] = it;
   }
   c.M -= c.nrem;
+  c.amax = 0ull;   // clear the accumulators for the round's first reduction
+  c.nzero = 0ull;
 }
 
 // recreate step j: add the word of largest gain (smallest index on ties).
@@ -404,6 +429,8 @@ __global__ void k_ccb_add(BCell *cells, int ncells, int j, BItem *items) {
     it.w = w;
   }
   items[ci] = it;
+  c.amax = 0ull;   // read: clear for the next reduction
+  c.nzero = 0ull;
 }
 
 // accept / revert: a round that uncovers more words is undone with
