@@ -98,18 +98,40 @@ int live_after(const BCell &c, int upto) {
 // the cell's sweeps: groups of consecutive axes whose tile (L live planes
 // x P words x TL columns, int32) fits the LDS budget; the first group is
 // contiguous (TL = 1), later ones stage up to 32 columns
+// axes per register round for a homogeneous Hamming group of axis size q
+// (cube of q^RR <= 16 words), 0 where the LDS tile runs
+int round_axes_b(const BCell &c, int q) {
+  if (c.prob != CC_HAMMING || !c.l16 || getenv("CCB_NO_ROUND")) return 0;
+  return q == 2 ? 4 : (q == 3 || q == 4 ? 2 : 0);
+}
+
+// LDS bytes of a round tile of pt words: layer pairs packed per dword, one
+// pad word per 16
+long long round_lds(const BCell &c, long long pt) {
+  return (long long)((c.L + 1) / 2) * (pt + (pt >> 4) + 1) * 4;
+}
+
 void plan_cell(int ci, const BCell &c) {
   int j = 0;
   while (j < c.n) {
     const long long s = c.pw[j];
     const int TL = j == 0 ? 1 : (int)std::min<long long>(32, s);
+    // a homogeneous Hamming group of 16-bit layers is sized by the round
+    // tile's packed footprint; binary / quaternary groups stop at 4,096
+    // words (three rounds)
+    const int q = c.ax[j], rrq = round_axes_b(c, q);
+    bool homog = true;
     long long P = 1;
     int k = j;
     while (k < c.n) {
       const long long P2 = P * c.ax[k];
-      if (k > j && ((long long)c.L * P2 * TL * 4 > B.budget || k - j >= CC_MAXSW))
-        break;
+      const bool hk = homog && c.ax[k] == q;
+      const bool packed = rrq > 0 && hk;
+      const long long need = packed ? round_lds(c, P2 * TL) : (long long)c.L * P2 * TL * 4;
+      const bool cap = packed && q != 3 && P2 * TL > 4096;
+      if (k > j && (need > B.budget || k - j >= CC_MAXSW || cap)) break;
       P = P2;
+      homog = hk;
       k++;
     }
     // strided groups: widen the tile to fill the budget (a group of one
@@ -124,7 +146,7 @@ void plan_cell(int ci, const BCell &c) {
     memset(&a, 0, sizeof(a));
     for (int d = 0; d < c.L; d++) {
       a.A[d] = c.A[d];
-      a.H[d] = c.l16 ? c.H + (size_t)d * c.X : nullptr;
+      a.H[d] = c.l16 ? c.H + (size_t)d * ccb_hpitch(c.X) : nullptr;
     }
     a.cnt = c.cnt;
     a.s = s;
@@ -143,6 +165,7 @@ void plan_cell(int ci, const BCell &c) {
       sk *= a.Q[i];
     }
     for (int i = 0; i <= a.nax; i++) a.live[i] = live_after(c, j + i);
+    a.rr = homog && a.nax >= rrq ? rrq : 0;
     const bool first = j == 0, last = k == c.n;
     for (int kind = 0; kind < 3; kind++) {
       SweepArgs b = a;
@@ -156,7 +179,10 @@ void plan_cell(int ci, const BCell &c) {
     }
     B.argcell.push_back(ci);
     B.need.push_back((int)a.ntiles);
-    B.lds = std::max(B.lds, (size_t)c.L * cc_gplane<false>((int)(P * TLw)) * 4);
+    if (a.rr)   // one round: registers only
+      B.lds = std::max(B.lds, a.nax > a.rr ? (size_t)round_lds(c, P * TLw) : (size_t)0);
+    else
+      B.lds = std::max(B.lds, (size_t)c.L * cc_gplane<false>((int)(P * TLw)) * 4);
     j = k;
   }
 }
@@ -392,7 +418,7 @@ int ccb_init(int ncells, const int *problem, const int *n, const int *axes,
     // device planes
     for (int d = 0; d < c.L; d++)
       if (balloc(&c.A[d], (size_t)c.X * 4)) { bfree_all(); return -3; }
-    if (c.l16 && balloc(&c.H, (size_t)c.L * c.X * 2)) { bfree_all(); return -3; }
+    if (c.l16 && balloc(&c.H, (size_t)c.L * ccb_hpitch(c.X) * 2)) { bfree_all(); return -3; }
     if (balloc(&c.cnt, (size_t)((c.X + 1) & ~1LL) * 2) ||
         balloc(&c.code, (size_t)cap * 8) || balloc(&c.best, (size_t)cap * 8))
       { bfree_all(); return -3; }

@@ -43,6 +43,10 @@ struct BCell {
   unsigned long long nzero;  // words with cnt == 0
 };
 
+// pitch (words) of the 16-bit layer planes: a multiple of 8, so every
+// plane starts 16-byte aligned
+inline long long ccb_hpitch(long long X) { return (X + 7) & ~7LL; }
+
 struct BItem {
   int cell, delta;   // delta 0: no-op slot
   long long w;
@@ -161,10 +165,27 @@ __device__ __forceinline__ void ccb_tile_t(const SweepArgs &a, int prob, int L,
   }
 }
 
+// homogeneous Hamming groups of 16-bit cells (axis size 2, 3 or 4, at
+// least rr axes): register rounds of rr axes per LDS round trip
+template <int R1>
+__device__ __forceinline__ void ccb_tile_round(const SweepArgs &a, long long tile,
+                                               int32_t *S) {
+  if (a.Q[0] == 2) cc_round_tile<CC_HAMMING, R1, 2, 4, uint16_t>(a, tile, S);
+  else if (a.Q[0] == 3) cc_round_tile<CC_HAMMING, R1, 3, 2, uint16_t>(a, tile, S);
+  else cc_round_tile<CC_HAMMING, R1, 4, 2, uint16_t>(a, tile, S);
+}
+
 __device__ __forceinline__ void ccb_tile(const SweepArgs &a, const BCell &c,
                                          long long tile, int32_t *S) {
-  if (c.l16) ccb_tile_t<uint16_t>(a, c.prob, c.L, tile, S);
-  else ccb_tile_t<int32_t>(a, c.prob, c.L, tile, S);
+  if (a.rr) {
+    if (c.L == 2) ccb_tile_round<2>(a, tile, S);
+    else if (c.L == 3) ccb_tile_round<3>(a, tile, S);
+    else ccb_tile_round<4>(a, tile, S);
+  } else if (c.l16) {
+    ccb_tile_t<uint16_t>(a, c.prob, c.L, tile, S);
+  } else {
+    ccb_tile_t<int32_t>(a, c.prob, c.L, tile, S);
+  }
 }
 
 // items[i] = (sweep argument index, tile), all of one sweep level

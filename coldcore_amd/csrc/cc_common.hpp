@@ -115,6 +115,7 @@ struct SweepArgs {
   // after the group): min(sum of the per-axis maximal distances so far, R)+1,
   // one per axis for Hamming and floor(ax/2) for Lee
   int live[CC_MAXSW + 1];
+  int rr;            // batched sweeps: axes per register round (0: LDS tile)
 };
 
 // LDS layout of a generic tile: layer plane d starts at d * cc_gplane(PT),

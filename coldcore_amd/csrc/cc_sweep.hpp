@@ -478,9 +478,11 @@ __device__ __forceinline__ void cc_cube_apply(uint32_t (&v)[R1][C], int ilo,
   }
 }
 
+// one tile of the round sweep (the batched transform calls it for the
+// homogeneous Hamming groups of many cells)
 template <int PROB, int R1, int Q, int RR, typename LT>
-__global__ void __launch_bounds__(CC_BLOCK) k_sweep_r(SweepArgs a) {
-  extern __shared__ int32_t S[];
+__device__ __forceinline__ void cc_round_tile(const SweepArgs &a, long long tile,
+                                              int32_t *__restrict__ S) {
   constexpr int C = RR == 1 ? Q : (RR == 2 ? Q * Q : (RR == 3 ? Q * Q * Q : Q * Q * Q * Q));
   const int m = a.nax;
   const int PT = a.P * a.TL;
@@ -489,47 +491,52 @@ __global__ void __launch_bounds__(CC_BLOCK) k_sweep_r(SweepArgs a) {
   const int Lin = a.live[0];
   const int nrounds = (m + RR - 1) / RR;
   const int ncube = PT / C;
-  for (long long tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
-    const long long h = tile / a.ncolb;
-    const long long l0 = (tile - h * a.ncolb) * a.TL;
-    const int ncol = (int)min((long long)a.TL, a.s - l0);
-    const long long base = h * a.s * a.P + l0;
-    // ---- round 0: HBM -> registers (cube = group axes 0..RR-1)
-    {
-      const int ca = min(0, m - RR);
-      for (int cu = threadIdx.x; cu < ncube; cu += CC_BLOCK) {
-        const int tl = cu % a.TL, pb = (cu / a.TL) * C;
-        uint32_t v[R1][C];
-        cc_cube_load<R1, C, LT>(a, base, pb, tl, tl < ncol, Lin, v);
-        cc_cube_apply<PROB, R1, Q, C, RR>(v, -ca, a.R);
-        if (nrounds == 1) {
-          if (tl < ncol) cc_cube_store<R1, C, LT>(a, base, pb, 1, tl, Lmax, v);
-        } else {
-          cc_cube_lds_st<R1, C, LT>(S, PTP, pb, 1, a.TL, tl, v);
-        }
+  const long long h = tile / a.ncolb;
+  const long long l0 = (tile - h * a.ncolb) * a.TL;
+  const int ncol = (int)min((long long)a.TL, a.s - l0);
+  const long long base = h * a.s * a.P + l0;
+  // ---- round 0: HBM -> registers (cube = group axes 0..RR-1)
+  {
+    const int ca = min(0, m - RR);
+    for (int cu = threadIdx.x; cu < ncube; cu += CC_BLOCK) {
+      const int tl = cu % a.TL, pb = (cu / a.TL) * C;
+      uint32_t v[R1][C];
+      cc_cube_load<R1, C, LT>(a, base, pb, tl, tl < ncol, Lin, v);
+      cc_cube_apply<PROB, R1, Q, C, RR>(v, -ca, a.R);
+      if (nrounds == 1) {
+        if (tl < ncol) cc_cube_store<R1, C, LT>(a, base, pb, 1, tl, Lmax, v);
+      } else {
+        cc_cube_lds_st<R1, C, LT>(S, PTP, pb, 1, a.TL, tl, v);
       }
-      __syncthreads();
     }
-    // ---- rounds 1..: LDS -> registers -> LDS, the last one -> HBM
-    for (int rd = 1; rd < nrounds; rd++) {
-      const bool last = rd == nrounds - 1;
-      const int lo = rd * RR;                     // first axis of this round
-      const int ca = min(lo, m - RR);             // cube's lowest group axis
-      int qa = 1;                                 // Q^ca: in-tile stride
-      for (int i = 0; i < ca; i++) qa *= Q;
-      for (int cu = threadIdx.x; cu < ncube; cu += CC_BLOCK) {
-        const int tl = cu % a.TL, u = cu / a.TL;
-        const int pb = (u / qa) * qa * C + (u % qa);
-        uint32_t v[R1][C];
-        cc_cube_lds_ld<R1, C, LT>(S, PTP, pb, qa, a.TL, tl, v);
-        cc_cube_apply<PROB, R1, Q, C, RR>(v, lo - ca, a.R);
-        if (last) {
-          if (tl < ncol) cc_cube_store<R1, C, LT>(a, base, pb, qa, tl, Lmax, v);
-        } else {
-          cc_cube_lds_st<R1, C, LT>(S, PTP, pb, qa, a.TL, tl, v);
-        }
-      }
-      __syncthreads();
-    }
+    __syncthreads();
   }
+  // ---- rounds 1..: LDS -> registers -> LDS, the last one -> HBM
+  for (int rd = 1; rd < nrounds; rd++) {
+    const bool last = rd == nrounds - 1;
+    const int lo = rd * RR;                     // first axis of this round
+    const int ca = min(lo, m - RR);             // cube's lowest group axis
+    int qa = 1;                                 // Q^ca: in-tile stride
+    for (int i = 0; i < ca; i++) qa *= Q;
+    for (int cu = threadIdx.x; cu < ncube; cu += CC_BLOCK) {
+      const int tl = cu % a.TL, u = cu / a.TL;
+      const int pb = (u / qa) * qa * C + (u % qa);
+      uint32_t v[R1][C];
+      cc_cube_lds_ld<R1, C, LT>(S, PTP, pb, qa, a.TL, tl, v);
+      cc_cube_apply<PROB, R1, Q, C, RR>(v, lo - ca, a.R);
+      if (last) {
+        if (tl < ncol) cc_cube_store<R1, C, LT>(a, base, pb, qa, tl, Lmax, v);
+      } else {
+        cc_cube_lds_st<R1, C, LT>(S, PTP, pb, qa, a.TL, tl, v);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <int PROB, int R1, int Q, int RR, typename LT>
+__global__ void __launch_bounds__(CC_BLOCK) k_sweep_r(SweepArgs a) {
+  extern __shared__ int32_t S[];
+  for (long long tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x)
+    cc_round_tile<PROB, R1, Q, RR, LT>(a, tile, S);
 }
