@@ -142,6 +142,22 @@ void plan_cell(int ci, const BCell &c) {
       if (tl >= 64) tl = tl / 32 * 32;
       TLw = (int)std::max<long long>(TL, tl);
     }
+    const int rr = homog && k - j >= rrq ? rrq : 0;
+    // contiguous groups that stopped short of the budget carry F "passenger"
+    // rows of the axes above (F divides the words above the group; the tile
+    // is F consecutive groups, transformed along the group axes only), so a
+    // tile is not a few hundred words
+    if (j == 0 && k < c.n && getenv("CCB_NO_ROWS") == nullptr) {
+      const long long rest = c.X / P;
+      long long F = 1;
+      for (long long f = 2; f <= rest && P * f <= 16384; f++) {
+        if (rest % f) continue;
+        const long long need = rr ? (k - j > rr ? round_lds(c, P * f) : 0)
+                                  : (long long)c.L * P * f * 4;
+        if (need <= B.budget) F = f;
+      }
+      P *= F;
+    }
     SweepArgs a;
     memset(&a, 0, sizeof(a));
     for (int d = 0; d < c.L; d++) {
@@ -165,7 +181,7 @@ void plan_cell(int ci, const BCell &c) {
       sk *= a.Q[i];
     }
     for (int i = 0; i <= a.nax; i++) a.live[i] = live_after(c, j + i);
-    a.rr = homog && a.nax >= rrq ? rrq : 0;
+    a.rr = rr;
     const bool first = j == 0, last = k == c.n;
     for (int kind = 0; kind < 3; kind++) {
       SweepArgs b = a;
