@@ -238,6 +238,16 @@ int build_plan(int budget) {
       for (int t = 0; t < B.need[a]; t++) B.items.push_back(make_int2(a, t));
     }
     if (!any) break;
+    // largest tiles first (the level ends with its last tile, so the
+    // longest ones are taken while every workgroup is still busy)
+    if (!getenv("CCB_NO_LPT")) {
+      auto cost = [&](const int2 &it) {
+        const SweepArgs &sa = B.args[0][it.x];
+        return (long long)sa.P * sa.TL * B.h[B.argcell[it.x]].L * (sa.nax + 2);
+      };
+      std::stable_sort(B.items.begin() + off, B.items.end(),
+                       [&](const int2 &x, const int2 &y) { return cost(x) > cost(y); });
+    }
     B.level_off.push_back(off);
   }
   B.nlevels = (int)B.level_off.size();
