@@ -89,7 +89,7 @@ class Searcher:
     """Ruin-and-recreate search over one backend (search.py:170-257)."""
 
     def __init__(self, backend, seed=1, deadline=None, on_log=None,
-                 on_round=None):
+                 on_round=None, on_solved=None):
         self.b = backend
         self.rng = np.random.default_rng(seed * 1000003 + 12345)
         self.deadline = deadline
@@ -97,6 +97,9 @@ class Searcher:
         #: called after every LNS round with (searcher, record) -- the move
         #: log the learned proposal trains on
         self.on_round = on_round
+        #: called with (M, code) whenever a pass establishes a feasible code
+        #: of a new size (the checkpoint hook, reference search.py:35, 188)
+        self.on_solved = on_solved or (lambda m, code: None)
         self.code = np.zeros(0, dtype=np.int64)
         self.uncov = None
         self.best_code = None
@@ -475,6 +478,7 @@ def descend_rounds(searcher, rounds, kmin=8, select="fixed"):
         used += s.stats["rounds"] - before
         if s.uncov == 0:
             sizes.append((used, len(s.code)))
+            s.on_solved(len(s.code), s.code.copy())
     return used, sizes
 
 
