@@ -53,94 +53,12 @@ struct BItem {
 };
 
 // ---------------------------------------------------------------- transform
-// runtime-Q window sums for the L-inf families with axes longer than the
-// unrolled table (king tori up to 32 x 32): one fiber per thread, in place,
-// with a shift register of the 2R+1 original values under the window and
-// the first R originals kept for the circular wrap (no indexed arrays, so
-// nothing spills to scratch)
-template <int R, bool TORUS>
-__device__ __forceinline__ void ccb_step_linf_rt(int32_t *__restrict__ S, int TL,
-                                                 int nf, int sk, int Q) {
-  const uint32_t mT = cc_magic((uint32_t)TL), mS = cc_magic((uint32_t)sk);
-  for (int fi = threadIdx.x; fi < nf; fi += blockDim.x) {
-    const int f = cc_div(fi, mT), tl = fi - f * TL;
-    const int hi = cc_div(f, mS), lo = f - hi * sk;
-    const int e0 = (hi * sk * Q + lo) * TL + tl, es = sk * TL;
-    uint32_t head[R], win[2 * R + 1];
-#pragma unroll
-    for (int i = 0; i < R; i++) head[i] = (uint32_t)S[cc_gpad<false>(e0 + i * es)];
-#pragma unroll
-    for (int i = 0; i <= 2 * R; i++) {
-      const int b = i - R;   // -R .. R  (Q >= 2R+1 for the torus)
-      if (b < 0) win[i] = TORUS ? (uint32_t)S[cc_gpad<false>(e0 + (b + Q) * es)] : 0u;
-      else win[i] = b < Q ? (uint32_t)S[cc_gpad<false>(e0 + b * es)] : 0u;
-    }
-    for (int a = 0; a < Q; a++) {
-      uint32_t s = 0;
-#pragma unroll
-      for (int i = 0; i <= 2 * R; i++) s += win[i];
-      S[cc_gpad<false>(e0 + a * es)] = (int32_t)s;
-      // slide: the next window ends at a + 1 + R
-      const int b = a + 1 + R;
-      uint32_t nx = 0;
-      if (b < Q) {
-        nx = (uint32_t)S[cc_gpad<false>(e0 + b * es)];
-      } else if (TORUS) {
-#pragma unroll
-        for (int i = 0; i < R; i++) nx = (b - Q == i) ? head[i] : nx;
-      }
-#pragma unroll
-      for (int i = 0; i < 2 * R; i++) win[i] = win[i + 1];
-      win[2 * R] = nx;
-    }
-  }
-}
-
 template <int PROB>
 __device__ __forceinline__ void ccb_tile_linf(const SweepArgs &a, long long tile,
                                               int32_t *__restrict__ S) {
-  bool big = false;
-  for (int k = 0; k < a.nax; k++) big = big || a.Q[k] > 16;
-  if (!big) {
-    cc_sweep_tile<PROB, 1, int32_t, false>(a, tile, S);
-    return;
-  }
-  // same stage / write-back as cc_sweep_tile, axes by the runtime-Q path
-  const int PT = a.P * a.TL;
-  const long long h = tile / a.ncolb;
-  const long long l0 = (tile - h * a.ncolb) * a.TL;
-  const int ncol = (int)min((long long)a.TL, a.s - l0);
-  const long long base = h * a.s * a.P + l0;
-  for (int e = threadIdx.x; e < PT; e += blockDim.x) {
-    const int p = e / a.TL, tl = e - p * a.TL;
-    const bool live = tl < ncol;
-    const long long w = base + (long long)p * a.s + tl;
-    if (a.in_kind == CC_IN_PRED)
-      S[cc_gpad<false>(e)] = (live && cc_pred(a.pmode, (int)a.cnt[w], 0, a.mu)) ? 1 : 0;
-    else
-      S[cc_gpad<false>(e)] = live ? a.A[0][w] : 0;
-  }
-  __syncthreads();
-  for (int k = 0; k < a.nax; k++) {
-    const int Q = a.Q[k];
-    const int nf = (a.P / Q) * a.TL;
-    constexpr bool T = PROB == CC_TORUS;
-    if (a.R == 1) ccb_step_linf_rt<1, T>(S, a.TL, nf, a.sk[k], Q);
-    else if (a.R == 2) ccb_step_linf_rt<2, T>(S, a.TL, nf, a.sk[k], Q);
-    else ccb_step_linf_rt<3, T>(S, a.TL, nf, a.sk[k], Q);
-    __syncthreads();
-  }
-  for (int e = threadIdx.x; e < PT; e += blockDim.x) {
-    const int p = e / a.TL, tl = e - p * a.TL;
-    if (tl >= ncol) continue;
-    const long long w = base + (long long)p * a.s + tl;
-    const int32_t x = S[cc_gpad<false>(e)];
-    if (a.out_kind == CC_OUT_LAYERS) a.A[0][w] = x;
-    else if (a.out_kind == CC_OUT_MAP) a.A[0][w] = x;
-    else a.cnt[w] = x > 65535 ? (uint16_t)65535 : (uint16_t)x;
-  }
-  __syncthreads();
+  cc_sweep_tile<PROB, 1, int32_t, false>(a, tile, S);
 }
+
 
 // LT: inter-sweep layer planes int32 (A) or uint16 (H, cells whose layer
 // values provably fit 16 bits; never with the runtime-length window path)

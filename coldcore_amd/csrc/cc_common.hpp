@@ -22,6 +22,9 @@
 // support-table formats the walks decode
 #define CC_FMT_V1 0     // uint64, byte k = pos<<4 | delta (pos < 16), <= 8
 #define CC_FMT_WIDE 1   // uint64, bits [10k, 10k+10) = pos<<4 | delta, <= 6
+#define CC_FMT_X16 2    // uint64, bits [16k, 16k+16) = pos<<10 | delta, <= 4
+#define CC_MAXPAT_X 4   // 16-bit fields per long-axis pattern
+#define CC_MAXAX_X 1024 // L-inf axis size reachable with the long-axis format
 #define CC_CSTRIDE 15   // per-position slots of the walk displacement table
 #define CC_BLOCK 256    // threads per workgroup (4 waves of 64)
 

@@ -183,10 +183,11 @@ __global__ void __launch_bounds__(CC_BLOCK)
                 long long lcap, unsigned long long *lcount) {
   __shared__ long long disp[CC_MAXN * CC_CSTRIDE];
   __shared__ unsigned char okf[CC_MAXN * CC_CSTRIDE];
-  __shared__ int digs[CC_MAXN];
+  __shared__ int digs[CC_MAXN], sax[CC_MAXN];
+  __shared__ long long spw[CC_MAXN];
   const int di = blockIdx.x / split, seg = blockIdx.x - di * split;
   const long long w = drops[di];
-  cc_walk_disp(g, w, disp, okf, digs);
+  cc_walk_disp(g, w, disp, okf, digs, sax, spw);
   __syncthreads();
   long long acc = 0;
   const long long step = (long long)split * CC_BLOCK;
@@ -194,7 +195,8 @@ __global__ void __launch_bounds__(CC_BLOCK)
     const long long t = base + threadIdx.x;
     bool ok = false;
     long long x = 0;
-    if (t < blen) x = cc_walk_word<GRID, FMT>(pat[t], w, disp, okf, &ok);
+    if (t < blen)
+      x = cc_walk_word<GRID, FMT>(pat[t], w, disp, okf, &ok, digs, sax, spw, g.R);
     const int c = ok ? (int)cnt[x] : -1;
     acc += (c >= 1 && c <= g.mu) ? 1 : 0;
     const bool hit = c == g.mu;

@@ -109,6 +109,50 @@ struct CcQRange {
   static constexpr int hi = PROB == CC_HAMMING ? 10 : 16;
 };
 
+// runtime-Q window sums for the L-inf families with axes longer than the
+// unrolled table (axes above 16, R <= 3): one fiber per thread, in place,
+// with a shift register of the 2R+1 original values under the window and
+// the first R originals kept for the circular wrap (no indexed arrays, so
+// nothing spills to scratch)
+template <int R, bool TORUS, bool PAD>
+__device__ __forceinline__ void cc_step_linf_rt(int32_t *__restrict__ S, int TL,
+                                                 int nf, int sk, int Q) {
+  const uint32_t mT = cc_magic((uint32_t)TL), mS = cc_magic((uint32_t)sk);
+  for (int fi = threadIdx.x; fi < nf; fi += blockDim.x) {
+    const int f = cc_div(fi, mT), tl = fi - f * TL;
+    const int hi = cc_div(f, mS), lo = f - hi * sk;
+    const int e0 = (hi * sk * Q + lo) * TL + tl, es = sk * TL;
+    uint32_t head[R], win[2 * R + 1];
+#pragma unroll
+    for (int i = 0; i < R; i++) head[i] = (uint32_t)S[cc_gpad<PAD>(e0 + i * es)];
+#pragma unroll
+    for (int i = 0; i <= 2 * R; i++) {
+      const int b = i - R;   // -R .. R  (Q >= 2R+1 for the torus)
+      if (b < 0) win[i] = TORUS ? (uint32_t)S[cc_gpad<PAD>(e0 + (b + Q) * es)] : 0u;
+      else win[i] = b < Q ? (uint32_t)S[cc_gpad<PAD>(e0 + b * es)] : 0u;
+    }
+    for (int a = 0; a < Q; a++) {
+      uint32_t s = 0;
+#pragma unroll
+      for (int i = 0; i <= 2 * R; i++) s += win[i];
+      S[cc_gpad<PAD>(e0 + a * es)] = (int32_t)s;
+      // slide: the next window ends at a + 1 + R
+      const int b = a + 1 + R;
+      uint32_t nx = 0;
+      if (b < Q) {
+        nx = (uint32_t)S[cc_gpad<PAD>(e0 + b * es)];
+      } else if (TORUS) {
+#pragma unroll
+        for (int i = 0; i < R; i++) nx = (b - Q == i) ? head[i] : nx;
+      }
+#pragma unroll
+      for (int i = 0; i < 2 * R; i++) win[i] = win[i + 1];
+      win[2 * R] = nx;
+    }
+  }
+}
+
+
 // Inter-sweep layer planes are int32 (LT = int32_t, planes A[d]) or, when
 // every layer value is provably below 2^16, uint16 (LT = uint16_t, planes
 // H[d]).  A layer value is a count of solution elements at one partial
@@ -192,7 +236,15 @@ __device__ __forceinline__ void cc_sweep_tile(const SweepArgs &a, long long tile
       CC_STEP_CASE(6) CC_STEP_CASE(7) CC_STEP_CASE(8) CC_STEP_CASE(9)
       CC_STEP_CASE(10) CC_STEP_CASE(11) CC_STEP_CASE(12) CC_STEP_CASE(13)
       CC_STEP_CASE(14) CC_STEP_CASE(15) CC_STEP_CASE(16)
-      default: break;
+      default:
+        // L-inf axes above 16 (R <= 3): the runtime-length window
+        if constexpr (PROB == CC_TORUS || PROB == CC_GRID) {
+          constexpr bool T = PROB == CC_TORUS;
+          if (a.R == 1) cc_step_linf_rt<1, T, PAD>(S, a.TL, nf, sk, Q);
+          else if (a.R == 2) cc_step_linf_rt<2, T, PAD>(S, a.TL, nf, sk, Q);
+          else cc_step_linf_rt<3, T, PAD>(S, a.TL, nf, sk, Q);
+        }
+        break;
     }
     __syncthreads();
   }

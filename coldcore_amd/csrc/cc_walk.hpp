@@ -7,9 +7,12 @@
 // 158-212).  The open grid resolves the same circular delta against the
 // centre digit and drops a pattern whose displacement leaves the L-inf
 // radius, which is a wrap past the board edge (dctcore_grid.cuh:239-286).
-// Two packed formats are decoded: v1 (8-bit pos<<4|delta fields, pos < 16,
-// the reference's wire format) and wide (10-bit fields, pos < 64, at most 6
-// per pattern) for the Hamming / Lee cells with more than 16 axes.
+// Three packed formats are decoded: v1 (8-bit pos<<4|delta fields, pos < 16,
+// the reference's wire format), wide (10-bit fields, pos < 64, at most 6
+// per pattern) for the Hamming / Lee cells with more than 16 axes, and
+// long-axis (16-bit pos<<10|delta fields, at most 4) for the L-inf cells
+// with axes longer than 16, whose displacement is computed per field from
+// the centre digit (a table of every (position, delta) would not fit).
 //
 // One workgroup walks one contiguous segment of one word's table
 // (blockIdx = word * split + segment).  The displacement table lives in LDS
@@ -36,8 +39,14 @@ struct Geom {
 // caller synchronises after it
 __device__ __forceinline__ void cc_walk_disp(const Geom &g, long long w,
                                              long long *disp,
-                                             unsigned char *okf, int *digs) {
+                                             unsigned char *okf, int *digs,
+                                             int *sax = nullptr,
+                                             long long *spw = nullptr) {
   for (int p = threadIdx.x; p < g.n; p += blockDim.x) {
+    if (sax) {
+      sax[p] = g.ax[p];
+      spw[p] = g.pw[p];
+    }
     int d;
     if (w < (1LL << 32)) {
       const uint32_t w32 = (uint32_t)w;
@@ -71,9 +80,27 @@ template <bool GRID, int FMT>
 __device__ __forceinline__ long long cc_walk_word(uint64_t pt, long long w,
                                                   const long long *disp,
                                                   const unsigned char *okf,
-                                                  bool *ok) {
+                                                  bool *ok,
+                                                  const int *digs = nullptr,
+                                                  const int *sax = nullptr,
+                                                  const long long *spw = nullptr,
+                                                  int R = 0) {
   long long x = w;
   bool good = true;
+  if constexpr (FMT == CC_FMT_X16) {
+#pragma unroll
+    for (int k = 0; k < CC_MAXPAT_X; k++) {
+      const unsigned b = (unsigned)(pt >> (16 * k)) & 0xFFFFu;
+      if (!b) break;
+      const int pos = (int)(b >> 10), dl = (int)(b & 1023u);
+      const int a = sax[pos];
+      const int sh = digs[pos] + dl < a ? dl : dl - a;
+      x += (long long)sh * spw[pos];
+      if (GRID) good = good && sh <= R && sh >= -R;
+    }
+    *ok = good;
+    return x;
+  }
   constexpr int FB = FMT == CC_FMT_V1 ? 8 : 10;
   constexpr int NF = FMT == CC_FMT_V1 ? CC_MAXPAT : CC_MAXPAT_W;
   constexpr unsigned FM = (1u << FB) - 1u;
@@ -97,17 +124,19 @@ __global__ void __launch_bounds__(CC_BLOCK)
            unsigned long long *trans) {
   __shared__ long long disp[CC_MAXN * CC_CSTRIDE];
   __shared__ unsigned char okf[CC_MAXN * CC_CSTRIDE];
-  __shared__ int digs[CC_MAXN];
+  __shared__ int digs[CC_MAXN], sax[CC_MAXN];
+  __shared__ long long spw[CC_MAXN];
   const int wi = blockIdx.x / split, seg = blockIdx.x - wi * split;
   const long long w = wptr ? wptr[wi] : wa.w[wi];
-  cc_walk_disp(g, w, disp, okf, digs);
+  cc_walk_disp(g, w, disp, okf, digs, sax, spw);
   __syncthreads();
   long long acc = 0;
   const long long step = (long long)split * CC_BLOCK;
   for (long long t = (long long)seg * CC_BLOCK + threadIdx.x; t < blen;
        t += step) {
     bool ok;
-    const long long x = cc_walk_word<GRID, FMT>(pat[t], w, disp, okf, &ok);
+    const long long x = cc_walk_word<GRID, FMT>(pat[t], w, disp, okf, &ok, digs,
+                                                sax, spw, g.R);
     if (GRID && !ok) continue;
     if (KIND == CC_WALK_GATHER) {
       acc += cc_pred(pmode, (int)cnt[x], target, g.mu) ? 1 : 0;

@@ -229,20 +229,32 @@ int pick_split(int nwords) {
 // A v1 table is walked without per-entry bounds checks, so it is proven
 // well-formed first: positions < n and distinct, 1 <= delta < ax_pos, zero
 // byte terminates.  Any such pattern keeps every walked index inside X.
+// field geometry of a table format: field bits, fields per pattern, and
+// the shift / mask that split a field into (position, delta)
+struct TblFmt {
+  int fb, nf, ps;
+  unsigned dm;
+};
+TblFmt tbl_fmt(int fmt) {
+  if (fmt == CC_FMT_V1) return {8, CC_MAXPAT, 4, 15u};
+  if (fmt == CC_FMT_WIDE) return {10, CC_MAXPAT_W, 4, 15u};
+  return {16, CC_MAXPAT_X, 10, 1023u};
+}
+
 bool table_ok(const uint64_t *pat, long long len, int fmt) {
-  const int fb = fmt == CC_FMT_V1 ? 8 : 10;
-  const int nf = fmt == CC_FMT_V1 ? CC_MAXPAT : CC_MAXPAT_W;
-  const uint64_t fm = (1ull << fb) - 1ull;
+  const TblFmt tf = tbl_fmt(fmt);
+  const int fb = tf.fb, nf = tf.nf;
+  const uint64_t fm = fb == 64 ? ~0ull : (1ull << fb) - 1ull;
   for (long long t = 0; t < len; t++) {
     const uint64_t p = pat[t];
-    if (fmt == CC_FMT_WIDE && (p >> (fb * nf))) return false;
+    if (fb * nf < 64 && (p >> (fb * nf))) return false;
     uint64_t seen = 0;
     bool ended = false;
     for (int k = 0; k < nf; k++) {
       const unsigned b = (unsigned)((p >> (fb * k)) & fm);
       if (!b) { ended = true; continue; }
       if (ended) return false;
-      const int pos = (int)(b >> 4), d = (int)(b & 15u);
+      const int pos = (int)(b >> tf.ps), d = (int)(b & tf.dm);
       if (pos >= E.n || ((seen >> pos) & 1ull) || d < 1 || d >= E.ax[pos])
         return false;
       seen |= 1ull << pos;
@@ -259,8 +271,8 @@ bool table_ok(const uint64_t *pat, long long len, int fmt) {
 // enumerator product; a single coefficient for the L-inf families).  Only
 // then does a set-side count equal the walk's.
 bool table_is_ball(const uint64_t *pat, long long len, int fmt) {
-  const int fb = fmt == CC_FMT_V1 ? 8 : 10;
-  const int nf = fmt == CC_FMT_V1 ? CC_MAXPAT : CC_MAXPAT_W;
+  const TblFmt tf = tbl_fmt(fmt);
+  const int fb = tf.fb, nf = tf.nf;
   const uint64_t fm = (1ull << fb) - 1ull;
   const bool linf = E.prob == CC_TORUS || E.prob == CC_GRID;
   auto wt = [&](int pos, int d) -> int {   // -1: outside the radius
@@ -277,13 +289,13 @@ bool table_is_ball(const uint64_t *pat, long long len, int fmt) {
     for (int i = 0; i < nf; i++) {
       const unsigned b = (unsigned)((p >> (fb * i)) & fm);
       if (!b) break;
-      const int e = wt((int)(b >> 4), (int)(b & 15u));
+      const int e = wt((int)(b >> tf.ps), (int)(b & tf.dm));
       if (e < 0) return false;
       w += e;
       f[k++] = b;
     }
     if (w > E.R) return false;
-    std::sort(f, f + k);   // position-major: pos << 4 | delta
+    std::sort(f, f + k);   // position-major: pos << shift | delta
     uint64_t key = 0;
     for (int i = 0; i < k; i++) key |= f[i] << (fb * i);
     keys[(size_t)t] = key;
@@ -928,11 +940,15 @@ hipError_t launch_walk(const WordsArg &wa, const long long *wptr, int nb,
   const Geom g = make_geom();
   dim3 grid((unsigned)nb * split);
   const bool wide = E.fmt == 0 && E.tfmt == CC_FMT_WIDE;
+  const bool x16 = E.fmt == 0 && E.tfmt == CC_FMT_X16;
 #define CC_WALK_LAUNCH(GR, FM)                                              \
   hipLaunchKernelGGL((k_walk<KIND, GR, FM>), grid, dim3(CC_BLOCK), 0, E.st, \
                      g, wa, wptr, split, delta, E.cnt, walk_table(), E.blen,\
                      pmode, target, out, E.d_scal + 1)
-  if (E.prob == CC_GRID) {
+  if (x16) {
+    if (E.prob == CC_GRID) CC_WALK_LAUNCH(true, CC_FMT_X16);
+    else CC_WALK_LAUNCH(false, CC_FMT_X16);
+  } else if (E.prob == CC_GRID) {
     CC_WALK_LAUNCH(true, CC_FMT_V1);   // grid cells have n <= 8
   } else if (wide) {
     CC_WALK_LAUNCH(false, CC_FMT_WIDE);
@@ -1235,6 +1251,11 @@ static int init_cell(int problem, int n, const int *ax, int R,
   if (n < 2 || n > nmax || R < 1 || !ax || !ballpat || balllen < 1 ||
       balllen > INT_MAX || extract_cap < 1)
     return -1;
+  // the long-axis table reaches L-inf axes up to CC_MAXAX_X (the runtime
+  // window of the sweeps holds R <= 3); the nibble formats stop at 16
+  const int axmax = tfmt == CC_FMT_X16 ? CC_MAXAX_X : CC_MAXAX;
+  bool longax = false;
+  for (int i = 0; i < n; i++) longax = longax || ax[i] > CC_MAXAX;
   int L;
   if (problem == CC_HAMMING) {
     if (R > CC_MAXR) return -1;
@@ -1243,8 +1264,8 @@ static int init_cell(int problem, int n, const int *ax, int R,
     L = R + 1;
   } else if (problem == CC_TORUS) {
     for (int i = 0; i < n; i++)
-      if (ax[i] < 3 || ax[i] > CC_MAXAX || R > (ax[i] - 1) / 2) return -1;
-    if (n > CC_MAXPAT) return -1;
+      if (ax[i] < 3 || ax[i] > axmax || R > (ax[i] - 1) / 2) return -1;
+    if (n > CC_MAXPAT || (longax && R > 3)) return -1;
     L = 1;
   } else if (problem == CC_LEE) {
     if (R > CC_MAXR) return -1;
@@ -1253,8 +1274,8 @@ static int init_cell(int problem, int n, const int *ax, int R,
     L = R + 1;
   } else if (problem == CC_GRID) {
     for (int i = 0; i < n; i++)
-      if (ax[i] < 2 || ax[i] > CC_MAXAX) return -1;
-    if (n > CC_MAXPAT) return -1;
+      if (ax[i] < 2 || ax[i] > axmax) return -1;
+    if (n > CC_MAXPAT || (longax && R > 3)) return -1;
     L = 1;
   } else {
     return -1;
@@ -1446,6 +1467,19 @@ int dct_init3w(int problem, int n, const int *ax, int R,
   (void)layers_mode;
   (void)cnt_mode;
   return init_cell(problem, n, ax, R, widepat, balllen, CC_FMT_WIDE,
+                   extract_cap, bytes, use_owner);
+}
+
+// additive: the same cell with a long-axis table (16-bit pos<<10|delta
+// fields, at most 4 per pattern), for king tori and grid boxes with axes
+// longer than 16 (up to 1024, R <= 3)
+int dct_init3x(int problem, int n, const int *ax, int R,
+               const uint64_t *xpat, long long balllen, int extract_cap,
+               long long *bytes, int layers_mode, int cnt_mode,
+               int use_owner) {
+  (void)layers_mode;
+  (void)cnt_mode;
+  return init_cell(problem, n, ax, R, xpat, balllen, CC_FMT_X16,
                    extract_cap, bytes, use_owner);
 }
 
@@ -1648,7 +1682,10 @@ int dct_swap_eval(const long long *drops, int k, const long long *cand,
     hipLaunchKernelGGL((k_drop_walk<GR, FM>), grid, dim3(CC_BLOCK), 0, E.st, g, \
                        dg, ddrop, split, E.cnt, walk_table(), E.blen, dloss,    \
                        E.lbuf, E.ldrop, E.lcap, dlcnt)
-    if (E.prob == CC_GRID) CC_DROP(true, CC_FMT_V1);
+    const bool x16 = E.fmt == 0 && E.tfmt == CC_FMT_X16;
+    if (x16 && E.prob == CC_GRID) CC_DROP(true, CC_FMT_X16);
+    else if (x16) CC_DROP(false, CC_FMT_X16);
+    else if (E.prob == CC_GRID) CC_DROP(true, CC_FMT_V1);
     else if (wide) CC_DROP(false, CC_FMT_WIDE);
     else CC_DROP(false, CC_FMT_V1);
 #undef CC_DROP

@@ -12,7 +12,8 @@ import os
 
 import numpy as np
 
-from .patterns import PROBLEM_IDS, support_patterns, support_patterns_wide
+from .patterns import (PROBLEM_IDS, support_patterns, support_patterns_long,
+                       support_patterns_wide)
 from .space import ProductSpace
 
 _LL = ctypes.c_longlong
@@ -31,7 +32,7 @@ OPTIONAL = ("dct_map_hist", "dct_set_mu", "dct_get_mu", "dct_ball_gather_mu",
             "dct_count_deficit", "dct_host_bytes", "dct_loss_owner",
             "dct_set_tbl2", "dct_use_fmt", "dct_ball_update_delta",
             "dct_swap_eval", "dct_stats")
-ADDITIVE = ("dct_init3w", "dct_sweep_plan", "dct_sweep_plan2",
+ADDITIVE = ("dct_init3w", "dct_init3x", "dct_sweep_plan", "dct_sweep_plan2",
             "dct_plan_probe", "dct_autotune", "dct_sweep_profile",
             "dct_sweep_times",
             "dct_transform_layers", "dct_layers_read_at")
@@ -82,7 +83,7 @@ def _bind(path):
                              _LL, ctypes.c_int, _LLP]
     lib.dct_init2.restype = ctypes.c_int
     lib.dct_init2.argtypes = lib.dct_init.argtypes + [ctypes.c_int] * 3
-    for f in ("dct_init3", "dct_init3w"):
+    for f in ("dct_init3", "dct_init3w", "dct_init3x"):
         getattr(lib, f).restype = ctypes.c_int
         getattr(lib, f).argtypes = [ctypes.c_int, ctypes.c_int,
                                     ctypes.POINTER(ctypes.c_int), ctypes.c_int,
@@ -108,16 +109,19 @@ def locality_order(pats, axes, bits=8):
     """Sort a v1 table by unwrapped offset so neighbouring table entries
     touch neighbouring words (reference gpu.py:106-129).  Walks are
     order-free, so results are unchanged; entry 0 stays the centre.
+    ``bits`` is the field width: 8 (v1), 10 (wide), 16 (long-axis, whose
+    fields are pos << 10 | delta).
     ``COLDCORE_BALL_ORDER=native`` keeps the builder's order."""
     if os.environ.get("COLDCORE_BALL_ORDER", "locality") == "native":
         return pats
     pats = np.ascontiguousarray(pats, dtype=np.uint64)
     stride = np.cumprod([1] + list(axes[:-1])).astype(np.int64)
     key = np.zeros(len(pats), dtype=np.int64)
+    ps = 10 if bits == 16 else 4
     for k in range(64 // bits):
         byte = ((pats >> np.uint64(bits * k))
                 & np.uint64((1 << bits) - 1)).astype(np.int64)
-        pos, delta = byte >> 4, byte & 15
+        pos, delta = byte >> ps, byte & ((1 << ps) - 1)
         key += np.where(byte != 0,
                         delta * stride[np.minimum(pos, len(axes) - 1)], 0)
     return np.ascontiguousarray(pats[np.argsort(key, kind="stable")])
@@ -150,9 +154,17 @@ class GpuBackend:
         self._lib = lib = load_plugin(plugin_path)
         self._ext_bufs = None
         # v1 tables address 16 axes; larger Hamming / Lee spaces use the
-        # additive wide format (10-bit fields) through dct_init3w
+        # additive wide format (10-bit fields) through dct_init3w, L-inf
+        # cells with axes longer than 16 the long-axis format (16-bit
+        # fields) through dct_init3x
         self.wide = self.space.n > 16
-        if self.wide:
+        self.longax = max(self.space.axes) > 16
+        if self.longax:
+            pats = locality_order(
+                support_patterns_long(self.space.axes, R, problem),
+                self.space.axes, bits=16)
+            init = lib.dct_init3x
+        elif self.wide:
             pats = locality_order(
                 support_patterns_wide(self.space.axes, R, problem),
                 self.space.axes, bits=10)

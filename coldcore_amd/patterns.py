@@ -1,4 +1,5 @@
-"""Host-side support-table ("ball") builders: the v1 and ox wire formats.
+"""Host-side support-table ("ball") builders: the v1 and ox wire formats
+and the additive wide (n > 16) and long-axis (ax > 16) formats.
 
 The packed formats are part of the plugin ABI v1 (reference
 docs/PLUGIN.md:142-152, src/include/coldcore/plugin.h:56-61), so they are
@@ -87,12 +88,12 @@ def _enumerate(axes, R, problem):
                 yield row[None, :], d
 
 
-def _pack(axes, R, problem, bits, nfields, maxpos):
+def _pack(axes, R, problem, bits, nfields, maxpos, pshift=4, maxax=16):
     axes = [int(a) for a in axes]
     if problem not in PROBLEM_IDS:
         raise ValueError(problem)
-    if max(axes) > 16:
-        raise ValueError("packed tables need ax_j <= 16")
+    if max(axes) > maxax:
+        raise ValueError(f"this table format needs ax_j <= {maxax}")
     if len(axes) > maxpos:
         raise ValueError(f"this table format addresses at most {maxpos} axes")
     parts = [np.zeros(1, dtype=np.uint64)]
@@ -103,7 +104,7 @@ def _pack(axes, R, problem, bits, nfields, maxpos):
                              "changed positions per pattern")
         out = np.zeros((len(subs), len(d)), dtype=np.uint64)
         for k in range(w):
-            fld = ((subs[:, k][:, None] << 4) | d[None, :, k]).astype(np.uint64)
+            fld = ((subs[:, k][:, None] << pshift) | d[None, :, k]).astype(np.uint64)
             out |= fld << np.uint64(bits * k)
         parts.append(out.ravel())
     return np.concatenate(parts)
@@ -121,6 +122,15 @@ def support_patterns_wide(axes, R, problem="hamming"):
     more than 16 axes (K_2(24,3), K_2(30,4)) have a table; the engine takes
     it through the additive ``dct_init3w``."""
     return _pack(axes, R, problem, 10, 6, 64)
+
+
+def support_patterns_long(axes, R, problem="torus_linf"):
+    """Long-axis support table: the same enumeration, each pattern a uint64
+    of up to four 16-bit fields (pos << 10 | delta) at bits [16k, 16k+16),
+    zero field terminates.  Deltas up to 1023, so king tori and grid boxes
+    with axes longer than 16 (the v1 / ox nibble limit) have a table; the
+    engine takes it through the additive ``dct_init3x``."""
+    return _pack(axes, R, problem, 16, 4, 64, pshift=10, maxax=1024)
 
 
 def support_patterns_ox(axes, R, problem="hamming"):
@@ -169,12 +179,14 @@ def decode_v1(pat):
 
 
 def decode_fields(pat, bits):
-    """One packed pattern of ``bits``-wide fields -> {pos: delta}."""
+    """One packed pattern of ``bits``-wide fields -> {pos: delta} (16-bit
+    fields hold pos << 10 | delta, narrower ones pos << 4 | delta)."""
     out = {}
     p = int(pat)
     m = (1 << bits) - 1
+    ps = 10 if bits == 16 else 4
     while p:
         b = p & m
-        out[b >> 4] = b & 0xF
+        out[b >> ps] = b & ((1 << ps) - 1)
         p >>= bits
     return out

@@ -138,6 +138,17 @@ int  dct_init3w(int problem, int n, const int *ax, int R,
                 const uint64_t *widepat, long long balllen, int extract_cap,
                 long long *bytes, int layers_mode, int cnt_mode,
                 int use_owner);
+/* dct_init3 with a long-axis table: each pattern a uint64 of up to four
+ * 16-bit fields (pos<<10 | delta) at bits [16k, 16k+16), zero field ends.
+ * v1 and ox hold a delta in a nibble, so the reference ABI stops at axes of
+ * 16 (reference docs/PLUGIN.md:142-152); this entry takes torus_linf and
+ * grid_linf cells with axes up to 1024 (R <= 3 once an axis exceeds 16),
+ * e.g. the k x k king tori for k = 17..31.  Same cell, same semantics,
+ * same return codes otherwise. */
+int  dct_init3x(int problem, int n, const int *ax, int R,
+                const uint64_t *xpat, long long balllen, int extract_cap,
+                long long *bytes, int layers_mode, int cnt_mode,
+                int use_owner);
 /* The transform of dct_transform(init_mode, 1) with its last sweep keeping
  * the exact distance layers: after it, layer d at x = number of seed words
  * at distance exactly d from x (init 0 codewords of set_code, 1 deficient,

@@ -7,9 +7,9 @@ import re
 import numpy as np
 import pytest
 
-from coldcore_amd.patterns import (ball_patterns, ball_size, decode_v1,
-                                   lee_ball_size, support_patterns,
-                                   support_patterns_ox)
+from coldcore_amd.patterns import (ball_patterns, ball_size, decode_fields,
+                                   decode_v1, lee_ball_size, support_patterns,
+                                   support_patterns_long, support_patterns_ox)
 from coldcore_amd.space import ProductSpace
 from oracle.dp import NumpyDP
 
@@ -91,6 +91,49 @@ def test_table_walks_the_oracle_ball(problem, axes, R):
             if ok:
                 got.add(x)
         assert got == set(dp.ball_indices(w).tolist())
+
+
+@pytest.mark.parametrize("problem,axes,R", [
+    ("torus_linf", [20, 24], 2), ("torus_linf", [17, 18, 19], 1),
+    ("grid_linf", [64, 9], 1), ("grid_linf", [200, 3], 3),
+    ("grid_linf", [33, 2, 17], 1), ("torus_linf", [100, 7], 3)])
+def test_long_axis_table_walks_the_oracle_ball(problem, axes, R):
+    """The long-axis table (16-bit pos<<10|delta fields) decodes and walks
+    to the oracle's ball, with the open grid's off-board moves dropped as
+    the kernels drop them; its order is a permutation of the same set."""
+    from coldcore_amd.gpu import locality_order
+    pats = support_patterns_long(axes, R, problem)
+    assert pats[0] == 0 and len(set(pats.tolist())) == len(pats)
+    dp = NumpyDP(axes, R, problem)
+    sp = ProductSpace(axes)
+    for w in (0, 1, sp.size // 3, sp.size - 1):
+        dig = sp.index_word([w])[0].astype(np.int64)
+        got = []
+        for v in pats.tolist():
+            x, ok = w, True
+            for p, d in decode_fields(v, 16).items():
+                a = axes[p]
+                sh = d if int(dig[p]) + d < a else d - a
+                if problem == "grid_linf" and abs(sh) > R:
+                    ok = False
+                    break
+                x += sh * int(sp.pow_q[p])
+            if ok:
+                got.append(x)
+        assert sorted(got) == sorted(dp.ball_indices(w).tolist())
+    out = locality_order(pats, axes, bits=16)
+    assert out[0] == 0 and (np.sort(out) == np.sort(pats)).all()
+
+
+def test_long_axis_table_limits():
+    with pytest.raises(ValueError):
+        support_patterns_long([1100, 2], 1, "grid_linf")      # delta field
+    with pytest.raises(ValueError):
+        support_patterns_long([17] * 5, 1, "torus_linf")      # 5 moved axes
+    # on short axes it is the v1 table in wider fields
+    a = support_patterns([9, 9], 2, "torus_linf")
+    b = support_patterns_long([9, 9], 2, "torus_linf")
+    assert [decode_v1(x) for x in a] == [decode_fields(y, 16) for y in b]
 
 
 def test_locality_order_is_permutation():

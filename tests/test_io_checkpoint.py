@@ -128,8 +128,8 @@ def test_seed_file_start_and_mixed_radix(tmp_path):
 def test_gpu_run_writes_the_oracle_files(gpu_cell, tmp_path):
     """The same seeded, clock-free run through the engine's C ABI writes
     the same checkpoints, byte for byte, as on the numpy oracle."""
-    for axes, R in (([2] * 6, 1), ([3] * 4, 1), ([3] * 5, 1)):
-        a, b = str(tmp_path / "gpu.txt"), str(tmp_path / "cpu.txt")
+    for i, (axes, R) in enumerate((([2] * 6, 1), ([3] * 4, 1), ([3] * 5, 1))):
+        a, b = str(tmp_path / f"gpu{i}.txt"), str(tmp_path / f"cpu{i}.txt")
         g = gpu_cell(axes, R)
         rg = run_cell(g, a, seed=5, rounds=120)
         g.close()
