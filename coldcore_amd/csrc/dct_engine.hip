@@ -2022,7 +2022,11 @@ int dct_autotune(int reps, float *best_us) {
     for (int col : {2, 1, 0})
       for (int first : {1, 0}) opts.push_back({lds, col, first, 1, E.pk_narrow});
   }
-  opts.push_back({E.lds_budget, E.pk_col, E.pk_first, 0, E.pk_narrow});
+  // the unpacked schedule under each LDS budget (its plan is rebuilt per
+  // budget: tile sizes, sweep grouping and the workgroups a CU holds move
+  // together)
+  for (int lds : {16384, 32768, 65536, 98304, 131072})
+    if (lds <= E.lds_max) opts.push_back({lds, E.pk_col, E.pk_first, 0, E.pk_narrow});
   const Opt saved{E.lds_budget, E.pk_col, E.pk_first, E.packed, E.pk_narrow};
   std::vector<std::string> seen;
   float best = 1e30f;
@@ -2034,8 +2038,15 @@ int dct_autotune(int reps, float *best_us) {
     E.lds_budget = o.lds; E.pk_col = o.col; E.pk_first = o.first; E.packed = o.packed;
     E.pk_narrow = o.narrow;
     PPlan pl = o.packed ? pk_plan(1.0, 1e300, true, false) : PPlan();
-    std::string sig = "unpacked";
-    if (pl.ok) {
+    std::string sig = "unpacked|";
+    if (!o.packed) {
+      build_plan();
+      for (const Sweep &w : E.plan) {
+        char b[96];
+        snprintf(b, sizeof(b), "%d.%d.%d.%d.%d|", w.j0, w.nax, w.P, w.TL, w.rq);
+        sig += b;
+      }
+    } else if (pl.ok) {
       sig.clear();
       for (const PSweep &w : pl.sw) {
         char b[96];
@@ -2068,6 +2079,7 @@ int dct_autotune(int reps, float *best_us) {
   E.packed = win.packed;
   E.pk_narrow = win.narrow;
   E.pfield = wplan;
+  build_plan();   // the unpacked plan of the winning budget
   if (best_us) *best_us = best * 1000.0f / reps;
   return timed;
 }

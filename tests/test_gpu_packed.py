@@ -121,3 +121,28 @@ def test_autotune_keeps_fields_exact(monkeypatch):
         after = fields()
     for a, b in zip(before, after):
         assert (a == b).all()
+
+
+def test_autotune_unpacked_budgets_keep_fields_exact(monkeypatch):
+    """A mixed-radix cell has no packed schedule: the autotuner times the
+    unpacked plan rebuilt under each LDS budget, and the winner's fields
+    equal the default plan's bit for bit."""
+    from coldcore_amd.gpu import GpuBackend
+    monkeypatch.setenv("CC_AUTOTUNE", "0")
+    axes, R = [4, 5, 6, 7, 8, 4, 5, 6], 3
+    size = int(np.prod(axes))
+    code = np.flatnonzero(np.random.default_rng(6).random(size) < 3e-4)
+    with GpuBackend(axes, R) as g:
+        def fields():
+            g.recount(code)
+            cnt = g.read_cnt().copy()
+            g.gain_map()
+            gain = g.map_all().copy()
+            g.loss_map()
+            return cnt, gain, g.map_all().copy()
+        before = fields()
+        n, us = g.autotune(2)
+        assert n >= 3 and us > 0          # several unpacked plans timed
+        after = fields()
+    for a, b in zip(before, after):
+        assert (a == b).all()
