@@ -198,7 +198,7 @@ void plan_cell(int ci, const BCell &c) {
     if (a.rr)   // one round: registers only
       B.lds = std::max(B.lds, a.nax > a.rr ? (size_t)round_lds(c, P * TLw) : (size_t)0);
     else
-      B.lds = std::max(B.lds, (size_t)c.L * cc_gplane<false>((int)(P * TLw)) * 4);
+      B.lds = std::max(B.lds, (size_t)c.L * cc_gplane<CCB_PAD>((int)(P * TLw)) * 4);
     j = k;
   }
 }

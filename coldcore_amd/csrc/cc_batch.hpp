@@ -17,6 +17,11 @@
 
 #define CCB_KMAX 8       // ruin size per LNS round
 #define CCB_WSPLIT 8     // workgroups per walked ball
+// LDS layout of the batch's generic tiles: unpadded (measured faster than
+// the singleton's one-pad-word-per-32 layout on the batched workloads)
+#ifndef CCB_PAD
+#define CCB_PAD false
+#endif
 
 struct BCell {
   long long pw[CC_MAXN + 1];
@@ -56,7 +61,7 @@ struct BItem {
 template <int PROB>
 __device__ __forceinline__ void ccb_tile_linf(const SweepArgs &a, long long tile,
                                               int32_t *__restrict__ S) {
-  cc_sweep_tile<PROB, 1, int32_t, false>(a, tile, S);
+  cc_sweep_tile<PROB, 1, int32_t, CCB_PAD>(a, tile, S);
 }
 
 
@@ -66,16 +71,16 @@ template <typename LT>
 __device__ __forceinline__ void ccb_tile_t(const SweepArgs &a, int prob, int L,
                                            long long tile, int32_t *S) {
   if (prob == CC_HAMMING) {
-    if (L == 2) cc_sweep_tile<CC_HAMMING, 2, LT, false>(a, tile, S);
-    else if (L == 3) cc_sweep_tile<CC_HAMMING, 3, LT, false>(a, tile, S);
-    else cc_sweep_tile<CC_HAMMING, 4, LT, false>(a, tile, S);
+    if (L == 2) cc_sweep_tile<CC_HAMMING, 2, LT, CCB_PAD>(a, tile, S);
+    else if (L == 3) cc_sweep_tile<CC_HAMMING, 3, LT, CCB_PAD>(a, tile, S);
+    else cc_sweep_tile<CC_HAMMING, 4, LT, CCB_PAD>(a, tile, S);
   } else if (prob == CC_LEE) {
-    if (L == 2) cc_sweep_tile<CC_LEE, 2, LT, false>(a, tile, S);
-    else if (L == 3) cc_sweep_tile<CC_LEE, 3, LT, false>(a, tile, S);
-    else cc_sweep_tile<CC_LEE, 4, LT, false>(a, tile, S);
+    if (L == 2) cc_sweep_tile<CC_LEE, 2, LT, CCB_PAD>(a, tile, S);
+    else if (L == 3) cc_sweep_tile<CC_LEE, 3, LT, CCB_PAD>(a, tile, S);
+    else cc_sweep_tile<CC_LEE, 4, LT, CCB_PAD>(a, tile, S);
   } else if constexpr (sizeof(LT) == 2) {
-    if (prob == CC_TORUS) cc_sweep_tile<CC_TORUS, 1, LT, false>(a, tile, S);
-    else cc_sweep_tile<CC_GRID, 1, LT, false>(a, tile, S);
+    if (prob == CC_TORUS) cc_sweep_tile<CC_TORUS, 1, LT, CCB_PAD>(a, tile, S);
+    else cc_sweep_tile<CC_GRID, 1, LT, CCB_PAD>(a, tile, S);
   } else if (prob == CC_TORUS) {
     ccb_tile_linf<CC_TORUS>(a, tile, S);
   } else {
