@@ -478,9 +478,9 @@ def main():
         copy_calibration(args.calib_copy)
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build runs the hot path on a "
-              "single MI355X (multi-GPU sharding is a design note, not "
-              "code); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the hot path runs on a single "
+              "MI355X (sharding across GPUs is described, not implemented); "
+              "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     from coldcore_amd.gpu import GpuBackend
