@@ -267,7 +267,8 @@ int build_plan(int budget) {
       palloc((void **)&B.d_items, 8 * B.nitems) ||
       palloc((void **)&B.d_counters, 4 * B.nlevels) ||
       hipMemcpy(B.d_argcell, B.argcell.data(), 4 * B.nargs, hipMemcpyHostToDevice) ||
-      hipMemcpy(B.d_items, B.items.data(), 8 * B.nitems, hipMemcpyHostToDevice))
+      hipMemcpy(B.d_items, B.items.data(), 8 * B.nitems, hipMemcpyHostToDevice) ||
+      hipMemset(B.d_counters, 0, 4 * B.nlevels))   // heads start at zero
     return -3;
   if (B.lds > 48 * 1024)   // beyond the default dynamic LDS of a launch
     CKB(hipFuncSetAttribute((const void *)k_ccb_sweeps,
@@ -284,11 +285,15 @@ int build_plan(int budget) {
 }
 
 int transform_launch(int kind) {
-  // the per-level queue heads are cleared by a kernel (graph-capturable; no
-  // memset nodes)
-  hipLaunchKernelGGL(k_ccb_fill, dim3(1), dim3(256), 0, B.st, (int *)B.d_counters,
-                     0, B.nlevels);
-  CKB(hipGetLastError());
+  // queue heads: the launch of level l clears the head of level l + 1 (mod
+  // the level count), so each head is zero when its level starts and no
+  // separate clearing launch (or memset node in a captured graph) is
+  // needed; a one-level plan keeps the clearing kernel
+  if (B.nlevels == 1) {
+    hipLaunchKernelGGL(k_ccb_fill, dim3(1), dim3(256), 0, B.st, (int *)B.d_counters,
+                       0, B.nlevels);
+    CKB(hipGetLastError());
+  }
   if (kind == 2) {
     const int gx = (int)std::min<long long>((B.maxX + 255) / 256, 1024);
     hipLaunchKernelGGL(k_ccb_zero_a0, dim3(gx, B.nc), dim3(256), 0, B.st, B.d);
@@ -301,9 +306,10 @@ int transform_launch(int kind) {
   for (int l = 0; l < B.nlevels; l++) {
     const int n = B.level_off[l + 1] - B.level_off[l];
     const int grid = std::max(1, std::min(B.per_cu * B.ncu, n));
+    unsigned *next = B.nlevels > 1 ? B.d_counters + (l + 1) % B.nlevels : nullptr;
     hipLaunchKernelGGL(k_ccb_sweeps, dim3(grid), dim3(CC_BLOCK), B.lds, B.st,
                        B.dargs[kind], B.d_argcell, B.d, B.d_items + B.level_off[l],
-                       n, B.d_counters + l);
+                       n, B.d_counters + l, next);
     CKB(hipGetLastError());
   }
   return 0;
@@ -483,7 +489,12 @@ int ccb_init(int ncells, const int *problem, const int *n, const int *axes,
     (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
     B.ncu = std::max(ncu, 1);
   }
-  if (build_plan(CCB_LDS)) { bfree_all(); return -3; }
+  // CCB_LDS (bytes) overrides the initial LDS budget per tile, within
+  // [4 KiB, 64 KiB]; the autotuner may replace it
+  int budget0 = CCB_LDS;
+  if (const char *lb = getenv("CCB_LDS"))
+    budget0 = std::min(64 * 1024, std::max(4096, atoi(lb)));
+  if (build_plan(budget0)) { bfree_all(); return -3; }
   // the set-up copies and clears ran outside the batch's (non-blocking)
   // stream: finish them before its first launch
   if (hipDeviceSynchronize() != hipSuccess) { bfree_all(); return -3; }

@@ -115,9 +115,10 @@ __device__ __forceinline__ void ccb_tile(const SweepArgs &a, const BCell &c,
 __global__ void __launch_bounds__(CC_BLOCK)
     k_ccb_sweeps(const SweepArgs *__restrict__ args, const int *__restrict__ argcell,
                  const BCell *__restrict__ cells, const int2 *__restrict__ items,
-                 int nitems, unsigned *counter) {
+                 int nitems, unsigned *counter, unsigned *next_counter) {
   extern __shared__ int32_t S[];
   __shared__ int cur;
+  if (next_counter && blockIdx.x == 0 && threadIdx.x == 0) *next_counter = 0u;
   for (;;) {
     if (threadIdx.x == 0) cur = (int)atomicAdd(counter, 1u);
     __syncthreads();

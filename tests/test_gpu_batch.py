@@ -116,6 +116,34 @@ def test_batch_autotune_keeps_fields_exact():
                                                   err_msg=f"{which} cell {i}")
 
 
+def test_passenger_rows_and_round_tiles_exact(monkeypatch):
+    """A 64 KiB budget gives contiguous first-sweep tiles of several groups
+    (passenger rows: a round tile of 2 x 4,096 binary words, generic tiles
+    of 6 x 840 and 2 x 3,024 words): recount, gain and loss still equal the
+    oracle."""
+    monkeypatch.setenv("CCB_LDS", "65536")
+    cells = [("hamming", [2] * 14, 1), ("hamming", [4, 5, 6, 7, 8, 9], 2),
+             ("grid_linf", [6, 7, 8, 9, 10], 1), ("hamming", [3] * 9, 3)]
+    codes = _codes(cells, frac=0.01, seed=9)
+    with _batch(cells) as b:
+        assert b.info()["lds_budget"] == 65536
+        for i, c in enumerate(codes):
+            b.set_code(i, c)
+        b.recount()
+        oracles = []
+        for i, (p, axes, R) in enumerate(cells):
+            o = NumpyDP(axes, R, problem=p)
+            o.recount(codes[i])
+            oracles.append(o)
+            np.testing.assert_array_equal(b.read_cnt(i), o.cnt.astype(np.uint16))
+        for which in ("gain", "loss"):
+            getattr(b, which + "_maps")()
+            for i, o in enumerate(oracles):
+                getattr(o, which + "_map")()
+                np.testing.assert_array_equal(b.read_map(i), o.map,
+                                              err_msg=f"{which} cell {i}")
+
+
 def test_batched_transform_repeats_exactly():
     """The persistent work list resets itself: repeated launches agree."""
     cells = BATCH_CELLS[:8]
