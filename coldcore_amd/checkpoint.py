@@ -30,6 +30,8 @@ def run_cell(backend, solution_path, seed=1, rounds=1000, budget_s=None,
     """Search one cell on ``backend`` (any object with the Backend
     protocol and a ``space`` with ``axes``), checkpointing every feasible
     code to ``solution_path``.  Returns the run record."""
+    if rounds is None and budget_s is None:
+        raise ValueError("run_cell needs a round bound or a time budget")
     t0 = time.time()
     rec = {"M": None, "uncov": None, "solutions": [], "resumed_from": None,
            "rounds": 0, "solution_file": os.fspath(solution_path),

@@ -111,6 +111,11 @@ def test_rerun_resumes_and_never_ends_larger(tmp_path):
     assert fresh["M"] == first["M"]            # clock-free: the same run again
 
 
+def test_unbounded_run_is_refused(tmp_path):
+    with pytest.raises(ValueError):
+        run_cell(NumpyDP([3] * 3, 1), str(tmp_path / "x.txt"), rounds=None)
+
+
 def test_seed_file_start_and_mixed_radix(tmp_path):
     axes, R = [2, 3, 4], 1
     o = NumpyDP(axes, R)
