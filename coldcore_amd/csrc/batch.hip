@@ -404,7 +404,7 @@ int ccb_init(int ncells, const int *problem, const int *n, const int *axes,
     c.pw[0] = 1;
     for (int j = 0; j < c.n; j++) {
       c.ax[j] = axes[aoff + j];
-      const int lim = linf ? 32 : (c.prob == CC_HAMMING ? 10 : 16);
+      const int lim = linf ? CC_MAXAX_X : (c.prob == CC_HAMMING ? 10 : 16);
       if (c.ax[j] < 2 || c.ax[j] > lim || (c.prob == CC_TORUS && c.ax[j] < 2 * c.R + 1))
         { bfree_all(); return -1; }
       c.pw[j + 1] = c.pw[j] * c.ax[j];

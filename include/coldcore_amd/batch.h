@@ -8,7 +8,7 @@
  *
  * A batch is many uneven cells at once: Hamming, Lee (R <= 3, the same v1 /
  * wide support tables as dct_init3 / dct_init3w), king torus and open grid
- * (n <= 8, axes <= 32, R <= 3; torus axes >= 2R + 1), each with at most
+ * (n <= 8, axes <= 1024, R <= 3; torus axes >= 2R + 1), each with at most
  * 2^26 words.  The exact transform of every cell runs as one persistent
  * launch per sweep level (the l-th sweep of every cell together; no
  * workgroup waits on another); a batched large-neighbourhood-search round

@@ -38,6 +38,9 @@ BATCH_CELLS = [
     ("grid_linf", [5, 6, 4], 2),
     ("grid_linf", [17, 5], 3),
     ("grid_linf", [2, 3, 4, 6, 8], 1),
+    # axes far beyond 32: the runtime-length window in the sweeps
+    ("torus_linf", [100, 7], 3),
+    ("grid_linf", [200, 3], 3),
 ]
 
 
