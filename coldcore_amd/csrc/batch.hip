@@ -45,7 +45,6 @@ struct Batch {
   std::vector<int> level_off;         // items of level l: [off[l], off[l+1])
   int *d_argcell = nullptr;
   int2 *d_items = nullptr;
-  unsigned *d_counters = nullptr;     // one queue head per level
   int nargs = 0, nitems = 0, nlevels = 0;
   size_t lds = 0;
   int per_cu = 1, ncu = 1;
@@ -238,8 +237,8 @@ int build_plan(int budget) {
       for (int t = 0; t < B.need[a]; t++) B.items.push_back(make_int2(a, t));
     }
     if (!any) break;
-    // largest tiles first (the level ends with its last tile, so the
-    // longest ones are taken while every workgroup is still busy)
+    // largest tiles first: dealt round-robin, a descending list gives every
+    // workgroup about the same mix of tile sizes
     if (!getenv("CCB_NO_LPT")) {
       auto cost = [&](const int2 &it) {
         const SweepArgs &sa = B.args[0][it.x];
@@ -265,10 +264,8 @@ int build_plan(int budget) {
       return -3;
   if (palloc((void **)&B.d_argcell, 4 * B.nargs) ||
       palloc((void **)&B.d_items, 8 * B.nitems) ||
-      palloc((void **)&B.d_counters, 4 * B.nlevels) ||
       hipMemcpy(B.d_argcell, B.argcell.data(), 4 * B.nargs, hipMemcpyHostToDevice) ||
-      hipMemcpy(B.d_items, B.items.data(), 8 * B.nitems, hipMemcpyHostToDevice) ||
-      hipMemset(B.d_counters, 0, 4 * B.nlevels))   // heads start at zero
+      hipMemcpy(B.d_items, B.items.data(), 8 * B.nitems, hipMemcpyHostToDevice))
     return -3;
   if (B.lds > 48 * 1024)   // beyond the default dynamic LDS of a launch
     CKB(hipFuncSetAttribute((const void *)k_ccb_sweeps,
@@ -285,15 +282,6 @@ int build_plan(int budget) {
 }
 
 int transform_launch(int kind) {
-  // queue heads: the launch of level l clears the head of level l + 1 (mod
-  // the level count), so each head is zero when its level starts and no
-  // separate clearing launch (or memset node in a captured graph) is
-  // needed; a one-level plan keeps the clearing kernel
-  if (B.nlevels == 1) {
-    hipLaunchKernelGGL(k_ccb_fill, dim3(1), dim3(256), 0, B.st, (int *)B.d_counters,
-                       0, B.nlevels);
-    CKB(hipGetLastError());
-  }
   if (kind == 2) {
     const int gx = (int)std::min<long long>((B.maxX + 255) / 256, 1024);
     hipLaunchKernelGGL(k_ccb_zero_a0, dim3(gx, B.nc), dim3(256), 0, B.st, B.d);
@@ -306,10 +294,9 @@ int transform_launch(int kind) {
   for (int l = 0; l < B.nlevels; l++) {
     const int n = B.level_off[l + 1] - B.level_off[l];
     const int grid = std::max(1, std::min(B.per_cu * B.ncu, n));
-    unsigned *next = B.nlevels > 1 ? B.d_counters + (l + 1) % B.nlevels : nullptr;
     hipLaunchKernelGGL(k_ccb_sweeps, dim3(grid), dim3(CC_BLOCK), B.lds, B.st,
                        B.dargs[kind], B.d_argcell, B.d, B.d_items + B.level_off[l],
-                       n, B.d_counters + l, next);
+                       n);
     CKB(hipGetLastError());
   }
   return 0;
