@@ -190,6 +190,8 @@ void plan_cell(int ci, const BCell &c) {
       // uint16 planes keep layer 0 in H[0] too: the first sweep writes it
       // whatever seeded it (A[0] holds the code indicator as int32)
       b.write_l0 = c.L == 1 || (first && (kind != 2 || c.l16));
+      // the gain map's last sweep folds the cell's argmax into its BCell
+      b.amax = kind == 0 && last ? &B.d[ci].amax : nullptr;
       B.args[kind].push_back(b);
     }
     B.argcell.push_back(ci);
@@ -332,8 +334,9 @@ int round_launch(int k) {
                      0, B.st, B.d, B.d_ruin, CCB_WSPLIT);
   CKB(hipGetLastError());
   for (int j = 0; j < k; j++) {
+    // the gain transform leaves each cell's argmax in amax (its map sweep
+    // folds it in; ruin / add cleared it), so no reduction runs here
     if (int rc = transform_launch(0)) return rc;
-    if (int rc = reduce_launch(1, false)) return rc;
     hipLaunchKernelGGL(k_ccb_add, cg, cb, 0, B.st, B.d, B.nc, j, B.d_add);
     CKB(hipGetLastError());
     hipLaunchKernelGGL(k_ccb_walk, dim3(B.nc * CCB_WSPLIT), dim3(CC_BLOCK), 0, B.st,

@@ -119,7 +119,33 @@ struct SweepArgs {
   // one per axis for Hamming and floor(ax/2) for Lee
   int live[CC_MAXSW + 1];
   int rr;            // batched sweeps: axes per register round (0: LDS tile)
+  // batched gain sweeps that write the map: the largest (gain << 32 |
+  // 0xFFFFFFFF - index) key of the written words is folded in here with
+  // atomicMax (the cell's argmax without a separate reduction); nullptr: off
+  unsigned long long *amax;
 };
+
+// fold (gain, index) into a running argmax key: larger gain first, then the
+// smaller index; zero gains never enter
+__device__ __forceinline__ void cc_key_max(unsigned long long &best, uint32_t g,
+                                           long long w) {
+  if ((int32_t)g > 0) {
+    const unsigned long long key = ((unsigned long long)g << 32) |
+                                   (0xFFFFFFFFull - (unsigned long long)w);
+    best = key > best ? key : best;
+  }
+}
+
+// wave maximum of the keys, one atomicMax per wave (every lane calls it)
+__device__ __forceinline__ void cc_key_flush(unsigned long long best,
+                                             unsigned long long *amax) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long b2 = __shfl_xor(best, o, 64);
+    best = b2 > best ? b2 : best;
+  }
+  if ((threadIdx.x & 63) == 0 && best) atomicMax(amax, best);
+}
 
 // LDS layout of a generic tile: layer plane d starts at d * cc_gplane(PT),
 // tile element e sits at cc_gpad(e).  With PAD, one pad word every 32

@@ -249,6 +249,7 @@ __device__ __forceinline__ void cc_sweep_tile(const SweepArgs &a, long long tile
     __syncthreads();
   }
   // ---- write back: layers, or the layer sum into the map / cnt plane
+  unsigned long long best = 0;
   for (int e = threadIdx.x; e < PT; e += blockDim.x) {
     const int p = cc_div(e, mTL), tl = e - p * a.TL;
     if (tl >= ncol) continue;
@@ -262,12 +263,14 @@ __device__ __forceinline__ void cc_sweep_tile(const SweepArgs &a, long long tile
       for (int d = 0; d < Lmax; d++) sum += (uint32_t)S[d * PTP + ep];
       if (a.out_kind == CC_OUT_MAP) {
         a.A[0][w] = (int32_t)sum;
+        if (a.amax) cc_key_max(best, sum, w);
       } else {
         const int32_t si = (int32_t)sum;
         a.cnt[w] = si > 65535 ? (uint16_t)65535 : (uint16_t)si;
       }
     }
   }
+  if (a.amax) cc_key_flush(best, a.amax);
   __syncthreads();
 }
 
@@ -445,7 +448,8 @@ __device__ __forceinline__ void cc_cube_load(const SweepArgs &a, long long base,
 template <int R1, int C, typename LT>
 __device__ __forceinline__ void cc_cube_store(const SweepArgs &a, long long base,
                                               int pb, int qa, int tl,
-                                              int Lmax, const uint32_t (&v)[R1][C]) {
+                                              int Lmax, const uint32_t (&v)[R1][C],
+                                              unsigned long long &best) {
   const long long sq = (long long)qa * a.s;
   const long long w0 = base + (long long)pb * a.s + tl;
   if (a.out_kind == CC_OUT_LAYERS) {
@@ -463,6 +467,7 @@ __device__ __forceinline__ void cc_cube_store(const SweepArgs &a, long long base
 #pragma unroll
       for (int d = 0; d < R1; d++) sum += v[d][c];
       a.A[0][w0 + c * sq] = (int32_t)sum;
+      if (a.amax) cc_key_max(best, sum, w0 + c * sq);
     }
   } else {
 #pragma unroll
@@ -547,6 +552,7 @@ __device__ __forceinline__ void cc_round_tile(const SweepArgs &a, long long tile
   const long long l0 = (tile - h * a.ncolb) * a.TL;
   const int ncol = (int)min((long long)a.TL, a.s - l0);
   const long long base = h * a.s * a.P + l0;
+  unsigned long long best = 0;   // argmax key of the map words this thread wrote
   // ---- round 0: HBM -> registers (cube = group axes 0..RR-1)
   {
     const int ca = min(0, m - RR);
@@ -556,7 +562,7 @@ __device__ __forceinline__ void cc_round_tile(const SweepArgs &a, long long tile
       cc_cube_load<R1, C, LT>(a, base, pb, tl, tl < ncol, Lin, v);
       cc_cube_apply<PROB, R1, Q, C, RR>(v, -ca, a.R);
       if (nrounds == 1) {
-        if (tl < ncol) cc_cube_store<R1, C, LT>(a, base, pb, 1, tl, Lmax, v);
+        if (tl < ncol) cc_cube_store<R1, C, LT>(a, base, pb, 1, tl, Lmax, v, best);
       } else {
         cc_cube_lds_st<R1, C, LT>(S, PTP, pb, 1, a.TL, tl, v);
       }
@@ -577,13 +583,14 @@ __device__ __forceinline__ void cc_round_tile(const SweepArgs &a, long long tile
       cc_cube_lds_ld<R1, C, LT>(S, PTP, pb, qa, a.TL, tl, v);
       cc_cube_apply<PROB, R1, Q, C, RR>(v, lo - ca, a.R);
       if (last) {
-        if (tl < ncol) cc_cube_store<R1, C, LT>(a, base, pb, qa, tl, Lmax, v);
+        if (tl < ncol) cc_cube_store<R1, C, LT>(a, base, pb, qa, tl, Lmax, v, best);
       } else {
         cc_cube_lds_st<R1, C, LT>(S, PTP, pb, qa, a.TL, tl, v);
       }
     }
     __syncthreads();
   }
+  if (a.amax) cc_key_flush(best, a.amax);
 }
 
 template <int PROB, int R1, int Q, int RR, typename LT>
