@@ -94,6 +94,49 @@ def run_cpu_baseline(code, passes):
                        f"os.cpu_count()={os.cpu_count()}")
 
 
+#: the reference's published single-thread numpy gain-transform times
+#: (reference docs/benchmarks.md:127-144): cell -> ms
+CPU_CALIB_CELLS = [
+    ("king 15x15 torus R=1", "torus_linf", [15, 15], 1, 0.04),
+    ("lee 9x9x9 R=2", "lee", [9, 9, 9], 2, 0.12),
+    ("K_2(12,2)", "hamming", [2] * 12, 2, 0.59),
+    ("K_3(8,2)", "hamming", [3] * 8, 2, 0.48),
+    ("K_2(16,3)", "hamming", [2] * 16, 3, 13.3),
+    ("torus 64^3 R=2", "torus_linf", [64, 64, 64], 2, 1.6),
+    ("K_5(8,3)", "hamming", [5] * 8, 3, 26.3),
+    ("K_3(12,3)", "hamming", [3] * 12, 3, 73.1),
+    ("K_7(8,3)", "hamming", [7] * 8, 3, 455.0),
+]
+
+
+def run_cpu_calibration(reps=3):
+    """The numpy oracle's gain transform (one thread, median of ``reps``
+    after a warm-up) on the cells the reference publishes single-thread
+    numpy times for, beside those times: the CPU baseline's restatement is
+    of the same speed class as the reference's own CPU backend."""
+    from oracle.dp import NumpyDP
+    try:
+        os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[0]})
+    except (AttributeError, OSError):
+        pass
+    out = []
+    for name, prob, axes, R, ref_ms in CPU_CALIB_CELLS:
+        c = NumpyDP(axes, R, problem=prob)
+        size = c.space.size
+        rng = np.random.default_rng(0)
+        c.recount(rng.choice(size, max(1, size // 100), replace=False).astype(np.int64))
+        c.gain_map()
+        ts = []
+        for _ in range(reps):
+            t = time.perf_counter()
+            c.gain_map()
+            ts.append(time.perf_counter() - t)
+        ms = sorted(ts)[len(ts) // 2] * 1e3
+        out.append({"cell": name, "words": int(size), "oracle_gain_ms": round(ms, 3),
+                    "reference_cpu_gain_ms": ref_ms, "ratio": round(ms / ref_ms, 2)})
+    return {"host": cpu_model(), "threads": 1, "cells": out}
+
+
 def schedule_key(plan):
     return " | ".join(describe(plan))
 
@@ -449,9 +492,12 @@ def main():
                          "(3 times): the reference kernel for scaling the "
                          "FETCH_SIZE / WRITE_SIZE counters")
     ap.add_argument("--only", choices=("near_cover", "batch64", "batch64-quick",
-                                       "cells", "learned", "walks"),
+                                       "cells", "learned", "walks", "cpu_calib"),
                     help="run one side section and print its JSON only")
     args = ap.parse_args()
+    if args.only == "cpu_calib":
+        print(json.dumps({"cpu_calib": run_cpu_calibration()}), flush=True)
+        return
     if args.only:
         from coldcore_amd.gpu import GpuBackend
         if args.only == "near_cover":
