@@ -446,19 +446,23 @@ OTHER_CELLS = [
     ("K_3(8,2)", [3] * 8, 2, ("choice", 40)),
     ("K_3(10,2)", [3] * 10, 2, ("choice", 400)),
     ("K_4(8,2)", [4] * 8, 2, ("choice", 400)),
+    # L-inf cells with axes beyond the v1 nibble (long-axis tables)
+    ("king torus 1024x1024 R=1", [1024, 1024], 1, ("p", 1e-3), "torus_linf"),
+    ("grid 1000x1000 R=3", [1000, 1000], 3, ("p", 1e-3), "grid_linf"),
 ]
 
 
 def time_cells(GpuBackend, steps):
     out = []
-    for name, axes, R, (kind, arg) in OTHER_CELLS:
+    for name, axes, R, (kind, arg), *prob in OTHER_CELLS:
+        problem = prob[0] if prob else "hamming"
         size = int(np.prod(np.array(axes, dtype=np.int64)))
         rng = np.random.default_rng(0)
         if kind == "p":
             code = np.flatnonzero(rng.random(size) < arg)
         else:
             code = rng.choice(size, arg, replace=False)
-        g = GpuBackend(axes, R)
+        g = GpuBackend(axes, R, problem=problem)
         try:
             g.recount(code.astype(np.int64))
             g.gain_map()                  # the first one tunes the schedule

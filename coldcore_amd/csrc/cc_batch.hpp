@@ -65,7 +65,7 @@ struct BItem {
 template <int PROB>
 __device__ __forceinline__ void ccb_tile_linf(const SweepArgs &a, long long tile,
                                               int32_t *__restrict__ S) {
-  cc_sweep_tile<PROB, 1, int32_t, CCB_PAD>(a, tile, S);
+  cc_sweep_tile<PROB, 1, int32_t, CCB_PAD, false>(a, tile, S);
 }
 
 
@@ -75,16 +75,16 @@ template <typename LT>
 __device__ __forceinline__ void ccb_tile_t(const SweepArgs &a, int prob, int L,
                                            long long tile, int32_t *S) {
   if (prob == CC_HAMMING) {
-    if (L == 2) cc_sweep_tile<CC_HAMMING, 2, LT, CCB_PAD>(a, tile, S);
-    else if (L == 3) cc_sweep_tile<CC_HAMMING, 3, LT, CCB_PAD>(a, tile, S);
-    else cc_sweep_tile<CC_HAMMING, 4, LT, CCB_PAD>(a, tile, S);
+    if (L == 2) cc_sweep_tile<CC_HAMMING, 2, LT, CCB_PAD, false>(a, tile, S);
+    else if (L == 3) cc_sweep_tile<CC_HAMMING, 3, LT, CCB_PAD, false>(a, tile, S);
+    else cc_sweep_tile<CC_HAMMING, 4, LT, CCB_PAD, false>(a, tile, S);
   } else if (prob == CC_LEE) {
-    if (L == 2) cc_sweep_tile<CC_LEE, 2, LT, CCB_PAD>(a, tile, S);
-    else if (L == 3) cc_sweep_tile<CC_LEE, 3, LT, CCB_PAD>(a, tile, S);
-    else cc_sweep_tile<CC_LEE, 4, LT, CCB_PAD>(a, tile, S);
+    if (L == 2) cc_sweep_tile<CC_LEE, 2, LT, CCB_PAD, false>(a, tile, S);
+    else if (L == 3) cc_sweep_tile<CC_LEE, 3, LT, CCB_PAD, false>(a, tile, S);
+    else cc_sweep_tile<CC_LEE, 4, LT, CCB_PAD, false>(a, tile, S);
   } else if constexpr (sizeof(LT) == 2) {
-    if (prob == CC_TORUS) cc_sweep_tile<CC_TORUS, 1, LT, CCB_PAD>(a, tile, S);
-    else cc_sweep_tile<CC_GRID, 1, LT, CCB_PAD>(a, tile, S);
+    if (prob == CC_TORUS) cc_sweep_tile<CC_TORUS, 1, LT, CCB_PAD, false>(a, tile, S);
+    else cc_sweep_tile<CC_GRID, 1, LT, CCB_PAD, false>(a, tile, S);
   } else if (prob == CC_TORUS) {
     ccb_tile_linf<CC_TORUS>(a, tile, S);
   } else {

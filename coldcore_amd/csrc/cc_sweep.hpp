@@ -110,12 +110,72 @@ struct CcQRange {
 };
 
 // runtime-Q window sums for the L-inf families with axes longer than the
-// unrolled table (axes above 16, R <= 3): one fiber per thread, in place,
-// with a shift register of the 2R+1 original values under the window and
-// the first R originals kept for the circular wrap (no indexed arrays, so
-// nothing spills to scratch)
+// unrolled table (axes above 16, up to 1024, R <= 3).  A fiber's words are
+// cut into segments of CC_LSEG outputs, one (fiber, segment) per thread, so
+// a long axis keeps the whole workgroup busy (one thread per fiber left a
+// 1024-long torus axis to a single lane).  A pass covers whole fibers: every
+// thread first reads its segment's window (CC_LSEG + 2R words, wrapped on
+// the torus, clamped on the grid) into registers, the workgroup
+// synchronises, then the outputs are written in place; fibers of different
+// passes share no words, so no pass reads what an earlier one wrote.
+#define CC_LSEG 8
 template <int R, bool TORUS, bool PAD>
 __device__ __forceinline__ void cc_step_linf_rt(int32_t *__restrict__ S, int TL,
+                                                 int nf, int sk, int Q) {
+  const int segs = (Q + CC_LSEG - 1) / CC_LSEG;           // <= 128 for Q <= 1024
+  const int fpp = max(1, (int)blockDim.x / segs);          // fibers per pass
+  const uint32_t mT = cc_magic((uint32_t)TL), mS = cc_magic((uint32_t)sk);
+  const uint32_t mG = cc_magic((uint32_t)segs);
+  const int es = sk * TL;
+  const int fl = cc_div((int)threadIdx.x, mG), sg = (int)threadIdx.x - fl * segs;
+  const int a0 = sg * CC_LSEG;
+  for (int f0 = 0; f0 < nf; f0 += fpp) {
+    const int fi = f0 + fl;
+    const bool act = fl < fpp && fi < nf;
+    uint32_t out[CC_LSEG];
+    int e0 = 0;
+    if (act) {
+      const int f = cc_div(fi, mT), tl = fi - f * TL;
+      const int hi = cc_div(f, mS), lo = f - hi * sk;
+      e0 = (hi * sk * Q + lo) * TL + tl;
+      uint32_t w[CC_LSEG + 2 * R];
+#pragma unroll
+      for (int i = 0; i < CC_LSEG + 2 * R; i++) {
+        int b = a0 + i - R;              // -R .. Q + CC_LSEG + R - 2
+        uint32_t v = 0u;
+        if (TORUS) {
+          b += b < 0 ? Q : (b >= Q ? -Q : 0);
+          v = (uint32_t)S[cc_gpad<PAD>(e0 + b * es)];
+        } else if (b >= 0 && b < Q) {
+          v = (uint32_t)S[cc_gpad<PAD>(e0 + b * es)];
+        }
+        w[i] = v;
+      }
+#pragma unroll
+      for (int i = 0; i < CC_LSEG; i++) {
+        uint32_t sum = 0u;
+#pragma unroll
+        for (int j = 0; j <= 2 * R; j++) sum += w[i + j];
+        out[i] = sum;
+      }
+    }
+    __syncthreads();
+    if (act) {
+#pragma unroll
+      for (int i = 0; i < CC_LSEG; i++)
+        if (a0 + i < Q) S[cc_gpad<PAD>(e0 + (a0 + i) * es)] = (int32_t)out[i];
+    }
+    __syncthreads();
+  }
+}
+
+// the same window sums with one fiber per thread, in place, through a
+// shift register of the 2R+1 original values under the window and the
+// first R originals kept for the circular wrap (no indexed arrays, so
+// nothing spills to scratch): fewer registers and no extra barriers, which
+// the batched level kernel (axes up to 32 in its workloads) measured faster
+template <int R, bool TORUS, bool PAD>
+__device__ __forceinline__ void cc_step_linf_seq(int32_t *__restrict__ S, int TL,
                                                  int nf, int sk, int Q) {
   const uint32_t mT = cc_magic((uint32_t)TL), mS = cc_magic((uint32_t)sk);
   for (int fi = threadIdx.x; fi < nf; fi += blockDim.x) {
@@ -176,7 +236,8 @@ __device__ __forceinline__ void cc_lay_st(const SweepArgs &a, int d,
 // back (the batched transform calls it for tiles of many cells).
 #define CC_STAGE 8   // tile elements per thread whose loads are issued together
 
-template <int PROB, int R1, typename LT, bool PAD = true>
+// SEG: long L-inf axes by segments (true) or one fiber per thread (false)
+template <int PROB, int R1, typename LT, bool PAD = true, bool SEG = true>
 __device__ __forceinline__ void cc_sweep_tile(const SweepArgs &a, long long tile,
                                               int32_t *__restrict__ S) {
   constexpr int QMIN = CcQRange<PROB>::lo, QMAX = CcQRange<PROB>::hi;
@@ -240,9 +301,15 @@ __device__ __forceinline__ void cc_sweep_tile(const SweepArgs &a, long long tile
         // L-inf axes above 16 (R <= 3): the runtime-length window
         if constexpr (PROB == CC_TORUS || PROB == CC_GRID) {
           constexpr bool T = PROB == CC_TORUS;
-          if (a.R == 1) cc_step_linf_rt<1, T, PAD>(S, a.TL, nf, sk, Q);
-          else if (a.R == 2) cc_step_linf_rt<2, T, PAD>(S, a.TL, nf, sk, Q);
-          else cc_step_linf_rt<3, T, PAD>(S, a.TL, nf, sk, Q);
+          if constexpr (SEG) {
+            if (a.R == 1) cc_step_linf_rt<1, T, PAD>(S, a.TL, nf, sk, Q);
+            else if (a.R == 2) cc_step_linf_rt<2, T, PAD>(S, a.TL, nf, sk, Q);
+            else cc_step_linf_rt<3, T, PAD>(S, a.TL, nf, sk, Q);
+          } else {
+            if (a.R == 1) cc_step_linf_seq<1, T, PAD>(S, a.TL, nf, sk, Q);
+            else if (a.R == 2) cc_step_linf_seq<2, T, PAD>(S, a.TL, nf, sk, Q);
+            else cc_step_linf_seq<3, T, PAD>(S, a.TL, nf, sk, Q);
+          }
         }
         break;
     }
