@@ -381,8 +381,15 @@ void build_plan() {
     sw.nax = k - j;
     sw.P = (int)P;
     sw.lmax = layers_after(k);
-    // widen short strided tiles to ~4096 words so every thread has work
+    // an L-inf group of one long axis (up to 1024 words) may not fit the
+    // budget at full row width: narrow its tile (fewer columns) until it
+    // does (the generic tile addresses every column; no contiguity assumed)
     int TL = TL0;
+    while (j > 0 && TL > 1 && (E.prob == CC_TORUS || E.prob == CC_GRID) &&
+           (size_t)sw.lmax * cc_gplane((int)std::min<long long>(P * TL, 1 << 24)) * 4 >
+               (size_t)E.lds_budget)
+      TL /= 2;
+    // widen short strided tiles to ~4096 words so every thread has work
     while (j > 0 && TL * 2LL <= sw.s && P * TL < 4096 &&
            (size_t)sw.lmax * cc_gplane((int)(P * TL * 2)) * 4 <= (size_t)E.lds_budget)
       TL *= 2;
