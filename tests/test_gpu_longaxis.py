@@ -112,3 +112,35 @@ def test_long_axis_limits_are_enforced(gpu_cell):
                              ("hamming", [17, 2], 1)):
         with pytest.raises((RuntimeError, ValueError)):
             gpu_cell(axes, R, problem=problem)
+
+
+def test_malformed_long_tables_are_rejected():
+    """dct_init3x proves the table walk-safe first: a delta equal to its
+    axis, a position past n, a repeated position or a field after the
+    terminating zero returns -1 and leaves no cell."""
+    import ctypes
+    from coldcore_amd.gpu import load_plugin, _ptr, _U64P
+    from coldcore_amd.patterns import support_patterns_long
+    lib = load_plugin()
+    axes, R = [20, 24], 1
+    good = support_patterns_long(axes, R, "torus_linf")
+    ax = (ctypes.c_int * 2)(*axes)
+
+    def init(tbl):
+        tbl = np.ascontiguousarray(tbl, dtype=np.uint64)
+        by = ctypes.c_longlong(0)
+        rc = lib.dct_init3x(1, 2, ax, R, _ptr(tbl, _U64P), len(tbl), 1 << 10,
+                            ctypes.byref(by), 0, 0, 0)
+        if rc == 0:
+            lib.dct_free()
+        return rc
+
+    assert init(good) == 0
+    for bad in ((0 << 10) | 20,                 # delta == axis size
+                (2 << 10) | 1,                  # position past n
+                ((0 << 10) | 1) | (((0 << 10) | 2) << 16),   # repeated position
+                ((1 << 10) | 1) << 16):         # a field after the zero field
+        tbl = np.concatenate([good, np.array([bad], dtype=np.uint64)])
+        assert tbl.dtype == np.uint64
+        assert init(tbl) == -1, hex(bad)
+
