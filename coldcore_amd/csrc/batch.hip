@@ -98,10 +98,10 @@ int live_after(const BCell &c, int upto) {
 // x P words x TL columns, int32) fits the LDS budget; the first group is
 // contiguous (TL = 1), later ones stage up to 32 columns
 // axes per register round for a homogeneous Hamming group of axis size q
-// (cube of q^RR <= 16 words), 0 where the LDS tile runs
+// (cubes of 8 or 9 words), 0 where the LDS tile runs
 int round_axes_b(const BCell &c, int q) {
   if (c.prob != CC_HAMMING || !c.l16 || getenv("CCB_NO_ROUND")) return 0;
-  return q == 2 ? 4 : (q == 3 || q == 4 ? 2 : 0);
+  return q == 2 ? 3 : (q == 3 ? 2 : 0);
 }
 
 // LDS bytes of a round tile of pt words: layer pairs packed per dword, one
@@ -116,8 +116,8 @@ void plan_cell(int ci, const BCell &c) {
     const long long s = c.pw[j];
     const int TL = j == 0 ? 1 : (int)std::min<long long>(32, s);
     // a homogeneous Hamming group of 16-bit layers is sized by the round
-    // tile's packed footprint; binary / quaternary groups stop at 4,096
-    // words (three rounds)
+    // tile's packed footprint; binary groups stop at 4,096 words
+    // (four rounds of three axes)
     const int q = c.ax[j], rrq = round_axes_b(c, q);
     bool homog = true;
     long long P = 1;

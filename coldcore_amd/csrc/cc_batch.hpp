@@ -92,14 +92,18 @@ __device__ __forceinline__ void ccb_tile_t(const SweepArgs &a, int prob, int L,
   }
 }
 
-// homogeneous Hamming groups of 16-bit cells (axis size 2, 3 or 4, at
-// least rr axes): register rounds of rr axes per LDS round trip
+// homogeneous Hamming groups of 16-bit cells (axis size 2 or 3, at least
+// rr axes): register rounds of rr axes per LDS round trip.  Cubes stay at
+// 8 or 9 words (rr = 3 binary, 2 ternary): the 16-word cubes of the
+// singleton's round kernel would set this whole kernel's register count
+// (187 VGPRs, 2 waves per SIMD, against 112 and 4), and every tile kind of
+// a level shares it (64 cells: 133 -> 121 us per transform, the grid boxes
+// 105 -> 78 us, with the smaller cubes).
 template <int R1>
 __device__ __forceinline__ void ccb_tile_round(const SweepArgs &a, long long tile,
                                                int32_t *S) {
-  if (a.Q[0] == 2) cc_round_tile<CC_HAMMING, R1, 2, 4, uint16_t>(a, tile, S);
-  else if (a.Q[0] == 3) cc_round_tile<CC_HAMMING, R1, 3, 2, uint16_t>(a, tile, S);
-  else cc_round_tile<CC_HAMMING, R1, 4, 2, uint16_t>(a, tile, S);
+  if (a.Q[0] == 2) cc_round_tile<CC_HAMMING, R1, 2, 3, uint16_t>(a, tile, S);
+  else cc_round_tile<CC_HAMMING, R1, 3, 2, uint16_t>(a, tile, S);
 }
 
 __device__ __forceinline__ void ccb_tile(const SweepArgs &a, const BCell &c,
