@@ -323,30 +323,39 @@ __device__ __forceinline__ unsigned long long ccb_next(BCell &c) {
 // tail); a feasible cell will re-add one fewer (notch descent).  Ruin and
 // add leave amax / nzero cleared, so the reductions of a round need no
 // separate reset launch.
-__global__ void k_ccb_ruin(BCell *cells, int ncells, int k, BItem *items) {
+__global__ void k_ccb_ruin(BCell *__restrict__ cells, int ncells, int k,
+                           BItem *__restrict__ items) {
   const int ci = blockIdx.x * blockDim.x + threadIdx.x;
   if (ci >= ncells) return;
   BCell &c = cells[ci];
-  c.pre_uncov = c.uncov;
-  const int kk = min(k, c.M - 1);
-  c.nrem = kk > 0 ? kk : 0;
+  // the cell's state in registers: one round trip for the fields, one per
+  // removed word for its two code slots
+  const int M = c.M, uncov = c.uncov;
+  long long *__restrict__ code = c.code;
+  unsigned long long rng = c.rng;
+  const int kk = min(k, M - 1);
+  const int nrem = kk > 0 ? kk : 0;
+  c.pre_uncov = uncov;
+  c.nrem = nrem;
   c.nadd = 0;
-  c.add_budget = kk <= 0 ? 0 : (c.uncov == 0 ? kk - 1 : kk);
+  c.add_budget = kk <= 0 ? 0 : (uncov == 0 ? kk - 1 : kk);
   for (int j = 0; j < CCB_KMAX; j++) {
     BItem it{ci, 0, 0};
-    if (j < c.nrem) {
-      const int top = c.M - 1 - j;
-      const int r = (int)(ccb_next(c) % (unsigned long long)(top + 1));
-      const long long t = c.code[r];
-      c.code[r] = c.code[top];
-      c.code[top] = t;
+    if (j < nrem) {
+      const int top = M - 1 - j;
+      rng = ccb_mix(rng);
+      const int r = (int)(rng % (unsigned long long)(top + 1));
+      const long long t = code[r], u = code[top];
+      code[r] = u;
+      code[top] = t;
       c.rem[j] = t;
       it.delta = -1;
       it.w = t;
     }
     items[ci * CCB_KMAX + j] = it;
   }
-  c.M -= c.nrem;
+  c.rng = rng;
+  c.M = M - nrem;
   c.amax = 0ull;   // clear the accumulators for the round's first reduction
   c.nzero = 0ull;
 }
@@ -354,7 +363,8 @@ __global__ void k_ccb_ruin(BCell *cells, int ncells, int k, BItem *items) {
 // recreate step j: add the word of largest gain (smallest index on ties).
 // j < 0: a greedy step instead (add while the cell has uncovered words; the
 // added words are not recorded for a revert)
-__global__ void k_ccb_add(BCell *cells, int ncells, int j, BItem *items) {
+__global__ void k_ccb_add(BCell *__restrict__ cells, int ncells, int j,
+                          BItem *__restrict__ items) {
   const int ci = blockIdx.x * blockDim.x + threadIdx.x;
   if (ci >= ncells) return;
   BCell &c = cells[ci];
@@ -375,7 +385,8 @@ __global__ void k_ccb_add(BCell *cells, int ncells, int j, BItem *items) {
 // accept / revert: a round that uncovers more words is undone with
 // probability 0.9 (added words out, removed words back); a new best
 // covering code is flagged for k_ccb_snapshot
-__global__ void k_ccb_decide(BCell *cells, int ncells, BItem *items) {
+__global__ void k_ccb_decide(BCell *__restrict__ cells, int ncells,
+                             BItem *__restrict__ items) {
   const int ci = blockIdx.x * blockDim.x + threadIdx.x;
   if (ci >= ncells) return;
   BCell &c = cells[ci];
