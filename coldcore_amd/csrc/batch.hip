@@ -330,8 +330,9 @@ int round_launch(int k) {
   const dim3 cg((B.nc + 63) / 64), cb(64);
   hipLaunchKernelGGL(k_ccb_ruin, cg, cb, 0, B.st, B.d, B.nc, k, B.d_ruin);
   CKB(hipGetLastError());
-  hipLaunchKernelGGL(k_ccb_walk, dim3(B.nc * CCB_KMAX * CCB_WSPLIT), dim3(CC_BLOCK),
-                     0, B.st, B.d, B.d_ruin, CCB_WSPLIT);
+  // only the first k of a cell's CCB_KMAX ruin slots can hold a word
+  hipLaunchKernelGGL(k_ccb_walk, dim3(B.nc * k * CCB_WSPLIT), dim3(CC_BLOCK),
+                     0, B.st, B.d, B.d_ruin, CCB_WSPLIT, k, CCB_KMAX);
   CKB(hipGetLastError());
   for (int j = 0; j < k; j++) {
     // the gain transform leaves each cell's argmax in amax (its map sweep
@@ -340,7 +341,7 @@ int round_launch(int k) {
     hipLaunchKernelGGL(k_ccb_add, cg, cb, 0, B.st, B.d, B.nc, j, B.d_add);
     CKB(hipGetLastError());
     hipLaunchKernelGGL(k_ccb_walk, dim3(B.nc * CCB_WSPLIT), dim3(CC_BLOCK), 0, B.st,
-                       B.d, B.d_add, CCB_WSPLIT);
+                       B.d, B.d_add, CCB_WSPLIT, 1, 1);
     CKB(hipGetLastError());
   }
   if (int rc = reduce_launch(0, false)) return rc;
@@ -348,7 +349,8 @@ int round_launch(int k) {
   CKB(hipGetLastError());
   if (int rc = snapshot_launch()) return rc;
   hipLaunchKernelGGL(k_ccb_walk, dim3(B.nc * 2 * CCB_KMAX * CCB_WSPLIT),
-                     dim3(CC_BLOCK), 0, B.st, B.d, B.d_rev, CCB_WSPLIT);
+                     dim3(CC_BLOCK), 0, B.st, B.d, B.d_rev, CCB_WSPLIT,
+                     2 * CCB_KMAX, 2 * CCB_KMAX);
   CKB(hipGetLastError());
   return 0;
 }
@@ -600,7 +602,7 @@ int ccb_greedy(int steps) {
                        B.nc, -1, B.d_add);
     CKB(hipGetLastError());
     hipLaunchKernelGGL(k_ccb_walk, dim3(B.nc * CCB_WSPLIT), dim3(CC_BLOCK), 0, B.st,
-                       B.d, B.d_add, CCB_WSPLIT);
+                       B.d, B.d_add, CCB_WSPLIT, 1, 1);
     CKB(hipGetLastError());
   }
   if (int rc = reduce_launch(0)) return rc;

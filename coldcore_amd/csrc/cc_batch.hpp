@@ -160,8 +160,10 @@ __global__ void k_ccb_scatter(BCell *cells) {
 // from high-multiply divisions (exact: offset * (2R + 1) < 2^32).
 __global__ void __launch_bounds__(CC_BLOCK)
     k_ccb_walk(const BCell *__restrict__ cells, const BItem *__restrict__ items,
-               int split) {
-  const int ii = blockIdx.x / split, seg = blockIdx.x - ii * split;
+               int split, int ipc, int istride) {
+  // the launch walks the first ipc of every cell's istride item slots
+  const int q = blockIdx.x / split, seg = blockIdx.x - q * split;
+  const int cell = q / ipc, ii = cell * istride + (q - cell * ipc);
   const BItem itm = items[ii];
   if (itm.delta == 0) return;
   const BCell &c = cells[itm.cell];
