@@ -595,9 +595,13 @@ int ccb_lns_run(int rounds, int k, int graph, float *ms) {
 // still uncovered after the steps (>= 0).
 int ccb_greedy(int steps) {
   if (!B.live || steps < 0) return -1;
+  // each step's gain transform folds the argmax into amax (cleared here
+  // once, then by every k_ccb_add), so a step launches no reduction
+  hipLaunchKernelGGL(k_ccb_reset_red, dim3((B.nc + 63) / 64), dim3(64), 0, B.st,
+                     B.d, B.nc);
+  CKB(hipGetLastError());
   for (int s = 0; s < steps; s++) {
     if (int rc = transform_launch(0)) return rc;
-    if (int rc = reduce_launch(1)) return rc;
     hipLaunchKernelGGL(k_ccb_add, dim3((B.nc + 63) / 64), dim3(64), 0, B.st, B.d,
                        B.nc, -1, B.d_add);
     CKB(hipGetLastError());

@@ -371,7 +371,9 @@ __global__ void k_ccb_add(BCell *__restrict__ cells, int ncells, int j,
   if (ci >= ncells) return;
   BCell &c = cells[ci];
   BItem it{ci, 0, 0};
-  const bool go = j < 0 ? c.nzero != 0ull : j < c.add_budget;
+  // greedy (j < 0): a cell with an uncovered word has a positive gain
+  // somewhere (mu = 1: the word itself), so amax != 0 below is the test
+  const bool go = j < 0 ? true : j < c.add_budget;
   if (go && c.amax != 0ull && c.M < c.cap) {
     const long long w = (long long)(0xFFFFFFFFull - (c.amax & 0xFFFFFFFFull));
     c.code[c.M++] = w;
