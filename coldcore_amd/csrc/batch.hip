@@ -348,9 +348,9 @@ int round_launch(int k) {
   hipLaunchKernelGGL(k_ccb_decide, cg, cb, 0, B.st, B.d, B.nc, B.d_rev);
   CKB(hipGetLastError());
   if (int rc = snapshot_launch()) return rc;
-  hipLaunchKernelGGL(k_ccb_walk, dim3(B.nc * 2 * CCB_KMAX * CCB_WSPLIT),
+  hipLaunchKernelGGL(k_ccb_walk, dim3(B.nc * 2 * k * CCB_WSPLIT),
                      dim3(CC_BLOCK), 0, B.st, B.d, B.d_rev, CCB_WSPLIT,
-                     2 * CCB_KMAX, 2 * CCB_KMAX);
+                     2 * k, 2 * CCB_KMAX);
   CKB(hipGetLastError());
   return 0;
 }

@@ -406,8 +406,10 @@ __global__ void k_ccb_decide(BCell *__restrict__ cells, int ncells,
     for (int j = 0; j < c.nadd; j++)
       items[ci * 2 * CCB_KMAX + j] = BItem{ci, -1, c.add[j]};
     c.M -= c.nadd;
+    // the removed words follow the added ones, so a round of ruin size k
+    // fills at most the first 2k slots (the revert walk launches only those)
     for (int j = 0; j < c.nrem; j++) {
-      items[ci * 2 * CCB_KMAX + CCB_KMAX + j] = BItem{ci, 1, c.rem[j]};
+      items[ci * 2 * CCB_KMAX + c.nadd + j] = BItem{ci, 1, c.rem[j]};
       c.code[c.M++] = c.rem[j];
     }
     c.uncov = c.pre_uncov;
