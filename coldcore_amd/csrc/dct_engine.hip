@@ -177,6 +177,11 @@ struct Engine {
 
 Engine E;
 
+// Forced sweep boundaries of the packed schedules (dct_sweep_force): bit k
+// set means a sweep ends after axis k - 1 (0 < k < n).  0: the planner
+// chooses.  Cleared by init and free.
+uint64_t g_cuts = 0;
+
 bool set_ok() { return E.fmt == 1 ? E.set_ok2 : E.set_ok1; }
 
 Dig make_dig() {
@@ -818,7 +823,12 @@ PPlan pk_plan(double s0, double cap, bool pred_seed, bool cnt_out) {
                               &prev.out_nd, &prev.out_w8, &prev.out_nar))
       continue;
     prev.out_bytes = prev.out_nar ? 1 + prev.out_nar : 4 * prev.out_nd;
+    // forced boundaries: the sweep from j ends at the next cut (or at n)
+    int kf = n;
+    for (int c = j + 1; g_cuts && c < n; c++)
+      if ((g_cuts >> c) & 1) { kf = c; break; }
     for (int k = j + 1; k <= std::min(n, j + CC_MAXSW); k++) {
+      if (g_cuts && k != kf) continue;
       PSweep w;
       const bool okw = pk_sweep(B, j, k, pred_seed, cnt_out, j ? &prev : nullptr, w);
       if (getenv("CC_PK_DEBUG"))
@@ -906,7 +916,15 @@ int run_ptransform(const PPlan &pl, int in_kind, int out_kind, int plo,
 PPlan pk_code_plan(bool cnt_out) {
   int mmax = 0;
   for (const auto &kv : E.uploaded) mmax = std::max(mmax, kv.second);
-  return pk_plan(std::max(mmax, 1), std::max(E.codeM, 1), false, cnt_out);
+  PPlan p = pk_plan(std::max(mmax, 1), std::max(E.codeM, 1), false, cnt_out);
+  if (!p.ok && g_cuts) {
+    // forced boundaries a code seed cannot use: the planner's own schedule
+    const uint64_t keep = g_cuts;
+    g_cuts = 0;
+    p = pk_plan(std::max(mmax, 1), std::max(E.codeM, 1), false, cnt_out);
+    g_cuts = keep;
+  }
+  return p;
 }
 
 int sync_stream() {
@@ -1211,6 +1229,7 @@ void release_all() {
   E.fmt = 0;
   E.nswept = 0;
   E.pfield = PPlan();
+  g_cuts = 0;
 }
 
 }  // namespace
@@ -1443,6 +1462,7 @@ static int init_cell(int problem, int n, const int *ax, int R,
                           "CC_PK_COL", "CC_PK_FIRST", "CC_PK_NARROW", "CC_LAYER16"})
       if (getenv(k) && !pa) E.autotune = 0;
     E.tuned = 0;
+    g_cuts = 0;
     E.pfield = pk_plan(1.0, 1e300, true, false);
   }
   E.uploaded.clear();
@@ -1860,6 +1880,16 @@ int dct_read_cnt(long long off, long long len, uint16_t *out) {
   return 0;
 }
 
+// additive: a contiguous range of the map plane to the host (the map-plane
+// counterpart of dct_read_cnt; one copy, no index list)
+int dct_map_read(long long off, long long len, int32_t *out) {
+  if (!E.live || off < 0 || len < 0 || off + len > E.X || (len && !out))
+    return -1;
+  CK(hipStreamSynchronize(E.st));
+  if (len) CK(hipMemcpy(out, E.A[0] + off, (size_t)len * 4, hipMemcpyDeviceToHost));
+  return 0;
+}
+
 // ---- optional ox table (reference dct_hc.cu:430-450) ----------------------
 // The ox table must enumerate the same support set; walks are order-free, so
 // it is converted to the v1 byte form once and walked by the same kernels.
@@ -2164,6 +2194,37 @@ int dct_sweep_plan2(int kind, int *out, int cap) {
     o[12] = 0;
   }
   return ns;
+}
+
+// dct_sweep_force(cuts, ncuts): pin the sweep boundaries of the packed
+// schedules (ascending axis counts 1..n-1; each sweep ends at the next
+// cut).  On a live cell the gain / loss schedule is replanned at once and
+// the automatic timing is switched off; a recount whose code seed cannot
+// use the cuts keeps the planner's own schedule.  ncuts = 0 clears them.
+// With no live cell the cuts steer dct_plan_probe.  Returns the gain / loss
+// sweep count (0 with no live cell), -1 on bad cuts or when no packed
+// schedule has these boundaries (the cuts are then cleared).
+int dct_sweep_force(const int *cuts, int ncuts) {
+  if (ncuts < 0 || ncuts >= CC_MAXN || (ncuts > 0 && !cuts)) return -1;
+  uint64_t m = 0;
+  int last = 0;
+  for (int i = 0; i < ncuts; i++) {
+    if (cuts[i] <= last || cuts[i] >= CC_MAXN || (E.live && cuts[i] >= E.n))
+      return -1;
+    last = cuts[i];
+    m |= 1ull << cuts[i];
+  }
+  g_cuts = m;
+  if (!E.live) return 0;
+  E.autotune = 0;
+  E.tuned = 1;
+  E.pfield = pk_plan(1.0, 1e300, true, false);
+  if (!E.pfield.ok && m) {
+    g_cuts = 0;
+    E.pfield = pk_plan(1.0, 1e300, true, false);
+    return -1;
+  }
+  return E.pfield.ok ? (int)E.pfield.sw.size() : 0;
 }
 
 // dct_sweep_profile(1): record HIP events around every sweep of later

@@ -34,7 +34,7 @@ OPTIONAL = ("dct_map_hist", "dct_set_mu", "dct_get_mu", "dct_ball_gather_mu",
             "dct_swap_eval", "dct_stats")
 ADDITIVE = ("dct_init3w", "dct_init3x", "dct_sweep_plan", "dct_sweep_plan2",
             "dct_plan_probe", "dct_autotune", "dct_sweep_profile",
-            "dct_sweep_times",
+            "dct_sweep_times", "dct_sweep_force", "dct_map_read",
             "dct_transform_layers", "dct_layers_read_at")
 
 
@@ -70,7 +70,7 @@ def _bind(path):
               "dct_ball_gather_mu", "dct_loss_owner", "dct_set_tbl2",
               "dct_use_fmt", "dct_stats", "dct_sweep_plan",
               "dct_sweep_plan2", "dct_plan_probe", "dct_autotune",
-              "dct_sweep_profile",
+              "dct_sweep_profile", "dct_sweep_force", "dct_map_read",
               "dct_sweep_times", "dct_transform_layers",
               "dct_layers_read_at"):
         getattr(lib, f).restype = ctypes.c_int
@@ -78,6 +78,7 @@ def _bind(path):
               "dct_map_extract64", "dct_host_bytes"):
         getattr(lib, f).restype = _LL
     lib.dct_read_cnt.argtypes = [_LL, _LL, ctypes.POINTER(ctypes.c_uint16)]
+    lib.dct_map_read.argtypes = [_LL, _LL, _I32P]
     lib.dct_init.restype = ctypes.c_int
     lib.dct_init.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _U64P,
                              _LL, ctypes.c_int, _LLP]
@@ -353,8 +354,13 @@ class GpuBackend:
             _ptr(idx, _LLP), len(idx), _ptr(out, _I32P)))
         return out
 
-    def map_all(self):
-        return self.map_read_at(np.arange(self.space.size, dtype=np.int64))
+    def map_all(self, off=0, length=None):
+        """Map-plane entries [off, off+length) (default: the whole plane)."""
+        length = self.space.size - off if length is None else length
+        out = np.empty(length, dtype=np.int32)
+        _check("map_read", self._lib.dct_map_read(int(off), int(length),
+                                                  _ptr(out, _I32P)))
+        return out
 
     def read_cnt(self, off=0, length=None):
         length = self.space.size - off if length is None else length
@@ -415,6 +421,17 @@ class GpuBackend:
             _check("autotune", n)
         return n, float(us.value)
 
+    def sweep_force(self, cuts):
+        """Pin the packed schedule's sweep boundaries (ascending axis
+        counts; [] clears) and return the gain / loss sweep count; raises
+        when no packed schedule has these boundaries."""
+        c = np.ascontiguousarray(list(cuts), dtype=np.int32)
+        ns = self._lib.dct_sweep_force(
+            c.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), len(c))
+        if ns < 0:
+            raise RuntimeError(f"dct_sweep_force{tuple(c)} rc={ns}")
+        return ns
+
     def sweep_profile(self, on=True):
         _check("sweep_profile", self._lib.dct_sweep_profile(1 if on else 0))
 
@@ -430,15 +447,23 @@ PLAN_KEYS = ("first_axis", "axes", "P", "TL", "RR", "W", "ND", "bytes_in",
              "bytes_out", "lds_bytes", "threads", "rounds", "kernel")
 
 
-def plan_probe(axes, R, kind=0, s0=1, M=1, path=None):
+def plan_probe(axes, R, kind=0, s0=1, M=1, path=None, cuts=None):
     """The packed schedule the engine would plan for a homogeneous Hamming
     cell, computed on the host (no device, no live cell).  [] when the cell
-    runs the unpacked kernels."""
+    runs the unpacked kernels.  ``cuts`` pins the sweep boundaries (as
+    GpuBackend.sweep_force)."""
     lib = load_plugin(path)
     ax = np.ascontiguousarray(axes, dtype=np.int32)
     buf = (ctypes.c_int * (13 * 41))()
-    ns = lib.dct_plan_probe(len(ax), ax.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
-                            int(R), int(kind), int(s0), int(M), buf, 40)
+    c = np.ascontiguousarray(list(cuts or ()), dtype=np.int32)
+    if lib.dct_sweep_force(c.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                           len(c)) < 0:
+        raise RuntimeError(f"dct_sweep_force{tuple(c)}: bad cuts")
+    try:
+        ns = lib.dct_plan_probe(len(ax), ax.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                int(R), int(kind), int(s0), int(M), buf, 40)
+    finally:
+        lib.dct_sweep_force(None, 0)
     if ns < 0:
         raise RuntimeError(f"plan_probe rc={ns}")
     return [dict(zip(PLAN_KEYS, buf[13 * i:13 * i + 13])) for i in range(ns)]

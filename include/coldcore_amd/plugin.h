@@ -192,6 +192,21 @@ int  dct_plan_probe(int n, const int *ax, int R, int kind, int s0, int M,
  * Returns the number of schedules timed (*best_us: microseconds per
  * transform of the winner), or < 0. */
 int  dct_autotune(int reps, float *best_us);
+/* Copy map-plane entries [off, off+len) (int32) to out.  The map-plane
+ * counterpart of dct_read_cnt (no reference counterpart: the reference
+ * reads the map only at listed indices, dct_map_read_at); 0 or -1. */
+int  dct_map_read(long long off, long long len, int32_t *out);
+/* Pin the sweep boundaries of the packed schedules: cuts[0..ncuts-1]
+ * ascending axis counts in 1..n-1, each sweep ends at the next cut (the
+ * planner still picks each sweep's kernel, tile and formats).  On a live
+ * cell the gain / loss schedule is replanned at once and automatic timing
+ * is switched off; a recount whose code seed cannot use the cuts keeps the
+ * planner's schedule.  ncuts = 0 clears.  With no live cell the cuts steer
+ * dct_plan_probe; init and free clear them.  Returns the gain / loss sweep
+ * count (0 with no live cell), -1 on bad cuts or when no packed schedule
+ * has these boundaries.  No reference counterpart: lets tests run every
+ * kernel instantiation a large cell's schedules use on a small cell. */
+int  dct_sweep_force(const int *cuts, int ncuts);
 /* Record HIP events around every sweep of later transforms (on the
  * engine's stream; also enabled by DCT_TIMING=1 at init). */
 int  dct_sweep_profile(int on);

@@ -359,3 +359,72 @@ def test_lns_best_codes_cover():
             assert 0 < st["best"] <= start[i]
             o.recount(best)
             assert int((o.cnt == 0).sum()) == 0, f"best code of cell {i}"
+
+
+def test_batch64_workload_matches_oracle():
+    """The 64-cell batched workload exactly as benchmarked (Hamming K_3(n,R)
+    for n = 8..12 and R = 1..3, Lee 9x9x9, king tori 9..31, 25 grid boxes up
+    to 2^20 words; the batch's autotuned plan; seeded random starts at 1.5x
+    the device greedy size): recount, gain and loss of all 64 cells equal
+    the oracle, then 20 LNS rounds equal the CPU restatement word for word
+    on every cell of at most 2^15 words."""
+    import bench
+    from coldcore_amd.batch import GpuBatch
+    cells = bench.batch_cells64()
+    assert len(cells) == 64
+    with GpuBatch(cells, cap=65535) as b:
+        b.autotune(reps=2)
+        empty = np.zeros(0, dtype=np.int64)
+        for i in range(len(cells)):
+            b.set_code(i, empty)
+        b.recount()
+        b.greedy_cover(256)
+        starts = []
+        for i, (_, a, _) in enumerate(cells):
+            size = int(np.prod(a))
+            greedy = b.stats(i)["M"]
+            assert b.stats(i)["uncovered"] == 0
+            m = min(size, 65535, int(round(1.5 * greedy)))
+            starts.append(np.random.default_rng(i).choice(size, m, replace=False)
+                          .astype(np.int64))
+        for i, c in enumerate(starts):
+            b.set_code(i, c)
+        b.recount()
+        oracles = []
+        for i, (p, axes, R) in enumerate(cells):
+            o = NumpyDP(axes, R, problem=p)
+            o.recount(starts[i])
+            oracles.append(o)
+            np.testing.assert_array_equal(b.read_cnt(i), o.cnt.astype(np.uint16),
+                                          err_msg=f"cnt cell {i} {p} {axes} R{R}")
+        for which in ("gain", "loss"):
+            getattr(b, which + "_maps")()
+            for i, o in enumerate(oracles):
+                getattr(o, which + "_map")()
+                np.testing.assert_array_equal(b.read_map(i), o.map,
+                                              err_msg=f"{which} cell {i} {cells[i]}")
+        # LNS from the same starts: the device runs all 64 cells, the CPU
+        # restatement follows the small ones (cells are independent)
+        small = [i for i, (_, a, _) in enumerate(cells) if int(np.prod(a)) <= 1 << 15]
+        assert len(small) >= 30
+        cpus = {}
+        for i in small:
+            p, a, R = cells[i]
+            c = CpuCell(p, a, R, 65535)
+            c.code = starts[i].tolist()
+            c.o.recount(starts[i])
+            cpus[i] = c
+        seed, k = 0, 2
+        b.lns_init(seed)
+        for i, c in cpus.items():
+            c.lns_init(seed, i)
+        b.lns_run(20, k=k, graph=True)
+        for i, c in cpus.items():
+            for _ in range(20):
+                c.lns_round(k)
+            st = b.stats(i)
+            assert b.code(i).tolist() == c.code, f"cell {i} {cells[i]} code"
+            np.testing.assert_array_equal(b.read_cnt(i), c.o.cnt.astype(np.uint16))
+            assert st["uncovered"] == c.un
+            assert (st["rounds"], st["accepted"], st["reverted"]) == \
+                (c.rounds, c.accepted, c.reverted)

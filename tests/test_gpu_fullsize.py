@@ -79,3 +79,64 @@ def test_large_cells_against_numpy_oracle(gpu_cell, axes, R):
     ci, cv, cf = c.map_extract(max(1, mx // 2), cap=1 << 18)
     assert gf == cf and (gi == ci).all() and (gv == cv).all()
     assert (g.map_hist(1024, mx) == c.map_hist(1024, mx)).all()
+
+
+def _ball_masks(n, R):
+    """xor masks of the radius-R Hamming ball in Z_2^n (every popcount <= R)."""
+    from itertools import combinations
+    out = [0]
+    for d in range(1, R + 1):
+        for c in combinations(range(n), d):
+            out.append(sum(1 << i for i in c))
+    return np.array(out, dtype=np.int64)
+
+
+@pytest.mark.parametrize("tuning", ["autotuned", "planner"])
+def test_k2_30_4_full_size_invariants(gpu_cell, monkeypatch, tuning):
+    """K_2(30,4), 2^30 words, the 40,000-word seeded code of the benchmark,
+    under the autotuner's schedule and under the planner's own: the
+    translation-invariance sums, a brute-force count at sampled words,
+    brute-force gain / loss at a few words from the downloaded cnt, and the
+    walks equal to the fields there."""
+    if tuning == "planner":
+        monkeypatch.setenv("CC_AUTOTUNE", "0")
+    n, R = 30, 4
+    size = 1 << n
+    code = np.random.default_rng(0).choice(size, 40000, replace=False).astype(np.int64)
+    masks = _ball_masks(n, R)
+    B = ball_size(2, n, R)
+    assert len(masks) == B == 31931
+    g = gpu_cell([2] * n, R)
+    g.recount(code)
+    cnt = g.read_cnt()
+    assert int(cnt.sum(dtype=np.int64)) == len(code) * B
+    rng = np.random.default_rng(5)
+    probe = np.concatenate([code[:64], rng.integers(0, size, 4096)]).astype(np.int64)
+    want = np.empty(len(probe), dtype=np.int64)
+    for i in range(0, len(probe), 128):
+        x = (probe[i:i + 128, None] ^ code[None, :]).astype(np.uint64)
+        want[i:i + 128] = (np.bitwise_count(x) <= R).sum(axis=1)
+    assert (cnt[probe].astype(np.int64) == want).all()
+    zeros = int((cnt == 0).sum())
+    ones = int((cnt == 1).sum())
+    assert g.count_eq(0) == zeros and g.count_eq(1) == ones
+    few = probe[::256]
+    g.gain_map()
+    plan = g.sweep_plan2(0)
+    print(f"K_2(30,4) {tuning} schedule:",
+          [(s["first_axis"], s["axes"], s["kernel"], s["RR"], s["W"], s["ND"],
+            s["bytes_in"], s["bytes_out"]) for s in plan], flush=True)
+    tot = 0
+    for off in range(0, size, 1 << 26):
+        tot += int(g.map_all(off, 1 << 26).sum(dtype=np.int64))
+    assert tot == B * zeros
+    gain = g.map_read_at(probe)
+    assert (g.ball_gather(probe, 0) == gain).all()
+    for x, v in zip(few, gain[::256]):
+        assert int((cnt[x ^ masks] == 0).sum()) == v
+    g.loss_map()
+    assert int(g.map_read_at(code).sum(dtype=np.int64)) == ones
+    loss = g.map_read_at(probe)
+    assert (g.ball_gather(probe, 1) == loss).all()
+    for x, v in zip(few, loss[::256]):
+        assert int((cnt[x ^ masks] == 1).sum()) == v

@@ -146,3 +146,83 @@ def test_autotune_unpacked_budgets_keep_fields_exact(monkeypatch):
         after = fields()
     for a, b in zip(before, after):
         assert (a == b).all()
+
+
+# ---- the kernel instantiations of the 2^30-word R = 4 schedules ----------
+# K_2(30,4) runs, depending on the autotuner's pick, a 10- or 12-axis first
+# sweep (k_pfirstp with 8-bit fields in 2 dwords, or 16-bit fields in 3)
+# and five- or six-axis column sweeps (one lane or a lane pair per cube)
+# whose compact inter-sweep words are one dword plane up to 15 axes and two
+# planes from 16 axes on (field d takes bitlen(C(c,d)) bits).  The cuts
+# below reproduce every one of those kernels, with one- and two-plane words
+# in and out, on 2^22 / 2^23-word cells, checked against the Walsh-Hadamard
+# convolution.  Rows per sweep: (first axis, axes, kernel, RR, W, ND, bytes
+# in, bytes out); kernel 5 = persistent first sweep, 2 / 3 = column kernel
+# with 1 / 2 lanes per cube, 1 = LDS-tiled.
+FORCED_R4 = {
+    "first12-col2": (22, [12, 17], "2", [(0, 12, 5, 5, 16, 3, 2, 4),
+                                         (12, 5, 3, 5, 16, 3, 4, 8),
+                                         (17, 5, 3, 5, 16, 3, 8, 4)]),
+    "first12-col1": (22, [12, 17], "1", [(0, 12, 5, 5, 16, 3, 2, 4),
+                                         (12, 5, 2, 5, 16, 3, 4, 8),
+                                         (17, 5, 2, 5, 16, 3, 8, 4)]),
+    "first10-col1": (22, [10, 15, 20], "1", [(0, 10, 5, 4, 8, 2, 2, 4),
+                                             (10, 5, 2, 5, 16, 3, 4, 4),
+                                             (15, 5, 2, 5, 16, 3, 4, 8),
+                                             (20, 2, 2, 2, 16, 3, 8, 4)]),
+    "first10-col2": (22, [10, 15, 20], "2", [(0, 10, 5, 4, 8, 2, 2, 4),
+                                             (10, 5, 3, 5, 16, 3, 4, 4),
+                                             (15, 5, 3, 5, 16, 3, 4, 8),
+                                             (20, 2, 2, 2, 16, 3, 8, 4)]),
+    "mid5-col1": (22, [10, 16, 21], "1", [(0, 10, 5, 4, 8, 2, 2, 4),
+                                          (10, 6, 1, 5, 16, 3, 4, 8),
+                                          (16, 5, 2, 5, 16, 3, 8, 8),
+                                          (21, 1, 1, 1, 16, 3, 8, 4)]),
+    "mid5-col2": (22, [10, 16, 21], "2", [(0, 10, 5, 4, 8, 2, 2, 4),
+                                          (10, 6, 3, 6, 16, 3, 4, 8),
+                                          (16, 5, 3, 5, 16, 3, 8, 8),
+                                          (21, 1, 1, 1, 16, 3, 8, 4)]),
+    "pair6-last": (22, [10, 16], "2", [(0, 10, 5, 4, 8, 2, 2, 4),
+                                       (10, 6, 3, 6, 16, 3, 4, 8),
+                                       (16, 6, 3, 6, 16, 3, 8, 4)]),
+    "pair6-mid": (23, [10, 16, 22], "2", [(0, 10, 5, 4, 8, 2, 2, 4),
+                                          (10, 6, 3, 6, 16, 3, 4, 8),
+                                          (16, 6, 3, 6, 16, 3, 8, 8),
+                                          (22, 1, 1, 1, 16, 3, 8, 4)]),
+}
+_WALSH = {}
+
+
+def _walsh_fields(n, R):
+    """(code, cnt, gain, loss) of a seeded multiset code on K_2(n,R) by the
+    Walsh-Hadamard convolution; computed once per n."""
+    from oracle.walsh import binary_field
+    if (n, R) not in _WALSH:
+        rng = np.random.default_rng(100 + n)
+        base = np.flatnonzero(rng.random(1 << n) < 4e-4).astype(np.int64)
+        code = np.repeat(base, rng.integers(1, 4, size=len(base)))
+        ind = np.bincount(code, minlength=1 << n).astype(np.int64)
+        cnt = binary_field(ind, n, R)
+        gain = binary_field((cnt == 0).astype(np.int64), n, R)
+        loss = binary_field((cnt == 1).astype(np.int64), n, R)
+        _WALSH[(n, R)] = (code, cnt, gain, loss)
+    return _WALSH[(n, R)]
+
+
+@pytest.mark.parametrize("case", list(FORCED_R4))
+def test_r4_schedule_kernels_match_walsh(gpu_cell, monkeypatch, case):
+    n, cuts, col, rows = FORCED_R4[case]
+    monkeypatch.setenv("CC_PK_COL", col)
+    print(f"forced R=4 schedule {case}: n={n} cuts={cuts} col={col}", flush=True)
+    code, cnt, gain, loss = _walsh_fields(n, 4)
+    g = gpu_cell([2] * n, 4)
+    assert g.sweep_force(cuts) == len(rows)
+    got = [(s["first_axis"], s["axes"], s["kernel"], s["RR"], s["W"], s["ND"],
+            s["bytes_in"], s["bytes_out"]) for s in g.sweep_plan2(0)]
+    assert got == rows
+    g.recount(code)
+    assert (g.read_cnt().astype(np.int64) == cnt).all()
+    g.gain_map()
+    assert (g.map_all().astype(np.int64) == gain).all()
+    g.loss_map()
+    assert (g.map_all().astype(np.int64) == loss).all()

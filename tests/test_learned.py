@@ -81,7 +81,8 @@ def test_logging_and_features_on_oracle():
 # ------------------------------------------------------------------ GPU
 @pytest.mark.gpu
 @pytest.mark.parametrize("problem,axes,R,env", [
-    ("hamming", [3] * 10, 2, {}), ("hamming", [2] * 14, 3, {}),
+    ("hamming", [3] * 10, 2, {}), ("hamming", [4] * 8, 2, {}),
+    ("hamming", [2] * 14, 3, {}),
     ("hamming", [4, 5, 4, 5, 3], 2, {}), ("lee", [5, 6, 7, 5], 3, {}),
     ("hamming", [3] * 10, 2, {"CC_LAYER16": "0"}),
     ("hamming", [2] * 14, 3, {"CC_SWEEP_LDS": "4096"})])
