@@ -15,7 +15,7 @@ CSRC = os.path.join(HERE, "csrc")
 UNITS = ("dct_engine.hip", "sweep_hamming.hip", "sweep_lee.hip",
          "sweep_linf.hip", "sweep_round.hip", "sweep_packed_q2.hip",
          "sweep_packed_q3.hip", "sweep_packed_q4.hip", "sweep_packed_q5.hip",
-         "mlp.hip", "batch.hip")
+         "mlp.hip", "batch.hip", "noalloc.hip")
 LIB = os.path.join(HERE, "libcoldcore_covering.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

@@ -196,6 +196,21 @@ int  dct_autotune(int reps, float *best_us);
  * counterpart of dct_read_cnt (no reference counterpart: the reference
  * reads the map only at listed indices, dct_map_read_at); 0 or -1. */
 int  dct_map_read(long long off, long long len, int32_t *out);
+/* Allocation-free ordered extraction: the dct_map_extract64 compaction of
+ * a caller's device map (int32 map[0..n), 16-byte aligned) into caller
+ * device buffers idx[0..cap) / val[0..cap), ascending index order, with
+ * the exact hit count written to the device int64 *found; queued on
+ * `stream` (a hipStream_t; NULL: the null stream) with no host
+ * synchronisation.  No reference counterpart: it is for callers that cannot
+ * allocate (code under HIP graph capture, serving loops).  It calls no
+ * hipMalloc / hipFree and needs no live cell: its chunk counts and offsets
+ * live in a static 1 MiB device array per GPU, so n is limited to 2^30
+ * words (larger n returns -1 with a message on stderr), and calls that
+ * overlap on one GPU race.  Returns 0 once queued, -1 on bad arguments,
+ * -(100 + hipError_t) on a device error. */
+int  dct_map_extract64_noalloc(const int32_t *map, long long n, int32_t thr,
+                               int64_t *idx, int32_t *val, long long cap,
+                               long long *found, void *stream);
 /* Pin the sweep boundaries of the packed schedules: cuts[0..ncuts-1]
  * ascending axis counts in 1..n-1, each sweep ends at the next cut (the
  * planner still picks each sweep's kernel, tile and formats).  On a live
