@@ -13,7 +13,8 @@ CSRC = os.path.join(HERE, "csrc")
 #: translation units: the ABI (walks, reductions, host logic) and one per
 #: sweep-kernel family, compiled in parallel and linked into one library
 UNITS = ("dct_engine.hip", "sweep_hamming.hip", "sweep_lee.hip",
-         "sweep_linf.hip", "sweep_round.hip", "sweep_packed_q2.hip",
+         "sweep_linf.hip", "sweep_round.hip", "sweep_round_q67.hip",
+         "sweep_round_q810.hip", "sweep_packed_q2.hip",
          "sweep_packed_q3.hip", "sweep_packed_q4.hip", "sweep_packed_q5.hip",
          "mlp.hip", "batch.hip", "noalloc.hip")
 LIB = os.path.join(HERE, "libcoldcore_covering.so")

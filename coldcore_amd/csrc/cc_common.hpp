@@ -97,6 +97,11 @@ __device__ __forceinline__ long long cc_block_sum(long long v) {
 #define CC_OUT_LAYERS 0  // write back the layers this sweep produced
 #define CC_OUT_MAP 1     // write sum_d A_d into A0 (gain / loss map)
 #define CC_OUT_CNT 2     // write min(sum_d A_d, 65535) into cnt (recount)
+// compact inter-sweep words: every layer of a word (layer 0 included) is a
+// bit field of the word's 1..4 dwords, sized by a proven bound of its value
+#define CC_IN_COMPACT 3
+#define CC_OUT_COMPACT 3
+#define CC_CMAXND 4      // dwords per compact word
 
 struct SweepArgs {
   int32_t *A[CC_MAXL];   // int32 planes: A[0] is the map / code indicator
@@ -119,6 +124,14 @@ struct SweepArgs {
   // one per axis for Hamming and floor(ax/2) for Lee
   int live[CC_MAXSW + 1];
   int rr;            // batched sweeps: axes per register round (0: LDS tile)
+  // compact words: dword k of word w is pk[k * pitch + w]; layer d is the
+  // field of bits[d] bits at bit off[d] of dword dw[d] (0 bits: the layer
+  // is provably zero); in_* describe the input words, o* the output
+  uint32_t *pk;
+  long long pitch;
+  int in_nd, out_nd;
+  unsigned char idw[CC_MAXL], ioff[CC_MAXL], ibits[CC_MAXL];
+  unsigned char odw[CC_MAXL], ooff[CC_MAXL], obits[CC_MAXL];
   // batched gain sweeps that write the map: the largest (gain << 32 |
   // 0xFFFFFFFF - index) key of the written words is folded in here with
   // atomicMax (the cell's argmax without a separate reduction); nullptr: off

@@ -237,7 +237,10 @@ __device__ __forceinline__ void cc_lay_st(const SweepArgs &a, int d,
 #define CC_STAGE 8   // tile elements per thread whose loads are issued together
 
 // SEG: long L-inf axes by segments (true) or one fiber per thread (false)
-template <int PROB, int R1, typename LT, bool PAD = true, bool SEG = true>
+// CMP: the sweep reads or writes compact words (a separate instantiation,
+// so the layer-plane kernels carry none of its code)
+template <int PROB, int R1, typename LT, bool PAD = true, bool SEG = true,
+          bool CMP = false>
 __device__ __forceinline__ void cc_sweep_tile(const SweepArgs &a, long long tile,
                                               int32_t *__restrict__ S) {
   constexpr int QMIN = CcQRange<PROB>::lo, QMAX = CcQRange<PROB>::hi;
@@ -265,6 +268,19 @@ __device__ __forceinline__ void cc_sweep_tile(const SweepArgs &a, long long tile
       const int p = cc_div(e, mTL), tl = e - p * a.TL;
       if (tl >= ncol) continue;
       const long long w = base + (long long)p * a.s + tl;
+      if (CMP && a.in_kind == CC_IN_COMPACT) {
+        // dword planes in turn, each field extracted from its own plane
+        // (uniform layout: scalar branches, one bit-field extract each)
+        for (int k = 0; k < CC_CMAXND; k++) {
+          if (k >= a.in_nd) break;
+          const uint32_t q = a.pk[(long long)k * a.pitch + w];
+#pragma unroll
+          for (int d = 0; d < R1; d++)
+            if (a.idw[d] == k && a.ibits[d])
+              val[u][d] = __builtin_amdgcn_ubfe(q, a.ioff[d], a.ibits[d]);
+        }
+        continue;
+      }
       if (a.in_kind == CC_IN_PRED)
         val[u][0] = cc_pred(a.pmode, (int)a.cnt[w], 0, a.mu) ? 1u : 0u;
       else if (a.in_kind == CC_IN_CODE)
@@ -322,7 +338,17 @@ __device__ __forceinline__ void cc_sweep_tile(const SweepArgs &a, long long tile
     if (tl >= ncol) continue;
     const long long w = base + (long long)p * a.s + tl;
     const int ep = cc_gpad<PAD>(e);
-    if (a.out_kind == CC_OUT_LAYERS) {
+    if (CMP && a.out_kind == CC_OUT_COMPACT) {
+      for (int k = 0; k < CC_CMAXND; k++) {
+        if (k >= a.out_nd) break;
+        uint32_t q = 0u;
+#pragma unroll
+        for (int d = 0; d < R1; d++)
+          if (d < Lmax && a.obits[d] && a.odw[d] == k)
+            q |= (uint32_t)S[d * PTP + ep] << a.ooff[d];
+        a.pk[(long long)k * a.pitch + w] = q;
+      }
+    } else if (a.out_kind == CC_OUT_LAYERS) {
       for (int d = a.write_l0 ? 0 : 1; d < Lmax; d++)
         cc_lay_st<LT>(a, d, w, (uint32_t)S[d * PTP + ep]);
     } else {
@@ -341,11 +367,11 @@ __device__ __forceinline__ void cc_sweep_tile(const SweepArgs &a, long long tile
   __syncthreads();
 }
 
-template <int PROB, int R1, typename LT>
+template <int PROB, int R1, typename LT, bool CMP = false>
 __global__ void __launch_bounds__(CC_BLOCK) k_sweep(SweepArgs a) {
   extern __shared__ int32_t S[];
   for (long long tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x)
-    cc_sweep_tile<PROB, R1, LT>(a, tile, S);
+    cc_sweep_tile<PROB, R1, LT, true, true, CMP>(a, tile, S);
 }
 #undef CC_STEP_CASE
 
@@ -439,7 +465,7 @@ __device__ __forceinline__ void cc_cube_round(uint32_t (&v)[R1][C], int ilo,
 // live layers (the seed kind is block-uniform and branched on once per
 // cube).  With TL = 1 the first round's cube is C consecutive words, read
 // with 16-byte vector loads.
-template <int R1, int C, typename LT>
+template <int R1, int C, typename LT, bool CMP = false>
 __device__ __forceinline__ void cc_cube_load(const SweepArgs &a, long long base,
                                              int pb, int tl, bool live,
                                              int Lin, uint32_t (&v)[R1][C]) {
@@ -447,6 +473,26 @@ __device__ __forceinline__ void cc_cube_load(const SweepArgs &a, long long base,
   for (int d = 0; d < R1; d++)
 #pragma unroll
     for (int c = 0; c < C; c++) v[d][c] = 0u;
+  if (CMP && a.in_kind == CC_IN_COMPACT) {
+    // dword plane k of the cube's words, then every field that lives in it
+    // (the field layout is uniform: the branches are scalar and outside the
+    // per-word loops, one bit-field extract per layer and word)
+    if (!live) return;
+    const uint32_t *w0 = a.pk + base + (long long)pb * a.s + tl;
+    for (int k = 0; k < CC_CMAXND; k++) {
+      if (k >= a.in_nd) break;
+      uint32_t q[C];
+#pragma unroll
+      for (int c = 0; c < C; c++) q[c] = w0[(long long)k * a.pitch + c * a.s];
+#pragma unroll
+      for (int d = 0; d < R1; d++)
+        if (a.idw[d] == k && a.ibits[d])
+#pragma unroll
+          for (int c = 0; c < C; c++)
+            v[d][c] = __builtin_amdgcn_ubfe(q[c], a.ioff[d], a.ibits[d]);
+    }
+    return;
+  }
   if (a.TL == 1 && C % 8 == 0) {
     const long long w0 = base + pb;              // C-aligned, 16-byte aligned
     if (a.in_kind == CC_IN_PRED) {
@@ -512,13 +558,31 @@ __device__ __forceinline__ void cc_cube_load(const SweepArgs &a, long long base,
 
 // Last-round output of one cube (pb, stride qa in p): the produced layers,
 // or the layer sum into the map / clamped into cnt.
-template <int R1, int C, typename LT>
+template <int R1, int C, typename LT, bool CMP = false>
 __device__ __forceinline__ void cc_cube_store(const SweepArgs &a, long long base,
                                               int pb, int qa, int tl,
                                               int Lmax, const uint32_t (&v)[R1][C],
                                               unsigned long long &best) {
   const long long sq = (long long)qa * a.s;
   const long long w0 = base + (long long)pb * a.s + tl;
+  if (CMP && a.out_kind == CC_OUT_COMPACT) {
+    // dword plane k of the cube's words from the fields that live in it
+    uint32_t *o0 = a.pk + w0;
+    for (int k = 0; k < CC_CMAXND; k++) {
+      if (k >= a.out_nd) break;
+      uint32_t q[C];
+#pragma unroll
+      for (int c = 0; c < C; c++) q[c] = 0u;
+#pragma unroll
+      for (int d = 0; d < R1; d++)
+        if (d < Lmax && a.obits[d] && a.odw[d] == k)
+#pragma unroll
+          for (int c = 0; c < C; c++) q[c] |= v[d][c] << a.ooff[d];
+#pragma unroll
+      for (int c = 0; c < C; c++) o0[(long long)k * a.pitch + c * sq] = q[c];
+    }
+    return;
+  }
   if (a.out_kind == CC_OUT_LAYERS) {
 #pragma unroll
     for (int d = 0; d < R1; d++) {
@@ -604,7 +668,7 @@ __device__ __forceinline__ void cc_cube_apply(uint32_t (&v)[R1][C], int ilo,
 
 // one tile of the round sweep (the batched transform calls it for the
 // homogeneous Hamming groups of many cells)
-template <int PROB, int R1, int Q, int RR, typename LT>
+template <int PROB, int R1, int Q, int RR, typename LT, bool CMP = false>
 __device__ __forceinline__ void cc_round_tile(const SweepArgs &a, long long tile,
                                               int32_t *__restrict__ S) {
   constexpr int C = RR == 1 ? Q : (RR == 2 ? Q * Q : (RR == 3 ? Q * Q * Q : Q * Q * Q * Q));
@@ -623,13 +687,13 @@ __device__ __forceinline__ void cc_round_tile(const SweepArgs &a, long long tile
   // ---- round 0: HBM -> registers (cube = group axes 0..RR-1)
   {
     const int ca = min(0, m - RR);
-    for (int cu = threadIdx.x; cu < ncube; cu += CC_BLOCK) {
+    for (int cu = threadIdx.x; cu < ncube; cu += blockDim.x) {
       const int tl = cu % a.TL, pb = (cu / a.TL) * C;
       uint32_t v[R1][C];
-      cc_cube_load<R1, C, LT>(a, base, pb, tl, tl < ncol, Lin, v);
+      cc_cube_load<R1, C, LT, CMP>(a, base, pb, tl, tl < ncol, Lin, v);
       cc_cube_apply<PROB, R1, Q, C, RR>(v, -ca, a.R);
       if (nrounds == 1) {
-        if (tl < ncol) cc_cube_store<R1, C, LT>(a, base, pb, 1, tl, Lmax, v, best);
+        if (tl < ncol) cc_cube_store<R1, C, LT, CMP>(a, base, pb, 1, tl, Lmax, v, best);
       } else {
         cc_cube_lds_st<R1, C, LT>(S, PTP, pb, 1, a.TL, tl, v);
       }
@@ -643,14 +707,14 @@ __device__ __forceinline__ void cc_round_tile(const SweepArgs &a, long long tile
     const int ca = min(lo, m - RR);             // cube's lowest group axis
     int qa = 1;                                 // Q^ca: in-tile stride
     for (int i = 0; i < ca; i++) qa *= Q;
-    for (int cu = threadIdx.x; cu < ncube; cu += CC_BLOCK) {
+    for (int cu = threadIdx.x; cu < ncube; cu += blockDim.x) {
       const int tl = cu % a.TL, u = cu / a.TL;
       const int pb = (u / qa) * qa * C + (u % qa);
       uint32_t v[R1][C];
       cc_cube_lds_ld<R1, C, LT>(S, PTP, pb, qa, a.TL, tl, v);
       cc_cube_apply<PROB, R1, Q, C, RR>(v, lo - ca, a.R);
       if (last) {
-        if (tl < ncol) cc_cube_store<R1, C, LT>(a, base, pb, qa, tl, Lmax, v, best);
+        if (tl < ncol) cc_cube_store<R1, C, LT, CMP>(a, base, pb, qa, tl, Lmax, v, best);
       } else {
         cc_cube_lds_st<R1, C, LT>(S, PTP, pb, qa, a.TL, tl, v);
       }
@@ -660,9 +724,9 @@ __device__ __forceinline__ void cc_round_tile(const SweepArgs &a, long long tile
   if (a.amax) cc_key_flush(best, a.amax);
 }
 
-template <int PROB, int R1, int Q, int RR, typename LT>
+template <int PROB, int R1, int Q, int RR, typename LT, bool CMP = false>
 __global__ void __launch_bounds__(CC_BLOCK) k_sweep_r(SweepArgs a) {
   extern __shared__ int32_t S[];
   for (long long tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x)
-    cc_round_tile<PROB, R1, Q, RR, LT>(a, tile, S);
+    cc_round_tile<PROB, R1, Q, RR, LT, CMP>(a, tile, S);
 }

@@ -59,6 +59,7 @@ struct Sweep {
   long long s, ncolb, ntiles;
   int lmax;
   int rq;       // round kernel: axis size Q of the homogeneous group, else 0
+  int threads;  // round kernel: workgroup size (one cube per thread)
   size_t lds;   // LDS bytes with int32 layers
   size_t lds16; // LDS bytes in 16-bit mode (round kernel packs layer pairs)
 };
@@ -142,6 +143,8 @@ struct Engine {
   int pk_narrow = 0;         // CC_PK_NARROW=1: 16 / 24-bit compact words (fewer
                              // bytes, but the sub-dword accesses measured
                              // slower on K_2(24,3): off by default)
+  int compact = 1;           // CC_COMPACT=0: unpacked sweeps exchange int32 /
+                             // uint16 layer planes instead of compact words
   int pk_first = 1;          // CC_PK_FIRST: first sweep 0 k_psweep, 1 persistent
                              // with prefetch, 2 LDS-free (cross-lane)
   PPlan pfield;              // packed plan of gain / loss (predicate seeds)
@@ -365,7 +368,7 @@ void build_plan() {
     // of binary or quaternary axes is capped at one 16-word cube per thread
     // (4,096 words per tile) to keep its round count low
     const int q = E.ax[j];
-    const bool round_ok = !E.generic_only && E.prob == CC_HAMMING && q <= 5;
+    const bool round_ok = !E.generic_only && E.prob == CC_HAMMING && q <= 10;
     long long P = 1;
     int k = j;
     while (k < E.n) {
@@ -411,7 +414,14 @@ void build_plan() {
     if (round_ok && homog && sw.nax >= round_axes(q) &&
         (E.l16_field ? lds_r16 : lds_r) <= (size_t)E.lds_max)
       sw.rq = q;
+    sw.threads = CC_BLOCK;
     if (sw.rq) {
+      // one cube per thread: a tile with few cubes (a first sweep of large
+      // axes: 64 cubes of 8 words) runs smaller workgroups, more per CU
+      int C = 1;
+      for (int i = 0; i < std::min(round_axes(q), sw.nax); i++) C *= q;
+      const long long ncube = (long long)P * sw.TL / C;
+      sw.threads = (int)std::min<long long>(CC_BLOCK, std::max<long long>(64, (ncube + 63) / 64 * 64));
       const bool one = sw.nax <= round_axes(q);
       sw.lds = one ? 0 : lds_r;
       sw.lds16 = one ? 0 : lds_r16;
@@ -424,19 +434,106 @@ void build_plan() {
 }
 
 hipError_t launch_round(int q, const SweepArgs &a, int blocks, size_t lds,
-                        bool l16) {
-  return cc_sweep_round(a, E.L, q, blocks, lds, l16, LaunchCtx{E.st, E.ncu, E.grid});
+                        bool l16, bool cmp, int threads) {
+  return cc_sweep_round(a, E.L, q, blocks, lds, l16, cmp,
+                        LaunchCtx{E.st, E.ncu, E.grid, threads});
 }
 
-hipError_t launch_sweep(const SweepArgs &a, int blocks, size_t lds, bool l16) {
+hipError_t launch_sweep(const SweepArgs &a, int blocks, size_t lds, bool l16,
+                        bool cmp) {
   const LaunchCtx lc{E.st, E.ncu, E.grid};
   switch (E.prob) {
     case CC_TORUS:
     case CC_GRID: return cc_sweep_linf(a, E.prob, blocks, lds, lc);
-    case CC_HAMMING: return cc_sweep_hamming(a, E.L, blocks, lds, l16, lc);
-    case CC_LEE: return cc_sweep_lee(a, E.L, blocks, lds, l16, lc);
+    case CC_HAMMING: return cc_sweep_hamming(a, E.L, blocks, lds, l16, cmp, lc);
+    case CC_LEE: return cc_sweep_lee(a, E.L, blocks, lds, l16, cmp, lc);
   }
   return hipErrorInvalidValue;
+}
+
+// ---- compact words between unpacked sweeps -------------------------------
+// After the first c axes, layer d at a word counts seed words that agree
+// with it on the axes >= c and lie at partial distance exactly d over the
+// axes < c.  There are N_d(c) such positions (Hamming: the elementary
+// symmetric polynomial e_d of (q_i - 1), i < c; Lee: the coefficient of x^d
+// in the product of the per-axis Lee weight enumerators), so the layer is at
+// most s0 * N_d(c) for a seed whose words are <= s0, and never more than the
+// seed's total cap.  Each layer gets bitlen of that bound; the fields are
+// packed into at most CC_CMAXND dwords (largest first, each into the first
+// dword with room), never straddling a dword.
+int bitlen(double x);
+
+struct CFmt {
+  int nd = 0;
+  unsigned char dw[CC_MAXL] = {}, off[CC_MAXL] = {}, bits[CC_MAXL] = {};
+};
+
+bool cmp_fmt(int c, double s0, double cap, CFmt &f) {
+  std::vector<double> poly(E.L, 0.0);
+  poly[0] = 1;
+  for (int i = 0; i < c; i++) {
+    std::vector<double> nx(E.L, 0.0);
+    for (int w = 0; w < E.L; w++) {
+      if (poly[w] == 0) continue;
+      for (int a = 0; a < E.ax[i]; a++) {
+        const int e = E.prob == CC_LEE ? std::min(a, E.ax[i] - a) : (a ? 1 : 0);
+        if (w + e < E.L) nx[w + e] += poly[w];
+      }
+    }
+    poly = nx;
+  }
+  f = CFmt();
+  int order[CC_MAXL];
+  for (int d = 0; d < E.L; d++) {
+    const double b = std::min(s0 * poly[d], cap);
+    const int nb = b >= 1 ? bitlen(b) : 0;
+    if (nb > 31) return false;
+    f.bits[d] = (unsigned char)nb;
+    order[d] = d;
+  }
+  std::stable_sort(order, order + E.L,
+                   [&](int x, int y) { return f.bits[x] > f.bits[y]; });
+  int used[CC_CMAXND] = {};
+  for (int i = 0; i < E.L; i++) {
+    const int d = order[i];
+    if (!f.bits[d]) continue;
+    int k = 0;
+    while (k < CC_CMAXND && used[k] + f.bits[d] > 32) k++;
+    if (k == CC_CMAXND) return false;
+    f.dw[d] = (unsigned char)k;
+    f.off[d] = (unsigned char)used[k];
+    used[k] += f.bits[d];
+    f.nd = std::max(f.nd, k + 1);
+  }
+  // the compact planes live in the layer block (R planes)
+  return f.nd >= 1 && f.nd <= E.R;
+}
+
+// formats after every sweep but the last of the unpacked plan, for a seed
+// bounded by (s0, cap); false when the plan cannot exchange compact words
+// (single-plane families, register-round sweeps, a bound past 31 bits or
+// more dwords than the layer block holds)
+bool compact_plan(double s0, double cap, std::vector<CFmt> &fm) {
+  fm.clear();
+  if (!E.compact || E.L < 2 || (E.prob != CC_HAMMING && E.prob != CC_LEE))
+    return false;
+  // (the register-round kernels of axis size <= 5 have no compact variant)
+  for (const Sweep &sw : E.plan)
+    if (sw.rq && sw.rq <= 5) return false;
+  for (size_t i = 0; i + 1 < E.plan.size(); i++) {
+    CFmt f;
+    if (!cmp_fmt(E.plan[i].j0 + E.plan[i].nax, s0, cap, f)) return false;
+    fm.push_back(f);
+  }
+  return true;
+}
+
+// seed bounds of the pending code (largest multiplicity, code size)
+void code_bounds(double *s0, double *cap) {
+  int mmax = 1;
+  for (const auto &kv : E.uploaded) mmax = std::max(mmax, kv.second);
+  *s0 = mmax;
+  *cap = std::max(E.codeM, 1);
 }
 
 // Run the whole transform: in_kind seeds layer 0, out_kind says where the
@@ -449,6 +546,13 @@ int run_transform(int in_kind, int out_kind, int pmode) {
                        : E.l16_field;
   E.layers_live = out_kind == CC_OUT_LAYERS ? (l16 ? 2 : 1) : 0;
   E.nswept = 0;
+  std::vector<CFmt> fm;
+  bool cmp = false;
+  if (out_kind != CC_OUT_LAYERS && ns > 1) {
+    double s0 = 1, cap = 1e300;
+    if (in_kind == CC_IN_CODE) code_bounds(&s0, &cap);
+    cmp = compact_plan(s0, cap, fm);
+  }
   for (int i = 0; i < ns; i++) {
     const Sweep &sw = E.plan[i];
     SweepArgs a;
@@ -482,10 +586,32 @@ int run_transform(int in_kind, int out_kind, int pmode) {
       sk *= a.Q[k];
     }
     for (int k = 0; k <= sw.nax; k++) a.live[k] = layers_after(sw.j0 + k);
+    if (cmp) {
+      a.pk = (uint32_t *)E.lay;
+      a.pitch = E.pitch;
+      if (i > 0) {
+        const CFmt &f = fm[i - 1];
+        a.in_kind = CC_IN_COMPACT;
+        a.in_nd = f.nd;
+        for (int d = 0; d < E.L; d++) {
+          a.idw[d] = f.dw[d]; a.ioff[d] = f.off[d]; a.ibits[d] = f.bits[d];
+        }
+      }
+      if (i < ns - 1) {
+        const CFmt &f = fm[i];
+        a.out_kind = CC_OUT_COMPACT;
+        a.out_nd = f.nd;
+        for (int d = 0; d < E.L; d++) {
+          a.odw[d] = f.dw[d]; a.ooff[d] = f.off[d]; a.obits[d] = f.bits[d];
+        }
+      }
+    }
     const int blocks = (int)std::min<long long>(sw.ntiles, 1LL << 20);
     if (E.prof) CK(hipEventRecord(E.ev[2 * i], E.st));
-    if (sw.rq) CK(launch_round(sw.rq, a, blocks, l16 ? sw.lds16 : sw.lds, l16));
-    else CK(launch_sweep(a, blocks, sw.lds, l16));
+    const bool cm = a.in_kind == CC_IN_COMPACT || a.out_kind == CC_OUT_COMPACT;
+    if (sw.rq)
+      CK(launch_round(sw.rq, a, blocks, l16 ? sw.lds16 : sw.lds, l16, cm, sw.threads));
+    else CK(launch_sweep(a, blocks, sw.lds, l16, cm));
     if (E.prof) CK(hipEventRecord(E.ev[2 * i + 1], E.st));
   }
   E.nswept = ns;
@@ -1455,11 +1581,14 @@ static int init_cell(int problem, int n, const int *ax, int R,
     E.pk_first = pf ? atoi(pf) : 1;
     const char *pn = getenv("CC_PK_NARROW");
     E.pk_narrow = pn ? atoi(pn) : 0;
+    const char *pcm = getenv("CC_COMPACT");
+    E.compact = pcm ? atoi(pcm) : 1;
     // an explicit schedule setting turns the automatic timing off
     const char *pa = getenv("CC_AUTOTUNE");
     E.autotune = pa ? atoi(pa) : 1;
     for (const char *k : {"CC_PACKED", "CC_SWEEP_KERNEL", "CC_SWEEP_LDS",
-                          "CC_PK_COL", "CC_PK_FIRST", "CC_PK_NARROW", "CC_LAYER16"})
+                          "CC_PK_COL", "CC_PK_FIRST", "CC_PK_NARROW", "CC_LAYER16",
+                          "CC_COMPACT"})
       if (getenv(k) && !pa) E.autotune = 0;
     E.tuned = 0;
     g_cuts = 0;
@@ -2052,19 +2181,25 @@ int dct_plan_probe(int n, const int *ax, int R, int kind, int s0, int M,
 int dct_autotune(int reps, float *best_us) {
   if (!E.live || reps < 1) return -1;
   E.tuned = 1;
-  struct Opt { int lds, col, first, packed, narrow; };
+  struct Opt { int lds, col, first, packed, narrow, cmp, gen; };
   std::vector<Opt> opts;
   for (int lds : {16384, 32768, 65536, 98304, 131072}) {
     if (lds > E.lds_max) continue;
     for (int col : {2, 1, 0})
-      for (int first : {1, 0}) opts.push_back({lds, col, first, 1, E.pk_narrow});
+      for (int first : {1, 0})
+        opts.push_back({lds, col, first, 1, E.pk_narrow, E.compact, E.generic_only});
   }
   // the unpacked schedule under each LDS budget (its plan is rebuilt per
   // budget: tile sizes, sweep grouping and the workgroups a CU holds move
-  // together)
+  // together), with compact words between sweeps and with layer planes,
+  // with register-round sweeps for homogeneous groups and with LDS tiles only
   for (int lds : {16384, 32768, 65536, 98304, 131072})
-    if (lds <= E.lds_max) opts.push_back({lds, E.pk_col, E.pk_first, 0, E.pk_narrow});
-  const Opt saved{E.lds_budget, E.pk_col, E.pk_first, E.packed, E.pk_narrow};
+    if (lds <= E.lds_max)
+      for (int gen : {0, 1})
+        for (int cmp : {1, 0})
+          opts.push_back({lds, E.pk_col, E.pk_first, 0, E.pk_narrow, cmp, gen});
+  const Opt saved{E.lds_budget, E.pk_col, E.pk_first, E.packed, E.pk_narrow,
+                  E.compact, E.generic_only};
   std::vector<std::string> seen;
   float best = 1e30f;
   Opt win = saved;
@@ -2074,10 +2209,18 @@ int dct_autotune(int reps, float *best_us) {
   for (const Opt &o : opts) {
     E.lds_budget = o.lds; E.pk_col = o.col; E.pk_first = o.first; E.packed = o.packed;
     E.pk_narrow = o.narrow;
+    E.compact = o.cmp;
+    E.generic_only = o.gen;
     PPlan pl = o.packed ? pk_plan(1.0, 1e300, true, false) : PPlan();
     std::string sig = "unpacked|";
     if (!o.packed) {
-      build_plan();
+      build_plan();   // (a plan equal to an earlier one has its signature)
+      std::vector<CFmt> fm;
+      if (!compact_plan(1.0, 1e300, fm)) {
+        if (o.cmp) continue;         // the same plan as without compact words
+      } else {
+        sig += o.cmp ? "compact|" : "planes|";
+      }
       for (const Sweep &w : E.plan) {
         char b[96];
         snprintf(b, sizeof(b), "%d.%d.%d.%d.%d|", w.j0, w.nax, w.P, w.TL, w.rq);
@@ -2115,6 +2258,8 @@ int dct_autotune(int reps, float *best_us) {
   E.lds_budget = win.lds; E.pk_col = win.col; E.pk_first = win.first;
   E.packed = win.packed;
   E.pk_narrow = win.narrow;
+  E.compact = win.cmp;
+  E.generic_only = win.gen;
   E.pfield = wplan;
   build_plan();   // the unpacked plan of the winning budget
   if (best_us) *best_us = best * 1000.0f / reps;
@@ -2175,6 +2320,13 @@ int dct_sweep_plan2(int kind, int *out, int cap) {
   const int ns = (int)E.plan.size();
   const bool field = kind == 0;
   const int lb = (field ? E.l16_field : E.l16_count) ? 2 : 4;
+  std::vector<CFmt> fm;
+  bool cmp = false;
+  if (ns > 1) {
+    double s0 = 1, cp = 1e300;
+    if (!field) code_bounds(&s0, &cp);
+    cmp = compact_plan(s0, cp, fm);
+  }
   for (int i = 0; i < ns && i < cap; i++) {
     const Sweep &sw = E.plan[i];
     const int lin = layers_after(sw.c0), lout = sw.lmax;
@@ -2187,10 +2339,14 @@ int dct_sweep_plan2(int kind, int *out, int cap) {
       rd = i == 0 ? (field ? 2 : 4) : 4;
       wr = i == ns - 1 ? (field ? 4 : 2) : 4;
     }
+    if (cmp) {                // compact words of 1..4 dwords between sweeps
+      rd = i == 0 ? (field ? 2 : 4) : 4 * fm[i - 1].nd;
+      wr = i == ns - 1 ? (field ? 4 : 2) : 4 * fm[i].nd;
+    }
     int *o = out + 13 * i;
     o[0] = sw.j0; o[1] = sw.nax; o[2] = sw.P; o[3] = sw.TL; o[4] = 0;
     o[5] = 32; o[6] = E.L; o[7] = rd; o[8] = wr;
-    o[9] = (int)(E.l16_field ? sw.lds16 : sw.lds); o[10] = CC_BLOCK; o[11] = 0;
+    o[9] = (int)(E.l16_field ? sw.lds16 : sw.lds); o[10] = sw.threads; o[11] = 0;
     o[12] = 0;
   }
   return ns;

@@ -19,9 +19,12 @@ pytestmark = pytest.mark.gpu
 # unpacked register-round kernels, the per-axis kernel only, and a tiny LDS
 # budget that forces short sweeps and multi-round strided tiles (the
 # multi-sweep paths on small cells)
+# (with small tiles the unpacked cells run several sweeps, which exchange
+# compact bit-field words by default; "planes" keeps the layer planes)
 VARIANTS = {"default": {}, "unpacked": {"CC_PACKED": "0"},
             "generic": {"CC_SWEEP_KERNEL": "generic"},
-            "small-tiles": {"CC_SWEEP_LDS": "4096"}}
+            "small-tiles": {"CC_SWEEP_LDS": "4096"},
+            "planes": {"CC_SWEEP_LDS": "4096", "CC_COMPACT": "0"}}
 
 
 @pytest.mark.parametrize("variant", list(VARIANTS))
@@ -271,3 +274,48 @@ def test_owner_loss_is_the_marginal_deficit(gpu_cell, problem, axes, R, mu):
     np.testing.assert_array_equal(out, expected)
     g.ball_update(code, -1)
     assert not g.read_cnt().any()
+
+
+# multi-sweep unpacked cells: q > 5 Hamming, mixed radix, Lee; the sweeps
+# exchange compact words (every layer a bit field sized by its proven bound)
+COMPACT_CELLS = [("hamming", [8] * 6, 4, 1), ("hamming", [10] * 5, 5, 1),
+                 ("hamming", [4, 5, 6, 7, 8, 4], 3, 1), ("hamming", [7] * 6, 3, 2),
+                 ("lee", [9] * 5, 4, 1), ("lee", [6, 7, 8, 9], 6, 2),
+                 ("hamming", [6] * 7, 6, 1)]
+
+
+@pytest.mark.parametrize("problem,axes,R,mu", COMPACT_CELLS,
+                         ids=[f"{p}-{'x'.join(map(str, a))}-R{r}-mu{m}"
+                              for p, a, r, m in COMPACT_CELLS])
+@pytest.mark.parametrize("lds", ["16384", "65536"])
+def test_compact_words_multi_sweep(gpu_cell, monkeypatch, problem, axes, R, mu, lds):
+    """Recount of multiset codes (seed bound = the largest multiplicity,
+    capped by the code size), gain and loss on cells that run several
+    unpacked sweeps, with compact words and with layer planes between the
+    sweeps: both bit-exact against the oracle, and the compact plan moves
+    fewer bytes per word."""
+    monkeypatch.setenv("CC_SWEEP_LDS", lds)
+    c = NumpyDP(axes, R, problem=problem, mu=mu)
+    size = c.space.size
+    every = np.arange(size, dtype=np.int64)
+    bpw = {}
+    for mode in ("1", "0"):
+        monkeypatch.setenv("CC_COMPACT", mode)
+        g = gpu_cell(axes, R, problem=problem, mu=mu)
+        plan = g.sweep_plan2(0)
+        assert len(plan) >= 2 and all(s["W"] == 32 for s in plan)
+        bpw[mode] = sum(s["bytes_in"] + s["bytes_out"] for s in plan)
+        rng = np.random.default_rng(size + R)
+        for m, mult in ((max(2, size // 3000), 1), (max(3, size // 200), 5),
+                        (20000, 3)):
+            base = rng.integers(0, size, size=m).astype(np.int64)
+            code = np.repeat(base, rng.integers(1, mult + 1, size=m))[:65535]
+            g.recount(code)
+            c.recount(code)
+            assert (g.read_cnt() == c.read_cnt()).all(), (mode, m, mult)
+            for which in ("gain_map", "loss_map"):
+                getattr(g, which)()
+                getattr(c, which)()
+                assert (g.map_all() == c.map_read_at(every)).all(), (mode, which, m)
+        g.close()
+    assert bpw["1"] < bpw["0"], bpw
