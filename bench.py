@@ -73,8 +73,10 @@ def cpu_model():
 
 
 def run_cpu_baseline(code, passes):
-    """numpy axis-DP oracle, one thread, a bounded sample of gain passes."""
+    """numpy axis-DP oracle, one thread, a bounded sample of gain passes
+    (the median of ``passes`` >= 3 after one warm-up)."""
     from oracle.dp import NumpyDP
+    passes = max(3, passes)
     try:
         os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[0]})
     except (AttributeError, OSError):
@@ -82,15 +84,18 @@ def run_cpu_baseline(code, passes):
     c = NumpyDP([2] * 24, 3)
     c.recount(code)
     c.gain_map()                      # warm-up
-    t = time.perf_counter()
+    ts = []
     for _ in range(passes):
+        t = time.perf_counter()
         c.gain_map()
-    dt = (time.perf_counter() - t) / passes
+        ts.append(time.perf_counter() - t)
+    dt = float(np.median(ts))
     return dict(value=round(1.0 / dt, 4), unit="field passes/s", cores=1,
                 kind="port",
-                sample=f"{passes} gain passes of K_2(24,3) on the numpy "
+                sample=f"median of {passes} gain passes of K_2(24,3) on the numpy "
                        f"axis-DP oracle after one warm-up, 1 thread "
-                       f"({dt * 1e3:.0f} ms/pass); host {cpu_model()}, "
+                       f"({dt * 1e3:.0f} ms/pass, range "
+                       f"{min(ts) * 1e3:.0f}-{max(ts) * 1e3:.0f}); host {cpu_model()}, "
                        f"os.cpu_count()={os.cpu_count()}")
 
 
@@ -141,18 +146,27 @@ def schedule_key(plan):
     return " | ".join(describe(plan))
 
 
+TRAFFIC_PROFILE = os.path.join("profiles", "k_pcol_K2_24_3.md")
+
+
 def measured_traffic(kernel):
-    """HBM bytes per launch of a sweep kernel, as measured by the rocprofv3
-    FETCH_SIZE / WRITE_SIZE runs summarised in profiles/ (copy-calibrated);
-    None when no summary covers that kernel."""
-    path = os.path.join(ROOT, "profiles", "k_pcol_K2_24_3.md")
-    try:
-        text = open(path).read()
-    except OSError:
-        return None
+    """HBM bytes per launch of a sweep kernel, taken from the copy-calibrated
+    rocprofv3 FETCH_SIZE / WRITE_SIZE runs summarised in the profile file
+    (not measured by this run: counters need their own profiler runs).
+    Returns (bytes, source); (None, reason) when the profile does not cover
+    the kernel this run found dominant, so a stale figure is never
+    reported."""
     import re
+    try:
+        text = open(os.path.join(ROOT, TRAFFIC_PROFILE)).read()
+    except OSError:
+        return None, f"{TRAFFIC_PROFILE} missing"
+    date = re.search(r"^measured: (.+)$", text, re.M)
     t = re.search(r"^traffic\[" + re.escape(kernel) + r"\]: (\d+)$", text, re.M)
-    return int(t.group(1)) if t else None
+    if not t:
+        return None, f"{TRAFFIC_PROFILE} has no counter run of {kernel}"
+    return int(t.group(1)), (f"{TRAFFIC_PROFILE}, kernel {kernel}, counters "
+                             f"measured {date.group(1) if date else '(undated)'}")
 
 
 def run_config1(GpuBackend, with_cpu):
@@ -292,7 +306,37 @@ def run_batch64(GpuBackend, rounds=200, k=2, quick=False):
     return out
 
 
-def run_walks(g, code, size, n_upd=100000, n_q=10 ** 6):
+def run_config2_cell(GpuBackend, axes, R, steps=10):
+    """The plain-transform recipe on one cell: code = flatnonzero(
+    default_rng(0).random(|X|) < 1e-4); gain, loss and recount passes
+    (median of ``steps`` after a warm-up that also tunes the schedule);
+    10^5 single-word updates and 10^6 exact gain queries (run_walks)."""
+    from coldcore_amd.patterns import ball_size
+    size = int(np.prod(np.array(axes, dtype=np.int64)))
+    code = np.flatnonzero(np.random.default_rng(0).random(size) < 1e-4).astype(np.int64)
+    out = {"axes": list(axes), "R": R, "words": size, "code_words": int(len(code))}
+    g = GpuBackend(axes, R)
+    try:
+        g.recount(code)
+        for name, fn in (("gain", g.gain_map), ("loss", g.loss_map),
+                         ("recount", lambda: g.recount(code))):
+            fn()
+            ts = []
+            for _ in range(steps):
+                t = time.perf_counter()
+                fn()
+                ts.append(time.perf_counter() - t)
+            out[f"{name}_pass_ms"] = round(float(np.median(ts)) * 1e3, 4)
+        out["gain_bytes_per_word"] = sum(sweep_bytes(g.sweep_plan2(0)))
+        out["recount_bytes_per_word"] = sum(sweep_bytes(g.sweep_plan2(1)))
+        ball = ball_size(axes[0], len(axes), R) if len(set(axes)) == 1 else None
+        out.update(run_walks(g, code, size, ball=ball or 0))
+    finally:
+        g.close()
+    return out
+
+
+def run_walks(g, code, size, n_upd=100000, n_q=10 ** 6, ball=2325):
     """Incremental updates and exact queries on a recounted K_2(24,3) cell:
     n_upd single-word updates alternating +1 (a random non-code word, seed
     1) and -1 (a random current codeword), 20 x (+1, -1) batches of 1,024
@@ -339,7 +383,7 @@ def run_walks(g, code, size, n_upd=100000, n_q=10 ** 6):
         g.ball_gather(q[lo:lo + 65536], 0)
     dq = time.perf_counter() - t
     out["queries_per_s"] = round(n_q / dq, 1)
-    out["query_cnt_reads_per_s"] = round(n_q * 2325 / dq, 1)
+    out["query_cnt_reads_per_s"] = round(n_q * ball / dq, 1)
     return out
 
 
@@ -443,6 +487,9 @@ OTHER_CELLS = [
     ("K_5(10,3)", [5] * 10, 3, ("p", 1e-4)),
     ("mixed [4,5,6,7,8,4,5,6,7] R=3", [4, 5, 6, 7, 8, 4, 5, 6, 7], 3, ("p", 1e-4)),
     ("K_2(30,4)", [2] * 30, 4, ("choice", 40000)),
+    # the reference's own published GPU gain-transform cells
+    ("K_8(9,4)", [8] * 9, 4, ("p", 2e-6)),
+    ("K_10(9,5)", [10] * 9, 5, ("p", 2e-6)),
     ("K_3(8,2)", [3] * 8, 2, ("choice", 40)),
     ("K_3(10,2)", [3] * 10, 2, ("choice", 400)),
     ("K_4(8,2)", [4] * 8, 2, ("choice", 400)),
@@ -450,6 +497,12 @@ OTHER_CELLS = [
     ("king torus 1024x1024 R=1", [1024, 1024], 1, ("p", 1e-3), "torus_linf"),
     ("grid 1000x1000 R=3", [1000, 1000], 3, ("p", 1e-3), "grid_linf"),
 ]
+
+
+#: the reference's published gain-transform times on one H100-class GPU
+#: (GH200), v0.3 pair schedule, and its bytes moved (reference
+#: docs/performance.md:386-391, docs/benchmarks.md:103-104)
+REFERENCE_GPU_MS = {"K_8(9,4)": (6.1, 18.5e9), "K_10(9,5)": (61.0, 154e9)}
 
 
 def time_cells(GpuBackend, steps):
@@ -467,17 +520,27 @@ def time_cells(GpuBackend, steps):
             g.recount(code.astype(np.int64))
             g.gain_map()                  # the first one tunes the schedule
             plan = g.sweep_plan2(0)
-            t = time.perf_counter()
+            ts = []
             for _ in range(steps):
+                t = time.perf_counter()
                 g.gain_map()
-            ms = (time.perf_counter() - t) * 1e3 / steps
+                ts.append(time.perf_counter() - t)
+            ms = float(np.median(ts)) * 1e3
             bpw = sum(sweep_bytes(plan))
-            out.append({"cell": name, "words": size, "code_words": int(len(code)),
-                        "gain_pass_ms": round(ms, 4),
-                        "passes_per_s": round(1e3 / ms, 1),
-                        "bytes_per_word": bpw,
-                        "GBps_algorithmic": round(bpw * size / (ms * 1e-3) / 1e9, 1),
-                        "sweeps": len(plan)})
+            gbs = bpw * size / (ms * 1e-3) / 1e9
+            row = {"cell": name, "words": size, "code_words": int(len(code)),
+                   "gain_pass_ms": round(ms, 4),
+                   "passes_per_s": round(1e3 / ms, 1),
+                   "bytes_per_word": bpw,
+                   "GBps_algorithmic": round(gbs, 1),
+                   "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 3),
+                   "sweeps": len(plan), "schedule": describe(plan)}
+            if name in REFERENCE_GPU_MS:
+                ref_ms, ref_bytes = REFERENCE_GPU_MS[name]
+                row.update(reference_gain_ms_gh200=ref_ms,
+                           reference_bytes=ref_bytes,
+                           speedup_vs_reference=round(ref_ms / ms, 2))
+            out.append(row)
         finally:
             g.close()
     return out
@@ -488,7 +551,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--cpu-passes", type=int, default=2)
+    ap.add_argument("--cpu-passes", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
@@ -578,7 +641,7 @@ def main():
         dom_key = max(groups, key=lambda k: groups[k][0])
         dom_ms, dom_bytes, dom_n = groups[dom_key]
         dom_gbs = dom_bytes / (dom_ms * 1e-3) / 1e9
-        traffic = measured_traffic(dom_key)
+        traffic, traffic_src = measured_traffic(dom_key)
 
         extra = {}
         if not args.no_extra:
@@ -627,6 +690,7 @@ def main():
             "unit": "GB/s",
             "frac": round(dom_gbs / HBM_PEAK_GBS, 4),
             "traffic": traffic,
+            "traffic_source": traffic_src,
             "kernel": f"{dom_key}: {dom_n} launches per pass, "
                       f"{dom_bytes / dom_n / 1e6:.1f} MB algorithmic per launch, "
                       f"{dom_ms / dom_n * 1e3:.1f} us average",
@@ -635,6 +699,7 @@ def main():
     rec.update(extra)
     if not args.no_extra:
         rec["other_cells"] = time_cells(GpuBackend, 10)
+        rec["config2_k5_10_3"] = run_config2_cell(GpuBackend, [5] * 10, 3)
         rec["config1"] = run_config1(GpuBackend, not args.no_cpu)
         rec["near_cover"] = run_near_cover(GpuBackend)
         try:
@@ -644,6 +709,9 @@ def main():
     if not args.no_cpu:
         rec["cpu_baseline"] = run_cpu_baseline(code, args.cpu_passes)
         rec["speedup_vs_cpu"] = round(passes_s / rec["cpu_baseline"]["value"], 1)
+        # the oracle against the reference's published single-thread numpy
+        # times, on this (the GPU box's) host
+        rec["cpu_calib"] = run_cpu_calibration()
     print(json.dumps(rec), flush=True)
 
 

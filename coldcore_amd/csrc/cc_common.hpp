@@ -58,6 +58,37 @@ __device__ __forceinline__ int cc_wave_max(int v) {
   return v;
 }
 
+// wave64 total by DPP: three row shifts give each lane the sum of itself
+// and the three lanes before it in its row of 16, shifts by 4 and 8 (upper
+// banks only) complete each row's sum in its lane 15, and two row
+// broadcasts (lane 15 into the next row, lane 31 into the upper two rows)
+// carry the rows into lane 63, which is read back as a wave-uniform value
+__device__ __forceinline__ uint32_t cc_dpp_wave_add(uint32_t v) {
+  const int x = (int)v;
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, x, 0x111, 0xF, 0xF, true);   // row_shr:1
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, x, 0x112, 0xF, 0xF, true);   // row_shr:2
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, x, 0x113, 0xF, 0xF, true);   // row_shr:3
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xE, false);  // row_shr:4
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xC, false);  // row_shr:8
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false);  // row_bcast:15
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false);  // row_bcast:31
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+
+// the same shape for the maximum of non-negative values (0 is the identity
+// of the lanes a shift leaves empty; the result is max(0, wave maximum))
+__device__ __forceinline__ int cc_dpp_wave_max0(int v) {
+  const int x = v;
+  v = max(v, __builtin_amdgcn_update_dpp(0, x, 0x111, 0xF, 0xF, true));
+  v = max(v, __builtin_amdgcn_update_dpp(0, x, 0x112, 0xF, 0xF, true));
+  v = max(v, __builtin_amdgcn_update_dpp(0, x, 0x113, 0xF, 0xF, true));
+  v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x114, 0xF, 0xE, false));
+  v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x118, 0xF, 0xC, false));
+  v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x142, 0xA, 0xF, false));
+  v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x143, 0xC, 0xF, false));
+  return max(0, __builtin_amdgcn_readlane(v, 63));
+}
+
 // inclusive prefix sum over the 64 lanes of a wave
 __device__ __forceinline__ int cc_wave_incl_scan(int v) {
   const int lane = threadIdx.x & 63;

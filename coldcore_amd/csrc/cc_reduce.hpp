@@ -1,7 +1,7 @@
 // cc_reduce.hpp -- whole-field reductions and queries.
 //
-// Semantics follow reference src/coldcore/protocol.py:181-219 and
-// src/plugins/covering/dctcore_fast.cuh:199-327:
+// Semantics follow reference src/coldcore/protocol.py:181-219 (the counts,
+// the binning rule and the ordered extraction the host backends define):
 //   count_eq(t)   sum [cnt == t]                   (int64)
 //   deficit       sum max(0, mu - cnt)             (int64)
 //   map_max       max(0, max A0)
@@ -42,83 +42,113 @@ __global__ void k_scatter_code(int32_t *A0, const long long *idx, int M) {
   if (i < M) atomicAdd(&A0[idx[i]], 1);
 }
 
+// ---- field scans -----------------------------------------------------
+// A workgroup owns one contiguous slab of the plane (a multiple of 8 x
+// CC_BLOCK words, so every slab starts 16-byte aligned), each thread folds
+// 16-byte loads of the slab into a 32-bit partial, the wave combines the
+// partials with DPP row shifts and row broadcasts (the total lands in lane
+// 63) and one atomic per wave lands in the result.  The host sizes the slabs
+// so no 32-bit partial can wrap (at most 2^15 words per thread).
+
+// the words of slab b: [b * slab, min(N, (b + 1) * slab))
+__device__ __forceinline__ void cc_slab(long long N, long long slab,
+                                        long long *w0, long long *w1) {
+  *w0 = (long long)blockIdx.x * slab;
+  *w1 = min(N, *w0 + slab);
+}
+
+// the predicate count of the two uint16 counters of dword x: EQ counts
+// halves equal to the target (pat = target in both halves: a zero half of
+// x ^ pat), DEF sums max(0, mu - c) over both halves
+template <int MODE>
+__device__ __forceinline__ uint32_t cc_census2(uint32_t x, uint32_t pat, int mu) {
+  if (MODE == CC_EQ) {
+    const uint32_t y = x ^ pat;
+    return (uint32_t)((y & 0xFFFFu) == 0u) + (uint32_t)((y >> 16) == 0u);
+  } else {
+    const int lo = mu - (int)(x & 0xFFFFu), hi = mu - (int)(x >> 16);
+    return (uint32_t)(lo > 0 ? lo : 0) + (uint32_t)(hi > 0 ? hi : 0);
+  }
+}
+
 // MODE CC_EQ: sum [cnt == target];  CC_DEF: sum max(0, mu - cnt)
 template <int MODE>
 __global__ void __launch_bounds__(CC_BLOCK)
-    k_cnt_reduce(const uint16_t *__restrict__ cnt, long long N, int target,
-                 int mu, unsigned long long *out) {
-  const long long nvec = N >> 3;
-  const long long gs = (long long)gridDim.x * CC_BLOCK;
-  const uint4 *v = (const uint4 *)cnt;
-  long long acc = 0;
-  for (long long i = (long long)blockIdx.x * CC_BLOCK + threadIdx.x; i < nvec;
-       i += gs) {
-    const uint4 q = v[i];
-    const uint32_t wv[4] = {q.x, q.y, q.z, q.w};
-#pragma unroll
-    for (int h = 0; h < 8; h++) {
-      const int c = (int)((wv[h >> 1] >> (16 * (h & 1))) & 0xFFFFu);
-      if (MODE == CC_EQ) acc += (c == target);
-      else { const int d = mu - c; acc += d > 0 ? d : 0; }
-    }
+    k_cnt_census(const uint16_t *__restrict__ cnt, long long N, long long slab,
+                 int target, int mu, unsigned long long *out) {
+  long long w0, w1;
+  cc_slab(N, slab, &w0, &w1);
+  const uint32_t pat = ((uint32_t)target & 0xFFFFu) * 0x00010001u;
+  const long long v1 = w0 + ((w1 - w0) & ~7LL);     // end of the whole 8-word runs
+  uint32_t part = 0;
+  for (long long i = w0 + 8LL * threadIdx.x; i < v1; i += 8LL * CC_BLOCK) {
+    const uint4 q = *(const uint4 *)(cnt + i);
+    part += cc_census2<MODE>(q.x, pat, mu) + cc_census2<MODE>(q.y, pat, mu) +
+            cc_census2<MODE>(q.z, pat, mu) + cc_census2<MODE>(q.w, pat, mu);
   }
-  if (blockIdx.x == 0 && threadIdx.x < (int)(N & 7)) {
-    const int c = (int)cnt[nvec * 8 + threadIdx.x];
-    if (MODE == CC_EQ) acc += (c == target);
-    else { const int d = mu - c; acc += d > 0 ? d : 0; }
+  const long long j = v1 + threadIdx.x;             // the slab's last < 8 words
+  if (j < w1) {
+    const int c = (int)cnt[j];
+    if (MODE == CC_EQ) part += (uint32_t)(c == target);
+    else part += (uint32_t)(mu > c ? mu - c : 0);
   }
-  acc = cc_block_sum(acc);
-  if (threadIdx.x == 0 && acc) atomicAdd(out, (unsigned long long)acc);
+  const uint32_t wsum = cc_dpp_wave_add(part);
+  if ((threadIdx.x & 63) == 0 && wsum) atomicAdd(out, (unsigned long long)wsum);
 }
 
+// max(0, max of the map plane): DPP wave maxima, one atomicMax per wave
 __global__ void __launch_bounds__(CC_BLOCK)
-    k_max_i32(const int32_t *__restrict__ A, long long N, int *out) {
-  const long long nvec = N >> 2;
-  const long long gs = (long long)gridDim.x * CC_BLOCK;
-  const int4 *v = (const int4 *)A;
+    k_map_peak(const int32_t *__restrict__ A, long long N, long long slab,
+               int *out) {
+  long long w0, w1;
+  cc_slab(N, slab, &w0, &w1);
+  const long long v1 = w0 + ((w1 - w0) & ~3LL);
   int m = 0;
-  for (long long i = (long long)blockIdx.x * CC_BLOCK + threadIdx.x; i < nvec;
-       i += gs) {
-    const int4 q = v[i];
+  for (long long i = w0 + 4LL * threadIdx.x; i < v1; i += 4LL * CC_BLOCK) {
+    const int4 q = *(const int4 *)(A + i);
     m = max(m, max(max(q.x, q.y), max(q.z, q.w)));
   }
-  if (blockIdx.x == 0 && threadIdx.x < (int)(N & 3))
-    m = max(m, A[nvec * 4 + threadIdx.x]);
-  m = cc_wave_max(m);
+  const long long j = v1 + threadIdx.x;
+  if (j < w1) m = max(m, A[j]);
+  m = cc_dpp_wave_max0(m);
   if ((threadIdx.x & 63) == 0 && m > 0) atomicMax(out, m);
 }
 
-__device__ __forceinline__ void cc_hist_one(int32_t v, int nbins, int32_t vmax,
-                                            int32_t *sh) {
-  if (v > 0) {
-    long long b = (long long)(v - 1) * nbins / vmax;
-    if (b >= nbins) b = nbins - 1;
-    atomicAdd(&sh[b], 1);
-  }
+// bin of a positive map value: floor((v - 1) * nbins / vmax), clipped to the
+// last bin.  (v - 1) * nbins < 2^43 and the quotient is below 4096, so the
+// correctly rounded double quotient never reaches the next integer (its gap
+// to it is at least 1 / vmax >= 2^-31, far above the rounding error) and the
+// truncation is exact.
+__device__ __forceinline__ int cc_bin(int32_t v, int nbins, int32_t vmax) {
+  const double q = (double)((long long)(v - 1) * nbins) / (double)vmax;
+  const int b = (int)q;
+  return b < nbins ? b : nbins - 1;
 }
 
+// histogram of the positive map values: a workgroup bins its slab into a
+// private LDS histogram (nbins <= 4096 counters), then adds the nonzero
+// counters to the global one
 __global__ void __launch_bounds__(CC_BLOCK)
-    k_hist(const int32_t *__restrict__ A, long long N, int nbins, int32_t vmax,
-           int32_t *hist) {
-  extern __shared__ int32_t sh[];
-  for (int i = threadIdx.x; i < nbins; i += CC_BLOCK) sh[i] = 0;
+    k_map_hist(const int32_t *__restrict__ A, long long N, long long slab,
+               int nbins, int32_t vmax, int32_t *hist) {
+  extern __shared__ int32_t bins[];
+  for (int i = threadIdx.x; i < nbins; i += CC_BLOCK) bins[i] = 0;
   __syncthreads();
-  const long long nvec = N >> 2;
-  const long long gs = (long long)gridDim.x * CC_BLOCK;
-  const int4 *v = (const int4 *)A;
-  for (long long i = (long long)blockIdx.x * CC_BLOCK + threadIdx.x; i < nvec;
-       i += gs) {
-    const int4 q = v[i];
-    cc_hist_one(q.x, nbins, vmax, sh);
-    cc_hist_one(q.y, nbins, vmax, sh);
-    cc_hist_one(q.z, nbins, vmax, sh);
-    cc_hist_one(q.w, nbins, vmax, sh);
+  long long w0, w1;
+  cc_slab(N, slab, &w0, &w1);
+  const long long v1 = w0 + ((w1 - w0) & ~3LL);
+  for (long long i = w0 + 4LL * threadIdx.x; i < v1; i += 4LL * CC_BLOCK) {
+    const int4 q = *(const int4 *)(A + i);
+    const int32_t e[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+      if (e[k] > 0) atomicAdd(&bins[cc_bin(e[k], nbins, vmax)], 1);
   }
-  if (blockIdx.x == 0 && threadIdx.x < (int)(N & 3))
-    cc_hist_one(A[nvec * 4 + threadIdx.x], nbins, vmax, sh);
+  const long long j = v1 + threadIdx.x;
+  if (j < w1 && A[j] > 0) atomicAdd(&bins[cc_bin(A[j], nbins, vmax)], 1);
   __syncthreads();
   for (int i = threadIdx.x; i < nbins; i += CC_BLOCK)
-    if (sh[i]) atomicAdd(&hist[i], sh[i]);
+    if (bins[i]) atomicAdd(&hist[i], bins[i]);
 }
 
 __global__ void k_read_at(const int32_t *A, const long long *idx, int k,

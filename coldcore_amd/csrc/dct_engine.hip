@@ -220,6 +220,18 @@ bool valid_words(const long long *w, int n) {
   return true;
 }
 
+// Slabs of a field scan (k_cnt_census, k_map_peak, k_map_hist): whole
+// multiples of 8 x CC_BLOCK words, at most 2^17 words (512 per thread, so a
+// wave's 32-bit deficit total stays below 2^32), at least N / maxb words so
+// the grid stays within maxb workgroups where that bound allows.
+struct Slabs { unsigned blocks; long long slab; };
+Slabs scan_slabs(long long N, long long maxb = 8192, long long cap = 1LL << 17) {
+  const long long unit = 8LL * CC_BLOCK;
+  long long slab = ((N + maxb - 1) / maxb + unit - 1) / unit * unit;
+  slab = std::max(unit, std::min(slab, cap));
+  return Slabs{(unsigned)std::max(1LL, (N + slab - 1) / slab), slab};
+}
+
 int pass_blocks(long long items) {
   long long b = (items + CC_BLOCK * 8LL - 1) / (CC_BLOCK * 8LL);
   return (int)std::max(1LL, std::min(b, 8192LL));
@@ -1160,13 +1172,15 @@ int ensure_zlist(int pmode, int target, long long lim) {
   // never pays for a listing of millions of words
   hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, E.st, E.d_scal + 2, 1);
   CK(hipGetLastError());
-  if (pmode == CC_DEF)
-    hipLaunchKernelGGL(k_cnt_reduce<CC_DEF>, dim3(pass_blocks(E.X / 8 + 1)),
-                       dim3(CC_BLOCK), 0, E.st, E.cnt, E.X, 0, E.mu, E.d_scal + 2);
-  else
-    hipLaunchKernelGGL(k_cnt_reduce<CC_EQ>, dim3(pass_blocks(E.X / 8 + 1)),
-                       dim3(CC_BLOCK), 0, E.st, E.cnt, E.X, target, E.mu,
-                       E.d_scal + 2);
+  {
+    const Slabs sl = scan_slabs(E.X);
+    if (pmode == CC_DEF)
+      hipLaunchKernelGGL(k_cnt_census<CC_DEF>, dim3(sl.blocks), dim3(CC_BLOCK), 0,
+                         E.st, E.cnt, E.X, sl.slab, 0, E.mu, E.d_scal + 2);
+    else
+      hipLaunchKernelGGL(k_cnt_census<CC_EQ>, dim3(sl.blocks), dim3(CC_BLOCK), 0,
+                         E.st, E.cnt, E.X, sl.slab, target, E.mu, E.d_scal + 2);
+  }
   CK(hipGetLastError());
   unsigned long long bound = 0;
   if (int rc = fetch_scalar(2, &bound)) return rc;
@@ -1920,9 +1934,9 @@ int dct_loss_owner(const long long *words, int M, int32_t *out) {
 long long dct_count_eq(int target) {
   if (!E.live) return -1;
   if (int rc = zero_scalars()) return rc;
-  hipLaunchKernelGGL(k_cnt_reduce<CC_EQ>, dim3(pass_blocks(E.X / 8 + 1)),
-                     dim3(CC_BLOCK), 0, E.st, E.cnt, E.X, target, E.mu,
-                     E.d_scal);
+  const Slabs sl = scan_slabs(E.X);
+  hipLaunchKernelGGL(k_cnt_census<CC_EQ>, dim3(sl.blocks), dim3(CC_BLOCK), 0,
+                     E.st, E.cnt, E.X, sl.slab, target, E.mu, E.d_scal);
   CK(hipGetLastError());
   unsigned long long r;
   if (int rc = fetch_scalar(0, &r)) return rc;
@@ -1932,8 +1946,9 @@ long long dct_count_eq(int target) {
 long long dct_count_deficit(void) {
   if (!E.live) return -1;
   if (int rc = zero_scalars()) return rc;
-  hipLaunchKernelGGL(k_cnt_reduce<CC_DEF>, dim3(pass_blocks(E.X / 8 + 1)),
-                     dim3(CC_BLOCK), 0, E.st, E.cnt, E.X, 0, E.mu, E.d_scal);
+  const Slabs sl = scan_slabs(E.X);
+  hipLaunchKernelGGL(k_cnt_census<CC_DEF>, dim3(sl.blocks), dim3(CC_BLOCK), 0,
+                     E.st, E.cnt, E.X, sl.slab, 0, E.mu, E.d_scal);
   CK(hipGetLastError());
   unsigned long long r;
   if (int rc = fetch_scalar(0, &r)) return rc;
@@ -1943,8 +1958,9 @@ long long dct_count_deficit(void) {
 int dct_map_max(int32_t *out_max) {
   if (!E.live || !out_max) return -1;
   if (int rc = zero_scalars()) return rc;
-  hipLaunchKernelGGL(k_max_i32, dim3(pass_blocks(E.X / 4 + 1)), dim3(CC_BLOCK),
-                     0, E.st, E.A[0], E.X, (int *)E.d_scal);
+  const Slabs sl = scan_slabs(E.X);
+  hipLaunchKernelGGL(k_map_peak, dim3(sl.blocks), dim3(CC_BLOCK), 0, E.st,
+                     E.A[0], E.X, sl.slab, (int *)E.d_scal);
   CK(hipGetLastError());
   unsigned long long r;
   if (int rc = fetch_scalar(0, &r)) return rc;
@@ -1958,9 +1974,11 @@ int dct_map_hist(int nbins, int32_t vmax, int32_t *out_hist) {
                      E.d_hist, 0, (long long)nbins);
   CK(hipGetLastError());
   if (vmax > 0) {
-    const int hb = std::min(pass_blocks(E.X / 4 + 1), 1024);
-    hipLaunchKernelGGL(k_hist, dim3(hb), dim3(CC_BLOCK), (size_t)nbins * 4,
-                       E.st, E.A[0], E.X, nbins, vmax, E.d_hist);
+    // at most 1,024 slabs (no 32-bit partials here): each adds its nonzero
+    // bins to the global ones
+    const Slabs sl = scan_slabs(E.X, 1024, 1LL << 40);
+    hipLaunchKernelGGL(k_map_hist, dim3(sl.blocks), dim3(CC_BLOCK), (size_t)nbins * 4,
+                       E.st, E.A[0], E.X, sl.slab, nbins, vmax, E.d_hist);
     CK(hipGetLastError());
   }
   CK(hipMemcpyAsync(E.h_out, E.d_hist, (size_t)nbins * 4,

@@ -132,6 +132,28 @@ def _ptr(a, t):
     return a.ctypes.data_as(t)
 
 
+#: candidates and drops one dct_swap_eval call takes (uint16 staging)
+_SWAP_LIMIT = (1 << 16) - 1
+
+#: dct_stats slots, in slot order; the names are the reference binding's
+#: (interface: callers read them)
+STAT_KEYS = ("gather_set_side", "gather_walk", "set_scans", "set_overflows",
+             "swap_eval_fast", "swap_eval_fallback", "count_eq0_listed",
+             "gather_set_tiled")
+
+
+def _swap_batches(sizes, limit):
+    """[lo, hi) runs of consecutive drops whose candidate total and drop
+    count stay within ``limit`` (each drop's own count is within it)."""
+    k = len(sizes)
+    lo = 0
+    while lo < k:
+        run = np.cumsum(sizes[lo:lo + limit])
+        hi = lo + max(1, int(np.searchsorted(run, limit, side="right")))
+        yield lo, hi
+        lo = hi
+
+
 class GpuBackend:
     """One problem cell resident in MI355X HBM (reference gpu.py:207-537).
 
@@ -183,13 +205,12 @@ class GpuBackend:
                            _ptr(pats, _U64P), len(pats), extract_cap,
                            ctypes.byref(by), self.layers_mode, self.cnt_mode,
                            1 if use_owner else 0)
-        _check("init", rc)
-        _check("set_mu", lib.dct_set_mu(mu))
+        _check("dct_init3", rc)
+        _check("dct_set_mu", lib.dct_set_mu(mu))
         self.gpu_bytes = by.value
         self.host_bytes = int(lib.dct_host_bytes())
         self._closed = False
 
-    # ---------------- lifecycle ----------------
     def close(self):
         if not self._closed:
             self._lib.dct_free()
@@ -207,25 +228,23 @@ class GpuBackend:
     def index_word(self, idx):
         return self.space.index_word(idx)
 
-    # ---------------- solution / fields ----------------
     def _set_code(self, idx):
         idx = np.ascontiguousarray(np.atleast_1d(idx), dtype=np.int64)
-        _check("set_code", self._lib.dct_set_code(_ptr(idx, _LLP), len(idx)))
+        _check("dct_set_code", self._lib.dct_set_code(_ptr(idx, _LLP), len(idx)))
 
     def recount(self, idx):
         self._set_code(idx)
-        _check("transform(code->cnt)", self._lib.dct_transform(0, 2))
+        _check("dct_transform(0, 2)", self._lib.dct_transform(0, 2))
 
     def gain_map(self):
-        _check("transform(uncov->gain)", self._lib.dct_transform(1, 1))
+        _check("dct_transform(1, 1)", self._lib.dct_transform(1, 1))
 
     def loss_map(self):
-        _check("transform(cnt1->loss)", self._lib.dct_transform(2, 1))
+        _check("dct_transform(2, 1)", self._lib.dct_transform(2, 1))
 
-    # ---------------- incremental / exact ----------------
     def ball_update(self, idx, delta):
         idx = np.ascontiguousarray(np.atleast_1d(idx), dtype=np.int64)
-        _check("ball_update", self._lib.dct_ball_update(
+        _check("dct_ball_update", self._lib.dct_ball_update(
             _ptr(idx, _LLP), len(idx), int(delta)))
 
     def ball_gather(self, idx, target):
@@ -235,77 +254,71 @@ class GpuBackend:
         idx = np.ascontiguousarray(np.atleast_1d(idx), dtype=np.int64)
         out = np.empty(len(idx), dtype=np.int32)
         if self.mu == 1 or target not in (0, 1):
-            _check("ball_gather", self._lib.dct_ball_gather(
+            _check("dct_ball_gather", self._lib.dct_ball_gather(
                 _ptr(idx, _LLP), len(idx), int(target), _ptr(out, _I32P)))
         else:
-            _check("ball_gather_mu", self._lib.dct_ball_gather_mu(
+            _check("dct_ball_gather_mu", self._lib.dct_ball_gather_mu(
                 _ptr(idx, _LLP), len(idx), int(target), _ptr(out, _I32P)))
         return out
 
     def ball_update_delta(self, idx, delta):
         idx = np.ascontiguousarray(np.atleast_1d(idx), dtype=np.int64)
         dd = _LL(0)
-        _check("ball_update_delta", self._lib.dct_ball_update_delta(
+        _check("dct_ball_update_delta", self._lib.dct_ball_update_delta(
             _ptr(idx, _LLP), len(idx), int(delta), ctypes.byref(dd)))
         return int(dd.value)
 
     def swap_eval(self, drops, cands):
-        """Exact costs of many swaps without changing state
-        (reference gpu.py:368-440)."""
+        """Exact costs of many swaps without changing state (reference
+        gpu.py:368-440): loss[i] of removing drops[i] and the gain of every
+        candidate once drops[i] is gone.  ``cands`` is one array shared by
+        every drop (gains come back as a (k, m) array) or one array per drop
+        (a list of arrays).  The ABI call takes at most 65,535 drops and
+        65,535 candidates, so consecutive drops are packed into calls up to
+        that limit."""
         drops = np.ascontiguousarray(np.atleast_1d(drops), dtype=np.int64)
         k = len(drops)
         shared = not isinstance(cands, (list, tuple))
         if shared:
-            c1 = np.ascontiguousarray(np.atleast_1d(cands), dtype=np.int64)
-            parts = [c1] * k
+            per = [np.ascontiguousarray(np.atleast_1d(cands), dtype=np.int64)] * k
+        elif len(cands) != k:
+            raise ValueError(f"{len(cands)} candidate arrays for {k} drops")
         else:
-            if len(cands) != k:
-                raise ValueError("need one candidate array per drop")
-            parts = [np.ascontiguousarray(np.atleast_1d(c), dtype=np.int64)
-                     for c in cands]
-        cap = (1 << 16) - 1
+            per = [np.ascontiguousarray(np.atleast_1d(c), dtype=np.int64)
+                   for c in cands]
+        limit = _SWAP_LIMIT
+        sizes = np.fromiter((len(c) for c in per), dtype=np.int64, count=k)
+        if k and int(sizes.max()) > limit:
+            raise ValueError(f"a drop has {int(sizes.max())} candidates; "
+                             f"one call takes at most {limit}")
         loss = np.empty(k, dtype=np.int32)
-        gains = [None] * k
-        i = 0
-        while i < k:
-            j, tot = i, 0
-            while j < k and j - i < cap and tot + len(parts[j]) <= cap:
-                tot += len(parts[j])
-                j += 1
-            if j == i:
-                raise ValueError(f"over {cap} candidates for one drop")
-            cat = (np.concatenate(parts[i:j]) if tot
-                   else np.zeros(1, dtype=np.int64))
-            off = np.zeros(j - i + 1, dtype=np.int64)
-            off[1:] = np.cumsum([len(p) for p in parts[i:j]])
-            g = np.empty(max(tot, 1), dtype=np.int32)
-            lo = np.empty(j - i, dtype=np.int32)
-            dsub = np.ascontiguousarray(drops[i:j])
-            _check("swap_eval", self._lib.dct_swap_eval(
-                _ptr(dsub, _LLP), j - i, _ptr(cat, _LLP), _ptr(off, _LLP),
-                _ptr(g, _I32P), _ptr(lo, _I32P)))
-            loss[i:j] = lo
-            for t in range(j - i):
-                gains[i + t] = g[off[t]:off[t + 1]].copy()
-            i = j
+        gains = []
+        for lo, hi in _swap_batches(sizes, limit):
+            bounds = np.concatenate(([0], np.cumsum(sizes[lo:hi]))).astype(np.int64)
+            flat = (np.concatenate(per[lo:hi]) if bounds[-1]
+                    else np.zeros(1, dtype=np.int64))
+            got = np.empty(max(int(bounds[-1]), 1), dtype=np.int32)
+            part = np.ascontiguousarray(drops[lo:hi])
+            _check("dct_swap_eval", self._lib.dct_swap_eval(
+                _ptr(part, _LLP), hi - lo, _ptr(flat, _LLP), _ptr(bounds, _LLP),
+                _ptr(got, _I32P), _ptr(loss[lo:hi], _I32P)))
+            gains.extend(np.split(got[:bounds[-1]], bounds[1:-1]))
         if shared:
-            return loss, (np.stack(gains) if k
-                          else np.zeros((0, len(c1)), dtype=np.int32))
+            m = len(per[0]) if k else len(np.atleast_1d(cands))
+            return loss, (np.vstack(gains) if k else np.zeros((0, m), dtype=np.int32))
         return loss, gains
 
     def gpu_stats(self):
-        buf = (_LL * 8)()
-        self._lib.dct_stats(buf, 8)
-        keys = ("gather_set_side", "gather_walk", "set_scans",
-                "set_overflows", "swap_eval_fast", "swap_eval_fallback",
-                "count_eq0_listed", "gather_set_tiled")
-        return {k: int(v) for k, v in zip(keys, buf)}
+        """The engine's event counters (dct_stats slots 0..7) under the
+        reference binding's key names (reference gpu.py:451-456)."""
+        slots = (_LL * len(STAT_KEYS))()
+        self._lib.dct_stats(slots, len(STAT_KEYS))
+        return dict(zip(STAT_KEYS, (int(v) for v in slots)))
 
-    # ---------------- queries ----------------
     def count_eq(self, target):
         r = int(self._lib.dct_count_eq(int(target)))
         if r < 0:
-            _check("count_eq", r)
+            _check("dct_count_eq", r)
         return r
 
     def uncovered(self):
@@ -313,17 +326,17 @@ class GpuBackend:
         r = int(self._lib.dct_count_eq(0) if self.mu == 1
                 else self._lib.dct_count_deficit())
         if r < 0:
-            _check("uncovered", r)
+            _check("dct_count_eq/deficit", r)
         return r
 
     def map_max(self):
         v = ctypes.c_int32(0)
-        _check("map_max", self._lib.dct_map_max(ctypes.byref(v)))
+        _check("dct_map_max", self._lib.dct_map_max(ctypes.byref(v)))
         return int(v.value)
 
     def map_hist(self, nbins, vmax):
         out = np.zeros(nbins, dtype=np.int32)
-        _check("map_hist", self._lib.dct_map_hist(
+        _check("dct_map_hist", self._lib.dct_map_hist(
             int(nbins), int(vmax), _ptr(out, _I32P)))
         return out
 
@@ -343,14 +356,14 @@ class GpuBackend:
                 int(thr), _ptr(oi, ctypes.POINTER(ctypes.c_uint32)),
                 _ptr(ov, _I32P), cap)
         if found < 0:
-            _check("map_extract", int(found))
+            _check("dct_map_extract", int(found))
         m = min(int(found), cap)
         return oi[:m].astype(np.int64), ov[:m], int(found)
 
     def map_read_at(self, idx):
         idx = np.ascontiguousarray(np.atleast_1d(idx), dtype=np.int64)
         out = np.empty(len(idx), dtype=np.int32)
-        _check("map_read_at", self._lib.dct_map_read_at(
+        _check("dct_map_read_at", self._lib.dct_map_read_at(
             _ptr(idx, _LLP), len(idx), _ptr(out, _I32P)))
         return out
 
@@ -358,18 +371,17 @@ class GpuBackend:
         """Map-plane entries [off, off+length) (default: the whole plane)."""
         length = self.space.size - off if length is None else length
         out = np.empty(length, dtype=np.int32)
-        _check("map_read", self._lib.dct_map_read(int(off), int(length),
+        _check("dct_map_read", self._lib.dct_map_read(int(off), int(length),
                                                   _ptr(out, _I32P)))
         return out
 
     def read_cnt(self, off=0, length=None):
         length = self.space.size - off if length is None else length
         out = np.empty(length, dtype=np.uint16)
-        _check("read_cnt", self._lib.dct_read_cnt(
+        _check("dct_read_cnt", self._lib.dct_read_cnt(
             int(off), int(length), _ptr(out, ctypes.POINTER(ctypes.c_uint16))))
         return out
 
-    # ---------------- additive: exact distance layers ---------------------
     def layers_at(self, init_mode, idx, code=None):
         """(k, R+1) exact distance layers at idx: entry [i, d] is the number
         of seed words at distance exactly d from idx[i]; init_mode 0 seeds
@@ -378,19 +390,18 @@ class GpuBackend:
         idx = np.ascontiguousarray(np.atleast_1d(idx), dtype=np.int64)
         if init_mode == 0:
             self._set_code(np.zeros(0, dtype=np.int64) if code is None else code)
-        _check("transform_layers", self._lib.dct_transform_layers(int(init_mode)))
+        _check("dct_transform_layers", self._lib.dct_transform_layers(int(init_mode)))
         L = self.R + 1
         out = np.empty((len(idx), L), dtype=np.int32)
-        _check("layers_read_at", self._lib.dct_layers_read_at(
+        _check("dct_layers_read_at", self._lib.dct_layers_read_at(
             _ptr(idx, _LLP), len(idx), _ptr(out, _I32P)))
         return out
 
-    # ---------------- additive: schedule and per-sweep timing -------------
     def sweep_plan(self):
         buf = (ctypes.c_int * (8 * 17))()
         ns = self._lib.dct_sweep_plan(buf, 16)
         if ns < 0:
-            _check("sweep_plan", ns)
+            _check("dct_sweep_plan", ns)
         keys = ("first_axis", "axes", "P", "TL", "layers_in", "layers_out",
                 "round_q", "lds_bytes")
         out = [dict(zip(keys, buf[8 * i:8 * i + 8])) for i in range(ns)]
@@ -406,7 +417,7 @@ class GpuBackend:
         buf = (ctypes.c_int * (13 * 41))()
         ns = self._lib.dct_sweep_plan2(kind, buf, 40)
         if ns < 0:
-            _check("sweep_plan2", ns)
+            _check("dct_sweep_plan2", ns)
         keys = ("first_axis", "axes", "P", "TL", "RR", "W", "ND", "bytes_in",
                 "bytes_out", "lds_bytes", "threads", "rounds", "kernel")
         return [dict(zip(keys, buf[13 * i:13 * i + 13])) for i in range(ns)]
@@ -418,7 +429,7 @@ class GpuBackend:
         us = ctypes.c_float(0.0)
         n = self._lib.dct_autotune(int(reps), ctypes.byref(us))
         if n < 0:
-            _check("autotune", n)
+            _check("dct_autotune", n)
         return n, float(us.value)
 
     def sweep_force(self, cuts):
@@ -433,13 +444,13 @@ class GpuBackend:
         return ns
 
     def sweep_profile(self, on=True):
-        _check("sweep_profile", self._lib.dct_sweep_profile(1 if on else 0))
+        _check("dct_sweep_profile", self._lib.dct_sweep_profile(1 if on else 0))
 
     def sweep_times(self):
         buf = (ctypes.c_float * 16)()
         ns = self._lib.dct_sweep_times(buf, 16)
         if ns < 0:
-            _check("sweep_times", ns)
+            _check("dct_sweep_times", ns)
         return [float(buf[i]) for i in range(ns)]
 
 
