@@ -134,3 +134,46 @@ def test_config1_prefix_identical_cpu_and_ref():
     a = config1_run(BruteForce([3] * 6, 2), start, rounds=40)
     b = config1_run(NumpyDP([3] * 6, 2), start, rounds=40)
     assert a["trace"] == b["trace"] and (a["final"] == b["final"]).all()
+
+
+# ---- the reference search core's own config-1 trajectory -----------------
+# tests/golden/lns_k3_8_2.json was written by tests/golden/make_lns_trace.py
+# from the reference's Searcher on its numpy backend (K_3(8,2), 40-word start,
+# greedy, peel, 1,000 LNS rounds of notch descent): per round the uncovered
+# count after accept / reject and the code size, and the final code.
+def _golden_lns():
+    import json
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                        "lns_k3_8_2.json")
+    with open(path) as f:
+        return json.load(f)
+
+
+def _check_against_reference_run(backend):
+    from coldcore_amd.search import config1_run
+    gold = _golden_lns()
+    start = np.random.default_rng(0).choice(3 ** 8, size=40, replace=False)
+    r = config1_run(backend, start, rounds=gold["rounds"])
+    assert r["greedy"] == gold["greedy"] and r["peeled"] == gold["peeled"]
+    got = [[int(post), int(size)] for _mode, post, size in r["trace"]]
+    want = [[u, size] for _M, u, size in gold["trace_M_uncov_size"]]
+    assert len(got) == len(want) == gold["rounds"]
+    first_bad = next((i for i, (a, b) in enumerate(zip(got, want)) if a != b), None)
+    assert first_bad is None, (first_bad, got[first_bad], want[first_bad])
+    assert [list(x) for x in r["sizes"]] == gold["sizes"]
+    assert r["uncov"] == gold["final_uncovered"]
+    final = np.asarray(r["final"], dtype=np.int64)
+    assert hashlib.sha256(final.tobytes()).hexdigest() == gold["final_sha256"]
+
+
+def test_config1_lns_equals_reference_trajectory_on_oracle():
+    """The restated greedy / peel / LNS, driven by the numpy oracle, follows
+    the reference's own 1,000-round trajectory round for round."""
+    _check_against_reference_run(NumpyDP([3] * 8, 2))
+
+
+@pytest.mark.gpu
+def test_config1_lns_equals_reference_trajectory_on_gpu(gpu_cell):
+    """The same 1,000 rounds through the engine's C ABI."""
+    _check_against_reference_run(gpu_cell([3] * 8, 2))
