@@ -41,12 +41,20 @@
 #include "cc_common.hpp"
 
 #define CC_IN_PACKED 3   // layers from the compact planes
+#define CC_IN_SPARSE 4   // layer 0 = the code's multiplicities, from its sorted
+                         // word list (k_pfirstp only: no plane is read)
 #define CC_OUT_PACKED 3  // layers to the compact planes
 #define CC_PK_MAXRD 12   // rounds per sweep
 
 struct PSweepArgs {
   const uint16_t *cnt;   // CC_IN_PRED: predicate source
   const int32_t *code;   // CC_IN_CODE: code multiplicity plane
+  // CC_IN_SPARSE: the code's distinct words ascending (cw) with their
+  // multiplicities (cm); the words of first-sweep tile t are entries
+  // [toff[t], toff[t + 1])
+  const long long *cw;
+  const int32_t *cm;
+  const int32_t *toff;
   uint32_t *pk;          // compact layer planes (in and out, in place)
   int32_t *map;          // CC_OUT_MAP
   uint16_t *cnt_out;     // CC_OUT_CNT
@@ -742,7 +750,8 @@ __global__ void __launch_bounds__(CC_BLOCK) k_pfirst(PSweepArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// First sweep with an LDS exchange on a persistent grid, predicate seed:
+// First sweep with an LDS exchange on a persistent grid, predicate seed (or
+// a code seed from the code's sorted word list):
 // one cube per thread, and while a tile runs its rounds each thread already
 // has the next tile's counters in flight (raw 16-byte loads into registers),
 // so the memory pipe never waits for a tile's compute.  Round 0 is the
@@ -769,21 +778,42 @@ __global__ void __launch_bounds__(CC_BLOCK) k_pfirstp(PSweepArgs a) {
   const int u = threadIdx.x;                 // this thread's round-0 cube
   const uint32_t lane0 = (uint32_t)u * C;
   const uint32_t lo = (uint32_t)a.plo, span = (uint32_t)(a.phi - a.plo);
+  const bool sparse = a.in_kind == CC_IN_SPARSE;
   uint32_t pf[C / 2];
   long long tile = blockIdx.x;
-  if (tile < a.ntiles) pk_cnt_raw<C>(a, tile * a.P, lane0, pf);
+  if (tile < a.ntiles && !sparse) pk_cnt_raw<C>(a, tile * a.P, lane0, pf);
   for (; tile < a.ntiles; tile += gridDim.x) {
     const long long base = tile * a.P;
     uint32_t v[C][ND];
+    if (sparse) {
+      // code seed from the sorted word list: the tile's entries (a
+      // workgroup-uniform range, usually empty or one word) are matched
+      // against this thread's cube; no plane is read
 #pragma unroll
-    for (int c = 0; c < C; c++) {
+      for (int c = 0; c < C; c++)
 #pragma unroll
-      for (int k = 0; k < ND; k++) v[c][k] = 0u;
-      const uint32_t cv = (pf[c >> 1] >> (16 * (c & 1))) & 0xFFFFu;
-      v[c][0] = (cv - lo) <= span ? 1u : 0u;
+        for (int k = 0; k < ND; k++) v[c][k] = 0u;
+      const int e1 = a.toff[tile + 1];
+      const long long cube0 = base + lane0;
+      for (int e = a.toff[tile]; e < e1; e++) {
+        const long long d = a.cw[e] - cube0;
+        if (d >= 0 && d < C) {
+          const uint32_t m = (uint32_t)a.cm[e];
+#pragma unroll
+          for (int c = 0; c < C; c++) v[c][0] = d == c ? m : v[c][0];
+        }
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < C; c++) {
+#pragma unroll
+        for (int k = 0; k < ND; k++) v[c][k] = 0u;
+        const uint32_t cv = (pf[c >> 1] >> (16 * (c & 1))) & 0xFFFFu;
+        v[c][0] = (cv - lo) <= span ? 1u : 0u;
+      }
+      const long long next = tile + gridDim.x;
+      if (next < a.ntiles) pk_cnt_raw<C>(a, next * a.P, lane0, pf);
     }
-    const long long next = tile + gridDim.x;
-    if (next < a.ntiles) pk_cnt_raw<C>(a, next * a.P, lane0, pf);
     for (int rd = 0; rd < a.nrounds; rd++) {
       const int qa = a.qa[rd];
       const int pb = (u / qa) * qa * C + (u % qa);

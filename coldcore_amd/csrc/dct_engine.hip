@@ -88,6 +88,56 @@ struct PPlan {
   long long bytes = 0;  // algorithmic bytes per word of one transform
 };
 
+// A code multiset kept as its distinct words ascending with their
+// multiplicities (what a sparse seed stages) and, built on first use, a hash
+// map (what update validation looks words up in).  Changes go through the
+// map, which then becomes the only valid form.
+struct Multiset {
+  std::vector<long long> w;
+  std::vector<int> m;
+  std::unordered_map<long long, int> map;
+  bool vec_ok = true, map_ok = true;
+  int maxm = 0;   // largest multiplicity (valid with vec_ok)
+
+  void clear() {
+    w.clear(); m.clear(); map.clear();
+    vec_ok = map_ok = true;
+    maxm = 0;
+  }
+  // from an unsorted word list with repeats
+  void assign(const long long *idx, int M) {
+    w.assign(idx, idx + M);
+    std::sort(w.begin(), w.end());
+    m.clear();
+    int k = 0;
+    for (int i = 0; i < M; i++) {
+      if (k && w[k - 1] == w[i]) { m[k - 1]++; continue; }
+      w[k++] = w[i];
+      m.push_back(1);
+    }
+    w.resize(k);
+    maxm = 0;
+    for (int c : m) maxm = std::max(maxm, c);
+    map.clear();
+    vec_ok = true;
+    map_ok = false;
+  }
+  const std::unordered_map<long long, int> &get() {
+    if (!map_ok) {
+      map.clear();
+      map.reserve(w.size());
+      for (size_t i = 0; i < w.size(); i++) map.emplace(w[i], m[i]);
+      map_ok = true;
+    }
+    return map;
+  }
+  std::unordered_map<long long, int> &mut() {
+    get();
+    vec_ok = false;
+    return map;
+  }
+};
+
 struct Engine {
   bool live = false;
   int prob = 0, n = 0, R = 0, L = 1, mu = 1;
@@ -110,6 +160,14 @@ struct Engine {
   int fmt = 0, use_owner = 0;
   long long *d_code = nullptr;
   int codeM = 0;
+  // sparse code seeds (recount without a code plane): the pending code's
+  // distinct words ascending, their multiplicities, and the entry offsets
+  // of every first-sweep tile
+  long long *d_cw = nullptr;   // CODECAP words, then CODECAP multiplicities
+  int32_t *d_cm = nullptr;     // the multiplicities of the last staging (in d_cw)
+  int32_t *d_toff = nullptr;
+  long long toff_cap = 0;
+  int sparse = 1;            // CC_SPARSE_CODE=0: code seeds through the A0 plane
   long long *d_words = nullptr;
   int32_t *d_out = nullptr;
   unsigned long long *d_scal = nullptr;  // [0] reductions [1] transitions
@@ -127,8 +185,19 @@ struct Engine {
   long long bytes_dev = 0;
   long long stats[8] = {};
   unsigned long long trans_last = 0;
-  std::unordered_map<long long, int> uploaded, active;
+  // the pending code (set_code) and the represented one (cnt's code)
+  Multiset uploaded, active;
+  std::vector<long long> code_raw;   // the pending list as given (repeats kept)
+  bool code_dev_ok = false;          // d_code holds code_raw
+  // after a recount the represented multiset equals the pending one: the
+  // flag stands for that instead of a copy (active_map / active_mut)
+  bool active_shared = false;
   int active_size = 0;
+  // the code seed's packed plan, kept while its inputs are unchanged
+  bool cp_ok = false;
+  double cp_s0 = 0, cp_cap = 0;
+  int cp_opts[8] = {};
+  PPlan cp_plan;
   std::vector<Sweep> plan;
   double ball = 0;           // exact metric ball size (closed form)
   int lds_budget = 64 * 1024;
@@ -179,6 +248,19 @@ struct Engine {
 };
 
 Engine E;
+
+// the represented multiset, for lookups / for changes (materialised from
+// the pending one on the first change after a recount)
+const std::unordered_map<long long, int> &active_map() {
+  return E.active_shared ? E.uploaded.get() : E.active.get();
+}
+std::unordered_map<long long, int> &active_mut() {
+  if (E.active_shared) {
+    E.active = E.uploaded;
+    E.active_shared = false;
+  }
+  return E.active.mut();
+}
 
 // Forced sweep boundaries of the packed schedules (dct_sweep_force): bit k
 // set means a sweep ends after axis k - 1 (0 < k < n).  0: the planner
@@ -542,9 +624,7 @@ bool compact_plan(double s0, double cap, std::vector<CFmt> &fm) {
 
 // seed bounds of the pending code (largest multiplicity, code size)
 void code_bounds(double *s0, double *cap) {
-  int mmax = 1;
-  for (const auto &kv : E.uploaded) mmax = std::max(mmax, kv.second);
-  *s0 = mmax;
+  *s0 = std::max(E.uploaded.maxm, 1);
   *cap = std::max(E.codeM, 1);
 }
 
@@ -827,7 +907,8 @@ bool pk_sweep(const PBounds &B, int j, int k, bool pred_seed, bool cnt_out,
     w.threads = std::min(256, (ncube + 63) / 64 * 64);
     // persistent grid with next-tile prefetch: one cube per thread, a
     // whole number of waves, vector counter loads, at least two rounds
-    if (E.pk_first == 1 && pred_seed && m > w.RR && C % 8 == 0 &&
+    // (a code seed takes it too, seeded from the code's sorted word list)
+    if (E.pk_first == 1 && (pred_seed || E.sparse) && m > w.RR && C % 8 == 0 &&
         ncube <= CC_BLOCK && ncube % 64 == 0)
       w.kern = 4;
   } else if (m == w.RR) {
@@ -876,7 +957,9 @@ bool pk_sweep(const PBounds &B, int j, int k, bool pred_seed, bool cnt_out,
     w.in_bytes = prev->out_bytes;
   } else {
     w.in_nd = 0;
-    w.in_bytes = pred_seed ? 2 : 4;
+    // a code seed read from the A0 plane, or (persistent first sweep) from
+    // the code's word list: no plane bytes
+    w.in_bytes = pred_seed ? 2 : (w.kern == 4 ? 0 : 4);
   }
   if (k == E.n) {
     w.out_nd = 0;
@@ -1007,6 +1090,9 @@ int run_ptransform(const PPlan &pl, int in_kind, int out_kind, int plo,
     memset(&a, 0, sizeof(a));
     a.cnt = E.cnt;
     a.code = E.A[0];
+    a.cw = E.d_cw;
+    a.cm = E.d_cm;
+    a.toff = E.d_toff;
     a.pk = (uint32_t *)E.lay;
     a.map = E.A[0];
     a.cnt_out = E.cnt;
@@ -1015,7 +1101,8 @@ int run_ptransform(const PPlan &pl, int in_kind, int out_kind, int plo,
     a.s = w.s; a.ncolb = w.ncolb; a.ntiles = w.ntiles;
     a.P = w.P; a.TL = w.TL; a.nax = w.nax; a.L = E.L;
     a.kern = w.kern;
-    a.in_kind = i == 0 ? in_kind : CC_IN_PACKED;
+    a.in_kind = i == 0 ? (in_kind == CC_IN_CODE && w.kern == 4 ? CC_IN_SPARSE : in_kind)
+                       : CC_IN_PACKED;
     a.out_kind = i == ns - 1 ? out_kind : CC_OUT_PACKED;
     a.plo = plo; a.phi = phi;
     a.in_nd = w.in_nd; a.out_nd = w.out_nd;
@@ -1050,18 +1137,89 @@ int run_ptransform(const PPlan &pl, int in_kind, int out_kind, int plo,
   return 0;
 }
 
+// toff[t] = first entry of the ascending word list cw[0..n) at or above
+// t * P (a lower bound by binary search), t = 0..ntiles
+__global__ void k_tile_offsets(const long long *__restrict__ cw, int n, long long P,
+                               long long ntiles, int32_t *toff) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t > ntiles) return;
+  const long long key = t * P;
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (cw[mid] < key) lo = mid + 1;
+    else hi = mid;
+  }
+  toff[t] = lo;
+}
+
+template <typename T>
+int dalloc(T **p, size_t bytes);
+
+// the pending code list on the device (for the indicator-plane scatter)
+int ensure_code_dev() {
+  if (E.code_dev_ok) return 0;
+  const int M = (int)E.code_raw.size();
+  for (int off = 0; off < M; off += CODECAP) {
+    const int nb = std::min(M - off, CODECAP);
+    memcpy(E.h_words, E.code_raw.data() + off, (size_t)nb * 8);
+    CK(hipMemcpyAsync(E.d_code + off, E.h_words, (size_t)nb * 8,
+                      hipMemcpyHostToDevice, E.st));
+    CK(hipStreamSynchronize(E.st));
+  }
+  E.code_dev_ok = true;
+  return 0;
+}
+
+// Stage the pending code for a sparse seed of first sweep w: its distinct
+// words ascending and their multiplicities (through the pinned swap
+// staging, free outside swap_eval), and the entry offsets of w's tiles.
+// Everything is queued on the engine's stream; the transform's closing
+// synchronisation releases the staging.
+int stage_sparse_code(const PSweep &w) {
+  // one upload: the n words, then their n multiplicities right after them
+  const int n = (int)E.uploaded.w.size();
+  long long *hw = (long long *)E.h_swap;
+  memcpy(hw, E.uploaded.w.data(), (size_t)n * 8);
+  memcpy(hw + n, E.uploaded.m.data(), (size_t)n * 4);
+  E.d_cm = (int32_t *)(E.d_cw + n);
+  if (n)
+    CK(hipMemcpyAsync(E.d_cw, hw, (size_t)n * 12, hipMemcpyHostToDevice, E.st));
+  const long long ntiles = w.ntiles;
+  if (ntiles + 1 > E.toff_cap) {
+    CK(hipStreamSynchronize(E.st));
+    if (E.d_toff) (void)hipFree(E.d_toff);
+    E.d_toff = nullptr;
+    E.toff_cap = 0;
+    if (dalloc(&E.d_toff, (size_t)(ntiles + 1) * 4)) return -3;
+    E.toff_cap = ntiles + 1;
+  }
+  hipLaunchKernelGGL(k_tile_offsets, dim3((unsigned)((ntiles + 256) / 256)), dim3(256), 0,
+                     E.st, E.d_cw, n, (long long)w.P, ntiles, E.d_toff);
+  CK(hipGetLastError());
+  return 0;
+}
+
 // packed plan of a code seed: multiplicities of the uploaded multiset
 PPlan pk_code_plan(bool cnt_out) {
-  int mmax = 0;
-  for (const auto &kv : E.uploaded) mmax = std::max(mmax, kv.second);
-  PPlan p = pk_plan(std::max(mmax, 1), std::max(E.codeM, 1), false, cnt_out);
+  const double s0 = std::max(E.uploaded.maxm, 1), cap = std::max(E.codeM, 1);
+  // the planner is a shortest path over every sweep boundary pair: keep its
+  // result while the seed bounds and the schedule options stay the same
+  const int opts[8] = {(int)cnt_out, E.lds_budget, E.pk_col, E.pk_first, E.pk_narrow,
+                       E.packed, E.sparse, (int)(g_cuts & 0x7FFFFFFF) ^ (int)(g_cuts >> 31)};
+  if (E.cp_ok && E.cp_s0 == s0 && E.cp_cap == cap && !memcmp(opts, E.cp_opts, sizeof(opts)))
+    return E.cp_plan;
+  PPlan p = pk_plan(s0, cap, false, cnt_out);
   if (!p.ok && g_cuts) {
     // forced boundaries a code seed cannot use: the planner's own schedule
     const uint64_t keep = g_cuts;
     g_cuts = 0;
-    p = pk_plan(std::max(mmax, 1), std::max(E.codeM, 1), false, cnt_out);
+    p = pk_plan(s0, cap, false, cnt_out);
     g_cuts = keep;
   }
+  E.cp_ok = true; E.cp_s0 = s0; E.cp_cap = cap;
+  memcpy(E.cp_opts, opts, sizeof(opts));
+  E.cp_plan = p;
   return p;
 }
 
@@ -1304,18 +1462,20 @@ bool update_ok(const long long *words, int nwords, int delta,
   for (int i = 0; i < nwords; i++) ++batch[words[i]];
   if (delta == -1)
     for (const auto &kv : batch) {
-      auto it = E.active.find(kv.first);
-      if (it == E.active.end() || it->second < kv.second) return false;
+      const auto &act = active_map();
+      auto it = act.find(kv.first);
+      if (it == act.end() || it->second < kv.second) return false;
     }
   return true;
 }
 
 void apply_batch(const std::unordered_map<long long, int> &batch, int delta,
                  int nwords) {
+  auto &act = active_mut();
   for (const auto &kv : batch) {
-    int &c = E.active[kv.first];
+    int &c = act[kv.first];
     c += delta * kv.second;
-    if (c == 0) E.active.erase(kv.first);
+    if (c == 0) act.erase(kv.first);
   }
   E.active_size += delta * nwords;
 }
@@ -1337,13 +1497,15 @@ void release_all() {
   if (E.lay) (void)hipFree(E.lay);
   E.lay = nullptr;
   for (int d = 0; d < CC_MAXL; d++) { E.A[d] = nullptr; E.H[d] = nullptr; }
-  void *bufs[] = {E.cnt, E.pat, E.pat2, E.d_code, E.d_words, E.d_out,
+  void *bufs[] = {E.cnt, E.pat, E.pat2, E.d_code, E.d_cw, E.d_toff,
+                  E.d_words, E.d_out,
                   E.d_scal, E.d_hist, E.d_eidx64, E.d_eidx32, E.d_eval,
                   E.d_bcnt, E.d_boff, E.zbuf, E.lbuf, E.ldrop, E.d_swap,
                   E.d_sres};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   E.cnt = nullptr; E.pat = E.pat2 = nullptr; E.d_code = E.d_words = nullptr;
+  E.d_cw = nullptr; E.d_cm = nullptr; E.d_toff = nullptr; E.toff_cap = 0;
   E.d_out = nullptr; E.d_scal = nullptr; E.d_hist = nullptr;
   E.d_eidx64 = nullptr; E.d_eidx32 = nullptr; E.d_eval = nullptr;
   E.d_bcnt = E.d_boff = nullptr;
@@ -1362,6 +1524,10 @@ void release_all() {
   if (E.st) { (void)hipStreamDestroy(E.st); E.st = nullptr; }
   E.uploaded.clear();
   E.active.clear();
+  E.code_raw.clear();
+  E.code_dev_ok = false;
+  E.active_shared = false;
+  E.cp_ok = false;
   E.active_size = 0;
   E.codeM = 0;
   E.live = false;
@@ -1500,6 +1666,7 @@ static int init_cell(int problem, int n, const int *ax, int R,
   if (dalloc(&E.pat, (size_t)balllen * 8)) { release_all(); return -3; }
   CK(hipMemcpy(E.pat, ballpat, (size_t)balllen * 8, hipMemcpyHostToDevice));
   if (dalloc(&E.d_code, (size_t)CODECAP * 8) ||
+      dalloc(&E.d_cw, (size_t)CODECAP * 12) ||
       dalloc(&E.d_words, (size_t)CODECAP * 8) ||
       dalloc(&E.d_out, (size_t)CODECAP * 4) || dalloc(&E.d_scal, 64) ||
       dalloc(&E.d_hist, 4096 * 4)) {
@@ -1597,6 +1764,8 @@ static int init_cell(int problem, int n, const int *ax, int R,
     E.pk_narrow = pn ? atoi(pn) : 0;
     const char *pcm = getenv("CC_COMPACT");
     E.compact = pcm ? atoi(pcm) : 1;
+    const char *psc = getenv("CC_SPARSE_CODE");
+    E.sparse = psc ? atoi(psc) : 1;
     // an explicit schedule setting turns the automatic timing off
     const char *pa = getenv("CC_AUTOTUNE");
     E.autotune = pa ? atoi(pa) : 1;
@@ -1610,6 +1779,10 @@ static int init_cell(int problem, int n, const int *ax, int R,
   }
   E.uploaded.clear();
   E.active.clear();
+  E.code_raw.clear();
+  E.code_dev_ok = false;
+  E.active_shared = false;
+  E.cp_ok = false;
   E.active_size = 0;
   E.codeM = 0;
   E.fmt = 0;
@@ -1688,16 +1861,18 @@ int dct_get_mu(void) { return E.mu; }
 // ---- solution and transforms (reference dct_hc.cu:452-521) ----------------
 int dct_set_code(const long long *idx, int M) {
   if (!E.live || !valid_words(idx, M)) return -1;
-  for (int off = 0; off < M; off += CODECAP) {
-    const int nb = std::min(M - off, CODECAP);
-    memcpy(E.h_words, idx + off, (size_t)nb * 8);
-    CK(hipMemcpyAsync(E.d_code + off, E.h_words, (size_t)nb * 8,
-                      hipMemcpyHostToDevice, E.st));
-    CK(hipStreamSynchronize(E.st));
-  }
   E.codeM = M;
-  E.uploaded.clear();
-  for (int i = 0; i < M; i++) ++E.uploaded[idx[i]];
+  // the represented multiset may still stand for the old pending one: hand
+  // it over before the pending one is rebuilt
+  if (E.active_shared) {
+    std::swap(E.active, E.uploaded);
+    E.active_shared = false;
+  }
+  E.uploaded.assign(idx, M);
+  // the device copy of the list is made only for a transform that reads
+  // the code through the indicator plane (ensure_code_dev)
+  E.code_raw.assign(idx, idx + M);
+  E.code_dev_ok = false;
   return 0;
 }
 
@@ -1710,15 +1885,25 @@ int dct_transform(int init_mode, int out_mode) {
       (out_mode == 2 && init_mode != 0))
     return -1;
   int in_kind = CC_IN_PRED, pmode = CC_DEF;
+  const int out_kind = out_mode == 1 ? CC_OUT_MAP : CC_OUT_CNT;
+  PPlan code_plan;
   if (init_mode == 0) {
     in_kind = CC_IN_CODE;
-    hipLaunchKernelGGL(k_fill32, dim3(pass_blocks(E.X)), dim3(CC_BLOCK), 0,
-                       E.st, E.A[0], 0, E.X);
-    CK(hipGetLastError());
-    if (E.codeM > 0) {
-      hipLaunchKernelGGL(k_scatter_code, dim3((E.codeM + 255) / 256), dim3(256),
-                         0, E.st, E.A[0], E.d_code, E.codeM);
+    code_plan = pk_code_plan(out_kind == CC_OUT_CNT);
+    if (code_plan.ok && code_plan.sw[0].kern == 4) {
+      // the persistent first sweep seeds from the code's word list: no
+      // plane fill, no indicator plane read
+      if (int rc = stage_sparse_code(code_plan.sw[0])) return rc;
+    } else {
+      hipLaunchKernelGGL(k_fill32, dim3(pass_blocks(E.X)), dim3(CC_BLOCK), 0,
+                         E.st, E.A[0], 0, E.X);
       CK(hipGetLastError());
+      if (E.codeM > 0) {
+        if (int rc = ensure_code_dev()) return rc;
+        hipLaunchKernelGGL(k_scatter_code, dim3((E.codeM + 255) / 256), dim3(256),
+                           0, E.st, E.A[0], E.d_code, E.codeM);
+        CK(hipGetLastError());
+      }
     }
   } else if (init_mode == 2) {
     pmode = CC_CRIT;
@@ -1730,9 +1915,6 @@ int dct_transform(int init_mode, int out_mode) {
     E.tuned = 1;
     if (int rc = dct_autotune(2, nullptr); rc < 0) return rc;
   }
-  const int out_kind = out_mode == 1 ? CC_OUT_MAP : CC_OUT_CNT;
-  PPlan code_plan;
-  if (in_kind == CC_IN_CODE) code_plan = pk_code_plan(out_kind == CC_OUT_CNT);
   const PPlan &pp = in_kind == CC_IN_PRED ? E.pfield : code_plan;
   if (pp.ok) {
     const int plo = pmode == CC_CRIT ? 1 : 0;
@@ -1746,7 +1928,7 @@ int dct_transform(int init_mode, int out_mode) {
     for (int i = 0; i < E.nswept; i++)
       CK(hipEventElapsedTime(&E.sweep_ms[i], E.ev[2 * i], E.ev[2 * i + 1]));
   if (out_mode == 2) {
-    E.active = E.uploaded;
+    E.active_shared = true;          // the represented multiset is the pending one
     E.active_size = E.codeM;
     E.cnt_ver++;
     E.o_mode = -1;
@@ -1815,7 +1997,7 @@ int dct_swap_eval(const long long *drops, int k, const long long *cand,
   if (nc > CODECAP || (nc && (!cand || !gain))) return -1;
   if (!valid_words(drops, k) || !valid_words(cand, (int)nc)) return -1;
   for (int i = 0; i < k; i++)
-    if (E.active.find(drops[i]) == E.active.end()) return -1;
+    if (active_map().find(drops[i]) == active_map().end()) return -1;
   const int lmode = E.mu == 1 ? CC_EQ : CC_CRIT, lt = E.mu == 1 ? 1 : 0;
   const int gmode = E.mu == 1 ? CC_EQ : CC_DEF;
   if (set_ok() && E.lbuf) {
@@ -1922,8 +2104,8 @@ int dct_loss_owner(const long long *words, int M, int32_t *out) {
   E.cnt_ver++;
   E.o_mode = -1;
   if (int rc = update_inner<CC_WALK_UPDATE>(words, M, +1)) return rc;
-  E.active.clear();
-  for (int i = 0; i < M; i++) ++E.active[words[i]];
+  E.active_shared = false;
+  E.active.assign(words, M);
   E.active_size = M;
   if (M == 0) return sync_stream();
   return gather_impl(words, M, E.mu == 1 ? CC_EQ : CC_CRIT, E.mu == 1 ? 1 : 0,
@@ -2087,6 +2269,7 @@ int dct_transform_layers(int init_mode) {
                        E.st, E.A[0], 0, E.X);
     CK(hipGetLastError());
     if (E.codeM > 0) {
+      if (int rc = ensure_code_dev()) return rc;
       hipLaunchKernelGGL(k_scatter_code, dim3((E.codeM + 255) / 256), dim3(256),
                          0, E.st, E.A[0], E.d_code, E.codeM);
       CK(hipGetLastError());
@@ -2166,6 +2349,8 @@ int dct_plan_probe(int n, const int *ax, int R, int kind, int s0, int M,
   E.pk_first = pf ? atoi(pf) : 1;
   const char *pn = getenv("CC_PK_NARROW");
   E.pk_narrow = pn ? atoi(pn) : 0;
+  const char *psc = getenv("CC_SPARSE_CODE");
+  E.sparse = psc ? atoi(psc) : 1;
   const PPlan pl = kind == 0 ? pk_plan(1.0, 1e300, true, false)
                              : pk_plan((double)s0, (double)M, false, true);
   int ns = 0;
@@ -2279,6 +2464,7 @@ int dct_autotune(int reps, float *best_us) {
   E.compact = win.cmp;
   E.generic_only = win.gen;
   E.pfield = wplan;
+  E.cp_ok = false;
   build_plan();   // the unpacked plan of the winning budget
   if (best_us) *best_us = best * 1000.0f / reps;
   return timed;
@@ -2389,6 +2575,7 @@ int dct_sweep_force(const int *cuts, int ncuts) {
     m |= 1ull << cuts[i];
   }
   g_cuts = m;
+  E.cp_ok = false;
   if (!E.live) return 0;
   E.autotune = 0;
   E.tuned = 1;

@@ -217,7 +217,11 @@ def test_plan_probe_schedules_are_consistent(q, ns, Rs):
                     j += sw["axes"]
                     assert sw["W"] in (8, 16) and 1 <= sw["ND"] <= 4
                     assert sw["kernel"] >= 1
-                    if i == 0:
+                    if i == 0 and kind == 1 and sw["kernel"] == 5:
+                        # the persistent first sweep seeds a code from
+                        # its sorted word list: no plane is read
+                        assert sw["bytes_in"] == 0
+                    elif i == 0:
                         assert sw["bytes_in"] == (2 if kind == 0 else 4)
                     else:
                         assert sw["bytes_in"] == plan[i - 1]["bytes_out"]
