@@ -336,6 +336,48 @@ def run_config2_cell(GpuBackend, axes, R, steps=10):
     return out
 
 
+def run_config5(GpuBackend, streams=8, slice_rounds=1, max_restarts=4,
+                ruin_frac=0.01, kmax=64):
+    """The largest single-GPU space: K_2(30,4), 2^30 words, the seeded
+    40,000-word random start (default_rng(0).choice); its recount and the
+    full gain / loss field passes, then the restart portfolio at M = 40,000
+    (``streams`` time-shared restarts of plain LNS with a shared incumbent,
+    ``slice_rounds`` rounds per slice, ``max_restarts`` restarts, a restart
+    kicks ``ruin_frac`` M incumbents, LNS ruins 8..``kmax`` words; reference
+    search.py:1536-1620), all on the HBM-resident cell."""
+    from coldcore_amd.search import Searcher
+    size = 1 << 30
+    start = np.random.default_rng(0).choice(size, 40000, replace=False).astype(np.int64)
+    out = {"words": size, "start_words": int(len(start)), "streams": streams,
+           "slice_rounds": slice_rounds, "max_restarts": max_restarts,
+           "ruin_frac": ruin_frac, "kmin": 8, "kmax": kmax}
+    g = GpuBackend([2] * 30, 4)
+    try:
+        s = Searcher(g, seed=1)
+        t = time.perf_counter()
+        s.load(start)
+        out["load_recount_ms"] = round((time.perf_counter() - t) * 1e3, 2)
+        out["start_uncovered"] = int(s.uncov)
+        for name, fn in (("gain", g.gain_map), ("loss", g.loss_map)):
+            fn()
+            t = time.perf_counter()
+            fn()
+            out[f"{name}_pass_ms"] = round((time.perf_counter() - t) * 1e3, 2)
+        t = time.perf_counter()
+        u = s.portfolio(len(start), streams=streams, slice_rounds=slice_rounds,
+                        max_restarts=max_restarts, kmin=8, kmax=kmax,
+                        ruin_frac=ruin_frac)
+        out["portfolio_s"] = round(time.perf_counter() - t, 2)
+        out["lns_rounds"] = int(s.stats["rounds"])
+        out["restarts"] = int(s.stats["restarts"])
+        out["incumbent_uncovered"] = int(u)
+        out["incumbent_words"] = int(len(s.code))
+        out["recount_check"] = bool(s.verify_full() == u)
+    finally:
+        g.close()
+    return out
+
+
 def run_walks(g, code, size, n_upd=100000, n_q=10 ** 6, ball=2325):
     """Incremental updates and exact queries on a recounted K_2(24,3) cell:
     n_upd single-word updates alternating +1 (a random non-code word, seed
@@ -559,7 +601,8 @@ def main():
                          "(3 times): the reference kernel for scaling the "
                          "FETCH_SIZE / WRITE_SIZE counters")
     ap.add_argument("--only", choices=("near_cover", "batch64", "batch64-quick",
-                                       "cells", "learned", "walks", "cpu_calib"),
+                                       "cells", "learned", "walks", "cpu_calib",
+                                       "config5"),
                     help="run one side section and print its JSON only")
     args = ap.parse_args()
     if args.only == "cpu_calib":
@@ -573,6 +616,8 @@ def main():
             r = time_cells(GpuBackend, 10)
         elif args.only == "learned":
             r = run_learned(GpuBackend)
+        elif args.only == "config5":
+            r = run_config5(GpuBackend)
         elif args.only == "walks":
             # a short run of the update / query kernels (for the profiler)
             size = 1 << 24
@@ -700,6 +745,7 @@ def main():
     if not args.no_extra:
         rec["other_cells"] = time_cells(GpuBackend, 10)
         rec["config2_k5_10_3"] = run_config2_cell(GpuBackend, [5] * 10, 3)
+        rec["config5_k2_30_4"] = run_config5(GpuBackend)
         rec["config1"] = run_config1(GpuBackend, not args.no_cpu)
         rec["near_cover"] = run_near_cover(GpuBackend)
         try:

@@ -85,6 +85,46 @@ def _digit_distance(digs, centre):
     return (digs != centre[None, :]).sum(axis=1)
 
 
+class _StaleQueue:
+    """The lazy greedy's priority queue of (-bound, word) pairs: the refresh's
+    candidates as one array sorted by (-bound, word) plus a binary heap of
+    the pairs pushed back with exact gains.  pop() takes the smaller head, so
+    the pop order is that of one heap holding every pair, without building a
+    heap of the whole (up to 2^18-entry) refresh."""
+
+    __slots__ = ("k", "i", "p", "n", "h")
+
+    def __init__(self, keys, ids):
+        self.k, self.i = keys, ids
+        self.p, self.n = 0, len(keys)
+        self.h = []
+
+    @classmethod
+    def empty(cls):
+        return cls(np.zeros(0, dtype=np.int64), np.zeros(0, dtype=np.int64))
+
+    def __bool__(self):
+        return self.p < self.n or bool(self.h)
+
+    def _head(self):
+        return (int(self.k[self.p]), int(self.i[self.p]))
+
+    def pop(self):
+        if self.p < self.n and (not self.h or self._head() < self.h[0]):
+            t = self._head()
+            self.p += 1
+            return t
+        return heapq.heappop(self.h)
+
+    def push(self, t):
+        heapq.heappush(self.h, t)
+
+    def top_key(self):
+        if self.p < self.n and (not self.h or self._head() < self.h[0]):
+            return int(self.k[self.p])
+        return self.h[0][0]
+
+
 class Searcher:
     """Ruin-and-recreate search over one backend (search.py:170-257)."""
 
@@ -105,7 +145,7 @@ class Searcher:
         self.best_code = None
         self.best_uncov = None
         self.stats = {"transforms": 0, "placements": 0, "removals": 0,
-                      "rounds": 0, "recounts": 0}
+                      "rounds": 0, "recounts": 0, "restarts": 0}
         self._scan_off = 0
         self._deficit_cache = (None, None)
         self.loop_reserve_s = 10.0
@@ -160,12 +200,12 @@ class Searcher:
         while found > cap and thr < mx:
             thr = min(mx, thr + max(1, (mx - thr) // 2))
             idx, val, found = b.map_extract(thr, cap=cap)
-        order = np.argsort(-val, kind="stable")
-        keys = (-np.asarray(val, dtype=np.int64)[order]).tolist()
-        ids = np.asarray(idx, dtype=np.int64)[order].tolist()
-        heap = [t for t in zip(keys, ids) if t[1] not in codeset]
-        heapq.heapify(heap)
-        return heap, thr
+        idx = np.asarray(idx, dtype=np.int64)
+        keys = -np.asarray(val, dtype=np.int64)
+        keep = ~np.isin(idx, np.fromiter(codeset, dtype=np.int64, count=len(codeset)))
+        idx, keys = idx[keep], keys[keep]
+        order = np.lexsort((idx, keys))
+        return _StaleQueue(keys[order], idx[order]), thr
 
     def greedy_fill(self, target_m, batch=64, refresh_every=512):
         """Exact lazy greedy until covered / target_m / deadline: a
@@ -174,7 +214,7 @@ class Searcher:
         b = self.b
         codeset = set(self.code.tolist())
         since = refresh_every
-        heap, thr = [], 1
+        heap, thr = _StaleQueue.empty(), 1
         cap = min(b.extract_cap, 1 << 18)
         while self.uncov > 0 and len(codeset) < target_m:
             if self.time_left() < self.build_reserve_s:
@@ -191,7 +231,7 @@ class Searcher:
                     break
             cand = []
             while heap and len(cand) < batch:
-                v, i = heapq.heappop(heap)
+                v, i = heap.pop()
                 if i not in codeset:
                     cand.append((-v, i))
             if not cand:
@@ -203,7 +243,7 @@ class Searcher:
             bi = int(np.argmax(exact))
             bg = int(exact[bi])
             ex = exact.tolist()
-            bound = -heap[0][0] if heap else 0
+            bound = -heap.top_key() if heap else 0
             if bg < 1:
                 since = refresh_every
                 continue
@@ -216,11 +256,11 @@ class Searcher:
                 since += 1
                 for k, (v, i) in enumerate(cand):
                     if i != w and ex[k] > 0:
-                        heapq.heappush(heap, (-ex[k], i))
+                        heap.push((-ex[k], i))
             else:
                 for k, (v, i) in enumerate(cand):
                     if ex[k] > 0:
-                        heapq.heappush(heap, (-ex[k], i))
+                        heap.push((-ex[k], i))
                 if bg < thr:
                     since = refresh_every
         self.code = np.array(sorted(codeset), dtype=np.int64)
@@ -459,6 +499,79 @@ class Searcher:
             self.code = self.best_code.copy()
             self.b.recount(self.code)
             self.uncov = self.b.uncovered()
+        return self.uncov
+
+    def portfolio(self, M, budget_s=np.inf, streams=4, slice_s=None, kmin=8,
+                  kmax=None, ruin_frac=0.15, select="fixed", slice_rounds=None,
+                  max_restarts=None):
+        """Time-sliced restart portfolio at fixed M with a shared incumbent
+        (reference search.py:1536-1620): ``streams`` solutions -- the current
+        one and perturbations of it (a kick of ruin_frac * M incumbents by
+        each ruin operator in turn, then the exact greedy refill) -- each
+        given an LNS slice per round, round-robin, on the one backend (a
+        stream is resumed by a recount of its code).  The best state seen is
+        the incumbent; after every round the worst stream is replaced by a
+        fresh perturbation of it.  Stops when a stream is feasible, at the
+        budget, or -- clock-free runs -- after ``max_restarts`` restarts,
+        with slices of ``slice_rounds`` LNS rounds instead of ``slice_s``
+        seconds.  Leaves the incumbent loaded; returns its uncovered
+        count."""
+        t0 = time.time()
+        streams = max(1, int(streams))
+        slice_s = slice_s or max(1.0, budget_s / (3.0 * streams))
+        if len(self.code) > M:
+            self.ruin(len(self.code) - M, "low")
+        if len(self.code) < M and self.uncov > 0:
+            self.greedy_fill(M)
+        kick = max(1, int(ruin_frac * M))
+        base = self.code.copy()
+
+        def fresh(code, mode):
+            self.load(code)
+            self.ruin(kick, mode)
+            self.greedy_fill(M)
+            return self.code.copy(), self.uncov
+
+        pop = [(base, self.uncov)]
+        if self.uncov > 0:
+            pop += [fresh(base, RUIN_MODES[i % len(RUIN_MODES)])
+                    for i in range(1, streams)]
+        inc, inc_u = min(pop, key=lambda cu: cu[1])
+
+        def out_of_time():
+            return (time.time() - t0 > budget_s or
+                    self.time_left() < self.loop_reserve_s)
+
+        restarts = 0
+        while inc_u > 0 and not out_of_time():
+            for i in range(len(pop)):
+                left = budget_s - (time.time() - t0)
+                if left <= 0 or self.time_left() < self.loop_reserve_s:
+                    break
+                self.load(pop[i][0])
+                u = self.lns(M, round_budget_s=min(slice_s, left), kmin=kmin,
+                             kmax=kmax, select=select, max_rounds=slice_rounds)
+                pop[i] = (self.code.copy(), u)
+                if u < inc_u:
+                    inc, inc_u = pop[i]
+                if u == 0:
+                    break
+            if inc_u == 0 or time.time() - t0 > budget_s:
+                break
+            if max_restarts is not None and restarts >= max_restarts:
+                break
+            worst = int(np.argmax([u for _, u in pop]))
+            self.stats["restarts"] += 1
+            restarts += 1
+            mode = RUIN_MODES[self.stats["restarts"] % len(RUIN_MODES)]
+            pop[worst] = fresh(inc, mode)
+            if pop[worst][1] < inc_u:
+                inc, inc_u = pop[worst]
+            self.on_log(f"portfolio M={M} streams={len(pop)} best={inc_u} "
+                        f"restart=s{worst}/{mode}")
+        self.best_uncov = None
+        self.best_code = None
+        self.load(inc)
         return self.uncov
 
 

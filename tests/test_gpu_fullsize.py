@@ -140,3 +140,23 @@ def test_k2_30_4_full_size_invariants(gpu_cell, monkeypatch, tuning):
     assert (g.ball_gather(probe, 1) == loss).all()
     for x, v in zip(few, loss[::256]):
         assert int((cnt[x ^ masks] == 1).sum()) == v
+
+
+def test_k2_30_4_restart_portfolio(gpu_cell):
+    """Config 5 as a workload: the 40,000-word start on K_2(30,4), then a
+    short restart portfolio (8 streams time-sharing the one HBM-resident
+    cell): the incumbent never has more words than M, its uncovered count
+    is a from-scratch recount's and no worse than the start's."""
+    from coldcore_amd.search import Searcher
+    start = np.random.default_rng(0).choice(1 << 30, 40000, replace=False).astype(np.int64)
+    g = gpu_cell([2] * 30, 4)
+    s = Searcher(g, seed=1)
+    s.load(start)
+    u0 = s.uncov
+    assert u0 == g.count_eq(0) > 0
+    u = s.portfolio(40000, streams=8, slice_rounds=1, max_restarts=2, kmin=8,
+                    kmax=64, ruin_frac=0.01)
+    print(f"K_2(30,4) portfolio: uncovered {u0} -> {u}, rounds "
+          f"{s.stats['rounds']}, restarts {s.stats['restarts']}", flush=True)
+    assert len(s.code) <= 40000 and u <= u0
+    assert s.verify_full() == u

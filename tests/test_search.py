@@ -177,3 +177,36 @@ def test_config1_lns_equals_reference_trajectory_on_oracle():
 def test_config1_lns_equals_reference_trajectory_on_gpu(gpu_cell):
     """The same 1,000 rounds through the engine's C ABI."""
     _check_against_reference_run(gpu_cell([3] * 8, 2))
+
+
+# ---- restart portfolio (reference search.py:1536-1620) --------------------
+def _portfolio_run(backend, seed=3):
+    s = Searcher(backend, seed=seed)
+    s.load(np.zeros(0, dtype=np.int64))
+    s.greedy_fill(10 ** 9)
+    s.peel()
+    M = len(s.code) - 3                      # a hard notch: several rounds
+    u = s.portfolio(M, streams=4, slice_rounds=4, max_restarts=3, kmin=2)
+    return s, M, u
+
+
+def test_portfolio_invariants_on_oracle():
+    """The incumbent is loaded at the end, its uncovered count is a
+    from-scratch recount's, and no stream exceeds M words."""
+    c = NumpyDP([3] * 6, 1)
+    s, M, u = _portfolio_run(c)
+    assert len(s.code) <= M
+    assert u == s.uncov == s.verify_full()
+    assert s.stats["restarts"] <= 3
+    assert (np.diff(s.code) > 0).all()
+
+
+@pytest.mark.gpu
+def test_portfolio_gpu_equals_oracle(gpu_cell):
+    """Clock-free, the portfolio is deterministic: the engine's run equals the
+    oracle's restart for restart."""
+    a, _, ua = _portfolio_run(NumpyDP([3] * 6, 1))
+    b, _, ub = _portfolio_run(gpu_cell([3] * 6, 1))
+    assert ua == ub
+    assert a.code.tolist() == b.code.tolist()
+    assert a.stats == b.stats
