@@ -489,15 +489,40 @@ def run_near_cover(GpuBackend):
     return out
 
 
-def run_learned(GpuBackend, seconds=15.0, seeds=(0, 1)):
+def run_learned(GpuBackend, seconds=5.0, seeds=(0, 1, 2, 3, 4), rounds=4000,
+                workers=8, epochs=10, per_cell=None, log=None):
     """Time-to-size of the learned ruin proposal against plain adaptive LNS
-    on K_3(10,2): move logs of plain LNS on K_3(10,2) and K_4(8,2) (3,000
-    rounds each), the network trained on the GPU, then per seed both arms
-    from the same greedy + peel cover for ``seconds`` of notch descent."""
-    from coldcore_amd.learned import learned_ab
-    cells = [lambda: GpuBackend([3] * 10, 2), lambda: GpuBackend([4] * 8, 2)]
-    return learned_ab(lambda: GpuBackend([3] * 10, 2), cells, list(seeds),
-                      seconds, train_rounds=3000)
+    on K_3(10,2): move logs of plain LNS on K_3(10,2) and K_4(8,2) (every
+    round logged: region features from the exact layer fields, outcome),
+    ``workers`` logging processes with their own cells, ``rounds`` rounds
+    per process; the network trained on the GPU by its own kernels; then
+    per seed both arms from the same greedy + peel cover for ``seconds`` of
+    notch descent (the exact fields judge every move in both)."""
+    from coldcore_amd.learned import collect_records_parallel, learned_ab
+    t = time.perf_counter()
+    per_cell = per_cell or max(1, workers // 2)
+    X, y = collect_records_parallel([([3] * 10, 2), ([4] * 8, 2)], per_cell,
+                                    rounds, workers=workers, log=log)
+    collect_s = time.perf_counter() - t
+    out = learned_ab(lambda: GpuBackend([3] * 10, 2), [], list(seeds), seconds,
+                     epochs=epochs, records=(X, y), log=log)
+    out["collect_s"] = round(collect_s, 2)
+    out["logging_processes"] = workers
+    out["log_jobs"] = 2 * per_cell
+    out["rounds_per_job"] = rounds
+    for r in out["runs"]:                 # keep the JSON line short
+        for arm in ("plain", "learned"):
+            r[arm].pop("hist", None)
+    out.pop("median_ratio")
+    # median over the seeds, a seed whose learned arm never reached the
+    # plain arm's final size counting as infinitely late
+    ratios = sorted(np.inf if r["ratio"] is None else r["ratio"] for r in out["runs"])
+    med = float(np.median(ratios))
+    out["time_to_size_ratio"] = None if not np.isfinite(med) else round(med, 3)
+    out["per_seed_ratio"] = [r["ratio"] for r in out["runs"]]
+    out["learned_final_smaller_or_equal"] = int(sum(
+        r["learned"]["final_size"] <= r["plain"]["final_size"] for r in out["runs"]))
+    return out
 
 
 def copy_calibration(mib):
@@ -601,8 +626,8 @@ def main():
                          "(3 times): the reference kernel for scaling the "
                          "FETCH_SIZE / WRITE_SIZE counters")
     ap.add_argument("--only", choices=("near_cover", "batch64", "batch64-quick",
-                                       "cells", "learned", "walks", "cpu_calib",
-                                       "config5"),
+                                       "cells", "learned", "learned-1m", "walks",
+                                       "cpu_calib", "config5"),
                     help="run one side section and print its JSON only")
     args = ap.parse_args()
     if args.only == "cpu_calib":
@@ -616,6 +641,12 @@ def main():
             r = time_cells(GpuBackend, 10)
         elif args.only == "learned":
             r = run_learned(GpuBackend)
+        elif args.only == "learned-1m":
+            # config 4 at its configured scale: 10^6 logged rounds (80 jobs
+            # of 12,500 rounds over 8 processes), 30 s per arm
+            r = run_learned(GpuBackend, seconds=30.0, rounds=12500, epochs=8,
+                            per_cell=40,
+                            log=lambda m: print(m, file=sys.stderr, flush=True))
         elif args.only == "config5":
             r = run_config5(GpuBackend)
         elif args.only == "walks":
@@ -752,6 +783,8 @@ def main():
             rec["batch64"] = run_batch64(GpuBackend)
         except RuntimeError as e:          # reported, never hidden
             rec["batch64"] = {"error": str(e)}
+        # config 4: the learned proposal's time-to-size against plain LNS
+        rec["learned"] = run_learned(GpuBackend)
     if not args.no_cpu:
         rec["cpu_baseline"] = run_cpu_baseline(code, args.cpu_passes)
         rec["speedup_vs_cpu"] = round(passes_s / rec["cpu_baseline"]["value"], 1)

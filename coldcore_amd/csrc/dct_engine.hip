@@ -2282,6 +2282,193 @@ int dct_transform_layers(int init_mode) {
   return 0;
 }
 
+// ---- additive: ruin-region features of the learned proposal ---------------
+// Five windows of R+1 exact layer fractions per region of k words (layout
+// [region][window][d]): 0 mean over the region of the critical layers, 1
+// mean and 2 max of the deficient layers, 3 mean of the codeword layers, 4
+// min of the critical layers; a layer count at distance d is divided by the
+// distance-d shell size (double precision, rounded to float at the end).
+}  // extern "C"
+
+struct FeatShell { double s[CC_MAXL]; };
+
+template <typename LT>
+__global__ void k_region_feat(const SweepArgs a, int L, int Lv,
+                              const long long *__restrict__ words, int nreg, int k,
+                              FeatShell sh, int kind, float *out) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nreg) return;
+  float *o = out + (long long)r * 5 * L;
+  for (int d = 0; d < L; d++) {
+    double sum = 0, mx = 0, mn = 0;
+    for (int j = 0; j < k; j++) {
+      const long long w = words[(long long)r * k + j];
+      double v = 0;
+      if (d < Lv) {
+        if constexpr (sizeof(LT) == 2) v = (double)a.H[d][w];
+        else v = (double)a.A[d][w];
+      }
+      v = v / sh.s[d];
+      sum += v;
+      mx = j == 0 ? v : (v > mx ? v : mx);
+      mn = j == 0 ? v : (v < mn ? v : mn);
+    }
+    const double mean = sum / k;
+    if (kind == 1) { o[1 * L + d] = (float)mean; o[2 * L + d] = (float)mx; }
+    else if (kind == 2) { o[0 * L + d] = (float)mean; o[4 * L + d] = (float)mn; }
+    else o[3 * L + d] = (float)mean;
+  }
+}
+
+// Ruin regions on the device: block b takes anchor code position anc[b]
+// and picks the k code words nearest to it in digit distance, ties broken
+// by a 24-bit hash of (seed, anchor position, word position) and then by
+// the position; the key (distance << 40 | hash << 16 | position) is unique,
+// so round r takes the smallest key above round r-1's.  Writes the
+// positions (ascending key) to pos[b * k ..] and the words to words[..].
+__device__ __forceinline__ uint64_t cc_mix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+__global__ void __launch_bounds__(256)
+    k_region_pick(Geom g, const long long *__restrict__ code, int M,
+                  const int32_t *__restrict__ anc, int k, uint64_t seed,
+                  int32_t *pos, long long *words) {
+  __shared__ unsigned long long red[4];
+  __shared__ int adig[CC_MAXN];
+  const int b = blockIdx.x;
+  const int a = anc[b];
+  const long long wa = code[a];
+  for (int i = threadIdx.x; i < g.n; i += blockDim.x)
+    adig[i] = (int)((wa / g.pw[i]) % g.ax[i]);
+  __syncthreads();
+  unsigned long long last = 0;
+  for (int r = 0; r < k; r++) {
+    unsigned long long best = ~0ull;
+    for (int j = threadIdx.x; j < M; j += blockDim.x) {
+      const long long w = code[j];
+      int d = 0;
+      for (int i = 0; i < g.n; i++) d += (int)((w / g.pw[i]) % g.ax[i]) != adig[i];
+      const uint64_t h = cc_mix64(seed ^ ((uint64_t)a << 32) ^ (uint64_t)j) & 0xFFFFFFull;
+      const unsigned long long key = ((unsigned long long)d << 40) | (h << 16) | (unsigned)j;
+      if ((r == 0 || key > last) && key < best) best = key;
+    }
+    // block minimum: wave minima through shuffles, then the four waves
+    for (int o = 32; o > 0; o >>= 1) {
+      const unsigned long long x = __shfl_xor(best, o, 64);
+      best = x < best ? x : best;
+    }
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = best;
+    __syncthreads();
+    unsigned long long m = red[0];
+    for (int i = 1; i < (int)(blockDim.x >> 6); i++) m = red[i] < m ? red[i] : m;
+    __syncthreads();
+    last = m;
+    if (threadIdx.x == 0) {
+      const int j = (int)(m & 0xFFFFull);
+      pos[b * k + r] = j;
+      words[b * k + r] = code[j];
+    }
+  }
+}
+
+// the five windows of nreg regions whose words are in d_words (region r:
+// d_words[r * k ..]) from the three layer transforms of the pending code,
+// into the feature block at d_out (no synchronisation)
+extern "C" int dct_transform_layers(int init_mode);
+int region_windows(int nreg, int k) {
+  FeatShell sh;
+  // distance-d shell sizes: the elementary symmetric polynomials of the
+  // (axis size - 1), at least 1
+  std::vector<double> poly(E.L, 0.0);
+  poly[0] = 1;
+  for (int i = 0; i < E.n; i++)
+    for (int d = E.L - 1; d >= 1; d--) poly[d] += (E.ax[i] - 1) * poly[d - 1];
+  for (int d = 0; d < CC_MAXL; d++) sh.s[d] = d < E.L ? std::max(poly[d], 1.0) : 1.0;
+  float *dfeat = (float *)E.d_out;
+  const int Lv = layers_after(E.n);
+  for (int kind : {1, 2, 0}) {
+    if (int rc = dct_transform_layers(kind)) return rc;
+    SweepArgs a;
+    memset(&a, 0, sizeof(a));
+    for (int d = 0; d < CC_MAXL; d++) { a.A[d] = E.A[d]; a.H[d] = E.H[d]; }
+    if (E.layers_live == 2)
+      hipLaunchKernelGGL(k_region_feat<uint16_t>, dim3((nreg + 63) / 64), dim3(64), 0,
+                         E.st, a, E.L, Lv, E.d_words, nreg, k, sh, kind, dfeat);
+    else
+      hipLaunchKernelGGL(k_region_feat<int32_t>, dim3((nreg + 63) / 64), dim3(64), 0,
+                         E.st, a, E.L, Lv, E.d_words, nreg, k, sh, kind, dfeat);
+    CK(hipGetLastError());
+  }
+  return 0;
+}
+
+extern "C" {
+
+// dct_region_propose(code, M, anchors, na, k, seed, regions, feats): for
+// each of the na anchor positions the region of the k nearest code words
+// (k_region_pick, positions into code, written to regions[b * k ..]) and
+// its five feature windows (as dct_region_features, written to
+// feats[b * 5 (R+1) ..]): one upload, one download, no host work per
+// candidate.  Hamming only; M <= 65,535, na <= 64, k <= min(M, 64).
+int dct_region_propose(const long long *code, int M, const int32_t *anchors,
+                       int na, int k, unsigned long long seed, int32_t *regions,
+                       float *feats) {
+  if (!E.live || E.prob != CC_HAMMING || E.L < 2 || M < 1 || na < 1 || na > 64 ||
+      k < 1 || k > 64 || k > M || !anchors || !regions || !feats)
+    return -1;
+  for (int i = 0; i < na; i++)
+    if (anchors[i] < 0 || anchors[i] >= M) return -1;
+  if (int rc = dct_set_code(code, M)) return rc;
+  if (int rc = ensure_code_dev()) return rc;
+  // anchors through the word staging (int32 in the int64 buffer)
+  int32_t *h_anc = (int32_t *)E.h_words;
+  memcpy(h_anc, anchors, (size_t)na * 4);
+  int32_t *d_anc = (int32_t *)E.d_words + 2 * CODECAP - 64;  // the buffer's tail
+  CK(hipMemcpyAsync(d_anc, h_anc, (size_t)na * 4, hipMemcpyHostToDevice, E.st));
+  int32_t *d_pos = E.d_out + 4096;          // after the feature block
+  hipLaunchKernelGGL(k_region_pick, dim3(na), dim3(256), 0, E.st, make_geom(),
+                     E.d_code, M, d_anc, k, (uint64_t)seed, d_pos, E.d_words);
+  CK(hipGetLastError());
+  if (int rc = region_windows(na, k)) return rc;
+  // one download: the features, then the positions
+  float *dfeat = (float *)E.d_out;
+  const size_t fb = (size_t)na * 5 * E.L * 4;
+  CK(hipMemcpyAsync(E.h_out, dfeat, fb, hipMemcpyDeviceToHost, E.st));
+  CK(hipMemcpyAsync(E.h_out + 4096, d_pos, (size_t)na * k * 4, hipMemcpyDeviceToHost,
+                    E.st));
+  CK(hipStreamSynchronize(E.st));
+  memcpy(feats, E.h_out, fb);
+  memcpy(regions, E.h_out + 4096, (size_t)na * k * 4);
+  return 0;
+}
+
+// dct_region_features(code, M, words, nreg, k, out): sets the pending code
+// to code[0..M) and, from three layer transforms (deficient, critical and
+// codeword seeds) on the device, writes the five windows of each of the
+// nreg regions words[r*k .. r*k+k) to out[r * 5 (R+1) ..]: one call, one
+// download, no layer planes to the host.  Hamming only, nreg * k <= 65,536;
+// cnt and the represented code are unchanged, the map plane is overwritten.
+int dct_region_features(const long long *code, int M, const long long *words,
+                        int nreg, int k, float *out) {
+  if (!E.live || E.prob != CC_HAMMING || E.L < 2 || nreg < 1 || k < 1 ||
+      (long long)nreg * k > CODECAP || (long long)nreg * 5 * E.L > CODECAP || !out)
+    return -1;
+  if (!valid_words(words, nreg * k)) return -1;
+  if (int rc = dct_set_code(code, M)) return rc;
+  if (int rc = stage_words(words, nreg * k)) return rc;
+  if (int rc = region_windows(nreg, k)) return rc;
+  float *dfeat = (float *)E.d_out;
+  const size_t nb = (size_t)nreg * 5 * E.L * 4;
+  CK(hipMemcpyAsync(E.h_out, dfeat, nb, hipMemcpyDeviceToHost, E.st));
+  CK(hipStreamSynchronize(E.st));
+  memcpy(out, E.h_out, nb);
+  return 0;
+}
+
 // out[i * (R+1) + d] = layer d at idx[i] after dct_transform_layers
 int dct_layers_read_at(const long long *idx, int k, int32_t *out) {
   if (!E.live || !E.layers_live) return -1;

@@ -35,7 +35,8 @@ OPTIONAL = ("dct_map_hist", "dct_set_mu", "dct_get_mu", "dct_ball_gather_mu",
 ADDITIVE = ("dct_init3w", "dct_init3x", "dct_sweep_plan", "dct_sweep_plan2",
             "dct_plan_probe", "dct_autotune", "dct_sweep_profile",
             "dct_sweep_times", "dct_sweep_force", "dct_map_read",
-            "dct_transform_layers", "dct_layers_read_at")
+            "dct_transform_layers", "dct_layers_read_at", "dct_region_features",
+            "dct_region_propose")
 
 
 def _rc_msg(rc):
@@ -72,13 +73,16 @@ def _bind(path):
               "dct_sweep_plan2", "dct_plan_probe", "dct_autotune",
               "dct_sweep_profile", "dct_sweep_force", "dct_map_read",
               "dct_sweep_times", "dct_transform_layers",
-              "dct_layers_read_at"):
+              "dct_layers_read_at", "dct_region_features", "dct_region_propose"):
         getattr(lib, f).restype = ctypes.c_int
     for f in ("dct_count_eq", "dct_count_deficit", "dct_map_extract",
               "dct_map_extract64", "dct_host_bytes"):
         getattr(lib, f).restype = _LL
     lib.dct_read_cnt.argtypes = [_LL, _LL, ctypes.POINTER(ctypes.c_uint16)]
     lib.dct_map_read.argtypes = [_LL, _LL, _I32P]
+    lib.dct_region_propose.argtypes = [_LLP, ctypes.c_int, _I32P, ctypes.c_int,
+                                       ctypes.c_int, ctypes.c_uint64, _I32P,
+                                       ctypes.POINTER(ctypes.c_float)]
     lib.dct_init.restype = ctypes.c_int
     lib.dct_init.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _U64P,
                              _LL, ctypes.c_int, _LLP]
@@ -396,6 +400,33 @@ class GpuBackend:
         _check("dct_layers_read_at", self._lib.dct_layers_read_at(
             _ptr(idx, _LLP), len(idx), _ptr(out, _I32P)))
         return out
+
+    def region_features(self, code, regions):
+        """(regions, 5 (R+1)) float32 features of ruin regions (rows of word
+        indices), computed on the device from the three exact layer fields
+        of ``code`` (dct_region_features)."""
+        code = np.ascontiguousarray(np.atleast_1d(code), dtype=np.int64)
+        W = np.ascontiguousarray(regions, dtype=np.int64)
+        nreg, k = W.shape
+        out = np.empty((nreg, 5 * (self.R + 1)), dtype=np.float32)
+        _check("dct_region_features", self._lib.dct_region_features(
+            _ptr(code, _LLP), len(code), _ptr(W, _LLP), nreg, k,
+            out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return out
+
+    def region_propose(self, code, anchors, k, seed):
+        """((na, k) region positions into ``code``, (na, 5 (R+1)) features):
+        the k code words nearest each anchor position and their feature
+        windows, picked and computed on the device (dct_region_propose)."""
+        code = np.ascontiguousarray(np.atleast_1d(code), dtype=np.int64)
+        anc = np.ascontiguousarray(np.atleast_1d(anchors), dtype=np.int32)
+        regions = np.empty((len(anc), k), dtype=np.int32)
+        feats = np.empty((len(anc), 5 * (self.R + 1)), dtype=np.float32)
+        _check("dct_region_propose", self._lib.dct_region_propose(
+            _ptr(code, _LLP), len(code), _ptr(anc, _I32P), len(anc), int(k),
+            int(seed) & (2 ** 64 - 1), _ptr(regions, _I32P),
+            feats.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return regions, feats
 
     def sweep_plan(self):
         buf = (ctypes.c_int * (8 * 17))()

@@ -155,6 +155,16 @@ class RegionScorer:
         C = b.layers_at(0, code, code=code) / self.shell
         return G, L, C
 
+    def region_features(self, code, regions):
+        """(regions, 5 (R+1)) features of equal-size regions (positions into
+        ``code``): on the engine one device call (dct_region_features: the
+        three layer transforms and the windows, one download); on a host
+        backend the layers at the code and the windows in numpy."""
+        W = np.asarray(regions, dtype=np.int64)
+        if hasattr(self.b, "region_features"):
+            return self.b.region_features(code, np.asarray(code, dtype=np.int64)[W])
+        return self.features(self.layers(code), W)
+
     def features(self, lay, regions):
         """(regions, 5 (R+1)) features of equal-size regions (positions
         into the code), all regions at once."""
@@ -165,17 +175,35 @@ class RegionScorer:
                                Lw.min(1)], axis=1).astype(np.float32)
 
 
-def cluster_regions(searcher, k, anchors):
-    """Anchor + its k-1 nearest incumbents (digit distance, random ties),
-    for all anchors at once: a (anchors, k) array of code positions."""
-    s = searcher
-    m = len(s.code)
-    digs = s.b.index_word(s.code).astype(np.int8)
+_M64 = (1 << 64) - 1
+
+
+def _mix64(x):
+    """splitmix64's finaliser on uint64 arrays (the tie-break hash the device
+    region picker uses)."""
+    x = (x + np.uint64(0x9E3779B97F4A7C15)) & np.uint64(_M64)
+    x = ((x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)) & np.uint64(_M64)
+    x = ((x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)) & np.uint64(_M64)
+    return x ^ (x >> np.uint64(31))
+
+
+def cluster_regions(space, code, k, anchors, seed):
+    """For every anchor position, the k code words nearest to it in digit
+    distance, ties by a 24-bit hash of (seed, anchor, position) and then by
+    position -- the device picker's order (dct_region_propose), restated on
+    the host for the CPU backends.  A (anchors, k) array of positions."""
+    code = np.asarray(code, dtype=np.int64)
     anchors = np.asarray(anchors, dtype=np.int64)
-    d = (digs[None, :, :] != digs[anchors][:, None, :]).sum(axis=2)
-    d = d + s.rng.random((len(anchors), m))
-    k = min(k, m)
-    return np.argpartition(d, k - 1, axis=1)[:, :k]
+    radix = np.asarray(space.axes, dtype=np.int64)
+    place = np.concatenate(([1], np.cumprod(radix[:-1]))).astype(np.int64)
+    digs = ((code[:, None] // place[None, :]) % radix[None, :]).astype(np.int16)
+    d = (digs[None, :, :] != digs[anchors][:, None, :]).sum(axis=2).astype(np.uint64)
+    j = np.arange(len(code), dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        h = _mix64(np.uint64(seed & _M64) ^ (anchors.astype(np.uint64)[:, None] << np.uint64(32))
+                   ^ j[None, :]) & np.uint64(0xFFFFFF)
+    key = (d << np.uint64(40)) | (h << np.uint64(16)) | j[None, :]
+    return np.argsort(key, axis=1, kind="stable")[:, :k].astype(np.int64)
 
 
 class LoggingOperator:
@@ -190,8 +218,7 @@ class LoggingOperator:
 
     def __call__(self, searcher, k, mode):
         pick = searcher.ruin_pick(k, mode)
-        lay = self.scorer.layers(searcher.code)
-        self._pending = self.scorer.features(lay, [pick])[0]
+        self._pending = self.scorer.region_features(searcher.code, [pick])[0]
         return pick
 
     def outcome(self, searcher, rec):
@@ -215,10 +242,15 @@ class LearnedProposer:
     def __call__(self, searcher, k, mode=None):
         s = searcher
         m = len(s.code)
+        k = min(k, m, 64)
         anchors = s.rng.choice(m, size=min(self.ncand, m), replace=False)
-        regions = cluster_regions(s, k, anchors)
-        lay = self.scorer.layers(s.code)
-        F = self.scorer.features(lay, regions)
+        seed = int(s.rng.integers(0, 2 ** 63))
+        if hasattr(s.b, "region_propose"):
+            # regions picked and featurised on the device, one call
+            regions, F = s.b.region_propose(s.code, anchors, k, seed)
+        else:
+            regions = cluster_regions(s.b.space, s.code, k, anchors, seed)
+            F = self.scorer.region_features(s.code, regions)
         if self.norm is not None:
             F = (F - self.norm[0]) / self.norm[1]
         y = self.net.forward(F)
@@ -255,6 +287,43 @@ def collect_records(make_backend, seeds, rounds, kmin=2, kmax=6, log=None):
         finally:
             b.close()
     return np.array(X, dtype=np.float32), np.array(t, dtype=np.float32)
+
+
+def _collect_worker(job):
+    """One logging process: its own engine cell (the ABI holds one cell per
+    process), plain-LNS notch descent on (axes, R) for every seed."""
+    axes, R, seeds, rounds = job
+    from .gpu import GpuBackend
+    return collect_records(lambda: GpuBackend(axes, R), seeds=seeds, rounds=rounds)
+
+
+def _collect_job(item):
+    i, job = item
+    return i, _collect_worker(job)
+
+
+def collect_records_parallel(cells, seeds_per_cell, rounds, workers=8, log=None):
+    """Move logs from ``workers`` processes at once (each with its own cell
+    on the one GPU): cell i of ``cells`` ((axes, R) pairs) runs seeds
+    100 i + j, j < seeds_per_cell, ``rounds`` logged LNS rounds each.
+    Returns (X, t) concatenated in job order (deterministic for a given
+    split)."""
+    import multiprocessing as mp
+    jobs = [(list(axes), int(R), [100 * i + j], int(rounds))
+            for i, (axes, R) in enumerate(cells) for j in range(seeds_per_cell)]
+    ctx = mp.get_context("spawn")
+    parts = [None] * len(jobs)
+    t0 = time.perf_counter()
+    with ctx.Pool(max(1, min(workers, len(jobs)))) as pool:
+        for done, (i, part) in enumerate(pool.imap_unordered(
+                _collect_job, list(enumerate(jobs)), chunksize=1)):
+            parts[i] = part
+            if log:
+                log(f"log job {done + 1}/{len(jobs)} done, "
+                    f"{time.perf_counter() - t0:.0f} s")
+    X = np.concatenate([x for x, _ in parts]) if parts else np.zeros((0, IN - 1), np.float32)
+    t = np.concatenate([y for _, y in parts]) if parts else np.zeros(0, np.float32)
+    return X, t
 
 
 def train(net, X, t, epochs=20, batch=256, lr=2e-3, seed=0):
@@ -303,6 +372,18 @@ def start_code(make_backend, seed):
         b.close()
 
 
+def rank_auc(score, label):
+    """Probability that a random positive scores above a random negative
+    (ties count half), from ranks: O(n log n) for held-out sets of any size."""
+    from scipy.stats import rankdata
+    pos = label == 1
+    npos, nneg = int(pos.sum()), int((~pos).sum())
+    if not npos or not nneg:
+        return None
+    r = rankdata(score)
+    return float((r[pos].sum() - npos * (npos + 1) / 2) / (npos * nneg))
+
+
 def time_to_size(hist, size):
     """First time (s) a run's feasible size reached ``size``; None if never."""
     for t, m in hist:
@@ -312,7 +393,7 @@ def time_to_size(hist, size):
 
 
 def learned_ab(make_backend, train_cells, seeds, seconds, train_rounds=2000,
-               epochs=40, ncand=32, tau=0.25, log=None):
+               epochs=40, ncand=32, tau=0.25, log=None, records=None):
     """Time-to-size of the learned ruin proposal against plain adaptive LNS.
 
     1. Move logs: plain LNS (fixed operator mix) on every cell of
@@ -328,13 +409,16 @@ def learned_ab(make_backend, train_cells, seeds, seconds, train_rounds=2000,
     T_learned / T_plain (< 1: the learned arm got there sooner; None when it
     never did)."""
     t0 = time.perf_counter()
-    Xs, ts = [], []
-    for i, mk in enumerate(train_cells):
-        X, t = collect_records(mk, seeds=[100 + i], rounds=train_rounds)
-        Xs.append(X)
-        ts.append(t)
-    X = np.concatenate(Xs)
-    t = np.concatenate(ts)
+    if records is not None:
+        X, t = records                       # a log collected beforehand
+    else:
+        Xs, ts = [], []
+        for i, mk in enumerate(train_cells):
+            X, t = collect_records(mk, seeds=[100 + i], rounds=train_rounds)
+            Xs.append(X)
+            ts.append(t)
+        X = np.concatenate(Xs)
+        t = np.concatenate(ts)
     collect_s = time.perf_counter() - t0
     # standardised features; target = the round improved (binary); a
     # quarter held out to measure how well the network ranks regions
@@ -351,9 +435,7 @@ def learned_ab(make_backend, train_cells, seeds, seconds, train_rounds=2000,
         losses = train(net, Xn[tr], y[tr], epochs=epochs, batch=256, lr=3e-3)
         train_s = time.perf_counter() - t1
         score = net.forward(Xn[te])
-        pos, neg = score[y[te] == 1], score[y[te] == 0]
-        auc = (float((pos[:, None] > neg[None, :]).mean())
-               if len(pos) and len(neg) else None)
+        auc = rank_auc(score, y[te])
         runs = []
         for seed in seeds:
             code = start_code(make_backend, seed)

@@ -158,6 +158,27 @@ int  dct_init3x(int problem, int n, const int *ax, int R,
 int  dct_transform_layers(int init_mode);
 /* out[i*(R+1) + d] = layer d at idx[i] after dct_transform_layers. */
 int  dct_layers_read_at(const long long *idx, int k, int32_t *out);
+/* Ruin-region features of the learned proposal, computed on the device:
+ * sets the pending code to code[0..M), runs the deficient, critical and
+ * codeword layer transforms and writes, for each of the nreg regions of k
+ * words (words[r*k .. r*k+k)), five windows of R+1 floats to out[r*5*(R+1)]:
+ * mean of the critical layers, mean and max of the deficient layers, mean
+ * of the codeword layers, min of the critical layers, each layer count
+ * divided by its distance shell size.  Hamming only, nreg*k <= 65,536.
+ * cnt and the represented code are unchanged; the map plane is overwritten.
+ * No reference counterpart (the reference has no learned operator). */
+int  dct_region_features(const long long *code, int M, const long long *words,
+                         int nreg, int k, float *out);
+/* Candidate ruin regions and their features in one device call: for each of
+ * the na anchor positions into code[0..M), the k code words nearest to the
+ * anchor in digit distance (ties by a 24-bit splitmix64 hash of (seed,
+ * anchor position, word position), then by position), written as positions
+ * to regions[b*k ..] in that order, and their five windows (as
+ * dct_region_features) to feats[b*5*(R+1) ..].  Hamming only; M <= 65,535,
+ * na <= 64, k <= min(M, 64).  No reference counterpart. */
+int  dct_region_propose(const long long *code, int M, const int32_t *anchors,
+                        int na, int k, unsigned long long seed, int32_t *regions,
+                        float *feats);
 /* Schedule of the LDS-tiled sweeps: 8 ints per sweep (first axis, axes in
  * the group, P, TL, layers in, layers out, axis size of the register-round
  * kernel or 0 for the per-axis kernel, LDS bytes); returns the sweep count.

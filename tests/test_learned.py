@@ -177,3 +177,54 @@ def test_trained_network_ranks_held_out_regions():
         assert auc > 0.55, auc
     finally:
         net.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("axes,R", [([3] * 8, 2), ([4] * 6, 2), ([2] * 12, 2),
+                                    ([3, 4, 5, 3, 4], 2), ([2] * 10, 1)])
+def test_device_region_features_equal_host_features(gpu_cell, axes, R):
+    """dct_region_features (three layer transforms and the windows on the
+    device, one download) equals the windows the host computes from the
+    oracle's exact layers."""
+    from coldcore_amd.learned import RegionScorer
+    c = NumpyDP(axes, R)
+    g = gpu_cell(axes, R)
+    size = c.space.size
+    rng = np.random.default_rng(size)
+    code = np.unique(rng.integers(0, size, max(4, size // 40)))
+    c.recount(code)
+    g.recount(code)
+    cnt0 = g.read_cnt().copy()
+    for nreg, k in ((1, 1), (7, 3), (32, 5), (64, 8)):
+        k = min(k, len(code))
+        regions = np.stack([rng.choice(len(code), k, replace=False)
+                            for _ in range(nreg)])
+        want = RegionScorer(c).features(RegionScorer(c).layers(code), regions)
+        got = RegionScorer(g).region_features(code, regions)
+        assert got.shape == want.shape
+        assert np.allclose(got, want, rtol=1e-6, atol=1e-7), (nreg, k)
+    assert (g.read_cnt() == cnt0).all()          # cnt unchanged
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("axes,R", [([3] * 10, 2), ([4] * 8, 2), ([2, 3, 4, 5, 3, 2], 2)])
+def test_device_region_proposal_equals_host(gpu_cell, axes, R):
+    """dct_region_propose picks the same regions (nearest code words by digit
+    distance, the same hashed tie-break) as the host restatement, and their
+    features equal the host windows of the oracle's layers."""
+    from coldcore_amd.learned import RegionScorer, cluster_regions
+    c = NumpyDP(axes, R)
+    g = gpu_cell(axes, R)
+    size = c.space.size
+    rng = np.random.default_rng(size + 1)
+    code = np.unique(rng.integers(0, size, max(8, size // 60)))
+    c.recount(code)
+    g.recount(code)
+    for na, k, seed in ((1, 1, 5), (32, 4, 2 ** 62 + 17), (64, 9, 123456789)):
+        anchors = rng.choice(len(code), size=min(na, len(code)), replace=False)
+        regions, feats = g.region_propose(code, anchors, k, seed)
+        want = cluster_regions(c.space, code, k, anchors, seed)
+        assert (regions == want).all(), (na, k)
+        sc = RegionScorer(c)
+        assert np.allclose(feats, sc.features(sc.layers(code), want),
+                           rtol=1e-6, atol=1e-7)
