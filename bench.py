@@ -489,8 +489,8 @@ def run_near_cover(GpuBackend):
     return out
 
 
-def run_learned(GpuBackend, seconds=5.0, seeds=(0, 1, 2, 3, 4), rounds=4000,
-                workers=8, epochs=10, per_cell=None, log=None):
+def run_learned(GpuBackend, seconds=10.0, seeds=(0, 1, 2, 3, 4), rounds=12500,
+                workers=8, epochs=8, per_cell=10, log=None):
     """Time-to-size of the learned ruin proposal against plain adaptive LNS
     on K_3(10,2): move logs of plain LNS on K_3(10,2) and K_4(8,2) (every
     round logged: region features from the exact layer fields, outcome),
@@ -784,7 +784,8 @@ def main():
         except RuntimeError as e:          # reported, never hidden
             rec["batch64"] = {"error": str(e)}
         # config 4: the learned proposal's time-to-size against plain LNS
-        rec["learned"] = run_learned(GpuBackend)
+        rec["learned"] = run_learned(
+            GpuBackend, log=lambda m: print(m, file=sys.stderr, flush=True))
     if not args.no_cpu:
         rec["cpu_baseline"] = run_cpu_baseline(code, args.cpu_passes)
         rec["speedup_vs_cpu"] = round(passes_s / rec["cpu_baseline"]["value"], 1)

@@ -25,7 +25,9 @@
 #pragma once
 #include "cc_common.hpp"
 
-template <int PROB, int R1, int Q, bool PAD>
+// RW > 0: the L-inf radius as a compile-time constant (the window's terms
+// are then fixed: 2R+1 adds per word instead of Q predicated ones)
+template <int PROB, int R1, int Q, bool PAD, int RW = 0>
 __device__ __forceinline__ void cc_fiber(int32_t *__restrict__ S, int PT,
                                          int e0, int es, int lv, int lw,
                                          int R) {
@@ -76,24 +78,42 @@ __device__ __forceinline__ void cc_fiber(int32_t *__restrict__ S, int PT,
       for (int b = 0; b < Q; b++) {
         int dd = a > b ? a - b : b - a;
         if (PROB == CC_TORUS && Q - dd < dd) dd = Q - dd;
-        s += (dd <= R) ? v[0][b] : 0u;
+        if constexpr (RW > 0) {
+          if (dd <= RW) s += v[0][b];
+        } else {
+          s += (dd <= R) ? v[0][b] : 0u;
+        }
       }
       S[cc_gpad<PAD>(e0 + a * es)] = (int32_t)s;
     }
   }
 }
 
-template <int PROB, int R1, int Q, bool PAD>
-__device__ __forceinline__ void cc_step(int32_t *__restrict__ S, int PT,
-                                        int TL, int nf, int sk, int lv,
-                                        int lw, int R) {
+template <int PROB, int R1, int Q, bool PAD, int RW>
+__device__ __forceinline__ void cc_step_r(int32_t *__restrict__ S, int PT,
+                                          int TL, int nf, int sk, int lv,
+                                          int lw, int R) {
   const uint32_t mT = cc_magic((uint32_t)TL), mS = cc_magic((uint32_t)sk);
   for (int fi = threadIdx.x; fi < nf; fi += CC_BLOCK) {
     const int f = cc_div(fi, mT), tl = fi - f * TL;
     const int hi = cc_div(f, mS), lo = f - hi * sk;
     const int e0 = (hi * sk * Q + lo) * TL + tl;
-    cc_fiber<PROB, R1, Q, PAD>(S, PT, e0, sk * TL, lv, lw, R);
+    cc_fiber<PROB, R1, Q, PAD, RW>(S, PT, e0, sk * TL, lv, lw, R);
   }
+}
+
+// one axis pass over the tile's fibers; the L-inf families take their
+// radius (1..3, the sizes the workloads use) as a compile-time constant
+template <int PROB, int R1, int Q, bool PAD>
+__device__ __forceinline__ void cc_step(int32_t *__restrict__ S, int PT,
+                                        int TL, int nf, int sk, int lv,
+                                        int lw, int R) {
+  if constexpr (PROB == CC_TORUS || PROB == CC_GRID) {
+    if (R == 1) return cc_step_r<PROB, R1, Q, PAD, 1>(S, PT, TL, nf, sk, lv, lw, R);
+    if (R == 2) return cc_step_r<PROB, R1, Q, PAD, 2>(S, PT, TL, nf, sk, lv, lw, R);
+    if (R == 3) return cc_step_r<PROB, R1, Q, PAD, 3>(S, PT, TL, nf, sk, lv, lw, R);
+  }
+  cc_step_r<PROB, R1, Q, PAD, 0>(S, PT, TL, nf, sk, lv, lw, R);
 }
 
 #define CC_STEP_CASE(QQ)                                                    \

@@ -462,7 +462,10 @@ void build_plan() {
     // of binary or quaternary axes is capped at one 16-word cube per thread
     // (4,096 words per tile) to keep its round count low
     const int q = E.ax[j];
-    const bool round_ok = !E.generic_only && E.prob == CC_HAMMING && q <= 10;
+    // (the round kernels of axis size <= 5 exchange layer planes only: with
+    // compact words on, such a group takes the LDS tile, which has them)
+    const bool round_ok = !E.generic_only && E.prob == CC_HAMMING && q <= 10 &&
+                          (q > 5 || !E.compact);
     long long P = 1;
     int k = j;
     while (k < E.n) {
@@ -1645,6 +1648,12 @@ static int init_cell(int problem, int n, const int *ax, int R,
   E.grid = sg ? atoi(sg) : 0;   // measured: one workgroup per tile is best
   const char *sk = getenv("CC_SWEEP_KERNEL");
   E.generic_only = sk && !strcmp(sk, "generic");
+  {
+    // the plan built below depends on it (a cell never inherits the last
+    // cell's tuned value)
+    const char *pcm = getenv("CC_COMPACT");
+    E.compact = pcm ? atoi(pcm) : 1;
+  }
   CK(hipStreamCreateWithFlags(&E.st, hipStreamNonBlocking));
   // A[0] (map / code indicator) on its own; layers 1..R as one block whose
   // planes are 64-word aligned, viewed as int32 planes A[1..R] or, in 16-bit
@@ -2561,8 +2570,9 @@ int dct_plan_probe(int n, const int *ax, int R, int kind, int s0, int M,
 // loss transform on the live cell and keep the fastest.  Candidates are the
 // planner's schedule under every combination of LDS budget (16-128 KB),
 // column-kernel mode (off / whole cube / lane pairs) and first-sweep kernel
-// (LDS-tiled / persistent with prefetch), plus the unpacked per-layer
-// kernels; duplicates are timed once.  Each is run once untimed and then
+// (LDS-tiled / persistent with prefetch / LDS-free across the lanes), plus
+// the unpacked kernels under every LDS budget with compact words or layer
+// planes and with or without register rounds; duplicates are timed once.  Each is run once untimed and then
 // reps times between HIP events on the engine's stream.  The winning
 // options also steer the recount schedule.  Like a transform, it
 // overwrites the map plane; cnt and the represented solution are unchanged.
@@ -2576,7 +2586,7 @@ int dct_autotune(int reps, float *best_us) {
   for (int lds : {16384, 32768, 65536, 98304, 131072}) {
     if (lds > E.lds_max) continue;
     for (int col : {2, 1, 0})
-      for (int first : {1, 0})
+      for (int first : {1, 0, 2})
         opts.push_back({lds, col, first, 1, E.pk_narrow, E.compact, E.generic_only});
   }
   // the unpacked schedule under each LDS budget (its plan is rebuilt per
