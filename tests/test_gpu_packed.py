@@ -185,6 +185,12 @@ FORCED_R4 = {
     "pair6-last": (22, [10, 16], "2", [(0, 10, 5, 4, 8, 2, 2, 4),
                                        (10, 6, 3, 6, 16, 3, 4, 8),
                                        (16, 6, 3, 6, 16, 3, 8, 4)]),
+    # the LDS-free first sweep (k_pfirst: 4 axes in-thread, 6 across the
+    # lanes), which the autotuner may pick for K_2(30,4)'s first 10 axes
+    "xlane10-col1": (22, [10, 15, 20], "1", [(0, 10, 4, 4, 8, 2, 2, 4),
+                                             (10, 5, 2, 5, 16, 3, 4, 4),
+                                             (15, 5, 2, 5, 16, 3, 4, 8),
+                                             (20, 2, 2, 2, 16, 3, 8, 4)], "2"),
     "pair6-mid": (23, [10, 16, 22], "2", [(0, 10, 5, 4, 8, 2, 2, 4),
                                           (10, 6, 3, 6, 16, 3, 4, 8),
                                           (16, 6, 3, 6, 16, 3, 8, 8),
@@ -211,8 +217,10 @@ def _walsh_fields(n, R):
 
 @pytest.mark.parametrize("case", list(FORCED_R4))
 def test_r4_schedule_kernels_match_walsh(gpu_cell, monkeypatch, case):
-    n, cuts, col, rows = FORCED_R4[case]
+    n, cuts, col, rows, *first = FORCED_R4[case]
     monkeypatch.setenv("CC_PK_COL", col)
+    if first:
+        monkeypatch.setenv("CC_PK_FIRST", first[0])
     print(f"forced R=4 schedule {case}: n={n} cuts={cuts} col={col}", flush=True)
     code, cnt, gain, loss = _walsh_fields(n, 4)
     g = gpu_cell([2] * n, 4)
