@@ -504,8 +504,10 @@ void build_plan() {
     sw.rq = 0;
     bool homog = true;
     for (int i = j; i < k; i++) homog = homog && E.ax[i] == q;
-    const size_t lds_r = (size_t)E.L * (padded(P * sw.TL) + 1) * 4;
-    const size_t lds_r16 = (size_t)((E.L + 1) / 2) * (padded(P * sw.TL) + 1) * 4;
+    // the round kernel's cubes hold the layers live after the group
+    const int Lr = std::max(2, sw.lmax);
+    const size_t lds_r = (size_t)Lr * (padded(P * sw.TL) + 1) * 4;
+    const size_t lds_r16 = (size_t)((Lr + 1) / 2) * (padded(P * sw.TL) + 1) * 4;
     // (in 16-bit mode every transform of the cell runs 16-bit, so only the
     // packed footprint has to fit)
     if (round_ok && homog && sw.nax >= round_axes(q) &&
@@ -530,9 +532,12 @@ void build_plan() {
   }
 }
 
-hipError_t launch_round(int q, const SweepArgs &a, int blocks, size_t lds,
+// L: the layers the sweep's cubes hold -- those that can be nonzero after
+// its axes (layers above them are zero and stay zero), so an early sweep
+// (distances below R) carries fewer registers and does less arithmetic
+hipError_t launch_round(int q, const SweepArgs &a, int L, int blocks, size_t lds,
                         bool l16, bool cmp, int threads) {
-  return cc_sweep_round(a, E.L, q, blocks, lds, l16, cmp,
+  return cc_sweep_round(a, L, q, blocks, lds, l16, cmp,
                         LaunchCtx{E.st, E.ncu, E.grid, threads});
 }
 
@@ -705,7 +710,8 @@ int run_transform(int in_kind, int out_kind, int pmode) {
     if (E.prof) CK(hipEventRecord(E.ev[2 * i], E.st));
     const bool cm = a.in_kind == CC_IN_COMPACT || a.out_kind == CC_OUT_COMPACT;
     if (sw.rq)
-      CK(launch_round(sw.rq, a, blocks, l16 ? sw.lds16 : sw.lds, l16, cm, sw.threads));
+      CK(launch_round(sw.rq, a, std::max(2, sw.lmax), blocks,
+                      l16 ? sw.lds16 : sw.lds, l16, cm, sw.threads));
     else CK(launch_sweep(a, blocks, sw.lds, l16, cm));
     if (E.prof) CK(hipEventRecord(E.ev[2 * i + 1], E.st));
   }
