@@ -48,7 +48,7 @@ def kernel_name(plan, i, q):
         return "k_sweep / k_sweep_r (unpacked layers)"
     last = i == len(plan) - 1
     name = {1: "k_psweep", 2: "k_pcol", 3: "k_pcol", 4: "k_pfirst",
-            5: "k_pfirstp"}[sw["kernel"]]
+            5: "k_pfirstp", 6: "k_pfirstw", 7: "k_pstrip"}[sw["kernel"]]
     return (f"{name}<Q={q},RR={sw['RR']},W={sw['W']},ND={sw['ND']},"
             f"{'sum' if last else 'compact'},{sw['bytes_in']}+{sw['bytes_out']}B>")
 
@@ -489,8 +489,42 @@ def run_near_cover(GpuBackend):
     return out
 
 
+#: the proposal network trained on 10^6 logged plain-LNS rounds (half
+#: K_3(10,2), half K_4(8,2)) by the engine's own MLP kernels:
+#: ``python bench.py --only learned-train`` writes it
+PROPOSAL = os.path.join(ROOT, "coldcore_amd", "proposal_k3_10_2.npz")
+
+
+def run_learned_saved(GpuBackend, path=PROPOSAL, seconds=10.0,
+                      seeds=(0, 1, 2, 3, 4), log=None):
+    """Time-to-size A/B of the saved proposal against plain adaptive LNS on
+    K_3(10,2) (both arms from the same greedy + peel cover per seed)."""
+    from coldcore_amd.learned import learned_ab, load_proposal
+    model = load_proposal(path)
+    out = learned_ab(lambda: GpuBackend([3] * 10, 2), [], list(seeds), seconds,
+                     model=model, log=log)
+    out["proposal"] = os.path.relpath(path, ROOT)
+    return _ab_summary(out)
+
+
+def _ab_summary(out):
+    for r in out["runs"]:                 # keep the JSON line short
+        for arm in ("plain", "learned"):
+            r[arm].pop("hist", None)
+    out.pop("median_ratio")
+    # median over the seeds, a seed whose learned arm never reached the
+    # plain arm's final size counting as infinitely late
+    ratios = sorted(np.inf if r["ratio"] is None else r["ratio"] for r in out["runs"])
+    med = float(np.median(ratios))
+    out["time_to_size_ratio"] = None if not np.isfinite(med) else round(med, 3)
+    out["per_seed_ratio"] = [r["ratio"] for r in out["runs"]]
+    out["learned_final_smaller_or_equal"] = int(sum(
+        r["learned"]["final_size"] <= r["plain"]["final_size"] for r in out["runs"]))
+    return out
+
+
 def run_learned(GpuBackend, seconds=10.0, seeds=(0, 1, 2, 3, 4), rounds=12500,
-                workers=8, epochs=8, per_cell=10, log=None):
+                workers=8, epochs=8, per_cell=10, log=None, save=None):
     """Time-to-size of the learned ruin proposal against plain adaptive LNS
     on K_3(10,2): move logs of plain LNS on K_3(10,2) and K_4(8,2) (every
     round logged: region features from the exact layer fields, outcome),
@@ -505,24 +539,14 @@ def run_learned(GpuBackend, seconds=10.0, seeds=(0, 1, 2, 3, 4), rounds=12500,
                                     rounds, workers=workers, log=log)
     collect_s = time.perf_counter() - t
     out = learned_ab(lambda: GpuBackend([3] * 10, 2), [], list(seeds), seconds,
-                     epochs=epochs, records=(X, y), log=log)
+                     epochs=epochs, records=(X, y), log=log, save=save)
     out["collect_s"] = round(collect_s, 2)
     out["logging_processes"] = workers
     out["log_jobs"] = 2 * per_cell
     out["rounds_per_job"] = rounds
-    for r in out["runs"]:                 # keep the JSON line short
-        for arm in ("plain", "learned"):
-            r[arm].pop("hist", None)
-    out.pop("median_ratio")
-    # median over the seeds, a seed whose learned arm never reached the
-    # plain arm's final size counting as infinitely late
-    ratios = sorted(np.inf if r["ratio"] is None else r["ratio"] for r in out["runs"])
-    med = float(np.median(ratios))
-    out["time_to_size_ratio"] = None if not np.isfinite(med) else round(med, 3)
-    out["per_seed_ratio"] = [r["ratio"] for r in out["runs"]]
-    out["learned_final_smaller_or_equal"] = int(sum(
-        r["learned"]["final_size"] <= r["plain"]["final_size"] for r in out["runs"]))
-    return out
+    if not seeds:
+        return out
+    return _ab_summary(out)
 
 
 def copy_calibration(mib):
@@ -627,8 +651,11 @@ def main():
                          "FETCH_SIZE / WRITE_SIZE counters")
     ap.add_argument("--only", choices=("near_cover", "batch64", "batch64-quick",
                                        "cells", "learned", "learned-1m", "walks",
-                                       "cpu_calib", "config5"),
+                                       "cpu_calib", "config5",
+                                       "learned-train"),
                     help="run one side section and print its JSON only")
+    ap.add_argument("--out", help="learned-train: where to write the proposal "
+                    "(default: the package's saved proposal)")
     args = ap.parse_args()
     if args.only == "cpu_calib":
         print(json.dumps({"cpu_calib": run_cpu_calibration()}), flush=True)
@@ -646,6 +673,11 @@ def main():
             # of 12,500 rounds over 8 processes), 30 s per arm
             r = run_learned(GpuBackend, seconds=30.0, rounds=12500, epochs=8,
                             per_cell=40,
+                            log=lambda m: print(m, file=sys.stderr, flush=True))
+        elif args.only == "learned-train":
+            # the saved proposal: 10^6 logged rounds, trained, no A/B
+            r = run_learned(GpuBackend, seeds=(), rounds=12500, epochs=8,
+                            per_cell=40, save=args.out or PROPOSAL,
                             log=lambda m: print(m, file=sys.stderr, flush=True))
         elif args.only == "config5":
             r = run_config5(GpuBackend)
@@ -773,25 +805,48 @@ def main():
         },
     }
     rec.update(extra)
+    t_sec = [time.perf_counter()]
+
+    def section(name):
+        # progress on stderr: one line per finished section
+        now = time.perf_counter()
+        print(f"bench section {name}: {now - t_sec[0]:.1f} s", file=sys.stderr,
+              flush=True)
+        t_sec[0] = now
+
+    section("K_2(24,3) passes, walks")
     if not args.no_extra:
         rec["other_cells"] = time_cells(GpuBackend, 10)
+        section("other cells")
         rec["config2_k5_10_3"] = run_config2_cell(GpuBackend, [5] * 10, 3)
+        section("config 2 K_5(10,3)")
         rec["config5_k2_30_4"] = run_config5(GpuBackend)
+        section("config 5 K_2(30,4)")
         rec["config1"] = run_config1(GpuBackend, not args.no_cpu)
+        section("config 1")
         rec["near_cover"] = run_near_cover(GpuBackend)
+        section("near cover")
         try:
             rec["batch64"] = run_batch64(GpuBackend)
         except RuntimeError as e:          # reported, never hidden
             rec["batch64"] = {"error": str(e)}
+        section("batch64")
         # config 4: the learned proposal's time-to-size against plain LNS
-        rec["learned"] = run_learned(
-            GpuBackend, log=lambda m: print(m, file=sys.stderr, flush=True))
+        # (the saved network when there is one; else a shorter log, trained
+        # here)
+        lg = lambda m: print(m, file=sys.stderr, flush=True)
+        rec["learned"] = (run_learned_saved(GpuBackend, log=lg)
+                          if os.path.exists(PROPOSAL) else
+                          run_learned(GpuBackend, log=lg))
+        section("learned")
     if not args.no_cpu:
         rec["cpu_baseline"] = run_cpu_baseline(code, args.cpu_passes)
         rec["speedup_vs_cpu"] = round(passes_s / rec["cpu_baseline"]["value"], 1)
+        section("cpu baseline")
         # the oracle against the reference's published single-thread numpy
         # times, on this (the GPU box's) host
         rec["cpu_calib"] = run_cpu_calibration()
+        section("cpu calibration")
     print(json.dumps(rec), flush=True)
 
 

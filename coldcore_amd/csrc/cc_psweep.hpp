@@ -45,6 +45,9 @@
                          // word list (k_pfirstp only: no plane is read)
 #define CC_OUT_PACKED 3  // layers to the compact planes
 #define CC_PK_MAXRD 12   // rounds per sweep
+// cache-policy bits of a store: 0 default, CC_ST_NT non-temporal (gfx950:
+// the line is kept in L2 for a short time only)
+#define CC_ST_NT 2
 
 struct PSweepArgs {
   const uint16_t *cnt;   // CC_IN_PRED: predicate source
@@ -75,6 +78,12 @@ struct PSweepArgs {
   long long nar_off;     // dwords from pk to the narrow planes (the uint8
                          // plane starts 2 * pitch bytes after the uint16 one)
   uint32_t out_bmask;    // byte format: bytes of real layers
+  int st_nt;             // non-temporal stores
+  // k_pstrip: the one-dword format of the LDS exchange between its rounds
+  // (layer d in bits [moff, moff + mbits)); mid_w8: one byte per layer
+  int mid_w8;
+  int moff[CC_MAXL], mbits[CC_MAXL];
+  int na;                // k_pfirstw: axes of its last (widening) round
   int nrounds;
   int ca[CC_PK_MAXRD];     // lowest group axis of each round's cube
   int amask[CC_PK_MAXRD];  // cube axes the round applies (bit i: axis ca + i)
@@ -176,13 +185,13 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t pk_rsrc(const void *plane,
 #define PK_LD16(r, c) \
   ((uint32_t)__builtin_amdgcn_raw_buffer_load_b16((r), (int)(lane * 2u), (int)((c) * usq * 2u), 0))
 #define PK_ST32(r, c, val) \
-  __builtin_amdgcn_raw_buffer_store_b32((uint32_t)(val), (r), (int)(lane * 4u), (int)((c) * usq * 4u), 0)
+  __builtin_amdgcn_raw_buffer_store_b32((uint32_t)(val), (r), (int)(lane * 4u), (int)((c) * usq * 4u), ST_AUX)
 #define PK_ST16(r, c, val) \
-  __builtin_amdgcn_raw_buffer_store_b16((unsigned short)(val), (r), (int)(lane * 2u), (int)((c) * usq * 2u), 0)
+  __builtin_amdgcn_raw_buffer_store_b16((unsigned short)(val), (r), (int)(lane * 2u), (int)((c) * usq * 2u), ST_AUX)
 #define PK_LD8(r, c) \
   ((uint32_t)__builtin_amdgcn_raw_buffer_load_b8((r), (int)lane, (int)((c) * usq), 0))
 #define PK_ST8(r, c, val) \
-  __builtin_amdgcn_raw_buffer_store_b8((unsigned char)(val), (r), (int)lane, (int)((c) * usq), 0)
+  __builtin_amdgcn_raw_buffer_store_b8((unsigned char)(val), (r), (int)lane, (int)((c) * usq), ST_AUX)
 
 // the narrow compact planes of a sweep
 __device__ __forceinline__ const uint16_t *pk_nar16(const PSweepArgs &a) {
@@ -402,10 +411,10 @@ __device__ __forceinline__ uint64_t pk_compact(const PSweepArgs &a,
 // C words out (addressed as in pk_load): compact layers, the layer sum into
 // the map, or the clamped sum into cnt.  VEC: the cube is C consecutive
 // words (usq == 1, 16-byte aligned): 16-byte vector stores.
-template <int C, int W, int ND, bool SUM, bool OUT2, bool VEC = false>
-__device__ __forceinline__ void pk_store(const PSweepArgs &a, long long ub,
-                                         uint32_t usq, uint32_t lane,
-                                         const uint32_t (&v)[C][ND]) {
+template <int C, int W, int ND, bool SUM, bool OUT2, bool VEC, int ST_AUX>
+__device__ __forceinline__ void pk_store_aux(const PSweepArgs &a, long long ub,
+                                             uint32_t usq, uint32_t lane,
+                                             const uint32_t (&v)[C][ND]) {
   constexpr int NS = ND * (32 / W);
   if constexpr (!SUM) {
     if (a.out_nar) {
@@ -443,7 +452,7 @@ __device__ __forceinline__ void pk_store(const PSweepArgs &a, long long ub,
           pk_u32x4 o;
 #pragma unroll
           for (int i = 0; i < 4; i++) o[i] = pk_to_bytes<W, ND>(v[c + i], a.out_bmask);
-          __builtin_amdgcn_raw_buffer_store_b128(o, r, (int)(lane * 4u), c * 4, 0);
+          __builtin_amdgcn_raw_buffer_store_b128(o, r, (int)(lane * 4u), c * 4, ST_AUX);
         }
       } else {
 #pragma unroll
@@ -455,9 +464,9 @@ __device__ __forceinline__ void pk_store(const PSweepArgs &a, long long ub,
             o[i] = (uint32_t)x;
             o1[i] = (uint32_t)(x >> 32);
           }
-          __builtin_amdgcn_raw_buffer_store_b128(o, r, (int)(lane * 4u), c * 4, 0);
+          __builtin_amdgcn_raw_buffer_store_b128(o, r, (int)(lane * 4u), c * 4, ST_AUX);
           if (OUT2)
-            __builtin_amdgcn_raw_buffer_store_b128(o1, r1, (int)(lane * 4u), c * 4, 0);
+            __builtin_amdgcn_raw_buffer_store_b128(o1, r1, (int)(lane * 4u), c * 4, ST_AUX);
         }
       }
     } else if (w8) {
@@ -490,14 +499,14 @@ __device__ __forceinline__ void pk_store(const PSweepArgs &a, long long ub,
         if (to_map) {
           const pk_u32x4 o0 = {sm[0], sm[1], sm[2], sm[3]};
           const pk_u32x4 o1 = {sm[4], sm[5], sm[6], sm[7]};
-          __builtin_amdgcn_raw_buffer_store_b128(o0, rm, (int)(lane * 4u), c * 4, 0);
-          __builtin_amdgcn_raw_buffer_store_b128(o1, rm, (int)(lane * 4u), c * 4 + 16, 0);
+          __builtin_amdgcn_raw_buffer_store_b128(o0, rm, (int)(lane * 4u), c * 4, ST_AUX);
+          __builtin_amdgcn_raw_buffer_store_b128(o1, rm, (int)(lane * 4u), c * 4 + 16, ST_AUX);
         } else {
           pk_u32x4 o;
 #pragma unroll
           for (int i = 0; i < 4; i++)
             o[i] = min(sm[2 * i], 65535u) | (min(sm[2 * i + 1], 65535u) << 16);
-          __builtin_amdgcn_raw_buffer_store_b128(o, rc, (int)(lane * 2u), c * 2, 0);
+          __builtin_amdgcn_raw_buffer_store_b128(o, rc, (int)(lane * 2u), c * 2, ST_AUX);
         }
       }
     } else {
@@ -510,6 +519,16 @@ __device__ __forceinline__ void pk_store(const PSweepArgs &a, long long ub,
       }
     }
   }
+}
+
+// the store cache policy is chosen per sweep (st_nt: non-temporal stores
+// for words that the next kernel does not read soon, such as the map)
+template <int C, int W, int ND, bool SUM, bool OUT2, bool VEC = false>
+__device__ __forceinline__ void pk_store(const PSweepArgs &a, long long ub,
+                                         uint32_t usq, uint32_t lane,
+                                         const uint32_t (&v)[C][ND]) {
+  if (a.st_nt) pk_store_aux<C, W, ND, SUM, OUT2, VEC, CC_ST_NT>(a, ub, usq, lane, v);
+  else pk_store_aux<C, W, ND, SUM, OUT2, VEC, 0>(a, ub, usq, lane, v);
 }
 
 // LDS exchange: element (row p, column tl) of the tile lives at
@@ -827,5 +846,163 @@ __global__ void __launch_bounds__(CC_BLOCK) k_pfirstp(PSweepArgs a) {
         pk_lds_st<C, ND>(Sp, PTP, pb, qa, 1, 0, lin, kp, v);
       __syncthreads();
     }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// First sweep whose layers outgrow bytes near its end (k_pfirstw): the
+// rounds that stay within byte bounds run as in k_pfirstp in the W = 8
+// single-dword format (the LDS tile holds one byte per layer), and the last
+// round, which applies only the top NA axes of its cube, widens each chunk
+// of Q^NA words to the (W2, ND2) format on the way out.  A 14-axis binary
+// first sweep (16,384-word tiles) keeps a 66 KB byte tile in LDS instead of
+// 133 KB of 16-bit fields.
+template <int Q, int RR, int W2, int ND2, int NA, bool SUM, bool OUT2>
+__global__ void __launch_bounds__(CC_BLOCK) k_pfirstw(PSweepArgs a) {
+  extern __shared__ uint32_t Sp[];
+  constexpr int C = cc_ipow(Q, RR);
+  constexpr int CA = cc_ipow(Q, NA);         // words of one chunk
+  constexpr int F = C / CA;                  // chunks of the last round's cube
+  static_assert(C % 8 == 0 && NA >= 1 && NA <= RR, "cube shape");
+  const int PTP = pk_pad(a.P) + 1;
+  const int u = threadIdx.x;
+  const uint32_t lane0 = (uint32_t)u * C;
+  const uint32_t lo = (uint32_t)a.plo, span = (uint32_t)(a.phi - a.plo);
+  const bool sparse = a.in_kind == CC_IN_SPARSE;
+  const int last = a.nrounds - 1;
+  uint32_t pf[C / 2];
+  long long tile = blockIdx.x;
+  if (tile < a.ntiles && !sparse) pk_cnt_raw<C>(a, tile * a.P, lane0, pf);
+  for (; tile < a.ntiles; tile += gridDim.x) {
+    const long long base = tile * a.P;
+    uint32_t v[C][1];
+#pragma unroll
+    for (int c = 0; c < C; c++) v[c][0] = 0u;
+    if (sparse) {
+      const int e1 = a.toff[tile + 1];
+      const long long cube0 = base + lane0;
+      for (int e = a.toff[tile]; e < e1; e++) {
+        const long long d = a.cw[e] - cube0;
+        if (d >= 0 && d < C) {
+          const uint32_t m = (uint32_t)a.cm[e];
+#pragma unroll
+          for (int c = 0; c < C; c++) v[c][0] = d == c ? m : v[c][0];
+        }
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < C; c++) {
+        const uint32_t cv = (pf[c >> 1] >> (16 * (c & 1))) & 0xFFFFu;
+        v[c][0] = (cv - lo) <= span ? 1u : 0u;
+      }
+      const long long next = tile + gridDim.x;
+      if (next < a.ntiles) pk_cnt_raw<C>(a, next * a.P, lane0, pf);
+    }
+    // byte rounds
+    for (int rd = 0; rd < last; rd++) {
+      const int qa = a.qa[rd];
+      const int pb = (u / qa) * qa * C + (u % qa);
+      if (rd > 0) pk_lds_ld<C, 1>(Sp, PTP, pb, qa, 1, 0, a.lin[rd] != 0, a.kp[rd], v);
+      pk_apply<Q, C, 0, RR, 8, 1>(v, a.amask[rd]);
+      pk_lds_st<C, 1>(Sp, PTP, pb, qa, 1, 0, a.lin[rd] != 0, a.kp[rd], v);
+      __syncthreads();
+    }
+    // last round: the top NA cube axes, chunk by chunk in the wide format;
+    // chunk f holds the cube elements f + F g (g < Q^NA), F * qa words apart
+    {
+      const int qa = a.qa[last];
+      const int pb = (u / qa) * qa * C + (u % qa);
+      pk_lds_ld<C, 1>(Sp, PTP, pb, qa, 1, 0, a.lin[last] != 0, a.kp[last], v);
+#pragma unroll
+      for (int f = 0; f < F; f++) {
+        uint32_t w[CA][ND2];
+#pragma unroll
+        for (int g = 0; g < CA; g++) pk_from_bytes<W2, ND2>(v[f + F * g][0], w[g]);
+        pk_apply<Q, CA, 0, NA, W2, ND2>(w, (1 << NA) - 1);
+        pk_store<CA, W2, ND2, SUM, OUT2>(a, base, (uint32_t)(F * qa),
+                                         (uint32_t)(pb + f * qa), w);
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Strided sweep of 2 RA axes in two register rounds (k_pstrip): a tile is TL
+// consecutive columns of one outer block, each a fiber of P = C^2 rows
+// (C = Q^RA) s words apart.  Round A: thread (tl, u) holds the cube of
+// rows e + C u, e < C (the low RA axes, coalesced loads: consecutive
+// threads take consecutive columns), applies them and writes the rows to
+// LDS in a one-dword compact format; round B: the same thread holds rows
+// u + C k, k < C (the top RA axes), applies them and stores.  LDS row r
+// lives at r TL + (r / C) PADR so that the C-row groups of a round-A wave
+// fall on different banks; round-B waves read consecutive dwords.
+#define CC_STRIP_PADR 16
+
+template <int W, int ND>
+__device__ __forceinline__ uint32_t pk_mid_pack(const PSweepArgs &a,
+                                                const uint32_t (&r)[ND]) {
+  if (a.mid_w8) return pk_to_bytes<W, ND>(r, a.out_bmask);
+  constexpr int NS = ND * (32 / W);
+  constexpr uint32_t M = (1u << W) - 1u;
+  uint32_t o = 0;
+#pragma unroll
+  for (int d = 0; d < NS; d++) {
+    if (d >= a.L) continue;
+    o |= ((r[d % ND] >> (W * (d / ND))) & M) << a.moff[d];
+  }
+  return o;
+}
+
+template <int W, int ND>
+__device__ __forceinline__ void pk_mid_unpack(const PSweepArgs &a, uint32_t x,
+                                              uint32_t (&r)[ND]) {
+  if (a.mid_w8) {
+    pk_from_bytes<W, ND>(x, r);
+    return;
+  }
+  constexpr int NS = ND * (32 / W);
+#pragma unroll
+  for (int k = 0; k < ND; k++) r[k] = 0u;
+#pragma unroll
+  for (int d = 0; d < NS; d++) {
+    if (d >= a.L) continue;
+    r[d % ND] |= __builtin_amdgcn_ubfe(x, a.moff[d], a.mbits[d]) << (W * (d / ND));
+  }
+}
+
+template <int Q, int RA, int W, int ND, bool SUM, bool IN2, bool OUT2>
+__global__ void __launch_bounds__(512) k_pstrip(PSweepArgs a) {
+  extern __shared__ uint32_t Sp[];
+  constexpr int C = cc_ipow(Q, RA);
+  const int TL = a.TL;
+  const int tl = (int)threadIdx.x % TL;
+  const int u = (int)threadIdx.x / TL;            // 0 .. C-1
+  const uint32_t s = (uint32_t)a.s;
+  const int rstride = C * TL + CC_STRIP_PADR;      // LDS dwords per C rows
+  for (long long tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
+    const long long h = tile / a.ncolb;
+    const long long base = h * a.s * a.P + (tile - h * a.ncolb) * TL;
+    uint32_t v[C][ND];
+    // round A: rows e + C u
+    pk_load<C, W, ND, false, IN2>(a, base, s, (uint32_t)tl + (uint32_t)(C * u) * s,
+                                  true, v);
+    pk_apply<Q, C, 0, RA, W, ND>(v, (1 << RA) - 1);
+#pragma unroll
+    for (int e = 0; e < C; e++) {
+      Sp[u * rstride + e * TL + tl] = pk_mid_pack<W, ND>(a, v[e]);
+      PK_FENCE(e);
+    }
+    __syncthreads();
+    // round B: rows u + C k
+#pragma unroll
+    for (int k = 0; k < C; k++) {
+      pk_mid_unpack<W, ND>(a, Sp[k * rstride + u * TL + tl], v[k]);
+      PK_FENCE(k);
+    }
+    pk_apply<Q, C, 0, RA, W, ND>(v, (1 << RA) - 1);
+    pk_store<C, W, ND, SUM, OUT2>(a, base, (uint32_t)C * s, (uint32_t)tl + (uint32_t)u * s,
+                                  v);
+    __syncthreads();                 // the next tile rewrites the LDS rows
   }
 }

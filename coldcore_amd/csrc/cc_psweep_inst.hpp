@@ -53,6 +53,67 @@ inline void pk_run(void (*kf)(PSweepArgs), int grid, int threads, size_t lds,
     hipLaunchKernelGGL(kf, dim3(grid), dim3(threads), lds, st, a);
 }
 
+// resident workgroups of a persistent launch, queried once per (kernel,
+// threads, LDS) -- the attribute and occupancy calls cost microseconds of
+// host time -- after allowing the kernel more than 64 KB of LDS
+inline long long pk_resident(void (*kf)(PSweepArgs), int threads, size_t lds) {
+  struct Ent { void (*kf)(PSweepArgs); int threads; size_t lds; long long res; };
+  static Ent cache[32];
+  static int n = 0;
+  for (int i = 0; i < n; i++)
+    if (cache[i].kf == kf && cache[i].threads == threads && cache[i].lds == lds)
+      return cache[i].res;
+  if (lds > 64 * 1024 &&
+      hipFuncSetAttribute((const void *)kf, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)lds) != hipSuccess)
+    return -1;
+  int per_cu = 0, dev = 0, ncu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)kf, threads,
+                                                   lds) != hipSuccess || per_cu < 1)
+    per_cu = 1;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) !=
+          hipSuccess || ncu < 1)
+    ncu = 256;
+  const long long res = (long long)per_cu * ncu;
+  cache[n < 32 ? n++ : 31] = Ent{kf, threads, lds, res};
+  return res;
+}
+
+// persistent launch of kf: at most as many workgroups as the GPU holds
+inline hipError_t pk_persist(void (*kf)(PSweepArgs), int blocks, int threads,
+                             size_t lds, hipStream_t st, hipEvent_t e0, hipEvent_t e1,
+                             const PSweepArgs &a) {
+  const long long resident = pk_resident(kf, threads, lds);
+  if (resident < 0) return hipErrorInvalidValue;
+  const int g = (int)(blocks < resident ? blocks : resident);
+  pk_run(kf, g, threads, lds, st, e0, e1, a);
+  return hipGetLastError();
+}
+
+template <int Q, int RR, int W2, int ND2, int NA>
+void (*pk_wide_select(bool sum, bool out2))(PSweepArgs) {
+  if (sum) return &k_pfirstw<Q, RR, W2, ND2, NA, true, false>;
+  if constexpr (ND2 > 1)
+    if (out2) return &k_pfirstw<Q, RR, W2, ND2, NA, false, true>;
+  return &k_pfirstw<Q, RR, W2, ND2, NA, false, false>;
+}
+
+template <int Q, int RA, int W, int ND>
+void (*pk_strip_select(bool sum, bool in2, bool out2))(PSweepArgs) {
+  if constexpr (ND > 1) {
+    if (sum) return in2 ? &k_pstrip<Q, RA, W, ND, true, true, false>
+                        : &k_pstrip<Q, RA, W, ND, true, false, false>;
+    if (in2) return out2 ? &k_pstrip<Q, RA, W, ND, false, true, true>
+                         : &k_pstrip<Q, RA, W, ND, false, true, false>;
+    return out2 ? &k_pstrip<Q, RA, W, ND, false, false, true>
+                : &k_pstrip<Q, RA, W, ND, false, false, false>;
+  } else {
+    if (sum) return &k_pstrip<Q, RA, W, ND, true, false, false>;
+    return &k_pstrip<Q, RA, W, ND, false, false, false>;
+  }
+}
+
 template <int Q, int RR, int W, int ND>
 hipError_t pk_go(const PSweepArgs &a, int blocks, int threads, size_t lds,
                  hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
@@ -112,6 +173,24 @@ hipError_t pk_go(const PSweepArgs &a, int blocks, int threads, size_t lds,
     const int g = (int)(blocks < resident ? blocks : resident);
     pk_run(kf, g, threads, lds, st, e0, e1, a);
     return hipGetLastError();
+  }
+  if (a.kern == 5) {
+    // first sweep whose last round widens the byte tile (binary cells)
+    if constexpr (Q == 2 && RR == 6 && W == 16 && ND == 2) {
+      if (a.na == 1) kf = pk_wide_select<Q, RR, W, ND, 1>(sum, out2);
+      else if (a.na == 2) kf = pk_wide_select<Q, RR, W, ND, 2>(sum, out2);
+    }
+    if (!kf) return hipErrorInvalidValue;
+    if (blocks == 0) return hipSuccess;
+    return pk_persist(kf, blocks, threads, lds, st, e0, e1, a);
+  }
+  if (a.kern == 6) {
+    // strided sweep of two register rounds through a compact LDS tile
+    if constexpr (Q == 2 && (RR == 4 || RR == 5) && W == 16 && ND == 2)
+      kf = pk_strip_select<Q, RR, W, ND>(sum, in2, out2);
+    if (!kf) return hipErrorInvalidValue;
+    if (blocks == 0) return hipSuccess;
+    return pk_persist(kf, blocks, threads, lds, st, e0, e1, a);
   }
   if (a.kern == 3) {
     // LDS-free first sweep (k_pfirst): RR in-thread axes + 6 lane axes

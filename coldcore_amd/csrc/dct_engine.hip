@@ -67,7 +67,8 @@ struct Sweep {
 // One sweep of the packed-layer path (cc_psweep.hpp).
 struct PSweep {
   int j0, nax, RR, W, ND, P, TL, threads;
-  int kern;  // 0 k_psweep, 1 / 2 k_pcol (lanes per cube), 3 k_pfirst
+  int kern;  // 0 k_psweep, 1 / 2 k_pcol (lanes per cube), 3 k_pfirst,
+             // 4 k_pfirstp, 5 k_pfirstw, 6 k_pstrip
   long long s, ncolb, ntiles;
   int nrounds;
   int ca[CC_PK_MAXRD], amask[CC_PK_MAXRD], qa[CC_PK_MAXRD];
@@ -80,6 +81,9 @@ struct PSweep {
   size_t lds;
   int in_bytes, out_bytes;  // algorithmic bytes per word
   double cost;              // estimated relative time per word
+  int na;                   // k_pfirstw: axes of the widening last round
+  int mid_w8;               // k_pstrip: LDS exchange format (one dword)
+  int moff[CC_MAXL], mbits[CC_MAXL];
 };
 
 struct PPlan {
@@ -214,9 +218,13 @@ struct Engine {
                              // slower on K_2(24,3): off by default)
   int compact = 1;           // CC_COMPACT=0: unpacked sweeps exchange int32 /
                              // uint16 layer planes instead of compact words
+  int pk_nt = ~1;            // CC_PK_NT: bit i = sweep i stores non-temporal
+  int pk_wide = 0;           // CC_PK_WIDE: the planner may take k_pfirstw /
+                             // k_pstrip sweeps
   int pk_first = 1;          // CC_PK_FIRST: first sweep 0 k_psweep, 1 persistent
                              // with prefetch, 2 LDS-free (cross-lane)
   PPlan pfield;              // packed plan of gain / loss (predicate seeds)
+
   // set-side gathers: exact when the walked table is exactly the metric
   // ball (proven at init / dct_set_tbl2); Z cached per cnt state
   int set_policy = 1;        // CC_SET_POLICY: 0 never, 1 auto, 2 always
@@ -1024,6 +1032,118 @@ bool pk_sweep(const PBounds &B, int j, int k, bool pred_seed, bool cnt_out,
 // shortest path over sweep boundaries (the compact format at a boundary
 // depends only on how many axes precede it, so a sweep's cost depends only
 // on its two ends).
+// Two sweep shapes that let a binary cell take fewer, larger sweeps
+// (kern 5 and 6, CC_PK_WIDE): a first sweep of 13-14 axes whose byte-format
+// LDS tile widens to 16-bit fields in its last round (k_pfirstw), and a
+// strided sweep of 2 RA axes in two register rounds through a one-dword
+// compact LDS tile (k_pstrip).  K_2(24,3): axes 0-13 and 14-23, 2 + 4 and
+// 4 + 4 bytes per word (the three-sweep schedule moves 22).
+bool pk_sweep_ext(const PBounds &B, int j, int k, bool pred_seed, bool cnt_out,
+                  const PSweep *prev, PSweep &w) {
+  const int q = B.Q, L = E.L, m = k - j;
+  if (!E.pk_wide || q != 2 || L > 4 || L < 3) return false;
+  memset(&w, 0, sizeof(w));
+  w.j0 = j; w.nax = m;
+  w.s = E.pw[j];
+  const long long P = E.pw[k] / E.pw[j];
+  if (P > (1LL << 20) || P * w.s > (1LL << 29)) return false;
+  w.P = (int)P;
+  if (!B.fits(16, k)) return false;
+  w.W = 16; w.ND = 2;
+  w.TL = 1;
+  auto add = [&](int ca, int mask, int C) {
+    const int r = w.nrounds;
+    const int qa = 1 << ca;
+    w.ca[r] = ca; w.amask[r] = mask; w.qa[r] = qa;
+    const long long K = qa;
+    if (K % 64 == 0) { w.lin[r] = 1; w.kp[r] = (int)(K + K / 64); }
+    else if (K == 1 && 64 % C == 0) { w.lin[r] = 1; w.kp[r] = 1; }
+    else { w.lin[r] = 0; w.kp[r] = 0; }
+    w.nrounds++;
+  };
+  if (j == 0) {
+    // k_pfirstw: 64-word cubes (RR = 6), byte rounds of 6 axes up to the
+    // last axis count whose layers fit bytes, then the top NA axes widened
+    if (m < 13 || m > 14 || !(pred_seed || E.sparse)) return false;
+    int kb = 0;
+    while (kb < m && B.fits(8, kb + 1)) kb++;
+    const int a0 = std::min(kb, m - 1);
+    w.na = m - a0;
+    if (w.na < 1 || w.na > 2 || a0 < m - 6) return false;
+    w.RR = 6;
+    const int C = 64;
+    for (int lo = 0; lo < a0; lo += 6) {
+      const int hi = std::min(lo + 6, a0);
+      const int ca = std::min(lo, m - 6);
+      add(ca, ((1 << (hi - lo)) - 1) << (lo - ca), C);
+    }
+    add(m - 6, ((1 << w.na) - 1) << (6 - w.na), C);
+    if (w.nrounds > CC_PK_MAXRD) return false;
+    w.threads = (int)(P / C);
+    w.lds = (size_t)(pk_pad_h(w.P) + 1) * 4;
+    if (w.lds > (size_t)E.lds_max) return false;
+    w.kern = 5;
+    w.ncolb = 1;
+    w.ntiles = E.X / P;
+  } else {
+    // k_pstrip: two rounds of RA axes, TL columns per tile, 512 threads
+    if (m % 2) return false;
+    const int RA = m / 2;
+    if (RA < 4 || RA > 5) return false;
+    const int C = 1 << RA;
+    w.RR = RA;
+    w.TL = 512 / C;
+    if (w.s % w.TL) return false;
+    int dw[CC_MAXL], nd = 0, nar = 0;
+    if (!compact_fmt(B, j + RA, L, dw, w.moff, w.mbits, &nd, &w.mid_w8, &nar) ||
+        nd != 1 || nar)
+      return false;
+    w.nrounds = 2;
+    w.threads = w.TL * C;
+    w.lds = (size_t)C * (C * w.TL + 16) * 4;
+    if (w.lds > (size_t)E.lds_max) return false;
+    w.kern = 6;
+    w.ncolb = w.s / w.TL;
+    w.ntiles = (E.X / (w.s * P)) * w.ncolb;
+  }
+  if (prev) {
+    w.in_nd = prev->out_nd;
+    w.in_w8 = prev->out_w8;
+    w.in_nar = prev->out_nar;
+    if (w.in_nar) return false;
+    for (int d = 0; d < L; d++) {
+      w.idw[d] = prev->odw[d]; w.ioff[d] = prev->ooff[d]; w.ibits[d] = prev->obits[d];
+    }
+    w.in_bytes = prev->out_bytes;
+  } else {
+    w.in_nd = 0;
+    w.in_bytes = pred_seed ? 2 : 0;
+  }
+  if (k == E.n) {
+    w.out_nd = 0;
+    w.out_bytes = cnt_out ? 2 : 4;
+  } else {
+    if (!compact_fmt(B, k, L, w.odw, w.ooff, w.obits, &w.out_nd, &w.out_w8,
+                     &w.out_nar))
+      return false;
+    if (w.out_nd > L - 1 || w.out_nar) return false;
+    w.out_bytes = 4 * w.out_nd;
+  }
+  {
+    PSweepArgs a;
+    memset(&a, 0, sizeof(a));
+    a.kern = w.kern;
+    a.na = w.na;
+    a.in_kind = prev ? CC_IN_PACKED : (pred_seed ? CC_IN_PRED : CC_IN_SPARSE);
+    a.out_kind = k == E.n ? (cnt_out ? CC_OUT_CNT : CC_OUT_MAP) : CC_OUT_PACKED;
+    a.in_nd = w.in_nd;
+    a.out_nd = w.out_nd;
+    if (pk_dispatch(w, a, 0) != hipSuccess) return false;
+  }
+  w.cost = (w.in_bytes + w.out_bytes) * (w.kern == 5 ? 1.25 : 1.1);
+  return true;
+}
+
 PPlan pk_plan(double s0, double cap, bool pred_seed, bool cnt_out) {
   PPlan best;
   if (!E.packed || E.prob != CC_HAMMING || E.L < 2) return best;
@@ -1064,10 +1184,17 @@ PPlan pk_plan(double s0, double cap, bool pred_seed, bool cnt_out) {
       if (getenv("CC_PK_DEBUG"))
         fprintf(stderr, "pk_plan [%d,%d) ok=%d kern=%d RR=%d W=%d ND=%d TL=%d cost=%.2f\n",
                 j, k, (int)okw, w.kern, w.RR, w.W, w.ND, w.TL, okw ? w.cost : -1.0);
-      if (!okw) continue;
-      const Cost c{dp[j].t + w.cost, dp[j].bytes + w.in_bytes + w.out_bytes,
-                   dp[j].sweeps + 1, true};
-      if (better(c, dp[k])) { dp[k] = c; via[k] = w; from[k] = j; }
+      if (okw) {
+        const Cost c{dp[j].t + w.cost, dp[j].bytes + w.in_bytes + w.out_bytes,
+                     dp[j].sweeps + 1, true};
+        if (better(c, dp[k])) { dp[k] = c; via[k] = w; from[k] = j; }
+      }
+      PSweep x;
+      if (pk_sweep_ext(B, j, k, pred_seed, cnt_out, j ? &prev : nullptr, x)) {
+        const Cost c{dp[j].t + x.cost, dp[j].bytes + x.in_bytes + x.out_bytes,
+                     dp[j].sweeps + 1, true};
+        if (better(c, dp[k])) { dp[k] = c; via[k] = x; from[k] = j; }
+      }
     }
   }
   if (!dp[n].ok) return best;
@@ -1110,7 +1237,8 @@ int run_ptransform(const PPlan &pl, int in_kind, int out_kind, int plo,
     a.s = w.s; a.ncolb = w.ncolb; a.ntiles = w.ntiles;
     a.P = w.P; a.TL = w.TL; a.nax = w.nax; a.L = E.L;
     a.kern = w.kern;
-    a.in_kind = i == 0 ? (in_kind == CC_IN_CODE && w.kern == 4 ? CC_IN_SPARSE : in_kind)
+    a.in_kind = i == 0 ? (in_kind == CC_IN_CODE && (w.kern == 4 || w.kern == 5)
+                              ? CC_IN_SPARSE : in_kind)
                        : CC_IN_PACKED;
     a.out_kind = i == ns - 1 ? out_kind : CC_OUT_PACKED;
     a.plo = plo; a.phi = phi;
@@ -1126,6 +1254,10 @@ int run_ptransform(const PPlan &pl, int in_kind, int out_kind, int plo,
     a.out_nar = w.out_nar;
     a.nar_off = (long long)(E.L - 2) * E.pitch;
     a.out_bmask = E.L >= 4 ? 0xFFFFFFFFu : (uint32_t)((1u << (8 * E.L)) - 1u);
+    a.st_nt = (E.pk_nt >> i) & 1;
+    a.na = w.na;
+    a.mid_w8 = w.mid_w8;
+    for (int d = 0; d < E.L; d++) { a.moff[d] = w.moff[d]; a.mbits[d] = w.mbits[d]; }
     // real-layer slots: layer d = slot * ND + dword (interleaved layout)
     const int S = 32 / w.W;
     for (int k = 0; k < w.ND; k++) {
@@ -1214,7 +1346,8 @@ PPlan pk_code_plan(bool cnt_out) {
   const double s0 = std::max(E.uploaded.maxm, 1), cap = std::max(E.codeM, 1);
   // the planner is a shortest path over every sweep boundary pair: keep its
   // result while the seed bounds and the schedule options stay the same
-  const int opts[8] = {(int)cnt_out, E.lds_budget, E.pk_col, E.pk_first, E.pk_narrow,
+  const int opts[8] = {(int)cnt_out, E.lds_budget, E.pk_col, E.pk_first,
+                       E.pk_narrow | (E.pk_wide << 4),
                        E.packed, E.sparse, (int)(g_cuts & 0x7FFFFFFF) ^ (int)(g_cuts >> 31)};
   if (E.cp_ok && E.cp_s0 == s0 && E.cp_cap == cap && !memcmp(opts, E.cp_opts, sizeof(opts)))
     return E.cp_plan;
@@ -1775,6 +1908,12 @@ static int init_cell(int problem, int n, const int *ax, int R,
     E.pk_col = pc ? atoi(pc) : 2;
     const char *pf = getenv("CC_PK_FIRST");
     E.pk_first = pf ? atoi(pf) : 1;
+    const char *pnt = getenv("CC_PK_NT");
+    // default: every sweep after the first (its words are read right away
+    // by the next sweep, which non-temporal stores measured slower for)
+    E.pk_nt = pnt ? (int)strtol(pnt, nullptr, 0) : ~1;
+    const char *pwd = getenv("CC_PK_WIDE");
+    E.pk_wide = pwd ? atoi(pwd) : 0;
     const char *pn = getenv("CC_PK_NARROW");
     E.pk_narrow = pn ? atoi(pn) : 0;
     const char *pcm = getenv("CC_COMPACT");
@@ -1905,7 +2044,7 @@ int dct_transform(int init_mode, int out_mode) {
   if (init_mode == 0) {
     in_kind = CC_IN_CODE;
     code_plan = pk_code_plan(out_kind == CC_OUT_CNT);
-    if (code_plan.ok && code_plan.sw[0].kern == 4) {
+    if (code_plan.ok && (code_plan.sw[0].kern == 4 || code_plan.sw[0].kern == 5)) {
       // the persistent first sweep seeds from the code's word list: no
       // plane fill, no indicator plane read
       if (int rc = stage_sparse_code(code_plan.sw[0])) return rc;
@@ -2553,6 +2692,8 @@ int dct_plan_probe(int n, const int *ax, int R, int kind, int s0, int M,
   E.pk_narrow = pn ? atoi(pn) : 0;
   const char *psc = getenv("CC_SPARSE_CODE");
   E.sparse = psc ? atoi(psc) : 1;
+  const char *pwd = getenv("CC_PK_WIDE");
+  E.pk_wide = pwd ? atoi(pwd) : 0;
   const PPlan pl = kind == 0 ? pk_plan(1.0, 1e300, true, false)
                              : pk_plan((double)s0, (double)M, false, true);
   int ns = 0;
@@ -2587,13 +2728,15 @@ int dct_plan_probe(int n, const int *ax, int R, int kind, int s0, int M,
 int dct_autotune(int reps, float *best_us) {
   if (!E.live || reps < 1) return -1;
   E.tuned = 1;
-  struct Opt { int lds, col, first, packed, narrow, cmp, gen; };
+  struct Opt { int lds, col, first, packed, narrow, cmp, gen, wide; };
   std::vector<Opt> opts;
   for (int lds : {16384, 32768, 65536, 98304, 131072}) {
     if (lds > E.lds_max) continue;
     for (int col : {2, 1, 0})
       for (int first : {1, 0, 2})
-        opts.push_back({lds, col, first, 1, E.pk_narrow, E.compact, E.generic_only});
+        for (int wide : {0, 1})
+          opts.push_back({lds, col, first, 1, E.pk_narrow, E.compact, E.generic_only,
+                          wide});
   }
   // the unpacked schedule under each LDS budget (its plan is rebuilt per
   // budget: tile sizes, sweep grouping and the workgroups a CU holds move
@@ -2603,9 +2746,10 @@ int dct_autotune(int reps, float *best_us) {
     if (lds <= E.lds_max)
       for (int gen : {0, 1})
         for (int cmp : {1, 0})
-          opts.push_back({lds, E.pk_col, E.pk_first, 0, E.pk_narrow, cmp, gen});
+          opts.push_back({lds, E.pk_col, E.pk_first, 0, E.pk_narrow, cmp, gen,
+                          E.pk_wide});
   const Opt saved{E.lds_budget, E.pk_col, E.pk_first, E.packed, E.pk_narrow,
-                  E.compact, E.generic_only};
+                  E.compact, E.generic_only, E.pk_wide};
   std::vector<std::string> seen;
   float best = 1e30f;
   Opt win = saved;
@@ -2615,6 +2759,7 @@ int dct_autotune(int reps, float *best_us) {
   for (const Opt &o : opts) {
     E.lds_budget = o.lds; E.pk_col = o.col; E.pk_first = o.first; E.packed = o.packed;
     E.pk_narrow = o.narrow;
+    E.pk_wide = o.wide;
     E.compact = o.cmp;
     E.generic_only = o.gen;
     PPlan pl = o.packed ? pk_plan(1.0, 1e300, true, false) : PPlan();
@@ -2664,6 +2809,7 @@ int dct_autotune(int reps, float *best_us) {
   E.lds_budget = win.lds; E.pk_col = win.col; E.pk_first = win.first;
   E.packed = win.packed;
   E.pk_narrow = win.narrow;
+  E.pk_wide = win.wide;
   E.compact = win.cmp;
   E.generic_only = win.gen;
   E.pfield = wplan;

@@ -392,8 +392,30 @@ def time_to_size(hist, size):
     return None
 
 
+def save_proposal(path, params, mu, sd, meta):
+    """The trained proposal as a numpy archive: network parameters, the
+    feature standardisation it was trained with, and a JSON note of how it
+    was trained (records, epochs, held-out AUC)."""
+    import json
+    np.savez(path, params=np.asarray(params, np.float32),
+             mu=np.asarray(mu, np.float32), sd=np.asarray(sd, np.float32),
+             meta=np.array(json.dumps(meta)))
+
+
+def load_proposal(path):
+    """(params, mu, sd, meta) of a proposal written by :func:`save_proposal`
+    (plain arrays only: nothing in the file is executed)."""
+    import json
+    with np.load(path, allow_pickle=False) as z:
+        params = z["params"].astype(np.float32)
+        if params.size != NPARAM:
+            raise ValueError(f"{path}: {params.size} parameters, expected {NPARAM}")
+        return params, z["mu"], z["sd"], json.loads(str(z["meta"]))
+
+
 def learned_ab(make_backend, train_cells, seeds, seconds, train_rounds=2000,
-               epochs=40, ncand=32, tau=0.25, log=None, records=None):
+               epochs=40, ncand=32, tau=0.25, log=None, records=None,
+               model=None, save=None):
     """Time-to-size of the learned ruin proposal against plain adaptive LNS.
 
     1. Move logs: plain LNS (fixed operator mix) on every cell of
@@ -407,7 +429,13 @@ def learned_ab(make_backend, train_cells, seeds, seconds, train_rounds=2000,
     Per seed: F = the smallest size plain reached, T_plain = when plain first
     reached F, T_learned = when the learned arm first reached F; the ratio
     T_learned / T_plain (< 1: the learned arm got there sooner; None when it
-    never did)."""
+    never did).
+
+    ``model``: a trained proposal (params, mu, sd, meta) from
+    :func:`load_proposal`; steps 1 and 2 are then skipped.  ``save``: write
+    the proposal trained in step 2 there."""
+    if model is not None:
+        return _ab_runs(make_backend, seeds, seconds, model, ncand, tau, log)
     t0 = time.perf_counter()
     if records is not None:
         X, t = records                       # a log collected beforehand
@@ -436,6 +464,28 @@ def learned_ab(make_backend, train_cells, seeds, seconds, train_rounds=2000,
         train_s = time.perf_counter() - t1
         score = net.forward(Xn[te])
         auc = rank_auc(score, y[te])
+        meta = {"records": int(len(X)), "improving": int(y.sum()),
+                "heldout_auc": None if auc is None else round(auc, 3),
+                "collect_s": round(collect_s, 2), "train_s": round(train_s, 2),
+                "epochs": int(epochs),
+                "train_loss": [round(float(losses[0]), 4), round(float(losses[-1]), 4)]}
+        params = net.get_params()
+        if save:
+            save_proposal(save, params, mu, sd, meta)
+    finally:
+        net.close()
+    if not seeds:
+        return meta
+    return _ab_runs(make_backend, seeds, seconds, (params, mu, sd, meta), ncand,
+                    tau, log)
+
+
+def _ab_runs(make_backend, seeds, seconds, model, ncand, tau, log):
+    """Step 3 of :func:`learned_ab` for a trained proposal ``model``."""
+    params, mu, sd, meta = model
+    net = Net(max_batch=max(1024, ncand), seed=7)
+    try:
+        net.set_params(params)
         runs = []
         for seed in seeds:
             code = start_code(make_backend, seed)
@@ -467,14 +517,12 @@ def learned_ab(make_backend, train_cells, seeds, seconds, train_rounds=2000,
                     f"{res['learned']['final_size']} (reached {F} at {tl}s)")
         ratios = [r["ratio"] for r in runs]
         done = [x for x in ratios if x is not None]
-        return {"records": int(len(X)), "improving": int(y.sum()),
-                "heldout_auc": None if auc is None else round(auc, 3),
-                "collect_s": round(collect_s, 2),
-                "train_s": round(train_s, 2),
-                "train_loss": [round(float(losses[0]), 4), round(float(losses[-1]), 4)],
-                "seconds_per_arm": seconds, "runs": runs,
-                "median_ratio": (float(np.median(done)) if len(done) == len(ratios)
-                                 else None),
-                "reached": f"{len(done)}/{len(ratios)}"}
+        out = dict(meta)
+        out.update({"seconds_per_arm": seconds, "runs": runs,
+                    "median_ratio": (float(np.median(done)) if len(done) == len(ratios)
+                                     else None),
+                    "reached": f"{len(done)}/{len(ratios)}"})
+        return out
     finally:
         net.close()
+

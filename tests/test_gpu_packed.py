@@ -234,3 +234,44 @@ def test_r4_schedule_kernels_match_walsh(gpu_cell, monkeypatch, case):
     assert (g.map_all().astype(np.int64) == gain).all()
     g.loss_map()
     assert (g.map_all().astype(np.int64) == loss).all()
+
+
+# ---- the two-sweep shapes (CC_PK_WIDE): k_pfirstw and k_pstrip ----------
+# A 13- or 14-axis first sweep whose byte tile widens to 16-bit fields in
+# its last round (kernel 6) and a strided sweep of 2 RA axes in two
+# register rounds through a compact LDS tile (kernel 7), against the
+# Walsh-Hadamard convolution; a set code (its recount takes the widening
+# first sweep too, seeded from the sorted word list) and mu = 2 predicates.
+WIDE = {"k2_22_3": (22, 3, 1, None, [(0, 12, 5), (12, 10, 7)]),
+        "k2_22_3_cut14": (22, 3, 1, [14], [(0, 14, 6), (14, 8, 7)]),
+        "k2_23_3": (23, 3, 1, None, [(0, 13, 6), (13, 10, 7)]),
+        "k2_22_3_cut14_mu2": (22, 3, 2, [14], [(0, 14, 6), (14, 8, 7)]),
+        "k2_24_3": (24, 3, 1, None, [(0, 14, 6), (14, 10, 7)])}
+
+
+@pytest.mark.parametrize("case", list(WIDE))
+def test_wide_schedule_kernels_match_walsh(gpu_cell, monkeypatch, case):
+    from oracle.walsh import binary_field
+    n, R, mu, cuts, rows = WIDE[case]
+    monkeypatch.setenv("CC_PK_WIDE", "1")
+    monkeypatch.setenv("CC_AUTOTUNE", "0")
+    print(f"two-sweep schedule {case}", flush=True)
+    rng = np.random.default_rng(7 + n)
+    code = np.flatnonzero(rng.random(1 << n) < 2e-4).astype(np.int64)
+    ind = np.zeros(1 << n, dtype=np.int64)
+    ind[code] = 1
+    cnt = binary_field(ind, n, R)
+    g = gpu_cell([2] * n, R, mu=mu)
+    if cuts:
+        g.sweep_force(cuts)
+    got = [(s["first_axis"], s["axes"], s["kernel"]) for s in g.sweep_plan2(0)]
+    assert got == rows
+    g.recount(code)
+    assert [s["kernel"] for s in g.sweep_plan2(1)] == [r[2] for r in rows]
+    assert (g.read_cnt().astype(np.int64) == cnt).all()
+    g.gain_map()
+    gain = binary_field((cnt < mu).astype(np.int64), n, R)
+    assert (g.map_all().astype(np.int64) == gain).all()
+    g.loss_map()
+    loss = binary_field(((cnt >= 1) & (cnt <= mu)).astype(np.int64), n, R)
+    assert (g.map_all().astype(np.int64) == loss).all()
