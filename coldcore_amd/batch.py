@@ -187,14 +187,15 @@ class GpuBatch:
         return dict(zip(keys, (int(v) for v in out)))
 
     def info(self):
-        out = (_LL * 6)()
+        out = (_LL * 7)()
         _check("ccb_info", self._lib.ccb_info(out))
         return dict(zip(("cells", "sweeps", "items", "levels", "workgroups",
-                         "lds_budget"), (int(v) for v in out)))
+                         "lds_budget", "threads"), (int(v) for v in out)))
 
     def autotune(self, reps=5):
-        """Time the batched gain transform under each LDS budget and keep
-        the fastest plan; returns (budget bytes, us per transform)."""
+        """Time the batched gain transform under each LDS budget and level
+        workgroup size and keep the fastest plan; returns (budget bytes, us
+        per transform)."""
         us = ctypes.c_float(0.0)
         r = self._lib.ccb_autotune(int(reps), ctypes.byref(us))
         if r < 0:

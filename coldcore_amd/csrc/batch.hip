@@ -45,9 +45,15 @@ struct Batch {
   std::vector<int> level_off;         // items of level l: [off[l], off[l+1])
   int *d_argcell = nullptr;
   int2 *d_items = nullptr;
+  // the items of the cells a greedy run still covers (ccb_greedy): level l
+  // is [open_off[l], open_off[l + 1]) of d_open
+  int2 *d_open = nullptr;
+  std::vector<int> open_off;
   int nargs = 0, nitems = 0, nlevels = 0;
   size_t lds = 0;
   int per_cu = 1, ncu = 1;
+  int threads = 512;                  // threads per level tile (CCB_THREADS,
+                                      // autotuned: 256, 512 or 1024)
   int budget = CCB_LDS;               // LDS budget the plan was built for
   std::vector<void *> plan_allocs;    // freed when the plan is rebuilt
   long long maxX = 0;
@@ -204,6 +210,13 @@ void plan_cell(int ci, const BCell &c) {
   }
 }
 
+// the level kernel for the batch's threads per tile
+const void *sweeps_kernel() {
+  if (B.threads == 1024) return (const void *)k_ccb_sweeps<1024>;
+  if (B.threads == 512) return (const void *)k_ccb_sweeps<512>;
+  return (const void *)k_ccb_sweeps<256>;
+}
+
 // The sweep plan of every cell for an LDS budget: sweeps, work items by
 // level, their device copies and the persistent grid.  Rebuilding drops a
 // captured graph (its launches hold the old plan).
@@ -266,24 +279,28 @@ int build_plan(int budget) {
       return -3;
   if (palloc((void **)&B.d_argcell, 4 * B.nargs) ||
       palloc((void **)&B.d_items, 8 * B.nitems) ||
+      palloc((void **)&B.d_open, 8 * B.nitems) ||
       hipMemcpy(B.d_argcell, B.argcell.data(), 4 * B.nargs, hipMemcpyHostToDevice) ||
       hipMemcpy(B.d_items, B.items.data(), 8 * B.nitems, hipMemcpyHostToDevice))
     return -3;
+  const void *kf = sweeps_kernel();
   if (B.lds > 48 * 1024)   // beyond the default dynamic LDS of a launch
-    CKB(hipFuncSetAttribute((const void *)k_ccb_sweeps,
-                            hipFuncAttributeMaxDynamicSharedMemorySize,
+    CKB(hipFuncSetAttribute(kf, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)B.lds + 1024));
   // persistent grids: what the CUs hold at once
   int per_cu = 1;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_ccb_sweeps,
-                                                   CC_BLOCK, B.lds) != hipSuccess ||
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kf, B.threads, B.lds) !=
+          hipSuccess ||
       per_cu < 1)
     per_cu = 1;
   B.per_cu = per_cu;
   return 0;
 }
 
-int transform_launch(int kind) {
+int reduce_launch(int with_map, bool reset);
+
+// open_only: the levels of d_open (cells a greedy run still works on)
+int transform_launch(int kind, bool open_only = false) {
   if (kind == 2) {
     const int gx = (int)std::min<long long>((B.maxX + 255) / 256, 1024);
     hipLaunchKernelGGL(k_ccb_zero_a0, dim3(gx, B.nc), dim3(256), 0, B.st, B.d);
@@ -293,20 +310,52 @@ int transform_launch(int kind) {
   }
   // one persistent launch per sweep level: at most what the CUs hold at
   // once, at most one workgroup per item
-  for (int l = 0; l < B.nlevels; l++) {
-    const int n = B.level_off[l + 1] - B.level_off[l];
+  const std::vector<int> &off = open_only ? B.open_off : B.level_off;
+  int2 *items = open_only ? B.d_open : B.d_items;
+  for (int l = 0; l + 1 < (int)off.size(); l++) {
+    const int n = off[l + 1] - off[l];
+    if (n == 0) continue;
     const int grid = std::max(1, std::min(B.per_cu * B.ncu, n));
-    hipLaunchKernelGGL(k_ccb_sweeps, dim3(grid), dim3(CC_BLOCK), B.lds, B.st,
-                       B.dargs[kind], B.d_argcell, B.d, B.d_items + B.level_off[l],
-                       n);
+    if (B.threads == 1024)
+      hipLaunchKernelGGL(k_ccb_sweeps<1024>, dim3(grid), dim3(1024), B.lds, B.st,
+                         B.dargs[kind], B.d_argcell, B.d, items + off[l], n);
+    else if (B.threads == 512)
+      hipLaunchKernelGGL(k_ccb_sweeps<512>, dim3(grid), dim3(512), B.lds, B.st,
+                         B.dargs[kind], B.d_argcell, B.d, items + off[l], n);
+    else
+      hipLaunchKernelGGL(k_ccb_sweeps<256>, dim3(grid), dim3(256), B.lds, B.st,
+                         B.dargs[kind], B.d_argcell, B.d, items + off[l], n);
     CKB(hipGetLastError());
   }
   return 0;
 }
 
+// d_open / open_off: the items of the cells that still have an uncovered
+// word (from a fresh reduction), in the level order of the full list
+int build_open_items(std::vector<char> &open) {
+  if (int rc = reduce_launch(0, true)) return rc;
+  CKB(hipStreamSynchronize(B.st));
+  std::vector<BCell> h(B.nc);
+  CKB(hipMemcpy(h.data(), B.d, sizeof(BCell) * B.nc, hipMemcpyDeviceToHost));
+  open.assign(B.nc, 0);
+  // (nzero counts cnt == 0: with mu > 1 a cell stays open while any word
+  // may still be deficient)
+  for (int i = 0; i < B.nc; i++) open[i] = h[i].nzero != 0ull || h[i].mu > 1;
+  std::vector<int2> sel;
+  B.open_off.assign(1, 0);
+  for (int l = 0; l < B.nlevels; l++) {
+    for (int i = B.level_off[l]; i < B.level_off[l + 1]; i++)
+      if (open[B.argcell[B.items[i].x]]) sel.push_back(B.items[i]);
+    B.open_off.push_back((int)sel.size());
+  }
+  if (!sel.empty())
+    CKB(hipMemcpy(B.d_open, sel.data(), 8 * sel.size(), hipMemcpyHostToDevice));
+  return 0;
+}
+
 // reset = false when the previous control kernel (k_ccb_ruin, k_ccb_add)
 // has already cleared every cell's accumulators
-int reduce_launch(int with_map, bool reset = true) {
+int reduce_launch(int with_map, bool reset) {
   if (reset) {
     hipLaunchKernelGGL(k_ccb_reset_red, dim3((B.nc + 63) / 64), dim3(64), 0, B.st,
                        B.d, B.nc);
@@ -481,11 +530,15 @@ int ccb_init(int ncells, const int *problem, const int *n, const int *axes,
     (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
     B.ncu = std::max(ncu, 1);
   }
+  if (const char *nt = getenv("CCB_THREADS")) {
+    const int t = atoi(nt);
+    B.threads = t == 1024 ? 1024 : (t == 256 ? 256 : 512);
+  }
   // CCB_LDS (bytes) overrides the initial LDS budget per tile, within
   // [4 KiB, 64 KiB]; the autotuner may replace it
   int budget0 = CCB_LDS;
   if (const char *lb = getenv("CCB_LDS"))
-    budget0 = std::min(64 * 1024, std::max(4096, atoi(lb)));
+    budget0 = std::min(128 * 1024, std::max(4096, atoi(lb)));
   if (build_plan(budget0)) { bfree_all(); return -3; }
   // the set-up copies and clears ran outside the batch's (non-blocking)
   // stream: finish them before its first launch
@@ -544,7 +597,7 @@ int ccb_read_map(int cell, int32_t *out) {
 // feasible
 int ccb_lns_init(unsigned long long seed) {
   if (!B.live) return -1;
-  if (int rc = reduce_launch(0)) return rc;
+  if (int rc = reduce_launch(0, true)) return rc;
   hipLaunchKernelGGL(k_ccb_lns_init, dim3((B.nc + 63) / 64), dim3(64), 0, B.st,
                      B.d, B.nc, seed);
   CKB(hipGetLastError());
@@ -595,13 +648,17 @@ int ccb_lns_run(int rounds, int k, int graph, float *ms) {
 // still uncovered after the steps (>= 0).
 int ccb_greedy(int steps) {
   if (!B.live || steps < 0) return -1;
+  // only the cells that still have an uncovered word are transformed (a
+  // covered cell has no positive gain, so its step would add nothing)
+  std::vector<char> todo;
+  if (int rc = build_open_items(todo)) return rc;
   // each step's gain transform folds the argmax into amax (cleared here
   // once, then by every k_ccb_add), so a step launches no reduction
   hipLaunchKernelGGL(k_ccb_reset_red, dim3((B.nc + 63) / 64), dim3(64), 0, B.st,
                      B.d, B.nc);
   CKB(hipGetLastError());
   for (int s = 0; s < steps; s++) {
-    if (int rc = transform_launch(0)) return rc;
+    if (int rc = transform_launch(0, true)) return rc;
     hipLaunchKernelGGL(k_ccb_add, dim3((B.nc + 63) / 64), dim3(64), 0, B.st, B.d,
                        B.nc, -1, B.d_add);
     CKB(hipGetLastError());
@@ -609,7 +666,7 @@ int ccb_greedy(int steps) {
                        B.d, B.d_add, CCB_WSPLIT, 1, 1);
     CKB(hipGetLastError());
   }
-  if (int rc = reduce_launch(0)) return rc;
+  if (int rc = reduce_launch(0, true)) return rc;
   if (int rc = sync_batch()) return rc;
   std::vector<BCell> h(B.nc);
   CKB(hipMemcpy(h.data(), B.d, sizeof(BCell) * B.nc, hipMemcpyDeviceToHost));
@@ -659,12 +716,15 @@ int ccb_info(long long *out) {
   out[3] = B.nlevels;
   out[4] = (long long)B.per_cu * B.ncu;
   out[5] = B.budget;
+  out[6] = B.threads;
   return 0;
 }
 
-// Time the batched gain transform under each LDS budget (8, 16, 32, 64 KiB
-// per tile: the axis grouping, tile widths and persistent grid follow from
-// it) and keep the fastest plan.  reps timed transforms per budget after
+// Time the batched gain transform under each LDS budget (8 to 128 KiB per
+// tile: the axis grouping, tile widths and persistent grid follow from it)
+// and each workgroup size of the level launches (256, 512, 1,024 threads
+// on one tile: more threads shorten the longest tiles, which set a level's
+// length) and keep the fastest plan.  reps timed transforms per plan after
 // one warm-up; the map planes are overwritten, cnt is not.  Returns the
 // chosen budget (bytes); *best_us = its time per transform.
 int ccb_autotune(int reps, float *best_us) {
@@ -673,8 +733,14 @@ int ccb_autotune(int reps, float *best_us) {
   CKB(hipEventCreate(&e0));
   CKB(hipEventCreate(&e1));
   float best = 1e30f;
-  int win = B.budget;
-  for (int budget : {8192, 16384, 32768, 65536}) {
+  int win = B.budget, wthreads = B.threads;
+  int dev = 0, lmax = 64 * 1024;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&lmax, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+  for (int threads : {256, 512, 1024})
+  for (int budget : {8192, 16384, 32768, 65536, 131072}) {
+    if (budget + 1024 > lmax) continue;
+    B.threads = threads;
     if (int rc = build_plan(budget)) return rc;
     if (int rc = transform_launch(0)) return rc;
     CKB(hipEventRecord(e0, B.st));
@@ -684,10 +750,11 @@ int ccb_autotune(int reps, float *best_us) {
     CKB(hipEventSynchronize(e1));
     float ms = 0;
     CKB(hipEventElapsedTime(&ms, e0, e1));
-    if (ms < best) { best = ms; win = budget; }
+    if (ms < best) { best = ms; win = budget; wthreads = threads; }
   }
   CKB(hipEventDestroy(e0));
   CKB(hipEventDestroy(e1));
+  B.threads = wthreads;
   if (int rc = build_plan(win)) return rc;
   if (best_us) *best_us = best * 1000.0f / reps;
   return win;

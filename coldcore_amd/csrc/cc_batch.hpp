@@ -119,8 +119,10 @@ __device__ __forceinline__ void ccb_tile(const SweepArgs &a, const BCell &c,
   }
 }
 
-// items[i] = (sweep argument index, tile), all of one sweep level
-__global__ void __launch_bounds__(CC_BLOCK)
+// items[i] = (sweep argument index, tile), all of one sweep level; NT
+// threads per workgroup work on one tile
+template <int NT>
+__global__ void __launch_bounds__(NT)
     k_ccb_sweeps(const SweepArgs *__restrict__ args, const int *__restrict__ argcell,
                  const BCell *__restrict__ cells, const int2 *__restrict__ items,
                  int nitems) {

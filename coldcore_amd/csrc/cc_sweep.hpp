@@ -94,7 +94,7 @@ __device__ __forceinline__ void cc_step_r(int32_t *__restrict__ S, int PT,
                                           int TL, int nf, int sk, int lv,
                                           int lw, int R) {
   const uint32_t mT = cc_magic((uint32_t)TL), mS = cc_magic((uint32_t)sk);
-  for (int fi = threadIdx.x; fi < nf; fi += CC_BLOCK) {
+  for (int fi = threadIdx.x; fi < nf; fi += blockDim.x) {
     const int f = cc_div(fi, mT), tl = fi - f * TL;
     const int hi = cc_div(f, mS), lo = f - hi * sk;
     const int e0 = (hi * sk * Q + lo) * TL + tl;

@@ -2814,6 +2814,30 @@ int dct_autotune(int reps, float *best_us) {
   E.generic_only = win.gen;
   E.pfield = wplan;
   E.cp_ok = false;
+  if (wplan.ok && !getenv("CC_PK_NT")) {
+    // the winner's stores: non-temporal after the first sweep, or plain
+    float tnt[2] = {0, 0};
+    const int masks[2] = {~1, 0};
+    const int prof = E.prof;
+    E.prof = 0;
+    for (int v = 0; v < 2; v++) {
+      E.pk_nt = masks[v];
+      for (int r = 0; r <= reps; r++) {
+        if (r == 1) CK(hipEventRecord(e0, E.st));
+        if (int rc = run_ptransform(wplan, CC_IN_PRED, CC_OUT_MAP, 0, E.mu - 1)) {
+          E.prof = prof;
+          return rc;
+        }
+      }
+      CK(hipEventRecord(e1, E.st));
+      CK(hipEventSynchronize(e1));
+      CK(hipEventElapsedTime(&tnt[v], e0, e1));
+      timed++;
+    }
+    E.prof = prof;
+    E.pk_nt = tnt[1] < tnt[0] ? masks[1] : masks[0];
+    best = std::min(best, std::min(tnt[0], tnt[1]));
+  }
   build_plan();   // the unpacked plan of the winning budget
   if (best_us) *best_us = best * 1000.0f / reps;
   return timed;

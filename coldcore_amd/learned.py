@@ -234,10 +234,11 @@ class LearnedProposer:
     """Scores ``ncand`` cluster regions with the network and ruins the best
     (ties and exploration by a softmax at temperature ``tau``)."""
 
-    def __init__(self, net, scorer, ncand=32, tau=0.25, norm=None):
+    def __init__(self, net, scorer, ncand=32, tau=0.25, norm=None, prior=2.0):
         self.net, self.scorer = net, scorer
         self.ncand, self.tau = ncand, tau
         self.norm = norm          # (mean, std) the network was trained with
+        self.prior = prior        # the operator bandit's prior weight for it
 
     def __call__(self, searcher, k, mode=None):
         s = searcher
@@ -415,7 +416,7 @@ def load_proposal(path):
 
 def learned_ab(make_backend, train_cells, seeds, seconds, train_rounds=2000,
                epochs=40, ncand=32, tau=0.25, log=None, records=None,
-               model=None, save=None):
+               model=None, save=None, prior=2.0):
     """Time-to-size of the learned ruin proposal against plain adaptive LNS.
 
     1. Move logs: plain LNS (fixed operator mix) on every cell of
@@ -435,7 +436,8 @@ def learned_ab(make_backend, train_cells, seeds, seconds, train_rounds=2000,
     :func:`load_proposal`; steps 1 and 2 are then skipped.  ``save``: write
     the proposal trained in step 2 there."""
     if model is not None:
-        return _ab_runs(make_backend, seeds, seconds, model, ncand, tau, log)
+        return _ab_runs(make_backend, seeds, seconds, model, ncand, tau, log,
+                        prior=prior)
     t0 = time.perf_counter()
     if records is not None:
         X, t = records                       # a log collected beforehand
@@ -477,10 +479,10 @@ def learned_ab(make_backend, train_cells, seeds, seconds, train_rounds=2000,
     if not seeds:
         return meta
     return _ab_runs(make_backend, seeds, seconds, (params, mu, sd, meta), ncand,
-                    tau, log)
+                    tau, log, prior=prior)
 
 
-def _ab_runs(make_backend, seeds, seconds, model, ncand, tau, log):
+def _ab_runs(make_backend, seeds, seconds, model, ncand, tau, log, prior=2.0):
     """Step 3 of :func:`learned_ab` for a trained proposal ``model``."""
     params, mu, sd, meta = model
     net = Net(max_batch=max(1024, ncand), seed=7)
@@ -496,7 +498,7 @@ def _ab_runs(make_backend, seeds, seconds, model, ncand, tau, log):
                     s = Searcher(b, seed=seed)
                     s.load(code)
                     prop = (LearnedProposer(net, RegionScorer(b), ncand=ncand,
-                                            tau=tau, norm=(mu, sd))
+                                            tau=tau, norm=(mu, sd), prior=prior)
                             if arm == "learned" else None)
                     hist = descend_timed(s, seconds, proposer=prop)
                     res[arm] = {"final_size": int(min(m for _, m in hist)),
@@ -518,6 +520,7 @@ def _ab_runs(make_backend, seeds, seconds, model, ncand, tau, log):
         ratios = [r["ratio"] for r in runs]
         done = [x for x in ratios if x is not None]
         out = dict(meta)
+        out.update({"ncand": ncand, "tau": tau, "prior": prior})
         out.update({"seconds_per_arm": seconds, "runs": runs,
                     "median_ratio": (float(np.median(done)) if len(done) == len(ratios)
                                      else None),

@@ -426,7 +426,8 @@ class Searcher:
         ops, prior = (RUIN_MODES_EXT, RUIN_MIX_EXT) if deficit else (RUIN_MODES, RUIN_MIX)
         if proposer is not None:
             draw = draw + ("learned", "learned")
-            ops, prior = ops + ("learned",), dict(prior, learned=2.0)
+            ops = ops + ("learned",)
+            prior = dict(prior, learned=float(getattr(proposer, "prior", 2.0)))
         bandit = None
         if select == "adaptive":
             bandit = (OperatorBandit(self.rng, ops=ops, prior=prior)

@@ -75,13 +75,15 @@ int ccb_get_code(int cell, int which, long long *out, int cap);
  * rounds, accepted rounds, reverted rounds */
 int ccb_stats(int cell, long long *out);
 
-/* out[0..5] = cells, sweeps, work items, sweep levels (persistent launches
- * per transform), workgroups per launch at most, LDS budget of the plan */
+/* out[0..6] = cells, sweeps, work items, sweep levels (persistent launches
+ * per transform), workgroups per launch at most, LDS budget of the plan,
+ * threads per workgroup of the level launches */
 int ccb_info(long long *out);
 
 /* the batch's schedule autotuner: times reps batched gain transforms under
- * each LDS budget (8, 16, 32, 64 KiB per tile; axis grouping, tile widths
- * and the persistent grid follow from it) and keeps the fastest plan.
+ * each LDS budget (8 to 128 KiB per tile; axis grouping, tile widths and
+ * the persistent grid follow from it) and each level-launch workgroup size
+ * (256, 512, 1,024 threads per tile) and keeps the fastest plan.
  * Overwrites the map planes (cnt unchanged) and drops a captured graph.
  * Returns the chosen budget in bytes; *best_us = its time per transform. */
 int ccb_autotune(int reps, float *best_us);

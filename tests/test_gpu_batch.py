@@ -59,11 +59,15 @@ def _codes(cells, frac=0.02, seed=0):
     return out
 
 
-def test_batched_transform_matches_oracle():
-    """Recount, gain and loss of every cell from one launch each."""
+@pytest.mark.parametrize("threads", ["256", "512", "1024"])
+def test_batched_transform_matches_oracle(monkeypatch, threads):
+    """Recount, gain and loss of every cell from one launch each (level
+    launches of 256, 512 and 1,024 threads per tile)."""
+    monkeypatch.setenv("CCB_THREADS", threads)
     codes = _codes(BATCH_CELLS)
     with _batch(BATCH_CELLS) as b:
         info = b.info()
+        assert info["threads"] == int(threads)
         assert info["cells"] == len(BATCH_CELLS)
         assert info["sweeps"] > len(BATCH_CELLS)     # multi-sweep cells
         assert info["levels"] >= 2
@@ -99,8 +103,9 @@ def test_batch_autotune_keeps_fields_exact():
             b.set_code(i, c)
         b.recount()
         budget, us = b.autotune(reps=2)
-        assert budget in (8192, 16384, 32768, 65536) and us > 0
+        assert budget in (8192, 16384, 32768, 65536, 131072) and us > 0
         assert b.info()["lds_budget"] == budget
+        assert b.info()["threads"] in (256, 512, 1024)
         b.recount()
         for which in ("recount", "gain", "loss"):
             if which == "gain":
