@@ -306,6 +306,48 @@ def run_batch64(GpuBackend, rounds=200, k=2, quick=False):
     return out
 
 
+def run_batch64_cells(top=16):
+    """Where the 64-cell batched transform's time goes: the autotuned
+    batched gain pass of each family's cells on their own, and of every cell
+    alone (its critical path inside the full batch), slowest first (starts:
+    |X| / 96 random words per cell, seed = cell index)."""
+    from coldcore_amd.batch import GpuBatch
+    cells = batch_cells64()
+
+    def run(sub):
+        with GpuBatch([c for _, c in sub], cap=65535) as b:
+            for j, (i, (_, a, _)) in enumerate(sub):
+                size = int(np.prod(a))
+                b.set_code(j, np.random.default_rng(i).choice(
+                    size, max(2, size // 96), replace=False))
+            b.recount()
+            b.autotune(reps=5)
+            b.gain_maps()
+            t = time.perf_counter()
+            for _ in range(20):
+                b.gain_maps()
+            info = b.info()
+            return round((time.perf_counter() - t) * 5e4, 1), info
+
+    indexed = list(enumerate(cells))
+    out = {"families": {}, "cells": []}
+    for name, fams in (("all", None), ("hamming", ("hamming",)),
+                       ("grid", ("grid_linf",)), ("torus+lee", ("torus_linf", "lee"))):
+        sub = [x for x in indexed if fams is None or x[1][0] in fams]
+        us, info = run(sub)
+        out["families"][name] = {"cells": len(sub), "us": us, "levels": info["levels"],
+                                 "threads": info["threads"],
+                                 "lds_budget": info["lds_budget"]}
+    rows = []
+    for x in indexed:
+        us, info = run([x])
+        rows.append({"cell": x[0], "problem": x[1][0], "axes": x[1][1], "R": x[1][2],
+                     "us": us, "levels": info["levels"], "threads": info["threads"],
+                     "lds_budget": info["lds_budget"]})
+    out["cells"] = sorted(rows, key=lambda r: -r["us"])[:top]
+    return out
+
+
 def run_config2_cell(GpuBackend, axes, R, steps=10):
     """The plain-transform recipe on one cell: code = flatnonzero(
     default_rng(0).random(|X|) < 1e-4); gain, loss and recount passes
@@ -560,16 +602,27 @@ def copy_calibration(mib):
     assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(n)) == 0
     assert hip.hipMemset(a, 1, ctypes.c_size_t(n)) == 0
     assert hip.hipDeviceSynchronize() == 0
-    times = []
-    for _ in range(3):
-        t = time.perf_counter()
-        assert hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3) == 0   # D2D
-        assert hip.hipDeviceSynchronize() == 0
-        times.append(time.perf_counter() - t)
+    # 3 untimed copies, then 20 between HIP events on the null stream (a
+    # copy of a few tens of MiB takes about as long as a host round trip)
+    e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipEventCreate(ctypes.byref(e0)) == 0
+    assert hip.hipEventCreate(ctypes.byref(e1)) == 0
+    for r in range(23):
+        if r == 3:
+            assert hip.hipEventRecord(e0, None) == 0
+        assert hip.hipMemcpyAsync(b, a, ctypes.c_size_t(n), 3, None) == 0   # D2D
+    assert hip.hipEventRecord(e1, None) == 0
+    assert hip.hipEventSynchronize(e1) == 0
+    ms = ctypes.c_float(0.0)
+    assert hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1) == 0
+    hip.hipEventDestroy(e0)
+    hip.hipEventDestroy(e1)
     hip.hipFree(a)
     hip.hipFree(b)
-    print(json.dumps({"calib_copy_bytes": n, "copies": 3,
-                      "copy_GBps_read_plus_write": round(2 * n / min(times) / 1e9, 1)}),
+    us = ms.value * 1e3 / 20
+    print(json.dumps({"calib_copy_bytes": n, "copies": 23, "timed_copies": 20,
+                      "copy_us": round(us, 2),
+                      "copy_GBps_read_plus_write": round(2 * n / (us * 1e-6) / 1e9, 1)}),
           flush=True)
 
 
@@ -650,6 +703,7 @@ def main():
                          "(3 times): the reference kernel for scaling the "
                          "FETCH_SIZE / WRITE_SIZE counters")
     ap.add_argument("--only", choices=("near_cover", "batch64", "batch64-quick",
+                                       "batch64-cells",
                                        "cells", "learned", "learned-1m", "walks",
                                        "cpu_calib", "config5",
                                        "learned-train"),
@@ -674,6 +728,8 @@ def main():
             r = run_learned(GpuBackend, seconds=30.0, rounds=12500, epochs=8,
                             per_cell=40,
                             log=lambda m: print(m, file=sys.stderr, flush=True))
+        elif args.only == "batch64-cells":
+            r = run_batch64_cells()
         elif args.only == "learned-train":
             # the saved proposal: 10^6 logged rounds, trained, no A/B
             r = run_learned(GpuBackend, seeds=(), rounds=12500, epochs=8,
