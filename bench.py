@@ -30,6 +30,14 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0        # MI355X HBM3E, published
 
+#: the proposal network trained on 10^6 logged plain-LNS rounds (half
+#: K_3(10,2), half K_4(8,2)) by the engine's own MLP kernels:
+#: ``python bench.py --only learned-train`` writes it
+PROPOSAL = os.path.join(ROOT, "coldcore_amd", "proposal_k3_10_2.npz")
+#: the proposal for radius-4 binary cells (banded windows), trained on plain-
+#: LNS logs of K_2(20,4): ``python bench.py --only learned-train-r4``
+PROPOSAL_R4 = os.path.join(ROOT, "coldcore_amd", "proposal_k2_r4.npz")
+
 
 def sweep_bytes(plan):
     """Algorithmic bytes per word of each sweep of one pass, from the
@@ -379,7 +387,7 @@ def run_config2_cell(GpuBackend, axes, R, steps=10):
 
 
 def run_config5(GpuBackend, streams=8, slice_rounds=1, max_restarts=4,
-                ruin_frac=0.01, kmax=64):
+                ruin_frac=0.01, kmax=64, proposal=PROPOSAL_R4):
     """The largest single-GPU space: K_2(30,4), 2^30 words, the seeded
     40,000-word random start (default_rng(0).choice); its recount and the
     full gain / loss field passes, then the restart portfolio at M = 40,000
@@ -415,6 +423,38 @@ def run_config5(GpuBackend, streams=8, slice_rounds=1, max_restarts=4,
         out["incumbent_uncovered"] = int(u)
         out["incumbent_words"] = int(len(s.code))
         out["recount_check"] = bool(s.verify_full() == u)
+        if proposal and os.path.exists(proposal):
+            # the same portfolio from the same start with the learned
+            # proposal as one more arm of each stream's adaptive operator
+            # bandit (the exact fields judge every move)
+            from coldcore_amd.learned import (LearnedProposer, Net, RegionScorer,
+                                              load_proposal)
+            params, mu, sd, meta = load_proposal(proposal)
+            net = Net(max_batch=1024, seed=7)
+            try:
+                net.set_params(params)
+                s = Searcher(g, seed=1)
+                s.load(start)
+                prop = LearnedProposer(net, RegionScorer(g), norm=(mu, sd))
+                t = time.perf_counter()
+                u = s.portfolio(len(start), streams=streams,
+                                slice_rounds=slice_rounds,
+                                max_restarts=max_restarts, kmin=8, kmax=kmax,
+                                ruin_frac=ruin_frac, select="adaptive",
+                                proposer=prop)
+                out["learned"] = {
+                    "proposal": os.path.relpath(proposal, ROOT),
+                    "trained_on": meta.get("cells"), "records": meta.get("records"),
+                    "heldout_auc": meta.get("heldout_auc"),
+                    "portfolio_s": round(time.perf_counter() - t, 2),
+                    "lns_rounds": int(s.stats["rounds"]),
+                    "learned_rounds": int(s.stats.get("learned_rounds", 0)),
+                    "restarts": int(s.stats["restarts"]),
+                    "incumbent_uncovered": int(u),
+                    "incumbent_words": int(len(s.code)),
+                    "recount_check": bool(s.verify_full() == u)}
+            finally:
+                net.close()
     finally:
         g.close()
     return out
@@ -531,10 +571,6 @@ def run_near_cover(GpuBackend):
     return out
 
 
-#: the proposal network trained on 10^6 logged plain-LNS rounds (half
-#: K_3(10,2), half K_4(8,2)) by the engine's own MLP kernels:
-#: ``python bench.py --only learned-train`` writes it
-PROPOSAL = os.path.join(ROOT, "coldcore_amd", "proposal_k3_10_2.npz")
 
 
 def run_learned_saved(GpuBackend, path=PROPOSAL, seconds=10.0,
@@ -589,6 +625,21 @@ def run_learned(GpuBackend, seconds=10.0, seeds=(0, 1, 2, 3, 4), rounds=12500,
     if not seeds:
         return out
     return _ab_summary(out)
+
+
+def run_learned_train(cells, per_cell, rounds, save, workers=8, epochs=8,
+                      log=None):
+    """Move logs of plain LNS on ``cells`` ((axes, R) pairs, ``per_cell``
+    logging jobs of ``rounds`` rounds each, ``workers`` processes), the
+    network trained on them on the GPU, saved to ``save`` (no A/B)."""
+    from coldcore_amd.learned import collect_records_parallel, learned_ab
+    t = time.perf_counter()
+    X, y = collect_records_parallel(cells, per_cell, rounds, workers=workers, log=log)
+    collect_s = time.perf_counter() - t
+    meta = learned_ab(None, [], [], 0.0, epochs=epochs, records=(X, y), save=save,
+                      meta_extra={"cells": [[list(a), int(R)] for a, R in cells],
+                                  "collect_s": round(collect_s, 2)})
+    return meta
 
 
 def copy_calibration(mib):
@@ -706,7 +757,7 @@ def main():
                                        "batch64-cells",
                                        "cells", "learned", "learned-1m", "walks",
                                        "cpu_calib", "config5",
-                                       "learned-train"),
+                                       "learned-train", "learned-train-r4"),
                     help="run one side section and print its JSON only")
     ap.add_argument("--out", help="learned-train: where to write the proposal "
                     "(default: the package's saved proposal)")
@@ -730,6 +781,11 @@ def main():
                             log=lambda m: print(m, file=sys.stderr, flush=True))
         elif args.only == "batch64-cells":
             r = run_batch64_cells()
+        elif args.only == "learned-train-r4":
+            # the radius-4 proposal: plain-LNS logs of K_2(20,4)
+            r = run_learned_train([([2] * 20, 4)], per_cell=16, rounds=12500,
+                                  save=args.out or PROPOSAL_R4,
+                                  log=lambda m: print(m, file=sys.stderr, flush=True))
         elif args.only == "learned-train":
             # the saved proposal: 10^6 logged rounds, trained, no A/B
             r = run_learned(GpuBackend, seeds=(), rounds=12500, epochs=8,

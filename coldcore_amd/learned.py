@@ -125,6 +125,20 @@ def init_params(seed, in_dim=IN):
     return p
 
 
+def band_features(F, R):
+    """Five (R+1)-layer windows as the network's inputs: for R > 2 each
+    window's layers become three bands -- layer 0, the mean of layers
+    1..R-1, layer R -- so every radius gives 15 inputs; R <= 2 is returned
+    unchanged (5 (R+1) <= 15).  Applied alike to the device and the host
+    windows, so both paths feed the network the same numbers."""
+    F = np.asarray(F, dtype=np.float32)
+    if R <= 2:
+        return F
+    W = F.reshape(len(F), 5, R + 1)
+    out = np.stack([W[:, :, 0], W[:, :, 1:R].mean(axis=2), W[:, :, R]], axis=2)
+    return out.reshape(len(F), 15).astype(np.float32)
+
+
 def shell_sizes(axes, R, problem="hamming"):
     """Words at exact distance d from any word, d = 0..R (Hamming: the
     elementary symmetric polynomials of a_j - 1)."""
@@ -143,8 +157,6 @@ class RegionScorer:
         b = backend
         self.b = b
         self.R = int(b.R)
-        if 5 * (self.R + 1) > IN:
-            raise ValueError("five (R+1)-layer windows must fit 16 inputs (R <= 2)")
         self.shell = shell_sizes(b.space.axes, self.R)
 
     def layers(self, code):
@@ -156,23 +168,29 @@ class RegionScorer:
         return G, L, C
 
     def region_features(self, code, regions):
-        """(regions, 5 (R+1)) features of equal-size regions (positions into
+        """(regions, inputs) features of equal-size regions (positions into
         ``code``): on the engine one device call (dct_region_features: the
         three layer transforms and the windows, one download); on a host
-        backend the layers at the code and the windows in numpy."""
+        backend the layers at the code and the windows in numpy; then
+        :func:`band_features`."""
         W = np.asarray(regions, dtype=np.int64)
         if hasattr(self.b, "region_features"):
-            return self.b.region_features(code, np.asarray(code, dtype=np.int64)[W])
+            F = self.b.region_features(code, np.asarray(code, dtype=np.int64)[W])
+            return band_features(F, self.R)
         return self.features(self.layers(code), W)
 
-    def features(self, lay, regions):
-        """(regions, 5 (R+1)) features of equal-size regions (positions
-        into the code), all regions at once."""
+    def windows(self, lay, regions):
+        """(regions, 5 (R+1)) windows of equal-size regions (positions into
+        the code), all regions at once."""
         G, L, C = lay
         W = np.asarray(regions, dtype=np.int64)          # (regions, k)
         Lw, Gw, Cw = L[W], G[W], C[W]                    # (regions, k, R+1)
         return np.concatenate([Lw.mean(1), Gw.mean(1), Gw.max(1), Cw.mean(1),
                                Lw.min(1)], axis=1).astype(np.float32)
+
+    def features(self, lay, regions):
+        """The network's inputs of equal-size regions: the windows, banded."""
+        return band_features(self.windows(lay, regions), self.R)
 
 
 _M64 = (1 << 64) - 1
@@ -249,6 +267,7 @@ class LearnedProposer:
         if hasattr(s.b, "region_propose"):
             # regions picked and featurised on the device, one call
             regions, F = s.b.region_propose(s.code, anchors, k, seed)
+            F = band_features(F, self.scorer.R)
         else:
             regions = cluster_regions(s.b.space, s.code, k, anchors, seed)
             F = self.scorer.region_features(s.code, regions)
@@ -416,7 +435,7 @@ def load_proposal(path):
 
 def learned_ab(make_backend, train_cells, seeds, seconds, train_rounds=2000,
                epochs=40, ncand=32, tau=0.25, log=None, records=None,
-               model=None, save=None, prior=2.0):
+               model=None, save=None, prior=2.0, meta_extra=None):
     """Time-to-size of the learned ruin proposal against plain adaptive LNS.
 
     1. Move logs: plain LNS (fixed operator mix) on every cell of
@@ -471,6 +490,7 @@ def learned_ab(make_backend, train_cells, seeds, seconds, train_rounds=2000,
                 "collect_s": round(collect_s, 2), "train_s": round(train_s, 2),
                 "epochs": int(epochs),
                 "train_loss": [round(float(losses[0]), 4), round(float(losses[-1]), 4)]}
+        meta.update(meta_extra or {})
         params = net.get_params()
         if save:
             save_proposal(save, params, mu, sd, meta)

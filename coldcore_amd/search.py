@@ -464,6 +464,8 @@ class Searcher:
                 pick = np.flatnonzero(~np.isin(before, self.code))
             self.greedy_fill(M)
             self.stats["rounds"] += 1
+            if mode == "learned":
+                self.stats["learned_rounds"] = self.stats.get("learned_rounds", 0) + 1
             r += 1
             accepted = True
             if self.uncov > pre and self.rng.random() > 0.10:
@@ -504,7 +506,7 @@ class Searcher:
 
     def portfolio(self, M, budget_s=np.inf, streams=4, slice_s=None, kmin=8,
                   kmax=None, ruin_frac=0.15, select="fixed", slice_rounds=None,
-                  max_restarts=None):
+                  max_restarts=None, proposer=None):
         """Time-sliced restart portfolio at fixed M with a shared incumbent
         (reference search.py:1536-1620): ``streams`` solutions -- the current
         one and perturbations of it (a kick of ruin_frac * M incumbents by
@@ -515,8 +517,9 @@ class Searcher:
         fresh perturbation of it.  Stops when a stream is feasible, at the
         budget, or -- clock-free runs -- after ``max_restarts`` restarts,
         with slices of ``slice_rounds`` LNS rounds instead of ``slice_s``
-        seconds.  Leaves the incumbent loaded; returns its uncovered
-        count."""
+        seconds.  ``proposer``: a learned ruin proposal, one more operator of
+        every stream's LNS (as in :meth:`lns`).  Leaves the incumbent
+        loaded; returns its uncovered count."""
         t0 = time.time()
         streams = max(1, int(streams))
         slice_s = slice_s or max(1.0, budget_s / (3.0 * streams))
@@ -551,7 +554,8 @@ class Searcher:
                     break
                 self.load(pop[i][0])
                 u = self.lns(M, round_budget_s=min(slice_s, left), kmin=kmin,
-                             kmax=kmax, select=select, max_rounds=slice_rounds)
+                             kmax=kmax, select=select, max_rounds=slice_rounds,
+                             proposer=proposer)
                 pop[i] = (self.code.copy(), u)
                 if u < inc_u:
                     inc, inc_u = pop[i]

@@ -42,6 +42,23 @@ def test_bf16_rounding_is_nearest_even():
     assert r[3] == -2.5 and abs(r[4] - 3.140625) < 1e-12
 
 
+def test_band_features():
+    """Radius <= 2 windows pass unchanged; for R > 2 each window's layers
+    become (layer 0, mean of layers 1..R-1, layer R): 15 inputs."""
+    from coldcore_amd.learned import band_features
+    rng = np.random.default_rng(0)
+    F2 = rng.random((7, 15)).astype(np.float32)
+    assert (band_features(F2, 2) == F2).all()
+    F4 = rng.random((7, 25)).astype(np.float32)
+    B = band_features(F4, 4)
+    assert B.shape == (7, 15)
+    for w in range(5):
+        x = F4[:, 5 * w:5 * w + 5]
+        assert np.allclose(B[:, 3 * w], x[:, 0])
+        assert np.allclose(B[:, 3 * w + 1], x[:, 1:4].mean(axis=1))
+        assert np.allclose(B[:, 3 * w + 2], x[:, 4])
+
+
 @pytest.mark.parametrize("problem,axes,R", [
     ("hamming", [3] * 5, 2), ("hamming", [2, 3, 4], 3), ("lee", [5, 6, 7], 3)])
 @pytest.mark.parametrize("init", [0, 1, 2])
@@ -181,7 +198,8 @@ def test_trained_network_ranks_held_out_regions():
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("axes,R", [([3] * 8, 2), ([4] * 6, 2), ([2] * 12, 2),
-                                    ([3, 4, 5, 3, 4], 2), ([2] * 10, 1)])
+                                    ([3, 4, 5, 3, 4], 2), ([2] * 10, 1),
+                                    ([2] * 12, 4), ([3] * 7, 3)])
 def test_device_region_features_equal_host_features(gpu_cell, axes, R):
     """dct_region_features (three layer transforms and the windows on the
     device, one download) equals the windows the host computes from the
@@ -207,7 +225,8 @@ def test_device_region_features_equal_host_features(gpu_cell, axes, R):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("axes,R", [([3] * 10, 2), ([4] * 8, 2), ([2, 3, 4, 5, 3, 2], 2)])
+@pytest.mark.parametrize("axes,R", [([3] * 10, 2), ([4] * 8, 2), ([2, 3, 4, 5, 3, 2], 2),
+                                    ([2] * 14, 4)])
 def test_device_region_proposal_equals_host(gpu_cell, axes, R):
     """dct_region_propose picks the same regions (nearest code words by digit
     distance, the same hashed tie-break) as the host restatement, and their
@@ -226,5 +245,30 @@ def test_device_region_proposal_equals_host(gpu_cell, axes, R):
         want = cluster_regions(c.space, code, k, anchors, seed)
         assert (regions == want).all(), (na, k)
         sc = RegionScorer(c)
-        assert np.allclose(feats, sc.features(sc.layers(code), want),
+        assert np.allclose(feats, sc.windows(sc.layers(code), want),
                            rtol=1e-6, atol=1e-7)
+
+
+@pytest.mark.gpu
+def test_portfolio_with_learned_proposal_radius4(gpu_cell):
+    """The restart portfolio with the learned proposal as one more operator
+    on a radius-4 binary cell (banded windows, 15 inputs): every stream's
+    moves are judged by the exact fields, so the incumbent it leaves is a
+    code of size M whose uncovered count a fresh recount confirms."""
+    from coldcore_amd.learned import LearnedProposer, Net, RegionScorer
+    from coldcore_amd.search import Searcher
+    g = gpu_cell([2] * 14, 4)
+    start = np.random.default_rng(4).choice(1 << 14, 8, replace=False).astype(np.int64)
+    net = Net(max_batch=1024, seed=3)
+    try:
+        s = Searcher(g, seed=2)
+        s.load(start)
+        prop = LearnedProposer(net, RegionScorer(g), ncand=16)
+        u = s.portfolio(len(start), streams=4, slice_rounds=2, max_restarts=3,
+                        kmin=2, kmax=5, ruin_frac=0.2, select="adaptive",
+                        proposer=prop)
+        assert len(s.code) == len(start)
+        assert s.verify_full() == u
+        assert s.stats.get("learned_rounds", 0) > 0
+    finally:
+        net.close()
