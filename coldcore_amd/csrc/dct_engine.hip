@@ -171,6 +171,7 @@ struct Engine {
   int32_t *d_cm = nullptr;     // the multiplicities of the last staging (in d_cw)
   int32_t *d_toff = nullptr;
   long long toff_cap = 0;
+  int32_t *toff_cur = nullptr;   // the tile offsets of the last staging
   int sparse = 1;            // CC_SPARSE_CODE=0: code seeds through the A0 plane
   long long *d_words = nullptr;
   int32_t *d_out = nullptr;
@@ -1228,7 +1229,7 @@ int run_ptransform(const PPlan &pl, int in_kind, int out_kind, int plo,
     a.code = E.A[0];
     a.cw = E.d_cw;
     a.cm = E.d_cm;
-    a.toff = E.d_toff;
+    a.toff = E.toff_cur;
     a.pk = (uint32_t *)E.lay;
     a.map = E.A[0];
     a.cnt_out = E.cnt;
@@ -1324,9 +1325,27 @@ int stage_sparse_code(const PSweep &w) {
   memcpy(hw, E.uploaded.w.data(), (size_t)n * 8);
   memcpy(hw + n, E.uploaded.m.data(), (size_t)n * 4);
   E.d_cm = (int32_t *)(E.d_cw + n);
+  const long long ntiles = w.ntiles;
+  const size_t with_offs = (size_t)n * 12 + (size_t)(ntiles + 1) * 4;
+  if (with_offs <= (size_t)CODECAP * 12 && with_offs <= SWAP_IN_BYTES) {
+    // few tiles: their entry offsets by one merge of the ascending words
+    // against the tile bounds here, uploaded behind the multiplicities in
+    // the same copy (no offset kernel)
+    int32_t *ht = (int32_t *)hw + 3LL * n;
+    const long long *cw = E.uploaded.w.data();
+    long long i = 0;
+    for (long long t = 0; t <= ntiles; t++) {
+      const long long key = t * (long long)w.P;
+      while (i < n && cw[i] < key) i++;
+      ht[t] = (int32_t)i;
+    }
+    E.toff_cur = (int32_t *)E.d_cw + 3LL * n;
+    CK(hipMemcpyAsync(E.d_cw, hw, with_offs, hipMemcpyHostToDevice, E.st));
+    return 0;
+  }
   if (n)
     CK(hipMemcpyAsync(E.d_cw, hw, (size_t)n * 12, hipMemcpyHostToDevice, E.st));
-  const long long ntiles = w.ntiles;
+  E.toff_cur = nullptr;
   if (ntiles + 1 > E.toff_cap) {
     CK(hipStreamSynchronize(E.st));
     if (E.d_toff) (void)hipFree(E.d_toff);
@@ -1338,6 +1357,7 @@ int stage_sparse_code(const PSweep &w) {
   hipLaunchKernelGGL(k_tile_offsets, dim3((unsigned)((ntiles + 256) / 256)), dim3(256), 0,
                      E.st, E.d_cw, n, (long long)w.P, ntiles, E.d_toff);
   CK(hipGetLastError());
+  E.toff_cur = E.d_toff;
   return 0;
 }
 
@@ -1648,6 +1668,7 @@ void release_all() {
     if (b) (void)hipFree(b);
   E.cnt = nullptr; E.pat = E.pat2 = nullptr; E.d_code = E.d_words = nullptr;
   E.d_cw = nullptr; E.d_cm = nullptr; E.d_toff = nullptr; E.toff_cap = 0;
+  E.toff_cur = nullptr;
   E.d_out = nullptr; E.d_scal = nullptr; E.d_hist = nullptr;
   E.d_eidx64 = nullptr; E.d_eidx32 = nullptr; E.d_eval = nullptr;
   E.d_bcnt = E.d_boff = nullptr;
