@@ -776,31 +776,54 @@ __global__ void __launch_bounds__(CC_BLOCK) k_pfirst(PSweepArgs a) {
 // so the memory pipe never waits for a tile's compute.  Round 0 is the
 // contiguous cube (vector loads), the last round the top cube (coalesced
 // stores), as in k_psweep.
+// a cube's counters held between tiles: packed pairs from 16-byte loads
+// when the cube is whole groups of 8 counters, else one per register
+template <int C>
+struct PkPf {
+  static constexpr int N = C % 8 == 0 ? C / 2 : C;
+};
+
 template <int C>
 __device__ __forceinline__ void pk_cnt_raw(const PSweepArgs &a, long long ub,
-                                           uint32_t lane, uint32_t (&pf)[C / 2]) {
+                                           uint32_t lane, uint32_t (&pf)[PkPf<C>::N]) {
   const __amdgpu_buffer_rsrc_t r = pk_rsrc(a.cnt, ub, a.X, 2);
+  if constexpr (C % 8 == 0) {
 #pragma unroll
-  for (int c = 0; c < C; c += 8) {
-    const pk_u32x4 q = __builtin_bit_cast(pk_u32x4,
-        __builtin_amdgcn_raw_buffer_load_b128(r, (int)(lane * 2u), c * 2, 0));
-    pf[c / 2] = q.x; pf[c / 2 + 1] = q.y; pf[c / 2 + 2] = q.z; pf[c / 2 + 3] = q.w;
+    for (int c = 0; c < C; c += 8) {
+      const pk_u32x4 q = __builtin_bit_cast(pk_u32x4,
+          __builtin_amdgcn_raw_buffer_load_b128(r, (int)(lane * 2u), c * 2, 0));
+      pf[c / 2] = q.x; pf[c / 2 + 1] = q.y; pf[c / 2 + 2] = q.z; pf[c / 2 + 3] = q.w;
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < C; c++)
+      pf[c] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b16(r, (int)(lane * 2u), c * 2, 0);
   }
+}
+
+template <int C>
+__device__ __forceinline__ uint32_t pk_cnt_at(const uint32_t (&pf)[PkPf<C>::N], int c) {
+  if constexpr (C % 8 == 0) return (pf[c >> 1] >> (16 * (c & 1))) & 0xFFFFu;
+  else return pf[c];
 }
 
 template <int Q, int RR, int W, int ND, bool SUM, bool OUT2>
 __global__ void __launch_bounds__(CC_BLOCK) k_pfirstp(PSweepArgs a) {
   extern __shared__ uint32_t Sp[];
   constexpr int C = cc_ipow(Q, RR);
-  static_assert(C % 8 == 0, "vector counter loads");
   const int PTP = pk_pad(a.P) + 1;
   const int u = threadIdx.x;                 // this thread's round-0 cube
+  // the tile's P / C cubes fill whole waves but the last: its idle lanes
+  // only join the barriers
+  const bool act = u < a.P / C;
   const uint32_t lane0 = (uint32_t)u * C;
   const uint32_t lo = (uint32_t)a.plo, span = (uint32_t)(a.phi - a.plo);
   const bool sparse = a.in_kind == CC_IN_SPARSE;
-  uint32_t pf[C / 2];
+  uint32_t pf[PkPf<C>::N];
+#pragma unroll
+  for (int i = 0; i < PkPf<C>::N; i++) pf[i] = 0u;
   long long tile = blockIdx.x;
-  if (tile < a.ntiles && !sparse) pk_cnt_raw<C>(a, tile * a.P, lane0, pf);
+  if (tile < a.ntiles && !sparse && act) pk_cnt_raw<C>(a, tile * a.P, lane0, pf);
   for (; tile < a.ntiles; tile += gridDim.x) {
     const long long base = tile * a.P;
     uint32_t v[C][ND];
@@ -814,7 +837,7 @@ __global__ void __launch_bounds__(CC_BLOCK) k_pfirstp(PSweepArgs a) {
         for (int k = 0; k < ND; k++) v[c][k] = 0u;
       const int e1 = a.toff[tile + 1];
       const long long cube0 = base + lane0;
-      for (int e = a.toff[tile]; e < e1; e++) {
+      for (int e = a.toff[tile]; e < e1 && act; e++) {
         const long long d = a.cw[e] - cube0;
         if (d >= 0 && d < C) {
           const uint32_t m = (uint32_t)a.cm[e];
@@ -827,23 +850,25 @@ __global__ void __launch_bounds__(CC_BLOCK) k_pfirstp(PSweepArgs a) {
       for (int c = 0; c < C; c++) {
 #pragma unroll
         for (int k = 0; k < ND; k++) v[c][k] = 0u;
-        const uint32_t cv = (pf[c >> 1] >> (16 * (c & 1))) & 0xFFFFu;
+        const uint32_t cv = pk_cnt_at<C>(pf, c);
         v[c][0] = (cv - lo) <= span ? 1u : 0u;
       }
       const long long next = tile + gridDim.x;
-      if (next < a.ntiles) pk_cnt_raw<C>(a, next * a.P, lane0, pf);
+      if (next < a.ntiles && act) pk_cnt_raw<C>(a, next * a.P, lane0, pf);
     }
     for (int rd = 0; rd < a.nrounds; rd++) {
       const int qa = a.qa[rd];
       const int pb = (u / qa) * qa * C + (u % qa);
       const bool lin = a.lin[rd] != 0;
       const int kp = a.kp[rd];
-      if (rd > 0) pk_lds_ld<C, ND>(Sp, PTP, pb, qa, 1, 0, lin, kp, v);
-      pk_apply<Q, C, 0, RR, W, ND>(v, a.amask[rd]);
-      if (rd == a.nrounds - 1)
-        pk_store<C, W, ND, SUM, OUT2>(a, base, (uint32_t)qa, (uint32_t)pb, v);
-      else
-        pk_lds_st<C, ND>(Sp, PTP, pb, qa, 1, 0, lin, kp, v);
+      if (act) {
+        if (rd > 0) pk_lds_ld<C, ND>(Sp, PTP, pb, qa, 1, 0, lin, kp, v);
+        pk_apply<Q, C, 0, RR, W, ND>(v, a.amask[rd]);
+        if (rd == a.nrounds - 1)
+          pk_store<C, W, ND, SUM, OUT2>(a, base, (uint32_t)qa, (uint32_t)pb, v);
+        else
+          pk_lds_st<C, ND>(Sp, PTP, pb, qa, 1, 0, lin, kp, v);
+      }
       __syncthreads();
     }
   }
@@ -870,7 +895,7 @@ __global__ void __launch_bounds__(CC_BLOCK) k_pfirstw(PSweepArgs a) {
   const uint32_t lo = (uint32_t)a.plo, span = (uint32_t)(a.phi - a.plo);
   const bool sparse = a.in_kind == CC_IN_SPARSE;
   const int last = a.nrounds - 1;
-  uint32_t pf[C / 2];
+  uint32_t pf[PkPf<C>::N];
   long long tile = blockIdx.x;
   if (tile < a.ntiles && !sparse) pk_cnt_raw<C>(a, tile * a.P, lane0, pf);
   for (; tile < a.ntiles; tile += gridDim.x) {
@@ -892,7 +917,7 @@ __global__ void __launch_bounds__(CC_BLOCK) k_pfirstw(PSweepArgs a) {
     } else {
 #pragma unroll
       for (int c = 0; c < C; c++) {
-        const uint32_t cv = (pf[c >> 1] >> (16 * (c & 1))) & 0xFFFFu;
+        const uint32_t cv = pk_cnt_at<C>(pf, c);
         v[c][0] = (cv - lo) <= span ? 1u : 0u;
       }
       const long long next = tile + gridDim.x;

@@ -136,7 +136,7 @@ hipError_t pk_go(const PSweepArgs &a, int blocks, int threads, size_t lds,
   }
   if (a.kern == 4) {
     // first sweep, LDS exchange, persistent grid with next-tile prefetch
-    if constexpr (cc_ipow(Q, RR) % 8 == 0 && pk_fits(Q, RR, ND)) {
+    if constexpr (pk_fits(Q, RR, ND)) {
       if (sum) kf = &k_pfirstp<Q, RR, W, ND, true, false>;
       else if constexpr (ND > 1) kf = out2 ? &k_pfirstp<Q, RR, W, ND, false, true>
                                            : &k_pfirstp<Q, RR, W, ND, false, false>;

@@ -923,11 +923,10 @@ bool pk_sweep(const PBounds &B, int j, int k, bool pred_seed, bool cnt_out,
     if (w.lds > (size_t)E.lds_budget) return false;
     const int ncube = w.P / C;
     w.threads = std::min(256, (ncube + 63) / 64 * 64);
-    // persistent grid with next-tile prefetch: one cube per thread, a
-    // whole number of waves, vector counter loads, at least two rounds
-    // (a code seed takes it too, seeded from the code's sorted word list)
-    if (E.pk_first == 1 && (pred_seed || E.sparse) && m > w.RR && C % 8 == 0 &&
-        ncube <= CC_BLOCK && ncube % 64 == 0)
+    // persistent grid with next-tile prefetch: one cube per thread (the
+    // last wave's spare lanes idle), at least two rounds (a code seed takes
+    // it too, seeded from the code's sorted word list)
+    if (E.pk_first == 1 && (pred_seed || E.sparse) && m > w.RR && ncube <= CC_BLOCK)
       w.kern = 4;
   } else if (m == w.RR) {
     w.lds = 0;
