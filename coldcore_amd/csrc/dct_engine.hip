@@ -185,6 +185,7 @@ struct Engine {
   long long nchunks = 0;
   long long *h_words = nullptr;
   int32_t *h_out = nullptr;
+  int32_t *h_out_dev = nullptr;  // h_out as the device addresses it
   unsigned long long *h_scal = nullptr;
   hipStream_t st = nullptr;
   long long bytes_dev = 0;
@@ -1602,6 +1603,21 @@ int gather_impl(const long long *words, int nwords, int pmode, int target,
                 int32_t *out) {
   if (nwords <= 0) return 0;
   if (!out) return -1;
+  if (nwords <= NV_WORDS && E.h_out_dev && pick_split(nwords) == 1 &&
+      !want_set(pmode, target, nwords)) {
+    // a small walked gather with one workgroup per word: the words travel
+    // in the kernel arguments and each sum is stored straight into the
+    // pinned host buffer -- one launch and the wait, no copies
+    E.stats[1]++;
+    WordsArg wa;
+    memset(&wa, 0, sizeof(wa));
+    memcpy(wa.w, words, (size_t)nwords * 8);
+    CK((launch_walk<CC_WALK_GATHER>(wa, nullptr, nwords, 1, 0, pmode, target,
+                                    E.h_out_dev)));
+    CK(hipStreamSynchronize(E.st));
+    memcpy(out, E.h_out, (size_t)nwords * 4);
+    return 0;
+  }
   for (int off = 0; off < nwords; off += CODECAP) {
     const int nb = std::min(nwords - off, CODECAP);
     if (int rc = stage_words(words + off, nb)) return rc;
@@ -1680,6 +1696,7 @@ void release_all() {
   if (E.h_sres) { (void)hipHostFree(E.h_sres); E.h_sres = nullptr; }
   if (E.h_words) { (void)hipHostFree(E.h_words); E.h_words = nullptr; }
   if (E.h_out) { (void)hipHostFree(E.h_out); E.h_out = nullptr; }
+  E.h_out_dev = nullptr;
   if (E.h_scal) { (void)hipHostFree(E.h_scal); E.h_scal = nullptr; }
   for (auto &e : E.ev)
     if (e) { (void)hipEventDestroy(e); e = nullptr; }
@@ -1856,6 +1873,10 @@ static int init_cell(int problem, int n, const int *ax, int R,
   CK(hipHostMalloc((void **)&E.h_sres, SWAP_RES_BYTES, hipHostMallocDefault));
   CK(hipHostMalloc((void **)&E.h_out, (size_t)std::max(CODECAP, 4096) * 4,
                    hipHostMallocDefault));
+  if (hipHostGetDevicePointer((void **)&E.h_out_dev, E.h_out, 0) != hipSuccess) {
+    (void)hipGetLastError();
+    E.h_out_dev = nullptr;     // small gathers then copy back like large ones
+  }
   CK(hipHostMalloc((void **)&E.h_scal, 64, hipHostMallocDefault));
   for (auto &e : E.ev) CK(hipEventCreate(&e));
   const char *pe = getenv("DCT_TIMING");
