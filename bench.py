@@ -573,14 +573,15 @@ def run_near_cover(GpuBackend):
 
 
 
-def run_learned_saved(GpuBackend, path=PROPOSAL, seconds=10.0,
+def run_learned_saved(GpuBackend, path=PROPOSAL, seconds=30.0,
                       seeds=(0, 1, 2, 3, 4), log=None):
     """Time-to-size A/B of the saved proposal against plain adaptive LNS on
-    K_3(10,2) (both arms from the same greedy + peel cover per seed)."""
+    K_3(10,2): per seed both arms from the same greedy + peel cover, run at
+    the same time in two processes on the one GPU for ``seconds`` each."""
     from coldcore_amd.learned import learned_ab, load_proposal
     model = load_proposal(path)
     out = learned_ab(lambda: GpuBackend([3] * 10, 2), [], list(seeds), seconds,
-                     model=model, log=log)
+                     model=model, log=log, cell=([3] * 10, 2))
     out["proposal"] = os.path.relpath(path, ROOT)
     return _ab_summary(out)
 
@@ -761,6 +762,8 @@ def main():
                     help="run one side section and print its JSON only")
     ap.add_argument("--out", help="learned-train: where to write the proposal "
                     "(default: the package's saved proposal)")
+    ap.add_argument("--learned-seconds", type=float, default=30.0,
+                    help="seconds per arm of the learned-proposal A/B")
     args = ap.parse_args()
     if args.only == "cpu_calib":
         print(json.dumps({"cpu_calib": run_cpu_calibration()}), flush=True)
@@ -772,7 +775,11 @@ def main():
         elif args.only == "cells":
             r = time_cells(GpuBackend, 10)
         elif args.only == "learned":
-            r = run_learned(GpuBackend)
+            # the saved proposal's A/B (seconds per arm: --learned-seconds)
+            lg = lambda m: print(m, file=sys.stderr, flush=True)
+            r = (run_learned_saved(GpuBackend, seconds=args.learned_seconds, log=lg)
+                 if os.path.exists(PROPOSAL) else
+                 run_learned(GpuBackend, seconds=args.learned_seconds, log=lg))
         elif args.only == "learned-1m":
             # config 4 at its configured scale: 10^6 logged rounds (80 jobs
             # of 12,500 rounds over 8 processes), 30 s per arm
@@ -947,9 +954,10 @@ def main():
         # (the saved network when there is one; else a shorter log, trained
         # here)
         lg = lambda m: print(m, file=sys.stderr, flush=True)
-        rec["learned"] = (run_learned_saved(GpuBackend, log=lg)
+        rec["learned"] = (run_learned_saved(GpuBackend, seconds=args.learned_seconds,
+                                            log=lg)
                           if os.path.exists(PROPOSAL) else
-                          run_learned(GpuBackend, log=lg))
+                          run_learned(GpuBackend, seconds=args.learned_seconds, log=lg))
         section("learned")
     if not args.no_cpu:
         rec["cpu_baseline"] = run_cpu_baseline(code, args.cpu_passes)

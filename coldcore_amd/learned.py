@@ -435,7 +435,7 @@ def load_proposal(path):
 
 def learned_ab(make_backend, train_cells, seeds, seconds, train_rounds=2000,
                epochs=40, ncand=32, tau=0.25, log=None, records=None,
-               model=None, save=None, prior=2.0, meta_extra=None):
+               model=None, save=None, prior=2.0, meta_extra=None, cell=None):
     """Time-to-size of the learned ruin proposal against plain adaptive LNS.
 
     1. Move logs: plain LNS (fixed operator mix) on every cell of
@@ -456,7 +456,7 @@ def learned_ab(make_backend, train_cells, seeds, seconds, train_rounds=2000,
     the proposal trained in step 2 there."""
     if model is not None:
         return _ab_runs(make_backend, seeds, seconds, model, ncand, tau, log,
-                        prior=prior)
+                        prior=prior, cell=cell)
     t0 = time.perf_counter()
     if records is not None:
         X, t = records                       # a log collected beforehand
@@ -499,33 +499,73 @@ def learned_ab(make_backend, train_cells, seeds, seconds, train_rounds=2000,
     if not seeds:
         return meta
     return _ab_runs(make_backend, seeds, seconds, (params, mu, sd, meta), ncand,
-                    tau, log, prior=prior)
+                    tau, log, prior=prior, cell=cell)
 
 
-def _ab_runs(make_backend, seeds, seconds, model, ncand, tau, log, prior=2.0):
-    """Step 3 of :func:`learned_ab` for a trained proposal ``model``."""
-    params, mu, sd, meta = model
-    net = Net(max_batch=max(1024, ncand), seed=7)
+def _arm(b, seed, code, seconds, net, model, ncand, tau, prior):
+    """One arm of the A/B on backend ``b``: notch descent from ``code`` for
+    ``seconds``; with ``net`` the proposal is one more bandit arm."""
+    _, mu, sd, _ = model
+    s = Searcher(b, seed=seed)
+    s.load(code)
+    prop = (LearnedProposer(net, RegionScorer(b), ncand=ncand, tau=tau,
+                            norm=(mu, sd), prior=prior) if net is not None else None)
+    hist = descend_timed(s, seconds, proposer=prop)
+    return {"final_size": int(min(m for _, m in hist)),
+            "rounds": int(s.stats["rounds"]),
+            "hist": [(round(a, 3), int(m)) for a, m in hist]}
+
+
+def _arm_job(job):
+    """An A/B arm in its own process (its own engine cell and network)."""
+    axes, R, seed, code, seconds, arm, model, ncand, tau, prior = job
+    from .gpu import GpuBackend
+    b = GpuBackend(axes, R)
+    net = None
     try:
+        if arm == "learned":
+            net = Net(max_batch=max(1024, ncand), seed=7)
+            net.set_params(model[0])
+        return arm, _arm(b, seed, code, seconds, net, model, ncand, tau, prior)
+    finally:
+        if net is not None:
+            net.close()
+        b.close()
+
+
+def _ab_runs(make_backend, seeds, seconds, model, ncand, tau, log, prior=2.0,
+             cell=None):
+    """Step 3 of :func:`learned_ab` for a trained proposal ``model``.  With
+    ``cell`` ((axes, R) of the backends) the two arms of a seed run at the
+    same time, each in a process of its own on the one GPU (same start
+    code, same conditions); otherwise one after the other."""
+    params, mu, sd, meta = model
+    pool = net = None
+    if cell is not None:
+        import multiprocessing as mp
+        pool = mp.get_context("spawn").Pool(2)
+    else:
+        net = Net(max_batch=max(1024, ncand), seed=7)
         net.set_params(params)
+    try:
         runs = []
         for seed in seeds:
             code = start_code(make_backend, seed)
             res = {"seed": int(seed), "start_size": int(len(code))}
-            for arm in ("plain", "learned"):
-                b = make_backend()
-                try:
-                    s = Searcher(b, seed=seed)
-                    s.load(code)
-                    prop = (LearnedProposer(net, RegionScorer(b), ncand=ncand,
-                                            tau=tau, norm=(mu, sd), prior=prior)
-                            if arm == "learned" else None)
-                    hist = descend_timed(s, seconds, proposer=prop)
-                    res[arm] = {"final_size": int(min(m for _, m in hist)),
-                                "rounds": int(s.stats["rounds"]),
-                                "hist": [(round(a, 3), int(m)) for a, m in hist]}
-                finally:
-                    b.close()
+            if pool is not None:
+                jobs = [(list(cell[0]), int(cell[1]), int(seed), code, seconds, arm,
+                         model, ncand, tau, prior) for arm in ("plain", "learned")]
+                for arm, r in pool.map(_arm_job, jobs, chunksize=1):
+                    res[arm] = r
+            else:
+                for arm in ("plain", "learned"):
+                    b = make_backend()
+                    try:
+                        res[arm] = _arm(b, seed, code, seconds,
+                                        net if arm == "learned" else None, model,
+                                        ncand, tau, prior)
+                    finally:
+                        b.close()
             F = res["plain"]["final_size"]
             tp = time_to_size(res["plain"]["hist"], F)
             tl = time_to_size(res["learned"]["hist"], F)
@@ -540,12 +580,18 @@ def _ab_runs(make_backend, seeds, seconds, model, ncand, tau, log, prior=2.0):
         ratios = [r["ratio"] for r in runs]
         done = [x for x in ratios if x is not None]
         out = dict(meta)
-        out.update({"ncand": ncand, "tau": tau, "prior": prior})
+        out.update({"ncand": ncand, "tau": tau, "prior": prior,
+                    "arms": "concurrent, one process each" if pool is not None
+                            else "one after the other"})
         out.update({"seconds_per_arm": seconds, "runs": runs,
                     "median_ratio": (float(np.median(done)) if len(done) == len(ratios)
                                      else None),
                     "reached": f"{len(done)}/{len(ratios)}"})
         return out
     finally:
-        net.close()
+        if net is not None:
+            net.close()
+        if pool is not None:
+            pool.close()
+            pool.join()
 
