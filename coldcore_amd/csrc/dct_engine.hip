@@ -235,6 +235,9 @@ struct Engine {
                              // is tried (-1: max(|X| / 8, 2^21))
   int set_ok1 = 0, set_ok2 = 0;  // init table / ox table proven to be the ball
   int dbits = 4;             // bits per packed digit
+  int region_walk = 0;       // region layers by ball walks: the table is the
+                             // Hamming ball and digits pack into 64 bits
+  int region_walk_mode = 1;  // CC_REGION_WALK: 0 never, 1 when cheaper, 2 always
   uint64_t *zbuf = nullptr;
   long long zcap = 0, zn = 0;
   int z_mode = -1, z_target = 0, z_mu = 0;
@@ -1895,9 +1898,13 @@ static int init_cell(int problem, int n, const int *ax, int R,
     for (int i = 0; i < n; i++) amax = std::max(amax, ax[i]);
     E.dbits = 1;
     while ((1 << E.dbits) < amax) E.dbits++;
-    E.set_ok1 = E.set_policy != 0 && n * E.dbits <= 64 &&
-                table_is_ball(ballpat, balllen, tfmt);
+    const bool ball = n * E.dbits <= 64 && table_is_ball(ballpat, balllen, tfmt);
+    E.set_ok1 = E.set_policy != 0 && ball;
     E.set_ok2 = 0;
+    E.region_walk = ball && E.prob == CC_HAMMING &&
+                    (tfmt == CC_FMT_V1 || tfmt == CC_FMT_WIDE);
+    const char *rw = getenv("CC_REGION_WALK");
+    E.region_walk_mode = rw ? atoi(rw) : 1;
     E.cnt_ver = 1;
     E.plus_words = 0;
     E.z_mode = E.o_mode = -1;
@@ -2515,6 +2522,131 @@ __global__ void k_region_feat(const SweepArgs a, int L, int Lv,
   }
 }
 
+// Exact layers at listed words without whole-space transforms, for cells
+// where walking each word's ball is far cheaper than three layer transforms
+// (the table must be exactly the Hamming ball, so a pattern's distance is
+// its number of fields): block i walks words[i]'s ball and counts the
+// deficient (cnt < mu) and critical (1 <= cnt <= mu) words at each
+// distance; out[i][0][d], out[i][1][d].
+template <int FMT>
+__global__ void __launch_bounds__(256)
+    k_region_walk(Geom g, const long long *__restrict__ words,
+                  const uint16_t *__restrict__ cnt, const uint64_t *__restrict__ pat,
+                  long long blen, int L, int32_t *out) {
+  __shared__ long long disp[CC_MAXN * CC_CSTRIDE];
+  __shared__ unsigned char okf[CC_MAXN * CC_CSTRIDE];
+  __shared__ int digs[CC_MAXN];
+  __shared__ int red[2 * CC_MAXL];
+  const long long w = words[blockIdx.x];
+  cc_walk_disp(g, w, disp, okf, digs);
+  if (threadIdx.x < 2 * CC_MAXL) red[threadIdx.x] = 0;
+  __syncthreads();
+  constexpr int FB = FMT == CC_FMT_V1 ? 8 : 10;
+  constexpr int NF = FMT == CC_FMT_V1 ? CC_MAXPAT : CC_MAXPAT_W;
+  constexpr uint64_t FM = (1u << FB) - 1u;
+  int def[CC_MAXL], crit[CC_MAXL];
+#pragma unroll
+  for (int d = 0; d < CC_MAXL; d++) def[d] = crit[d] = 0;
+  for (long long t = threadIdx.x; t < blen; t += blockDim.x) {
+    const uint64_t pt = pat[t];
+    int nf = 0;
+#pragma unroll
+    for (int f = 0; f < NF; f++) nf += ((pt >> (FB * f)) & FM) != 0;
+    bool ok;
+    const long long x = cc_walk_word<false, FMT>(pt, w, disp, okf, &ok);
+    const int c = (int)cnt[x];
+#pragma unroll
+    for (int d = 0; d < CC_MAXL; d++)
+      if (d == nf) {
+        def[d] += c < g.mu;
+        crit[d] += c >= 1 && c <= g.mu;
+      }
+  }
+  for (int d = 0; d < L; d++) {
+    int a = def[d], b = crit[d];
+    for (int o = 32; o > 0; o >>= 1) {
+      a += __shfl_xor(a, o, 64);
+      b += __shfl_xor(b, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+      atomicAdd(&red[d], a);
+      atomicAdd(&red[CC_MAXL + d], b);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < L) {
+    out[(long long)blockIdx.x * 2 * L + threadIdx.x] = red[threadIdx.x];
+    out[(long long)blockIdx.x * 2 * L + L + threadIdx.x] = red[CC_MAXL + threadIdx.x];
+  }
+}
+
+// the code's words as packed digits
+__global__ void k_pack_words(Dig g, const long long *__restrict__ w, int n,
+                             uint64_t *out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = cc_pack(g, w[i]);
+}
+
+// codeword layers at listed words: out[i][d] = code words at Hamming
+// distance exactly d from words[i] (packed digits: xor, fold each digit's
+// bits onto its low bit, popcount)
+__global__ void __launch_bounds__(256)
+    k_region_code(Dig g, const long long *__restrict__ words,
+                  const uint64_t *__restrict__ cpk, int M, int L, int32_t *out) {
+  __shared__ int red[CC_MAXL];
+  if (threadIdx.x < CC_MAXL) red[threadIdx.x] = 0;
+  __syncthreads();
+  const uint64_t pw = cc_pack(g, words[blockIdx.x]);
+  int acc[CC_MAXL];
+#pragma unroll
+  for (int d = 0; d < CC_MAXL; d++) acc[d] = 0;
+  for (int j = threadIdx.x; j < M; j += blockDim.x) {
+    const uint64_t x = pw ^ cpk[j];
+    uint64_t y = x;
+    for (int sft = 1; sft < g.b; sft++) y |= x >> sft;
+    const int d = __popcll(y & g.lsb);
+#pragma unroll
+    for (int e = 0; e < CC_MAXL; e++) acc[e] += d == e;
+  }
+  for (int d = 0; d < L; d++) {
+    int a = acc[d];
+    for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(&red[d], a);
+  }
+  __syncthreads();
+  if (threadIdx.x < L) out[(long long)blockIdx.x * L + threadIdx.x] = red[threadIdx.x];
+}
+
+// the five windows of nreg regions of k words from per-word layers (walk:
+// [word][deficient, critical][d], code: [word][d]), as k_region_feat
+__global__ void k_region_feat_words(const int32_t *__restrict__ walk,
+                                    const int32_t *__restrict__ codel, int L, int Lv,
+                                    int nreg, int k, FeatShell sh, float *out) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nreg) return;
+  float *o = out + (long long)r * 5 * L;
+  for (int kind = 0; kind < 3; kind++)
+    for (int d = 0; d < L; d++) {
+      double sum = 0, mx = 0, mn = 0;
+      for (int j = 0; j < k; j++) {
+        const long long i = (long long)r * k + j;
+        double v = 0;
+        if (d < Lv)
+          v = kind == 1 ? (double)walk[i * 2 * L + d]
+            : kind == 2 ? (double)walk[i * 2 * L + L + d]
+                        : (double)codel[i * L + d];
+        v = v / sh.s[d];
+        sum += v;
+        mx = j == 0 ? v : (v > mx ? v : mx);
+        mn = j == 0 ? v : (v < mn ? v : mn);
+      }
+      const double mean = sum / k;
+      if (kind == 1) { o[1 * L + d] = (float)mean; o[2 * L + d] = (float)mx; }
+      else if (kind == 2) { o[0 * L + d] = (float)mean; o[4 * L + d] = (float)mn; }
+      else o[3 * L + d] = (float)mean;
+    }
+}
+
 // Ruin regions on the device: block b takes anchor code position anc[b]
 // and picks the k code words nearest to it in digit distance, ties broken
 // by a 24-bit hash of (seed, anchor position, word position) and then by
@@ -2585,6 +2717,41 @@ int region_windows(int nreg, int k) {
   for (int d = 0; d < CC_MAXL; d++) sh.s[d] = d < E.L ? std::max(poly[d], 1.0) : 1.0;
   float *dfeat = (float *)E.d_out;
   const int Lv = layers_after(E.n);
+  const long long nw = (long long)nreg * k;
+  // scratch behind the feature block and the positions region_propose
+  // keeps at d_out + 4096: per-word walk and code layers
+  const bool fits = 8192 + nw * 3 * E.L <= CODECAP &&
+                    (long long)E.codeM * 8 <= (long long)CODECAP * 12;
+  const bool walk = E.region_walk && E.fmt == 0 && E.codeM > 0 && fits &&
+                    (E.region_walk_mode == 2 ||
+                     (E.region_walk_mode == 1 && nw * E.blen * 8 < E.X));
+  if (walk) {
+    // per-word layers by ball walks and code comparisons (the words are in
+    // d_words): no layer transform of the whole space
+    if (int rc = ensure_code_dev()) return rc;
+    int32_t *d_walk = E.d_out + 8192;
+    int32_t *d_codel = d_walk + nw * 2 * E.L;
+    uint64_t *d_cpk = (uint64_t *)E.d_cw;   // the code packed (sparse-seed staging)
+    const Dig dg = make_dig();
+    hipLaunchKernelGGL(k_pack_words, dim3((E.codeM + 255) / 256), dim3(256), 0, E.st, dg,
+                       E.d_code, E.codeM, d_cpk);
+    CK(hipGetLastError());
+    const Geom g = make_geom();
+    if (E.tfmt == CC_FMT_WIDE)
+      hipLaunchKernelGGL(k_region_walk<CC_FMT_WIDE>, dim3((unsigned)nw), dim3(256), 0,
+                         E.st, g, E.d_words, E.cnt, walk_table(), E.blen, E.L, d_walk);
+    else
+      hipLaunchKernelGGL(k_region_walk<CC_FMT_V1>, dim3((unsigned)nw), dim3(256), 0,
+                         E.st, g, E.d_words, E.cnt, walk_table(), E.blen, E.L, d_walk);
+    CK(hipGetLastError());
+    hipLaunchKernelGGL(k_region_code, dim3((unsigned)nw), dim3(256), 0, E.st, dg,
+                       E.d_words, d_cpk, E.codeM, E.L, d_codel);
+    CK(hipGetLastError());
+    hipLaunchKernelGGL(k_region_feat_words, dim3((nreg + 63) / 64), dim3(64), 0, E.st,
+                       d_walk, d_codel, E.L, Lv, nreg, k, sh, dfeat);
+    CK(hipGetLastError());
+    return 0;
+  }
   for (int kind : {1, 2, 0}) {
     if (int rc = dct_transform_layers(kind)) return rc;
     SweepArgs a;
@@ -2654,6 +2821,9 @@ int dct_region_features(const long long *code, int M, const long long *words,
     return -1;
   if (!valid_words(words, nreg * k)) return -1;
   if (int rc = dct_set_code(code, M)) return rc;
+  // the code goes up first: its upload reuses the pinned word staging, so it
+  // must be done before the region words are staged there
+  if (int rc = ensure_code_dev()) return rc;
   if (int rc = stage_words(words, nreg * k)) return rc;
   if (int rc = region_windows(nreg, k)) return rc;
   float *dfeat = (float *)E.d_out;

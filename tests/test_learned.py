@@ -272,3 +272,37 @@ def test_portfolio_with_learned_proposal_radius4(gpu_cell):
         assert s.stats.get("learned_rounds", 0) > 0
     finally:
         net.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("axes,R", [([3] * 8, 2), ([2] * 12, 3), ([2] * 18, 4),
+                                    ([4] * 6, 2)])
+def test_region_features_by_walks_equal_transforms(gpu_cell, monkeypatch, axes, R):
+    """Region windows from per-word ball walks and code comparisons (the
+    path large cells take) equal the windows from the three whole-space
+    layer transforms, exactly, for dct_region_features and
+    dct_region_propose; both equal the host windows of the oracle."""
+    from coldcore_amd.learned import RegionScorer
+    size = int(np.prod(axes))
+    rng = np.random.default_rng(size + R)
+    code = np.unique(rng.integers(0, size, max(8, size // 300)))
+    got = {}
+    for mode in ("0", "2"):
+        monkeypatch.setenv("CC_REGION_WALK", mode)
+        g = gpu_cell(axes, R)
+        g.recount(code)
+        regions = np.stack([np.random.default_rng(7).choice(len(code), 5, replace=False)
+                            for _ in range(24)])
+        anchors = np.arange(min(16, len(code)))
+        got[mode] = (RegionScorer(g).region_features(code, regions),
+                     g.region_propose(code, anchors, 4, 99))
+        g.close()
+    assert (got["0"][0] == got["2"][0]).all()
+    assert (got["0"][1][0] == got["2"][1][0]).all()
+    assert (got["0"][1][1] == got["2"][1][1]).all()
+    if size <= 1 << 16:
+        c = NumpyDP(axes, R)
+        c.recount(code)
+        sc = RegionScorer(c)
+        assert np.allclose(got["2"][0], sc.features(sc.layers(code), regions),
+                           rtol=1e-6, atol=1e-7)
