@@ -2722,9 +2722,12 @@ int region_windows(int nreg, int k) {
   // keeps at d_out + 4096: per-word walk and code layers
   const bool fits = 8192 + nw * 3 * E.L <= CODECAP &&
                     (long long)E.codeM * 8 <= (long long)CODECAP * 12;
+  // walks when they read less than an eighth of one whole-space pass, or
+  // at most 2^22 entries (three layer transforms then cost more in launches
+  // and waits than the walks in reads)
+  const bool cheap = nw * E.blen * 8 < E.X || nw * E.blen <= (1LL << 22);
   const bool walk = E.region_walk && E.fmt == 0 && E.codeM > 0 && fits &&
-                    (E.region_walk_mode == 2 ||
-                     (E.region_walk_mode == 1 && nw * E.blen * 8 < E.X));
+                    (E.region_walk_mode == 2 || (E.region_walk_mode == 1 && cheap));
   if (walk) {
     // per-word layers by ball walks and code comparisons (the words are in
     // d_words): no layer transform of the whole space
