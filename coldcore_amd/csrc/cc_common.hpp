@@ -89,6 +89,33 @@ __device__ __forceinline__ int cc_dpp_wave_max0(int v) {
   return max(0, __builtin_amdgcn_readlane(v, 63));
 }
 
+// Census slots: CC_NSLOT uint64 accumulators `stride` words apart.  A
+// workgroup combines its waves' totals in LDS and adds (or maxes) the
+// workgroup total into slot blockIdx % CC_NSLOT, so the same-address
+// atomics of a launch spread over CC_NSLOT addresses (and memory channels);
+// the host reads the slots back and combines them.  Every thread of the
+// workgroup calls it (whole waves).
+#define CC_NSLOT 32
+template <bool MAX>
+__device__ __forceinline__ void cc_census_flush(uint32_t v, unsigned long long *slots,
+                                                int stride) {
+  __shared__ uint32_t cw[16];
+  const uint32_t t = MAX ? (uint32_t)cc_dpp_wave_max0((int)v) : cc_dpp_wave_add(v);
+  if ((threadIdx.x & 63) == 0) cw[threadIdx.x >> 6] = t;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long s = 0;
+    const int nw = (int)(blockDim.x + 63) >> 6;
+    for (int i = 0; i < nw; i++)
+      s = MAX ? (s > cw[i] ? s : (unsigned long long)cw[i]) : s + cw[i];
+    if (s) {
+      unsigned long long *p = slots + (size_t)(blockIdx.x % CC_NSLOT) * stride;
+      if (MAX) atomicMax(p, s);
+      else atomicAdd(p, s);
+    }
+  }
+}
+
 // inclusive prefix sum over the 64 lanes of a wave
 __device__ __forceinline__ int cc_wave_incl_scan(int v) {
   const int lane = threadIdx.x & 63;

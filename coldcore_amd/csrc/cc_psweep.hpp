@@ -61,6 +61,14 @@ struct PSweepArgs {
   uint32_t *pk;          // compact layer planes (in and out, in place)
   int32_t *map;          // CC_OUT_MAP
   uint16_t *cnt_out;     // CC_OUT_CNT
+  // the last sweep folds a census of what it writes into the census slots
+  // at fold (null: none; cc_census_flush): the map maximum or the cnt
+  // deficit sum max(0, fold_mu - cnt); the first sweep of the transform
+  // zeroes the slots at fold_zero first
+  unsigned long long *fold;
+  unsigned long long *fold_zero;
+  int fold_mu;
+  int fold_stride;
   long long pitch;       // dwords between compact planes
   long long X;           // words of every plane (buffer bounds)
   long long s;           // stride (words) of the group's first axis
@@ -84,6 +92,8 @@ struct PSweepArgs {
   int mid_w8;
   int moff[CC_MAXL], mbits[CC_MAXL];
   int na;                // k_pfirstw: axes of its last (widening) round
+  int lcmp;              // k_psweep with two rounds: the LDS tile holds one
+                         // compact dword per word (format moff / mbits)
   int nrounds;
   int ca[CC_PK_MAXRD];     // lowest group axis of each round's cube
   int amask[CC_PK_MAXRD];  // cube axes the round applies (bit i: axis ca + i)
@@ -412,7 +422,7 @@ __device__ __forceinline__ uint64_t pk_compact(const PSweepArgs &a,
 // the map, or the clamped sum into cnt.  VEC: the cube is C consecutive
 // words (usq == 1, 16-byte aligned): 16-byte vector stores.
 template <int C, int W, int ND, bool SUM, bool OUT2, bool VEC, int ST_AUX>
-__device__ __forceinline__ void pk_store_aux(const PSweepArgs &a, long long ub,
+__device__ __forceinline__ uint32_t pk_store_aux(const PSweepArgs &a, long long ub,
                                              uint32_t usq, uint32_t lane,
                                              const uint32_t (&v)[C][ND]) {
   constexpr int NS = ND * (32 / W);
@@ -433,7 +443,7 @@ __device__ __forceinline__ void pk_store_aux(const PSweepArgs &a, long long ub,
         if (two) PK_ST8(r8, c, o >> 16);
         PK_FENCE(c);
       }
-      return;
+      return 0u;
     }
     const __amdgpu_buffer_rsrc_t r = pk_rsrc(a.pk, ub, a.X, 4);
     // layer d goes to bit ooff[d] of plane odw[d] (OUT2: two planes);
@@ -484,18 +494,26 @@ __device__ __forceinline__ void pk_store_aux(const PSweepArgs &a, long long ub,
         PK_FENCE(c);
       }
     }
+    return 0u;
   } else {
     // one loop for both sums, so the per-word sums are formed next to
     // their stores (not hoisted ahead of the branch for all C words)
+    // f: the census of the stored words (their maximum, or their deficits'
+    // sum below fold_mu)
     const bool to_map = a.out_kind == CC_OUT_MAP;
     const __amdgpu_buffer_rsrc_t rm = pk_rsrc(a.map, ub, a.X, 4);
     const __amdgpu_buffer_rsrc_t rc = pk_rsrc(a.cnt_out, ub, a.X, 2);
+    const uint32_t mu = (uint32_t)a.fold_mu;
+    uint32_t f = 0u;
     if constexpr (VEC && C % 8 == 0) {
 #pragma unroll
       for (int c = 0; c < C; c += 8) {
         uint32_t sm[8];
 #pragma unroll
         for (int i = 0; i < 8; i++) sm[i] = pk_sum<W, ND>(a, v[c + i]);
+#pragma unroll
+        for (int i = 0; i < 8; i++)
+          f = to_map ? max(f, sm[i]) : f + (mu > min(sm[i], 65535u) ? mu - min(sm[i], 65535u) : 0u);
         if (to_map) {
           const pk_u32x4 o0 = {sm[0], sm[1], sm[2], sm[3]};
           const pk_u32x4 o1 = {sm[4], sm[5], sm[6], sm[7]};
@@ -513,22 +531,48 @@ __device__ __forceinline__ void pk_store_aux(const PSweepArgs &a, long long ub,
 #pragma unroll
       for (int c = 0; c < C; c++) {
         const uint32_t s = pk_sum<W, ND>(a, v[c]);
+        const uint32_t sc = s > 65535u ? 65535u : s;
         if (to_map) PK_ST32(rm, c, s);
-        else PK_ST16(rc, c, (s > 65535u ? 65535u : s));
+        else PK_ST16(rc, c, sc);
+        f = to_map ? max(f, s) : f + (mu > sc ? mu - sc : 0u);
         PK_FENCE(c);
       }
     }
+    return f;
   }
 }
 
 // the store cache policy is chosen per sweep (st_nt: non-temporal stores
 // for words that the next kernel does not read soon, such as the map)
 template <int C, int W, int ND, bool SUM, bool OUT2, bool VEC = false>
-__device__ __forceinline__ void pk_store(const PSweepArgs &a, long long ub,
-                                         uint32_t usq, uint32_t lane,
-                                         const uint32_t (&v)[C][ND]) {
-  if (a.st_nt) pk_store_aux<C, W, ND, SUM, OUT2, VEC, CC_ST_NT>(a, ub, usq, lane, v);
-  else pk_store_aux<C, W, ND, SUM, OUT2, VEC, 0>(a, ub, usq, lane, v);
+__device__ __forceinline__ uint32_t pk_store(const PSweepArgs &a, long long ub,
+                                             uint32_t usq, uint32_t lane,
+                                             const uint32_t (&v)[C][ND]) {
+  if (a.st_nt) return pk_store_aux<C, W, ND, SUM, OUT2, VEC, CC_ST_NT>(a, ub, usq, lane, v);
+  return pk_store_aux<C, W, ND, SUM, OUT2, VEC, 0>(a, ub, usq, lane, v);
+}
+
+// a thread's running census over the cubes it stored
+__device__ __forceinline__ uint32_t pk_fold(const PSweepArgs &a, uint32_t acc, uint32_t f) {
+  return a.out_kind == CC_OUT_MAP ? max(acc, f) : acc + f;
+}
+
+// kernel prologue: the first sweep clears the census slots
+__device__ __forceinline__ void pk_fold_zero(const PSweepArgs &a) {
+  if (a.fold_zero && blockIdx.x == 0 && threadIdx.x < CC_NSLOT)
+    a.fold_zero[(size_t)threadIdx.x * a.fold_stride] = 0ull;
+}
+
+// kernel epilogue, reached by every thread of the workgroup (all launches
+// are whole waves): the workgroup's census, one atomic per workgroup
+template <bool SUM>
+__device__ __forceinline__ void pk_fold_flush(const PSweepArgs &a, uint32_t acc) {
+  if constexpr (SUM) {
+    if (a.fold) {
+      if (a.out_kind == CC_OUT_MAP) cc_census_flush<true>(acc, a.fold, a.fold_stride);
+      else cc_census_flush<false>(acc, a.fold, a.fold_stride);
+    }
+  }
 }
 
 // LDS exchange: element (row p, column tl) of the tile lives at
@@ -583,6 +627,65 @@ __device__ __forceinline__ void pk_lds_ld(const uint32_t *__restrict__ S,
   }
 }
 
+template <int W, int ND>
+__device__ __forceinline__ uint32_t pk_mid_pack(const PSweepArgs &a,
+                                                const uint32_t (&r)[ND]) {
+  if (a.mid_w8) return pk_to_bytes<W, ND>(r, a.out_bmask);
+  constexpr int NS = ND * (32 / W);
+  constexpr uint32_t M = (1u << W) - 1u;
+  uint32_t o = 0;
+#pragma unroll
+  for (int d = 0; d < NS; d++) {
+    if (d >= a.L) continue;
+    o |= ((r[d % ND] >> (W * (d / ND))) & M) << a.moff[d];
+  }
+  return o;
+}
+
+template <int W, int ND>
+__device__ __forceinline__ void pk_mid_unpack(const PSweepArgs &a, uint32_t x,
+                                              uint32_t (&r)[ND]) {
+  if (a.mid_w8) {
+    pk_from_bytes<W, ND>(x, r);
+    return;
+  }
+  constexpr int NS = ND * (32 / W);
+#pragma unroll
+  for (int k = 0; k < ND; k++) r[k] = 0u;
+#pragma unroll
+  for (int d = 0; d < NS; d++) {
+    if (d >= a.L) continue;
+    r[d % ND] |= __builtin_amdgcn_ubfe(x, a.moff[d], a.mbits[d]) << (W * (d / ND));
+  }
+}
+
+// the LDS exchange of k_psweep in one compact dword per word (addresses as
+// pk_lds_st / pk_lds_ld, plane 0 only)
+template <int C, int W, int ND>
+__device__ __forceinline__ void pk_lds_st_c(const PSweepArgs &a, uint32_t *__restrict__ S,
+                                            int pb, int qa, int TL, int tl, bool lin,
+                                            int kp, const uint32_t (&v)[C][ND]) {
+#pragma unroll
+  for (int c = 0; c < C; c++) {
+    const int e = lin ? pk_pad(pb * TL + tl) + c * kp : pk_pad((pb + c * qa) * TL + tl);
+    S[e] = pk_mid_pack<W, ND>(a, v[c]);
+    PK_FENCE(c);
+  }
+}
+
+template <int C, int W, int ND>
+__device__ __forceinline__ void pk_lds_ld_c(const PSweepArgs &a,
+                                            const uint32_t *__restrict__ S, int pb,
+                                            int qa, int TL, int tl, bool lin, int kp,
+                                            uint32_t (&v)[C][ND]) {
+#pragma unroll
+  for (int c = 0; c < C; c++) {
+    const int e = lin ? pk_pad(pb * TL + tl) + c * kp : pk_pad((pb + c * qa) * TL + tl);
+    pk_mid_unpack<W, ND>(a, S[e], v[c]);
+    PK_FENCE(c);
+  }
+}
+
 // SEED: the sweep starts the transform (reads the predicate or code plane).
 // SUM: the sweep ends the transform (layer sum into the map or cnt);
 // otherwise it writes compact layers.  IN2 / OUT2: its compact input /
@@ -591,6 +694,8 @@ __device__ __forceinline__ void pk_lds_ld(const uint32_t *__restrict__ S,
 template <int Q, int RR, int W, int ND, bool SEED, bool SUM, bool IN2, bool OUT2>
 __global__ void __launch_bounds__(CC_BLOCK) k_psweep(PSweepArgs a) {
   extern __shared__ uint32_t Sp[];
+  pk_fold_zero(a);
+  uint32_t facc = 0u;
   constexpr int C = cc_ipow(Q, RR);
   const int PT = a.P * a.TL;
   const int PTP = pk_pad(PT) + 1;
@@ -614,11 +719,15 @@ __global__ void __launch_bounds__(CC_BLOCK) k_psweep(PSweepArgs a) {
         uint32_t v[C][ND];
         if (first)
           pk_load<C, W, ND, SEED, IN2>(a, base, usq, lane, tl < ncol, v);
+        else if (ND > 1 && a.lcmp)
+          pk_lds_ld_c<C, W, ND>(a, Sp, pb, qa, a.TL, tl, lin, kp, v);
         else
           pk_lds_ld<C, ND>(Sp, PTP, pb, qa, a.TL, tl, lin, kp, v);
         pk_apply<Q, C, 0, RR, W, ND>(v, a.amask[rd]);
         if (last) {
-          if (tl < ncol) pk_store<C, W, ND, SUM, OUT2>(a, base, usq, lane, v);
+          if (tl < ncol) facc = pk_fold(a, facc, pk_store<C, W, ND, SUM, OUT2>(a, base, usq, lane, v));
+        } else if (ND > 1 && a.lcmp) {
+          pk_lds_st_c<C, W, ND>(a, Sp, pb, qa, a.TL, tl, lin, kp, v);
         } else {
           pk_lds_st<C, ND>(Sp, PTP, pb, qa, a.TL, tl, lin, kp, v);
         }
@@ -626,6 +735,7 @@ __global__ void __launch_bounds__(CC_BLOCK) k_psweep(PSweepArgs a) {
       __syncthreads();
     }
   }
+  pk_fold_flush<SUM>(a, facc);
 }
 
 // ---------------------------------------------------------------------------
@@ -651,6 +761,8 @@ __device__ __forceinline__ void pk_add_up_xlane(uint32_t (&x)[ND]) {
 
 template <int Q, int RR, int W, int ND, bool SUM, bool IN2, bool OUT2, int SPLIT>
 __global__ void __launch_bounds__(CC_BLOCK) k_pcol(PSweepArgs a) {
+  pk_fold_zero(a);
+  uint32_t facc = 0u;
   constexpr int C = cc_ipow(Q, RR);
   constexpr int CH = C / SPLIT;              // words per lane
   constexpr int RI = SPLIT == 2 ? RR - 1 : RR;  // axes inside the lane
@@ -672,8 +784,9 @@ __global__ void __launch_bounds__(CC_BLOCK) k_pcol(PSweepArgs a) {
 #pragma unroll
       for (int c = 0; c < CH; c++) pk_add_up_xlane<W, ND>(v[c]);
     }
-    pk_store<CH, W, ND, SUM, OUT2>(a, base, usq, lane, v);
+    facc = pk_fold(a, facc, pk_store<CH, W, ND, SUM, OUT2>(a, base, usq, lane, v));
   }
+  pk_fold_flush<SUM>(a, facc);
 }
 
 // ---------------------------------------------------------------------------
@@ -733,6 +846,8 @@ __device__ __forceinline__ void pk_xlane_swap(uint32_t (&a)[ND], uint32_t (&b)[N
 
 template <int RR, int W, int ND, bool SUM, bool OUT2>
 __global__ void __launch_bounds__(CC_BLOCK) k_pfirst(PSweepArgs a) {
+  pk_fold_zero(a);
+  uint32_t facc = 0u;
   constexpr int C = 1 << RR;
   constexpr int TW = C * 64;                  // words per wave tile
   const int wave = (int)threadIdx.x >> 6;
@@ -764,8 +879,9 @@ __global__ void __launch_bounds__(CC_BLOCK) k_pfirst(PSweepArgs a) {
       pk_xlane_swap<W, ND, 32>(v[c], v[c + C / 2]);
       if ((c & 7) == 7) __builtin_amdgcn_sched_barrier(0);
     }
-    pk_store<C, W, ND, SUM, OUT2, true>(a, base, 1u, lane, v);
+    facc = pk_fold(a, facc, pk_store<C, W, ND, SUM, OUT2, true>(a, base, 1u, lane, v));
   }
+  pk_fold_flush<SUM>(a, facc);
 }
 
 // ---------------------------------------------------------------------------
@@ -810,6 +926,8 @@ __device__ __forceinline__ uint32_t pk_cnt_at(const uint32_t (&pf)[PkPf<C>::N], 
 template <int Q, int RR, int W, int ND, bool SUM, bool OUT2>
 __global__ void __launch_bounds__(CC_BLOCK) k_pfirstp(PSweepArgs a) {
   extern __shared__ uint32_t Sp[];
+  pk_fold_zero(a);
+  uint32_t facc = 0u;
   constexpr int C = cc_ipow(Q, RR);
   const int PTP = pk_pad(a.P) + 1;
   const int u = threadIdx.x;                 // this thread's round-0 cube
@@ -865,13 +983,14 @@ __global__ void __launch_bounds__(CC_BLOCK) k_pfirstp(PSweepArgs a) {
         if (rd > 0) pk_lds_ld<C, ND>(Sp, PTP, pb, qa, 1, 0, lin, kp, v);
         pk_apply<Q, C, 0, RR, W, ND>(v, a.amask[rd]);
         if (rd == a.nrounds - 1)
-          pk_store<C, W, ND, SUM, OUT2>(a, base, (uint32_t)qa, (uint32_t)pb, v);
+          facc = pk_fold(a, facc, pk_store<C, W, ND, SUM, OUT2>(a, base, (uint32_t)qa, (uint32_t)pb, v));
         else
           pk_lds_st<C, ND>(Sp, PTP, pb, qa, 1, 0, lin, kp, v);
       }
       __syncthreads();
     }
   }
+  pk_fold_flush<SUM>(a, facc);
 }
 
 // ---------------------------------------------------------------------------
@@ -885,6 +1004,8 @@ __global__ void __launch_bounds__(CC_BLOCK) k_pfirstp(PSweepArgs a) {
 template <int Q, int RR, int W2, int ND2, int NA, bool SUM, bool OUT2>
 __global__ void __launch_bounds__(CC_BLOCK) k_pfirstw(PSweepArgs a) {
   extern __shared__ uint32_t Sp[];
+  pk_fold_zero(a);
+  uint32_t facc = 0u;
   constexpr int C = cc_ipow(Q, RR);
   constexpr int CA = cc_ipow(Q, NA);         // words of one chunk
   constexpr int F = C / CA;                  // chunks of the last round's cube
@@ -944,12 +1065,13 @@ __global__ void __launch_bounds__(CC_BLOCK) k_pfirstw(PSweepArgs a) {
 #pragma unroll
         for (int g = 0; g < CA; g++) pk_from_bytes<W2, ND2>(v[f + F * g][0], w[g]);
         pk_apply<Q, CA, 0, NA, W2, ND2>(w, (1 << NA) - 1);
-        pk_store<CA, W2, ND2, SUM, OUT2>(a, base, (uint32_t)(F * qa),
-                                         (uint32_t)(pb + f * qa), w);
+        facc = pk_fold(a, facc, pk_store<CA, W2, ND2, SUM, OUT2>(a, base, (uint32_t)(F * qa),
+                                         (uint32_t)(pb + f * qa), w));
       }
       __syncthreads();
     }
   }
+  pk_fold_flush<SUM>(a, facc);
 }
 
 // ---------------------------------------------------------------------------
@@ -964,41 +1086,11 @@ __global__ void __launch_bounds__(CC_BLOCK) k_pfirstw(PSweepArgs a) {
 // fall on different banks; round-B waves read consecutive dwords.
 #define CC_STRIP_PADR 16
 
-template <int W, int ND>
-__device__ __forceinline__ uint32_t pk_mid_pack(const PSweepArgs &a,
-                                                const uint32_t (&r)[ND]) {
-  if (a.mid_w8) return pk_to_bytes<W, ND>(r, a.out_bmask);
-  constexpr int NS = ND * (32 / W);
-  constexpr uint32_t M = (1u << W) - 1u;
-  uint32_t o = 0;
-#pragma unroll
-  for (int d = 0; d < NS; d++) {
-    if (d >= a.L) continue;
-    o |= ((r[d % ND] >> (W * (d / ND))) & M) << a.moff[d];
-  }
-  return o;
-}
-
-template <int W, int ND>
-__device__ __forceinline__ void pk_mid_unpack(const PSweepArgs &a, uint32_t x,
-                                              uint32_t (&r)[ND]) {
-  if (a.mid_w8) {
-    pk_from_bytes<W, ND>(x, r);
-    return;
-  }
-  constexpr int NS = ND * (32 / W);
-#pragma unroll
-  for (int k = 0; k < ND; k++) r[k] = 0u;
-#pragma unroll
-  for (int d = 0; d < NS; d++) {
-    if (d >= a.L) continue;
-    r[d % ND] |= __builtin_amdgcn_ubfe(x, a.moff[d], a.mbits[d]) << (W * (d / ND));
-  }
-}
-
 template <int Q, int RA, int W, int ND, bool SUM, bool IN2, bool OUT2>
 __global__ void __launch_bounds__(512) k_pstrip(PSweepArgs a) {
   extern __shared__ uint32_t Sp[];
+  pk_fold_zero(a);
+  uint32_t facc = 0u;
   constexpr int C = cc_ipow(Q, RA);
   const int TL = a.TL;
   const int tl = (int)threadIdx.x % TL;
@@ -1026,8 +1118,9 @@ __global__ void __launch_bounds__(512) k_pstrip(PSweepArgs a) {
       PK_FENCE(k);
     }
     pk_apply<Q, C, 0, RA, W, ND>(v, (1 << RA) - 1);
-    pk_store<C, W, ND, SUM, OUT2>(a, base, (uint32_t)C * s, (uint32_t)tl + (uint32_t)u * s,
-                                  v);
+    facc = pk_fold(a, facc, pk_store<C, W, ND, SUM, OUT2>(a, base, (uint32_t)C * s, (uint32_t)tl + (uint32_t)u * s,
+                                  v));
     __syncthreads();                 // the next tile rewrites the LDS rows
   }
+  pk_fold_flush<SUM>(a, facc);
 }

@@ -31,6 +31,11 @@ __global__ void k_fill16(uint16_t *p, uint16_t v, long long n) {
     p[i] = v;
 }
 
+// zero the CC_NSLOT census slots of one region
+__global__ void k_clear_slots(unsigned long long *slots, int stride) {
+  if (threadIdx.x < CC_NSLOT) slots[(size_t)threadIdx.x * stride] = 0ull;
+}
+
 __global__ void k_fill_u64(unsigned long long *p, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0ull;
@@ -47,7 +52,8 @@ __global__ void k_scatter_code(int32_t *A0, const long long *idx, int M) {
 // CC_BLOCK words, so every slab starts 16-byte aligned), each thread folds
 // 16-byte loads of the slab into a 32-bit partial, the wave combines the
 // partials with DPP row shifts and row broadcasts (the total lands in lane
-// 63) and one atomic per wave lands in the result.  The host sizes the slabs
+// 63), the waves' totals meet in LDS and one atomic per workgroup lands in
+// one of CC_NSLOT census slots (cc_census_flush).  The host sizes the slabs
 // so no 32-bit partial can wrap (at most 2^15 words per thread).
 
 // the words of slab b: [b * slab, min(N, (b + 1) * slab))
@@ -75,7 +81,7 @@ __device__ __forceinline__ uint32_t cc_census2(uint32_t x, uint32_t pat, int mu)
 template <int MODE>
 __global__ void __launch_bounds__(CC_BLOCK)
     k_cnt_census(const uint16_t *__restrict__ cnt, long long N, long long slab,
-                 int target, int mu, unsigned long long *out) {
+                 int target, int mu, unsigned long long *slots, int stride) {
   long long w0, w1;
   cc_slab(N, slab, &w0, &w1);
   const uint32_t pat = ((uint32_t)target & 0xFFFFu) * 0x00010001u;
@@ -92,14 +98,13 @@ __global__ void __launch_bounds__(CC_BLOCK)
     if (MODE == CC_EQ) part += (uint32_t)(c == target);
     else part += (uint32_t)(mu > c ? mu - c : 0);
   }
-  const uint32_t wsum = cc_dpp_wave_add(part);
-  if ((threadIdx.x & 63) == 0 && wsum) atomicAdd(out, (unsigned long long)wsum);
+  cc_census_flush<false>(part, slots, stride);
 }
 
-// max(0, max of the map plane): DPP wave maxima, one atomicMax per wave
+// max(0, max of the map plane): DPP wave maxima, one atomicMax per workgroup
 __global__ void __launch_bounds__(CC_BLOCK)
     k_map_peak(const int32_t *__restrict__ A, long long N, long long slab,
-               int *out) {
+               unsigned long long *slots, int stride) {
   long long w0, w1;
   cc_slab(N, slab, &w0, &w1);
   const long long v1 = w0 + ((w1 - w0) & ~3LL);
@@ -110,8 +115,7 @@ __global__ void __launch_bounds__(CC_BLOCK)
   }
   const long long j = v1 + threadIdx.x;
   if (j < w1) m = max(m, A[j]);
-  m = cc_dpp_wave_max0(m);
-  if ((threadIdx.x & 63) == 0 && m > 0) atomicMax(out, m);
+  cc_census_flush<true>((uint32_t)max(m, 0), slots, stride);
 }
 
 // bin of a positive map value: floor((v - 1) * nbins / vmax), clipped to the

@@ -83,6 +83,7 @@ struct PSweep {
   double cost;              // estimated relative time per word
   int na;                   // k_pfirstw: axes of the widening last round
   int mid_w8;               // k_pstrip: LDS exchange format (one dword)
+  int lcmp;                 // k_psweep, two rounds: LDS words in that format
   int moff[CC_MAXL], mbits[CC_MAXL];
 };
 
@@ -187,6 +188,12 @@ struct Engine {
   int32_t *h_out = nullptr;
   int32_t *h_out_dev = nullptr;  // h_out as the device addresses it
   unsigned long long *h_scal = nullptr;
+  // census slot regions (cc_census_flush): CEN_* regions of CC_NSLOT
+  // uint64 slots, cen_stride words apart (CC_SLOT_STRIDE), read back into
+  // h_cen
+  unsigned long long *d_cen = nullptr;
+  unsigned long long *h_cen = nullptr;
+  int cen_stride = 32;
   hipStream_t st = nullptr;
   long long bytes_dev = 0;
   long long stats[8] = {};
@@ -223,6 +230,9 @@ struct Engine {
   int pk_nt = ~1;            // CC_PK_NT: bit i = sweep i stores non-temporal
   int pk_wide = 0;           // CC_PK_WIDE: the planner may take k_pfirstw /
                              // k_pstrip sweeps
+  int pk_lcmp = 0;           // CC_PK_LCMP=1: two-round LDS sweeps exchange one
+                             // compact dword per word (when it holds one;
+                             // half the LDS, but the packing measured slower)
   int pk_first = 1;          // CC_PK_FIRST: first sweep 0 k_psweep, 1 persistent
                              // with prefetch, 2 LDS-free (cross-lane)
   PPlan pfield;              // packed plan of gain / loss (predicate seeds)
@@ -245,6 +255,13 @@ struct Engine {
   int o_mode = -1, o_target = 0, o_mu = 0;   // last listing that overflowed
   long long o_F = 0, o_plus = 0;
   unsigned long long cnt_ver = 1;            // bumped by every cnt change
+  // census folded into the last packed sweep (dct_set_fold, CC_FOLD=1 at
+  // init): region CEN_FMAP holds max(0, max A0) while map_fold, CEN_FDEF
+  // the cnt deficit of cnt state fold_cnt_ver
+  int fold = 0;
+  bool map_fold = false;
+  bool cnt_fold_pending = false;
+  unsigned long long fold_cnt_ver = 0;
   long long plus_words = 0;                  // words placed by +1 updates
   // swap evaluation fast path: drop lists (packed digits, drop index)
   uint64_t *lbuf = nullptr;
@@ -652,6 +669,8 @@ void code_bounds(double *s0, double *cap) {
 // Run the whole transform: in_kind seeds layer 0, out_kind says where the
 // result goes.  pmode is the predicate for CC_IN_PRED.
 int run_transform(int in_kind, int out_kind, int pmode) {
+  E.map_fold = false;
+  E.cnt_fold_pending = false;
   const int ns = (int)E.plan.size();
   const bool l16 = out_kind == CC_OUT_CNT ? E.l16_count
                    : out_kind == CC_OUT_LAYERS && in_kind == CC_IN_CODE
@@ -949,10 +968,21 @@ bool pk_sweep(const PBounds &B, int j, int k, bool pred_seed, bool cnt_out,
       w.threads = std::min(256, (w.TL + 63) / 64 * 64);
     }
   } else {
+    // two rounds (round 0 the low RR axes, round 1 the rest): the tile may
+    // cross LDS as one compact dword per word, the format of the layers
+    // after round 0, instead of ND register dwords
+    w.lcmp = 0;
+    if (E.pk_lcmp && w.ND > 1 && m <= 2 * w.RR) {
+      int dw[CC_MAXL], nd = 0, nar = 0;
+      if (compact_fmt(B, j + w.RR, L, dw, w.moff, w.mbits, &nd, &w.mid_w8, &nar) &&
+          nd == 1 && !nar)
+        w.lcmp = 1;
+    }
+    const int lnd = w.lcmp ? 1 : w.ND;
     w.TL = 0;
     for (int tl : {64, 32, 16}) {
       if (tl > w.s) continue;
-      const size_t lds = (size_t)w.ND * (pk_pad_h(w.P * tl) + 1) * 4;
+      const size_t lds = (size_t)lnd * (pk_pad_h(w.P * tl) + 1) * 4;
       if (lds <= (size_t)E.lds_budget) { w.TL = tl; w.lds = lds; break; }
     }
     if (!w.TL) return false;
@@ -1208,6 +1238,40 @@ PPlan pk_plan(double s0, double cap, bool pred_seed, bool cnt_out) {
   return best;
 }
 
+// census regions: field scans, the set-side size census, the folded map
+// maximum and the folded cnt deficit
+enum { CEN_SCAN = 0, CEN_ZL = 1, CEN_FMAP = 2, CEN_FDEF = 3, CEN_N = 4 };
+constexpr int CEN_MAXSTRIDE = 512;   // uint64 words (4 KB)
+
+unsigned long long *cen(int r) {
+  return E.d_cen + (size_t)r * CC_NSLOT * CEN_MAXSTRIDE;
+}
+
+int cen_clear(int r) {
+  hipLaunchKernelGGL(k_clear_slots, dim3(1), dim3(64), 0, E.st, cen(r), E.cen_stride);
+  CK(hipGetLastError());
+  return 0;
+}
+
+// the slots of region r back on the host, combined (sum, or mx: maximum)
+int cen_fetch(int r, bool mx, unsigned long long *out) {
+  CK(hipMemcpy2DAsync(E.h_cen, 8, cen(r), (size_t)E.cen_stride * 8, 8, CC_NSLOT,
+                      hipMemcpyDeviceToHost, E.st));
+  CK(hipStreamSynchronize(E.st));
+  unsigned long long v = 0;
+  for (int i = 0; i < CC_NSLOT; i++)
+    v = mx ? std::max(v, E.h_cen[i]) : v + E.h_cen[i];
+  *out = v;
+  return 0;
+}
+
+// the deficit that the recount which made this cnt state folded (the
+// deficit at mu = 1 is count_eq(0)); false when none is held
+bool folded_deficit(unsigned long long *r) {
+  if (!E.fold_cnt_ver || E.fold_cnt_ver != E.cnt_ver) return false;
+  return cen_fetch(CEN_FDEF, false, r) == 0;
+}
+
 hipError_t pk_launch(const PSweep &w, const PSweepArgs &a, hipEvent_t e0 = nullptr,
                      hipEvent_t e1 = nullptr) {
   const long long units = w.kern == 3 ? (w.ntiles + CC_BLOCK / 64 - 1) / (CC_BLOCK / 64)
@@ -1224,10 +1288,26 @@ int run_ptransform(const PPlan &pl, int in_kind, int out_kind, int plo,
   const int ns = (int)pl.sw.size();
   E.layers_live = 0;
   E.nswept = 0;
+  E.map_fold = false;
+  E.cnt_fold_pending = false;
+  // the last sweep folds the map maximum or the cnt deficit into census
+  // region CEN_FMAP or CEN_FDEF (32-bit wave deficit sums: X mu < 2^32),
+  // which the first sweep clears.  A one-sweep transform (a small cell)
+  // does not fold: clearing would cost a launch of its own, as much as the
+  // scan it saves.
+  unsigned long long *fslot = nullptr;
+  if (E.fold && ns > 1 &&
+      (out_kind == CC_OUT_MAP ||
+       (out_kind == CC_OUT_CNT && (double)E.X * E.mu < 4294967296.0)))
+    fslot = cen(out_kind == CC_OUT_MAP ? CEN_FMAP : CEN_FDEF);
   for (int i = 0; i < ns; i++) {
     const PSweep &w = pl.sw[i];
     PSweepArgs a;
     memset(&a, 0, sizeof(a));
+    a.fold = i == ns - 1 ? fslot : nullptr;
+    a.fold_zero = i == 0 ? fslot : nullptr;
+    a.fold_mu = E.mu;
+    a.fold_stride = E.cen_stride;
     a.cnt = E.cnt;
     a.code = E.A[0];
     a.cw = E.d_cw;
@@ -1261,6 +1341,7 @@ int run_ptransform(const PPlan &pl, int in_kind, int out_kind, int plo,
     a.st_nt = (E.pk_nt >> i) & 1;
     a.na = w.na;
     a.mid_w8 = w.mid_w8;
+    a.lcmp = w.lcmp;
     for (int d = 0; d < E.L; d++) { a.moff[d] = w.moff[d]; a.mbits[d] = w.mbits[d]; }
     // real-layer slots: layer d = slot * ND + dword (interleaved layout)
     const int S = 32 / w.W;
@@ -1279,6 +1360,8 @@ int run_ptransform(const PPlan &pl, int in_kind, int out_kind, int plo,
     else CK(pk_launch(w, a));
   }
   E.nswept = ns;
+  E.map_fold = fslot && out_kind == CC_OUT_MAP;
+  E.cnt_fold_pending = fslot && out_kind == CC_OUT_CNT;
   return 0;
 }
 
@@ -1370,7 +1453,7 @@ PPlan pk_code_plan(bool cnt_out) {
   // the planner is a shortest path over every sweep boundary pair: keep its
   // result while the seed bounds and the schedule options stay the same
   const int opts[8] = {(int)cnt_out, E.lds_budget, E.pk_col, E.pk_first,
-                       E.pk_narrow | (E.pk_wide << 4),
+                       E.pk_narrow | (E.pk_wide << 4) | (E.pk_lcmp << 5),
                        E.packed, E.sparse, (int)(g_cuts & 0x7FFFFFFF) ^ (int)(g_cuts >> 31)};
   if (E.cp_ok && E.cp_s0 == s0 && E.cp_cap == cap && !memcmp(opts, E.cp_opts, sizeof(opts)))
     return E.cp_plan;
@@ -1407,6 +1490,7 @@ int fetch_scalar(int i, unsigned long long *r) {
   *r = E.h_scal[0];
   return 0;
 }
+
 
 // copy nb words to d_words through pinned staging (caller synchronises
 // before the staging buffer is reused)
@@ -1493,20 +1577,20 @@ int ensure_zlist(int pmode, int target, long long lim) {
   // or for the deficient predicate the deficit, an upper bound on the
   // count) decides whether a listing can fit, so a state far from a cover
   // never pays for a listing of millions of words
-  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, E.st, E.d_scal + 2, 1);
-  CK(hipGetLastError());
-  {
+  unsigned long long bound = 0;
+  if (!((pmode == CC_DEF || (target == 0 && E.mu == 1)) && folded_deficit(&bound))) {
+    if (int rc = cen_clear(CEN_ZL)) return rc;
     const Slabs sl = scan_slabs(E.X);
     if (pmode == CC_DEF)
       hipLaunchKernelGGL(k_cnt_census<CC_DEF>, dim3(sl.blocks), dim3(CC_BLOCK), 0,
-                         E.st, E.cnt, E.X, sl.slab, 0, E.mu, E.d_scal + 2);
+                         E.st, E.cnt, E.X, sl.slab, 0, E.mu, cen(CEN_ZL), E.cen_stride);
     else
       hipLaunchKernelGGL(k_cnt_census<CC_EQ>, dim3(sl.blocks), dim3(CC_BLOCK), 0,
-                         E.st, E.cnt, E.X, sl.slab, target, E.mu, E.d_scal + 2);
+                         E.st, E.cnt, E.X, sl.slab, target, E.mu, cen(CEN_ZL),
+                         E.cen_stride);
+    CK(hipGetLastError());
+    if (int rc = cen_fetch(CEN_ZL, false, &bound)) return rc;
   }
-  CK(hipGetLastError());
-  unsigned long long bound = 0;
-  if (int rc = fetch_scalar(2, &bound)) return rc;
   if ((long long)bound > lim) {
     E.stats[3]++;
     E.o_mode = pmode; E.o_target = target; E.o_mu = E.mu;
@@ -1679,7 +1763,7 @@ void release_all() {
   for (int d = 0; d < CC_MAXL; d++) { E.A[d] = nullptr; E.H[d] = nullptr; }
   void *bufs[] = {E.cnt, E.pat, E.pat2, E.d_code, E.d_cw, E.d_toff,
                   E.d_words, E.d_out,
-                  E.d_scal, E.d_hist, E.d_eidx64, E.d_eidx32, E.d_eval,
+                  E.d_scal, E.d_cen, E.d_hist, E.d_eidx64, E.d_eidx32, E.d_eval,
                   E.d_bcnt, E.d_boff, E.zbuf, E.lbuf, E.ldrop, E.d_swap,
                   E.d_sres};
   for (void *b : bufs)
@@ -1687,7 +1771,7 @@ void release_all() {
   E.cnt = nullptr; E.pat = E.pat2 = nullptr; E.d_code = E.d_words = nullptr;
   E.d_cw = nullptr; E.d_cm = nullptr; E.d_toff = nullptr; E.toff_cap = 0;
   E.toff_cur = nullptr;
-  E.d_out = nullptr; E.d_scal = nullptr; E.d_hist = nullptr;
+  E.d_out = nullptr; E.d_scal = nullptr; E.d_hist = nullptr; E.d_cen = nullptr;
   E.d_eidx64 = nullptr; E.d_eidx32 = nullptr; E.d_eval = nullptr;
   E.d_bcnt = E.d_boff = nullptr;
   E.zbuf = nullptr; E.lbuf = nullptr; E.ldrop = nullptr;
@@ -1701,6 +1785,7 @@ void release_all() {
   if (E.h_out) { (void)hipHostFree(E.h_out); E.h_out = nullptr; }
   E.h_out_dev = nullptr;
   if (E.h_scal) { (void)hipHostFree(E.h_scal); E.h_scal = nullptr; }
+  if (E.h_cen) { (void)hipHostFree(E.h_cen); E.h_cen = nullptr; }
   for (auto &e : E.ev)
     if (e) { (void)hipEventDestroy(e); e = nullptr; }
   if (E.st) { (void)hipStreamDestroy(E.st); E.st = nullptr; }
@@ -1815,7 +1900,8 @@ static int init_cell(int problem, int n, const int *ax, int R,
     int dev = 0, mx = 0;
     CK(hipGetDevice(&dev));
     CK(hipDeviceGetAttribute(&mx, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-    E.lds_max = mx > 0 ? mx : 64 * 1024;
+    // (less the packed kernels' static census LDS)
+    E.lds_max = (mx > 0 ? mx : 64 * 1024) - 256;
     int ncu = 0;
     CK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     E.ncu = ncu > 0 ? ncu : 256;
@@ -1857,6 +1943,7 @@ static int init_cell(int problem, int n, const int *ax, int R,
       dalloc(&E.d_cw, (size_t)CODECAP * 12) ||
       dalloc(&E.d_words, (size_t)CODECAP * 8) ||
       dalloc(&E.d_out, (size_t)CODECAP * 4) || dalloc(&E.d_scal, 64) ||
+      dalloc(&E.d_cen, (size_t)CEN_N * CC_NSLOT * CEN_MAXSTRIDE * 8) ||
       dalloc(&E.d_hist, 4096 * 4)) {
     release_all();
     return -3;
@@ -1881,6 +1968,12 @@ static int init_cell(int problem, int n, const int *ax, int R,
     E.h_out_dev = nullptr;     // small gathers then copy back like large ones
   }
   CK(hipHostMalloc((void **)&E.h_scal, 64, hipHostMallocDefault));
+  CK(hipHostMalloc((void **)&E.h_cen, CC_NSLOT * 8, hipHostMallocDefault));
+  {
+    const char *cs = getenv("CC_SLOT_STRIDE");
+    const int st = cs ? atoi(cs) : 32;
+    E.cen_stride = st >= 1 && st <= CEN_MAXSTRIDE ? st : 32;
+  }
   for (auto &e : E.ev) CK(hipEventCreate(&e));
   const char *pe = getenv("DCT_TIMING");
   E.prof = pe ? atoi(pe) : 0;
@@ -1906,6 +1999,11 @@ static int init_cell(int problem, int n, const int *ax, int R,
     const char *rw = getenv("CC_REGION_WALK");
     E.region_walk_mode = rw ? atoi(rw) : 1;
     E.cnt_ver = 1;
+    E.map_fold = false;
+    E.cnt_fold_pending = false;
+    E.fold_cnt_ver = 0;
+    const char *fo = getenv("CC_FOLD");
+    E.fold = fo ? (atoi(fo) != 0) : 0;
     E.plus_words = 0;
     E.z_mode = E.o_mode = -1;
     if (E.set_ok1) {
@@ -1962,6 +2060,8 @@ static int init_cell(int problem, int n, const int *ax, int R,
     E.pk_nt = pnt ? (int)strtol(pnt, nullptr, 0) : ~1;
     const char *pwd = getenv("CC_PK_WIDE");
     E.pk_wide = pwd ? atoi(pwd) : 0;
+    const char *plc = getenv("CC_PK_LCMP");
+    E.pk_lcmp = plc ? atoi(plc) : 0;
     const char *pn = getenv("CC_PK_NARROW");
     E.pk_narrow = pn ? atoi(pn) : 0;
     const char *pcm = getenv("CC_COMPACT");
@@ -2134,6 +2234,8 @@ int dct_transform(int init_mode, int out_mode) {
     E.active_size = E.codeM;
     E.cnt_ver++;
     E.o_mode = -1;
+    E.fold_cnt_ver = E.cnt_fold_pending ? E.cnt_ver : 0;
+    E.cnt_fold_pending = false;
   }
   return 0;
 }
@@ -2317,37 +2419,47 @@ int dct_loss_owner(const long long *words, int M, int32_t *out) {
 // ---- reductions and queries (reference dct_hc.cu:979-1157) ---------------
 long long dct_count_eq(int target) {
   if (!E.live) return -1;
-  if (int rc = zero_scalars()) return rc;
+  unsigned long long fr;
+  if (target == 0 && E.mu == 1 && folded_deficit(&fr)) return (long long)fr;
+  if (int rc = cen_clear(CEN_SCAN)) return rc;
   const Slabs sl = scan_slabs(E.X);
   hipLaunchKernelGGL(k_cnt_census<CC_EQ>, dim3(sl.blocks), dim3(CC_BLOCK), 0,
-                     E.st, E.cnt, E.X, sl.slab, target, E.mu, E.d_scal);
+                     E.st, E.cnt, E.X, sl.slab, target, E.mu, cen(CEN_SCAN), E.cen_stride);
   CK(hipGetLastError());
   unsigned long long r;
-  if (int rc = fetch_scalar(0, &r)) return rc;
+  if (int rc = cen_fetch(CEN_SCAN, false, &r)) return rc;
   return (long long)r;
 }
 
 long long dct_count_deficit(void) {
   if (!E.live) return -1;
-  if (int rc = zero_scalars()) return rc;
+  unsigned long long fr;
+  if (folded_deficit(&fr)) return (long long)fr;
+  if (int rc = cen_clear(CEN_SCAN)) return rc;
   const Slabs sl = scan_slabs(E.X);
   hipLaunchKernelGGL(k_cnt_census<CC_DEF>, dim3(sl.blocks), dim3(CC_BLOCK), 0,
-                     E.st, E.cnt, E.X, sl.slab, 0, E.mu, E.d_scal);
+                     E.st, E.cnt, E.X, sl.slab, 0, E.mu, cen(CEN_SCAN), E.cen_stride);
   CK(hipGetLastError());
   unsigned long long r;
-  if (int rc = fetch_scalar(0, &r)) return rc;
+  if (int rc = cen_fetch(CEN_SCAN, false, &r)) return rc;
   return (long long)r;
 }
 
 int dct_map_max(int32_t *out_max) {
   if (!E.live || !out_max) return -1;
-  if (int rc = zero_scalars()) return rc;
+  unsigned long long r;
+  if (E.map_fold) {
+    // folded by the transform's last sweep
+    if (int rc = cen_fetch(CEN_FMAP, true, &r)) return rc;
+    *out_max = (int32_t)(r & 0xFFFFFFFFull);
+    return 0;
+  }
+  if (int rc = cen_clear(CEN_SCAN)) return rc;
   const Slabs sl = scan_slabs(E.X);
   hipLaunchKernelGGL(k_map_peak, dim3(sl.blocks), dim3(CC_BLOCK), 0, E.st,
-                     E.A[0], E.X, sl.slab, (int *)E.d_scal);
+                     E.A[0], E.X, sl.slab, cen(CEN_SCAN), E.cen_stride);
   CK(hipGetLastError());
-  unsigned long long r;
-  if (int rc = fetch_scalar(0, &r)) return rc;
+  if (int rc = cen_fetch(CEN_SCAN, true, &r)) return rc;
   *out_max = (int32_t)(r & 0xFFFFFFFFull);
   return 0;
 }
@@ -2892,7 +3004,7 @@ int dct_plan_probe(int n, const int *ax, int R, int kind, int s0, int M,
     if (E.pw[i + 1] > (1LL << 40)) { E = saved; return -1; }
   }
   E.X = E.pw[n];
-  E.lds_max = 160 * 1024;
+  E.lds_max = 160 * 1024 - 256;
   const char *lb = getenv("CC_SWEEP_LDS");
   E.lds_budget = std::min(lb ? std::max(4096, atoi(lb)) : 64 * 1024, E.lds_max);
   const char *pe = getenv("CC_PACKED");
@@ -2908,6 +3020,8 @@ int dct_plan_probe(int n, const int *ax, int R, int kind, int s0, int M,
   E.sparse = psc ? atoi(psc) : 1;
   const char *pwd = getenv("CC_PK_WIDE");
   E.pk_wide = pwd ? atoi(pwd) : 0;
+  const char *plc = getenv("CC_PK_LCMP");
+  E.pk_lcmp = plc ? atoi(plc) : 0;
   const PPlan pl = kind == 0 ? pk_plan(1.0, 1e300, true, false)
                              : pk_plan((double)s0, (double)M, false, true);
   int ns = 0;
@@ -3188,6 +3302,18 @@ int dct_sweep_times(float *ms, int cap) {
   if (!E.live || !ms) return -1;
   for (int i = 0; i < E.nswept && i < cap; i++) ms[i] = E.sweep_ms[i];
   return E.nswept;
+}
+
+// dct_set_fold(1): later packed transforms fold their census into the last
+// sweep (gain / loss: the map maximum; recount: the cnt deficit), which
+// dct_map_max, dct_count_eq(0) and dct_count_deficit then read instead of
+// scanning the field.  The transform pays a workgroup epilogue and one
+// atomic per workgroup.  Returns the previous setting.
+int dct_set_fold(int on) {
+  if (!E.live || on < 0 || on > 1) return -1;
+  const int was = E.fold;
+  E.fold = on;
+  return was;
 }
 
 }  // extern "C"

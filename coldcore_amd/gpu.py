@@ -36,7 +36,7 @@ ADDITIVE = ("dct_init3w", "dct_init3x", "dct_sweep_plan", "dct_sweep_plan2",
             "dct_plan_probe", "dct_autotune", "dct_sweep_profile",
             "dct_sweep_times", "dct_sweep_force", "dct_map_read",
             "dct_transform_layers", "dct_layers_read_at", "dct_region_features",
-            "dct_region_propose")
+            "dct_region_propose", "dct_set_fold")
 
 
 def _rc_msg(rc):
@@ -73,7 +73,8 @@ def _bind(path):
               "dct_sweep_plan2", "dct_plan_probe", "dct_autotune",
               "dct_sweep_profile", "dct_sweep_force", "dct_map_read",
               "dct_sweep_times", "dct_transform_layers",
-              "dct_layers_read_at", "dct_region_features", "dct_region_propose"):
+              "dct_layers_read_at", "dct_region_features", "dct_region_propose",
+              "dct_set_fold"):
         getattr(lib, f).restype = ctypes.c_int
     for f in ("dct_count_eq", "dct_count_deficit", "dct_map_extract",
               "dct_map_extract64", "dct_host_bytes"):
@@ -476,6 +477,15 @@ class GpuBackend:
 
     def sweep_profile(self, on=True):
         _check("dct_sweep_profile", self._lib.dct_sweep_profile(1 if on else 0))
+
+    def set_fold(self, on=True):
+        """Fold the map maximum / cnt deficit into later transforms (read
+        by map_max / count_eq(0) / uncovered without a field scan); returns
+        the previous setting."""
+        r = int(self._lib.dct_set_fold(1 if on else 0))
+        if r < 0:
+            _check("dct_set_fold", r)
+        return bool(r)
 
     def sweep_times(self):
         buf = (ctypes.c_float * 16)()

@@ -131,6 +131,10 @@ class Searcher:
     def __init__(self, backend, seed=1, deadline=None, on_log=None,
                  on_round=None, on_solved=None):
         self.b = backend
+        # every recount here is followed by uncovered() and most gain maps
+        # by map_max(): let the transforms fold those censuses
+        if hasattr(backend, "set_fold"):
+            backend.set_fold(True)
         self.rng = np.random.default_rng(seed * 1000003 + 12345)
         self.deadline = deadline
         self.on_log = on_log or (lambda s: None)

@@ -248,6 +248,16 @@ int  dct_sweep_force(const int *cuts, int ncuts);
 int  dct_sweep_profile(int on);
 /* Per-sweep milliseconds of the last transform; returns the sweep count. */
 int  dct_sweep_times(float *ms, int cap);
+/* on = 1: later packed transforms fold a census of what their last sweep
+ * writes (gain / loss: max(0, max A0); recount: sum max(0, mu - cnt)) into
+ * device slots, and dct_map_max / dct_count_eq(0) / dct_count_deficit read
+ * it instead of scanning the field while the field is the one that
+ * transform wrote.  Off at init (CC_FOLD=1 turns it on); costs the
+ * transform a workgroup epilogue and one atomic per workgroup (~1.5 us on
+ * 2^24 words), saves a 4 / 2 B/word scan per query.  Returns the previous
+ * setting, -1 with no live cell or a bad argument.  No reference
+ * counterpart (its reductions are separate passes). */
+int  dct_set_fold(int on);
 
 /* ---- additive: the learned ruin proposal's network -------------------
  * No counterpart in the reference (its search has no learned operator).

@@ -67,6 +67,70 @@ def test_packed_mu2_predicates(gpu_cell, axes, R):
     _check_cell(g, NumpyDP(axes, R, mu=2), rng, mults=2)
 
 
+@pytest.mark.parametrize("lcmp", ["1", "0"])
+@pytest.mark.parametrize("axes,R", [([5] * 8, 3), ([3] * 12, 3)],
+                         ids=["5^8-R3", "3^12-R3"])
+def test_compact_lds_two_round_sweep(gpu_cell, monkeypatch, axes, R, lcmp):
+    """Two-round LDS-tiled sweeps with two-dword register words: the tile
+    crosses LDS as one compact dword per word (CC_PK_LCMP=1, half the LDS)
+    or as both register dwords; fields exact either way."""
+    monkeypatch.setenv("CC_PK_LCMP", lcmp)
+    monkeypatch.setenv("CC_AUTOTUNE", "0")
+    print(f"compact LDS {axes[0]}^{len(axes)} R{R} lcmp={lcmp}", flush=True)
+    g = gpu_cell(axes, R)
+    last = g.sweep_plan2(0)[-1]
+    assert (last["kernel"], last["ND"], last["rounds"], last["TL"]) == (1, 2, 2, 64)
+    per_word = 4 if lcmp == "1" else 8
+    words = last["P"] * last["TL"]
+    assert per_word * words <= last["lds_bytes"] <= per_word * (words + words // 64 + 1)
+    rng = np.random.default_rng(len(axes) + R)
+    _check_cell(g, NumpyDP(axes, R), rng, mults=3)
+
+
+@pytest.mark.parametrize("fold", ["api", "env", "off"])
+@pytest.mark.parametrize("axes,R,mu", [([2] * 16, 3, 1), ([2] * 10, 3, 1),
+                                       ([5] * 8, 3, 1), ([3] * 9, 3, 2),
+                                       ([2] * 14, 4, 2)],
+                         ids=["2^16-R3", "2^10-R3-one-sweep", "5^8-R3",
+                              "3^9-R3-mu2", "2^14-R4-mu2"])
+def test_folded_census_equals_field_scans(gpu_cell, monkeypatch, axes, R, mu, fold):
+    """The last packed sweep folds the map maximum (gain / loss) and the cnt
+    deficit (recount) into the transform; map_max, count_eq(0) and the
+    deficit then equal what a scan of the fields gives, also after updates
+    change cnt (the folded deficit no longer applies) and across gain /
+    loss / recount in any order.  Off (the default) scans every time."""
+    if fold == "env":
+        monkeypatch.setenv("CC_FOLD", "1")
+    monkeypatch.setenv("CC_AUTOTUNE", "0")
+    print(f"folded census {axes[0]}^{len(axes)} R{R} mu{mu} fold={fold}", flush=True)
+    rng = np.random.default_rng(len(axes) * 7 + R)
+    g = gpu_cell(axes, R, mu=mu)
+    if fold == "api":
+        assert g.set_fold(True) is False          # off at init
+        assert g.set_fold(True) is True
+    size = int(np.prod(axes))
+
+    def census():
+        cnt = g.read_cnt().astype(np.int64)
+        assert g.count_eq(0) == int((cnt == 0).sum())
+        assert g.uncovered() == int(np.maximum(0, mu - cnt).sum())
+
+    for m in (3, max(4, size // 200), max(5, size // 20)):
+        code = rng.integers(0, size, size=m).astype(np.int64)
+        g.recount(code)
+        census()
+        for which in ("gain_map", "loss_map", "gain_map"):
+            getattr(g, which)()
+            assert g.map_max() == max(0, int(g.map_all().max())), which
+        census()
+        g.ball_update(rng.integers(0, size, size=2).astype(np.int64), +1)
+        census()
+        g.loss_map()
+        g.ball_update(code[:1], -1)
+        assert g.map_max() == max(0, int(g.map_all().max()))
+        census()
+
+
 def test_packed_equals_unpacked_engine_k2_20_3(monkeypatch):
     """The two GPU paths agree on a 2^20-word cell (gain, loss, recount)."""
     from coldcore_amd.gpu import GpuBackend
