@@ -314,6 +314,47 @@ def run_batch64(GpuBackend, rounds=200, k=2, quick=False):
     return out
 
 
+def run_census(GpuBackend, reps=60):
+    """The search loop's query pairs on K_2(24,3) (the headline code): a
+    gain pass followed by map_max, and a recount followed by the deficit
+    (uncovered), each timed through the ABI with the field scans and with
+    the census folded into the transform (dct_set_fold); both answers are
+    checked against a host scan of the fields."""
+    size = 1 << 24
+    code = np.flatnonzero(np.random.default_rng(0).random(size) < 1e-4).astype(np.int64)
+    out = {"cell": "K_2(24,3)", "code_words": int(len(code)), "calls": reps}
+    g = GpuBackend([2] * 24, 3)
+    try:
+        g.recount(code)
+        g.gain_map()
+        ref_max = max(0, int(g.map_all().max()))
+        ref_unc = int((g.read_cnt() == 0).sum())
+        for fold in (False, True):
+            g.set_fold(fold)
+            g.recount(code)
+            assert g.uncovered() == ref_unc
+            g.gain_map()
+            assert g.map_max() == ref_max
+            tg, tr = [], []
+            for _ in range(reps):
+                t = time.perf_counter()
+                g.gain_map()
+                g.map_max()
+                tg.append(time.perf_counter() - t)
+            for _ in range(reps // 2):
+                t = time.perf_counter()
+                g.recount(code)
+                g.uncovered()
+                tr.append(time.perf_counter() - t)
+            key = "folded" if fold else "scans"
+            out[key] = {"gain_plus_map_max_us": round(float(np.median(tg)) * 1e6, 1),
+                        "recount_plus_uncovered_us": round(float(np.median(tr)) * 1e6, 1)}
+        g.set_fold(False)
+    finally:
+        g.close()
+    return out
+
+
 def run_batch64_cells(top=16):
     """Where the 64-cell batched transform's time goes: the autotuned
     batched gain pass of each family's cells on their own, and of every cell
@@ -757,7 +798,7 @@ def main():
     ap.add_argument("--only", choices=("near_cover", "batch64", "batch64-quick",
                                        "batch64-cells",
                                        "cells", "learned", "learned-1m", "walks",
-                                       "cpu_calib", "config5",
+                                       "cpu_calib", "config5", "census",
                                        "learned-train", "learned-train-r4"),
                     help="run one side section and print its JSON only")
     ap.add_argument("--out", help="learned-train: where to write the proposal "
@@ -800,6 +841,8 @@ def main():
                             log=lambda m: print(m, file=sys.stderr, flush=True))
         elif args.only == "config5":
             r = run_config5(GpuBackend)
+        elif args.only == "census":
+            r = run_census(GpuBackend)
         elif args.only == "walks":
             # a short run of the update / query kernels (for the profiler)
             size = 1 << 24
@@ -945,6 +988,8 @@ def main():
         section("config 1")
         rec["near_cover"] = run_near_cover(GpuBackend)
         section("near cover")
+        rec["census"] = run_census(GpuBackend)
+        section("census")
         try:
             rec["batch64"] = run_batch64(GpuBackend)
         except RuntimeError as e:          # reported, never hidden
