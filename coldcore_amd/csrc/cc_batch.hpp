@@ -291,11 +291,7 @@ __global__ void __launch_bounds__(CC_BLOCK)
     if (with_map) ccb_key_max(best, map[i], i);
   }
   const long long zs = cc_block_sum((long long)z);
-  for (int o = 32; o > 0; o >>= 1) {
-    const unsigned long long b2 = __shfl_xor(best, o, 64);
-    best = b2 > best ? b2 : best;
-  }
-  if ((threadIdx.x & 63) == 0 && best) atomicMax(&c.amax, best);
+  cc_key_flush(best, &c.amax);
   if (threadIdx.x == 0 && zs) atomicAdd(&c.nzero, (unsigned long long)zs);
 }
 

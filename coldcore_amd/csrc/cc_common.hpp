@@ -207,15 +207,27 @@ __device__ __forceinline__ void cc_key_max(unsigned long long &best, uint32_t g,
   }
 }
 
-// wave maximum of the keys, one atomicMax per wave (every lane calls it)
+// workgroup maximum of the keys, one atomicMax per workgroup: the waves'
+// maxima meet in LDS (every thread of the workgroup calls it, after a
+// barrier that follows the previous call's read of the LDS slots).  One
+// atomic per wave made a large cell's tiles queue ~10 ns each on the one
+// key address.
 __device__ __forceinline__ void cc_key_flush(unsigned long long best,
                                              unsigned long long *amax) {
+  __shared__ unsigned long long kb[16];
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
     const unsigned long long b2 = __shfl_xor(best, o, 64);
     best = b2 > best ? b2 : best;
   }
-  if ((threadIdx.x & 63) == 0 && best) atomicMax(amax, best);
+  if ((threadIdx.x & 63) == 0) kb[threadIdx.x >> 6] = best;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int nw = (int)(blockDim.x + 63) >> 6;
+    unsigned long long m = 0;
+    for (int i = 0; i < nw; i++) m = kb[i] > m ? kb[i] : m;
+    if (m) atomicMax(amax, m);
+  }
 }
 
 // LDS layout of a generic tile: layer plane d starts at d * cc_gplane(PT),
