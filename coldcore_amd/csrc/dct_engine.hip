@@ -1258,15 +1258,35 @@ int cen_clear(int r) {
   return 0;
 }
 
-// the slots of region r back on the host, combined (sum, or mx: maximum)
-int cen_fetch(int r, bool mx, unsigned long long *out) {
-  CK(hipMemcpy2DAsync(E.h_cen, 8, cen(r), (size_t)E.cen_stride * 8, 8, CC_NSLOT,
-                      hipMemcpyDeviceToHost, E.st));
-  CK(hipStreamSynchronize(E.st));
+// enqueue the copy of region r's slots to its host mirror h_cen[r]
+int cen_copy(int r) {
+  CK(hipMemcpy2DAsync(E.h_cen + (size_t)r * CC_NSLOT, 8, cen(r),
+                      (size_t)E.cen_stride * 8, 8, CC_NSLOT, hipMemcpyDeviceToHost,
+                      E.st));
+  return 0;
+}
+
+// region r's host mirror combined (sum, or mx: maximum)
+unsigned long long cen_combine(int r, bool mx) {
+  const unsigned long long *h = E.h_cen + (size_t)r * CC_NSLOT;
   unsigned long long v = 0;
-  for (int i = 0; i < CC_NSLOT; i++)
-    v = mx ? std::max(v, E.h_cen[i]) : v + E.h_cen[i];
-  *out = v;
+  for (int i = 0; i < CC_NSLOT; i++) v = mx ? std::max(v, h[i]) : v + h[i];
+  return v;
+}
+
+// the slots of region r back on the host, combined
+int cen_fetch(int r, bool mx, unsigned long long *out) {
+  if (int rc = cen_copy(r)) return rc;
+  CK(hipStreamSynchronize(E.st));
+  *out = cen_combine(r, mx);
+  return 0;
+}
+
+// a folded region's value: the transform that folded it also queued the copy
+// of its slots, so the host mirror is final once the stream is idle
+int cen_folded(int r, bool mx, unsigned long long *out) {
+  CK(hipStreamSynchronize(E.st));
+  *out = cen_combine(r, mx);
   return 0;
 }
 
@@ -1274,7 +1294,7 @@ int cen_fetch(int r, bool mx, unsigned long long *out) {
 // deficit at mu = 1 is count_eq(0)); false when none is held
 bool folded_deficit(unsigned long long *r) {
   if (!E.fold_cnt_ver || E.fold_cnt_ver != E.cnt_ver) return false;
-  return cen_fetch(CEN_FDEF, false, r) == 0;
+  return cen_folded(CEN_FDEF, false, r) == 0;
 }
 
 hipError_t pk_launch(const PSweep &w, const PSweepArgs &a, hipEvent_t e0 = nullptr,
@@ -1365,6 +1385,8 @@ int run_ptransform(const PPlan &pl, int in_kind, int out_kind, int plo,
     else CK(pk_launch(w, a));
   }
   E.nswept = ns;
+  if (fslot)   // the census comes back with the transform (no second round trip)
+    if (int rc = cen_copy(out_kind == CC_OUT_MAP ? CEN_FMAP : CEN_FDEF)) return rc;
   E.map_fold = fslot && out_kind == CC_OUT_MAP;
   E.cnt_fold_pending = fslot && out_kind == CC_OUT_CNT;
   return 0;
@@ -1973,7 +1995,8 @@ static int init_cell(int problem, int n, const int *ax, int R,
     E.h_out_dev = nullptr;     // small gathers then copy back like large ones
   }
   CK(hipHostMalloc((void **)&E.h_scal, 64, hipHostMallocDefault));
-  CK(hipHostMalloc((void **)&E.h_cen, CC_NSLOT * 8, hipHostMallocDefault));
+  CK(hipHostMalloc((void **)&E.h_cen, (size_t)CEN_N * CC_NSLOT * 8,
+                   hipHostMallocDefault));
   {
     const char *cs = getenv("CC_SLOT_STRIDE");
     const int st = cs ? atoi(cs) : 32;
@@ -2457,7 +2480,7 @@ int dct_map_max(int32_t *out_max) {
   unsigned long long r;
   if (E.map_fold) {
     // folded by the transform's last sweep
-    if (int rc = cen_fetch(CEN_FMAP, true, &r)) return rc;
+    if (int rc = cen_folded(CEN_FMAP, true, &r)) return rc;
     *out_max = (int32_t)(r & 0xFFFFFFFFull);
     return 0;
   }
